@@ -1,0 +1,195 @@
+/*
+ * summit_gpe.h — C ABI of libsummit_gpe.so, the MI355X (gfx950) genotype-probe
+ * engine behind SUMMIT's stochastic LD scores.
+ *
+ * Conventions
+ *   - Every entry point returns 0 on success and a nonzero status otherwise;
+ *     gpe_last_error() returns the message of the calling thread's last
+ *     failure (the reference raises std::runtime_error with the same texts,
+ *     surfaced as Python RuntimeError by its bindings).
+ *   - Pointers named *_dev are device (HBM) pointers; *_host are host
+ *     pointers.  Sizes are element counts.  Matrices are described by a
+ *     leading dimension; "column-major (N, Q)" means element (n, q) at
+ *     q * ld + n, matching the reference's Fortran arrays.
+ *   - stream is a hipStream_t (NULL = legacy default stream).  Calls enqueue
+ *     work and return; results are complete after the stream is synchronized,
+ *     except where a host output is written (those calls synchronize).
+ *   - A gpe_geno handle owns one genotype matrix resident in HBM in two packed
+ *     2-bit layouts (SNP-major and sample-major) plus its per-SNP statistics.
+ *
+ * Reference paths below are relative to the SUMMIT repository root.
+ */
+#ifndef SUMMIT_GPE_H
+#define SUMMIT_GPE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gpe_geno gpe_geno;
+
+enum {
+    GPE_OK = 0,
+    GPE_EINVAL = 1,   /* bad argument / shape (reference: std::runtime_error) */
+    GPE_EDEVICE = 2,  /* HIP runtime failure */
+    GPE_EINTERNAL = 3
+};
+
+enum { GPE_IMPUTE_MEAN = 0, GPE_IMPUTE_HWE = 1 };      /* src/native/common/genotype.hpp:33-36 */
+enum { GPE_RAND_RADEMACHER = 0, GPE_RAND_GAUSSIAN = 1 }; /* src/native/gwldcore.cpp:1681-1682 */
+enum { GPE_LAYOUT_SNP_RAW = 0, GPE_LAYOUT_SNP_WORK = 1, GPE_LAYOUT_SAMPLE_WORK = 2 };
+
+/* Message of the calling thread's last failure ("" if none). */
+const char* gpe_last_error(void);
+
+/* Library version string. */
+const char* gpe_version(void);
+
+/* Select the HIP device used by subsequent calls of this thread. */
+int gpe_set_device(int device);
+
+/* ------------------------------------------------------------------------
+ * Genotype residency
+ * ------------------------------------------------------------------------ */
+
+/* Allocate a zeroed device genotype matrix of n_samples x n_snps.
+ * Replaces the mmap'd BED cache entry (src/native/common/genotype.cpp:350-410)
+ * with an HBM-resident matrix. */
+int gpe_geno_create(int64_t n_samples, int64_t n_snps, gpe_geno** out);
+
+/* Copy SNP rows [snp_begin, snp_begin + snp_count) of a host BED body (the
+ * bytes after the 3-byte magic; host_stride = ceil(n_total/4) bytes per SNP)
+ * into the handle, keeping only the sorted global sample ids rows_host
+ * (NULL = all n_total samples; then n_total must equal n_samples).
+ * Follows the decode of src/native/common/genotype.cpp:474-574. */
+int gpe_geno_upload_bed(gpe_geno* g, const uint8_t* bed_host, int64_t n_total, int64_t host_stride,
+                        int64_t snp_begin, int64_t snp_count, const int64_t* rows_host, void* stream);
+
+/* Fill the handle with synthetic genotypes (no reference counterpart:
+ * workload generator).  copy_prob = 0 gives i.i.d. Binomial(2, MAF). */
+int gpe_geno_synthesize(gpe_geno* g, uint64_t seed, double maf_lo, double maf_hi, double miss_rate,
+                        int32_t block_len, double copy_prob, void* stream);
+
+/* Derive the working codes and per-SNP statistics: observed counts, mean and
+ * inverse sd (src/native/common/genotype.cpp:629-689), HWE hash completion of
+ * missing calls for impute_mode == GPE_IMPUTE_HWE
+ * (src/native/common/genotype.cpp:122-184, :576-595; snp_offset = global
+ * index of the handle's SNP 0, global_rows_host = global sample ids or NULL),
+ * the sample-major layout, and sorted per-sample missing-call lists. */
+int gpe_geno_prepare(gpe_geno* g, int32_t ddof, int32_t impute_mode, uint64_t impute_seed,
+                     int64_t snp_offset, const int64_t* global_rows_host, void* stream);
+
+/* Host copies of the per-SNP statistics (any pointer may be NULL).  Counts
+ * are of the working codes (after HWE completion when prepared that way). */
+int gpe_geno_stats(gpe_geno* g, int32_t* nobs_host, int64_t* sum_host, int64_t* sumsq_host,
+                   double* mean_host, double* inv_sd_host);
+
+/* Download one packed layout (rows of `host_stride` bytes; SNP layouts have
+ * n_snps rows of ceil(n_samples/4) used bytes, the sample layout n_samples rows
+ * of ceil(n_snps/4) used bytes). */
+int gpe_geno_download(gpe_geno* g, int32_t layout, uint8_t* host, int64_t host_stride);
+
+/* Download rows [row_begin, row_begin + row_count) of one packed layout. */
+int gpe_geno_download_rows(gpe_geno* g, int32_t layout, int64_t row_begin, int64_t row_count, uint8_t* host,
+                           int64_t host_stride);
+
+/* Sizes of the handle. */
+int gpe_geno_shape(gpe_geno* g, int64_t* n_samples, int64_t* n_snps, int64_t* n_missing);
+
+int gpe_geno_free(gpe_geno* g);
+
+/* ------------------------------------------------------------------------
+ * Exact integer products over the working genotype counts g (missing -> 0)
+ * ------------------------------------------------------------------------ */
+
+/* U[b][n] (+)= sum_{i in [snp_begin, snp_end)} g[i][n] * Z[i - snp_begin][b]
+ * Z: int8 row-major (snp_end - snp_begin) x n_cols (ldz >= n_cols), device.
+ * U: int32 column-major (n_samples, n_cols), ldu >= n_samples, device.
+ * |U| must stay below 2^31 (checked from max|Z| and the SNP count).
+ * The unit-weight form of phase 1 (src/native/gwldcore.cpp:1809-1834). */
+int gpe_gz_i32(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const int8_t* z_dev, int64_t ldz,
+               int32_t n_cols, int32_t* u_dev, int64_t ldu, int32_t accumulate, void* stream);
+
+/* W[i - snp_begin][b] = sum_n g[i][n] * U[b][n], exact int64, row-major
+ * (ldw >= n_cols).  U as produced by gpe_gz_i32 with |U| < 2^23 (three int8
+ * limbs).  The unit-weight form of phase 2's GEMM
+ * (src/native/gwldcore.cpp:2485-2491). */
+int gpe_gtu_i64(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const int32_t* u_dev, int64_t ldu,
+                int32_t n_cols, int64_t* w_dev, int64_t ldw, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Stochastic LD-score phases (reference estimator, FP64 semantics)
+ * ------------------------------------------------------------------------ */
+
+/* inv_out_dev[j - snp_begin] = 1/sqrt(max(||x_j||^2/(N-1), eps)) for the
+ * standardized SNPs of the prepared handle, no covariates.
+ * Replaces compute_inv_resvar_from_geno_block (src/native/gwldcore.cpp:3220-3333)
+ * as used by precompute_residual_variances_bed (:3335-3566) and the fused
+ * inv_out of phase 1. */
+int gpe_resvar_inv(gpe_geno* g, int64_t snp_begin, int64_t snp_end, double eps, double* inv_out_dev,
+                   void* stream);
+
+/* Phase 1, X.Z: for SNPs [snp_begin, snp_end) in blocks of step_size (block b
+ * draws its probes from make_seed(root_seed, block_start, v_start)),
+ *   Xz[:, k*v_count + v] (+)= sum_i x_i * inv_i * sqrt(a_ik) * z_iv
+ * annot_dev: (snp_end - snp_begin) x n_bins row-major FP64; inv_dev: FP64
+ * per SNP; Xz: column-major (n_samples, n_bins * v_count), ldxz >= n_samples.
+ * bin_mode_dev (may be NULL = accumulate every bin): per bin 1 = overwrite,
+ * 2 = accumulate.  limbs: 5 or 6 int8 limbs per coefficient.
+ * Replaces phase1_compute_Xz_bed_chunk (src/native/gwldcore.cpp:1582-1923,
+ * exported at :3673-3688); one call may span many reference blocks. */
+int gpe_phase1_xz(gpe_geno* g, int64_t snp_begin, int64_t snp_end, int64_t step_size,
+                  const double* annot_dev, int32_t n_bins, const double* inv_dev, int32_t v_start,
+                  int32_t v_count, int32_t rand_dist, uint64_t root_seed, double* xz_dev, int64_t ldxz,
+                  const uint8_t* bin_mode_dev, int32_t limbs, void* stream);
+
+/* Phase 2, X^T(X.Z): for SNPs j in [snp_begin, snp_end) and bins k,
+ *   meansq[j][k] += sum_v (x_j . Xz[:, k*tile_nvecs + v])^2 * inv_j^2/(N_denom-1)^2
+ * with the non-finite / extreme-update guards of src/native/gwldcore.cpp:295-417
+ * (extreme_warn <= 0 disables the extreme test), and optional probe fourth-moment
+ * sums (mc_aggregate_sumsq_dev[k], mc_per_snp_sumsq_dev[j][k]; NULL to skip).
+ * meansq: row-major (n_snps_total, n_bins) with ld_meansq >= n_bins, indexed by
+ * absolute SNP j.  inv_left_dev: per SNP of the range.  guard_counts_host
+ * (may be NULL) receives {nonfinite, extreme} counts (the call synchronizes).
+ * Replaces phase2_accum_XtXz_bed (src/native/gwldcore.cpp:2329-2649, exported
+ * at :3759-3770) and, with HWE-prepared handles, the Mailman variants
+ * (:2651-3119). */
+int gpe_phase2_accum(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const double* inv_left_dev,
+                     int32_t tile_nvecs, const double* xz_dev, int64_t ldxz, int32_t n_bins,
+                     int32_t n_denom, double* meansq_dev, int64_t ld_meansq, double* mc_aggregate_sumsq_dev,
+                     double* mc_per_snp_sumsq_dev, double extreme_warn, int32_t drop_extreme, int32_t limbs,
+                     unsigned long long* guard_counts_host, void* stream);
+
+/* Probe values of the reference's genome-wide streams for blocks starting at
+ * blk_starts_host[0..n_blocks) (last block ends at blk_end): Rademacher on the
+ * GPU, Gaussian with the host std::normal_distribution.  z_host: row-major
+ * (blk_end - blk_starts_host[0]) x v_count FP64.  Follows
+ * src/native/gwldcore.cpp:445-458, :1679-1716. */
+int gpe_probes(int32_t rand_dist, uint64_t root_seed, const int64_t* blk_starts_host, int32_t n_blocks,
+               int64_t blk_end, int32_t v_start, int32_t v_count, double* z_host);
+
+/* ------------------------------------------------------------------------
+ * Kernel timing (no reference counterpart: the reference's per-block phase
+ * timers, src/native/gwldcore.cpp:225-258, print wall time to stderr; these
+ * record HIP events around the contraction kernels on their own stream).
+ * ------------------------------------------------------------------------ */
+enum {
+    GPE_PROF_P1_CONTRACT = 0, /* phase-1 packed contraction (X.Z) */
+    GPE_PROF_P2_CONTRACT = 1, /* phase-2 packed contraction with fused Work^2 epilogue */
+    GPE_PROF_GZ = 2,          /* exact G.Z */
+    GPE_PROF_GTU = 3,         /* exact G^T U */
+    GPE_PROF_KINDS = 4
+};
+int gpe_profile_enable(int32_t on);
+/* Accumulated milliseconds and launch count of one kernel kind (synchronizes
+ * on the recorded events). */
+int gpe_profile_read(int32_t kind, double* total_ms, int64_t* launches);
+int gpe_profile_reset(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SUMMIT_GPE_H */

@@ -1,0 +1,469 @@
+"""ORACLE — CPU restatement of SUMMIT's genome-wide stochastic LD-score engine.
+
+This module is test infrastructure. Only ``tests/``, ``__graft_entry__.smoke()``
+and the ``cpu_baseline`` leg of ``bench.py`` may import it, and only as the
+checker or the timed CPU baseline; the product (``summit_amd``) never does.
+
+Every function restates one piece of the reference and cites it. Paths are
+relative to the reference repository root.
+
+Pins (each function is checked by something other than a retyping of itself):
+
+* BED layout / decode / stats / mean and HWE standardization
+  (``decode_block``, ``hwe_impute``, ``standardize_block``): checked against the
+  reference's own ``read_block_standardized_double`` compiled from
+  ``src/native/common/genotype.cpp`` (``oracle/_ref``), live where the
+  reference tree exists and through committed golden vectors elsewhere, and
+  against the textbook column standardization of
+  ``tests/test_pgen_gwld.py:244-262``.
+* Probe streams (``probe_block``): MT19937-64 known answer from the C++
+  standard; libstdc++'s own ``std::mt19937_64``/``std::normal_distribution``
+  through ``oracle/_ref``; golden vectors.
+* Phase 1 / phase 2 / finalize (``phase1_chunk``, ``phase2_accum``,
+  ``gw_ldscore``): the dense matrix oracle of ``tests/test_gwld_mc.py:42-128``
+  (hand-checked small matrices), invariance of the estimate to block size and
+  V-tiling where the reference says it is invariant, and the statistical pin of
+  ``tests/test_pgen_gwld.py:715-772`` (error against exact LD within 5 Monte
+  Carlo standard errors, variance ``2[(sum r^2)^2 - sum r^4]/B``).
+* Integer products (``gz_int``, ``gtu_int``): brute-force int64 loops on tiny
+  inputs and the identity ``sum_b U[:,b]^2 == (Z^T G G^T Z)`` trace checks.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+U64 = np.uint64
+MASK53 = U64((1 << 53) - 1)
+
+
+def _lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "libgpe_oracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle C library missing: build it with `make -C {_HERE}`")
+        lib = ctypes.CDLL(path)
+        lib.or_mt64_nth.restype = ctypes.c_uint64
+        lib.or_mt64_nth.argtypes = [ctypes.c_uint64, ctypes.c_int64]
+        lib.or_make_seed.restype = ctypes.c_uint64
+        lib.or_make_seed.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int]
+        for fn in (lib.or_rademacher_block, lib.or_gaussian_block):
+            fn.restype = None
+            fn.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                           ctypes.c_void_p]
+        lib.or_stream_raw.restype = None
+        lib.or_stream_raw.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int64,
+                                      ctypes.c_void_p]
+        _LIB = lib
+    return _LIB
+
+
+# ----------------------------------------------------------------------------
+# PLINK BED (SNP-major, magic 6C 1B 01, 4 samples per byte, low bits first).
+# Codes: 00 -> 0, 01 -> missing, 10 -> 1, 11 -> 2 copies of the counted allele
+# (src/native/common/genotype.cpp:135-144, :384-389).
+# ----------------------------------------------------------------------------
+BED_MAGIC = bytes([0x6C, 0x1B, 0x01])
+CODE_MISSING = 1
+
+
+def geno_to_codes(g: np.ndarray) -> np.ndarray:
+    """Map values {0,1,2,-9/negative=missing} to 2-bit codes (genotype.cpp:142-144)."""
+    g = np.asarray(g)
+    codes = np.full(g.shape, CODE_MISSING, dtype=np.uint8)
+    codes[g == 0] = 0
+    codes[g == 1] = 2
+    codes[g == 2] = 3
+    return codes
+
+
+def codes_to_geno(codes: np.ndarray) -> np.ndarray:
+    """2-bit codes to int8 genotype values with missing -> -9."""
+    lut = np.array([0, -9, 1, 2], dtype=np.int8)
+    return lut[codes]
+
+
+def pack_codes(codes_snp_major: np.ndarray) -> np.ndarray:
+    """codes (M, N) uint8 -> packed (M, ceil(N/4)) uint8."""
+    c = np.asarray(codes_snp_major, dtype=np.uint8)
+    M, N = c.shape
+    bps = (N + 3) // 4
+    pad = np.zeros((M, bps * 4), dtype=np.uint8)
+    pad[:, :N] = c
+    q = pad.reshape(M, bps, 4)
+    return (q[:, :, 0] | (q[:, :, 1] << 2) | (q[:, :, 2] << 4) | (q[:, :, 3] << 6)).astype(np.uint8)
+
+
+def unpack_codes(packed: np.ndarray, N: int) -> np.ndarray:
+    """packed (M, bps) -> codes (M, N)."""
+    p = np.asarray(packed, dtype=np.uint8)
+    out = np.empty((p.shape[0], p.shape[1] * 4), dtype=np.uint8)
+    for t in range(4):
+        out[:, t::4] = (p >> (2 * t)) & 3
+    return out[:, :N]
+
+
+def write_bed(prefix: str, codes_snp_major: np.ndarray, chrom=None, bp=None) -> None:
+    """Write .bed/.bim/.fam for codes (M, N)."""
+    M, N = codes_snp_major.shape
+    with open(prefix + ".bed", "wb") as f:
+        f.write(BED_MAGIC)
+        f.write(pack_codes(codes_snp_major).tobytes())
+    if chrom is None:
+        chrom = np.ones(M, dtype=np.int64)
+    if bp is None:
+        bp = np.arange(1, M + 1, dtype=np.int64) * 1000
+    with open(prefix + ".bim", "w") as f:
+        for j in range(M):
+            f.write(f"{int(chrom[j])}\trs{j}\t0\t{int(bp[j])}\tA\tG\n")
+    with open(prefix + ".fam", "w") as f:
+        for n in range(N):
+            f.write(f"f{n}\ti{n}\t0\t0\t0\t-9\n")
+
+
+def read_bed_matrix(prefix: str, n_total: int, m_total: int) -> np.ndarray:
+    raw = np.fromfile(prefix + ".bed", dtype=np.uint8)
+    if raw.size < 3 or bytes(raw[:3]) != BED_MAGIC:
+        raise RuntimeError("Invalid BED magic/mode (expected 6C 1B 01)")
+    bps = (n_total + 3) // 4
+    return raw[3:3 + bps * m_total].reshape(m_total, bps)
+
+
+# ----------------------------------------------------------------------------
+# Decode + stats: src/native/common/genotype.cpp:474-574
+# ----------------------------------------------------------------------------
+def decode_block(bed: np.ndarray, n_total: int, blk_start: int, blk_end: int, rows=None):
+    """Returns codes (L, N_sel) and per-SNP (nobs, sum, sumsq) as int64."""
+    codes = unpack_codes(bed[blk_start:blk_end], n_total)
+    if rows is not None:
+        codes = codes[:, np.asarray(rows, dtype=np.int64)]
+    obs = codes != CODE_MISSING
+    val = np.array([0, 0, 1, 2], dtype=np.int64)[codes]
+    nobs = obs.sum(axis=1).astype(np.int64)
+    s = val.sum(axis=1).astype(np.int64)
+    ss = (val * val).sum(axis=1).astype(np.int64)
+    return codes, nobs, s, ss
+
+
+# ----------------------------------------------------------------------------
+# HWE hash imputation: src/native/common/genotype.cpp:122-184, :576-595
+# ----------------------------------------------------------------------------
+def mix64(x):
+    x = np.asarray(x, dtype=U64)
+    with np.errstate(over="ignore"):
+        x = x + U64(0x9E3779B97F4A7C15)
+        x = (x ^ (x >> U64(30))) * U64(0xBF58476D1CE4E5B9)
+        x = (x ^ (x >> U64(27))) * U64(0x94D049BB133111EB)
+    return x ^ (x >> U64(31))
+
+
+def u01_from_u64(x):
+    return ((np.asarray(x, dtype=U64) >> U64(11)) & MASK53).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def hwe_seed_base(impute_seed: int, global_snp):
+    with np.errstate(over="ignore"):
+        h = mix64(U64(impute_seed))
+        gs = np.asarray(global_snp, dtype=np.int64).astype(np.uint32).astype(U64)
+        return h ^ mix64(gs + U64(0x243F6A8885A308D3))
+
+
+def hwe_thresholds(obs_sum, nobs):
+    obs_sum = np.asarray(obs_sum, dtype=np.float64)
+    nobs = np.asarray(nobs, dtype=np.float64)
+    p = np.where(nobs > 0, obs_sum / np.maximum(2.0 * nobs, 1.0), 0.0)
+    p = np.clip(p, 0.0, 1.0)
+    q0 = (1.0 - p) * (1.0 - p)
+    q01 = q0 + 2.0 * p * (1.0 - p)
+    return q0, q01
+
+
+def hwe_impute(codes, nobs, s, impute_seed: int, blk_start: int, rows=None):
+    """Complete missing codes by the deterministic HWE hash; returns completed
+    codes and the completed (sum, sumsq)."""
+    codes = codes.copy()
+    L, N = codes.shape
+    row_ids = np.arange(N, dtype=np.int64) if rows is None else np.asarray(rows, dtype=np.int64)
+    q0, q01 = hwe_thresholds(s, nobs)
+    for j in range(L):
+        miss = np.flatnonzero(codes[j] == CODE_MISSING)
+        if miss.size == 0:
+            continue
+        if q0[j] >= 1.0:
+            vals = np.zeros(miss.size, dtype=np.int64)
+        elif q01[j] <= 0.0:
+            vals = np.full(miss.size, 2, dtype=np.int64)
+        else:
+            base = hwe_seed_base(impute_seed, blk_start + j)
+            with np.errstate(over="ignore"):
+                gr = row_ids[miss].astype(np.uint32).astype(U64)
+                h = base ^ mix64(gr + U64(0x9E3779B97F4A7C15))
+            u = u01_from_u64(mix64(h))
+            vals = np.where(u < q0[j], 0, np.where(u < q01[j], 1, 2)).astype(np.int64)
+        codes[j, miss] = np.array([0, 2, 3], dtype=np.uint8)[vals]
+    val = np.array([0, 0, 1, 2], dtype=np.int64)[codes]
+    return codes, val.sum(axis=1), (val * val).sum(axis=1)
+
+
+# ----------------------------------------------------------------------------
+# Standardization: src/native/common/genotype.cpp:629-689, :802-838
+# ----------------------------------------------------------------------------
+def _mean_invsd(nobs, s, ss, ddof):
+    nobs = np.asarray(nobs, dtype=np.int64)
+    dn = nobs.astype(np.float64)
+    sf = np.asarray(s, dtype=np.float64)
+    ssf = np.asarray(ss, dtype=np.float64)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        mu = np.where(nobs > 0, sf / np.where(nobs > 0, dn, 1.0), 0.0)
+        M2 = np.where(nobs > 0, ssf - (sf * sf) / np.where(nobs > 0, dn, 1.0), 0.0)
+    M2 = np.where((M2 < 0.0) & (M2 > -1e-12), 0.0, M2)
+    denom = nobs - ddof
+    inv = np.ones_like(mu)
+    ok = (denom > 0) & (M2 > 0.0)
+    var = np.where(ok, M2 / np.where(denom > 0, denom, 1).astype(np.float64), 0.0)
+    ok &= var > 0.0
+    inv[ok] = 1.0 / np.sqrt(var[ok])
+    return mu, inv
+
+
+def standardize_block(codes, nobs, s, ss, ddof: int, mode: str = "mean", impute_seed: int = 0,
+                      blk_start: int = 0, rows=None):
+    """Standardized block X (N, L) in Fortran order, plus per-SNP mean/inv_sd.
+
+    mean mode: mu, inv_sd from observed calls; missing -> 0
+    (genotype.cpp:652-689). hwe mode: complete by the HWE hash first, then mu,
+    inv_sd over all N (genotype.cpp:802-830, :629-650)."""
+    L, N = codes.shape
+    if mode == "mean":
+        mu, inv = _mean_invsd(nobs, s, ss, ddof)
+        v = np.stack([(0.0 - mu) * inv, np.zeros_like(mu), (1.0 - mu) * inv, (2.0 - mu) * inv], axis=1)
+        X = np.take_along_axis(v, codes.astype(np.int64), axis=1)
+        return np.asfortranarray(X.T), mu, inv
+    if mode == "hwe":
+        comp, s2, ss2 = hwe_impute(codes, nobs, s, impute_seed, blk_start, rows)
+        full = np.full(L, N, dtype=np.int64)
+        mu, inv = _mean_invsd(full, s2, ss2, ddof)
+        # N<=0 or N-ddof<=0 keeps inv 1 (genotype.cpp:636-638)
+        v = np.stack([(0.0 - mu) * inv, np.zeros_like(mu), (1.0 - mu) * inv, (2.0 - mu) * inv], axis=1)
+        X = np.take_along_axis(v, comp.astype(np.int64), axis=1)
+        return np.asfortranarray(X.T), mu, inv
+    raise ValueError("impute_mode must be 'mean' or 'hwe'")
+
+
+def read_block_standardized(bed, n_total, blk_start, blk_end, rows=None, ddof=1, mode="mean",
+                            impute_seed=0):
+    codes, nobs, s, ss = decode_block(bed, n_total, blk_start, blk_end, rows)
+    X, _, _ = standardize_block(codes, nobs, s, ss, ddof, mode, impute_seed, blk_start, rows)
+    return X
+
+
+# ----------------------------------------------------------------------------
+# Probes: src/native/gwldcore.cpp:445-458, :1679-1716
+# ----------------------------------------------------------------------------
+def make_seed(root: int, block: int, v0: int) -> int:
+    return int(_lib().or_make_seed(root, block, v0))
+
+
+def mt64_nth(seed: int, n: int) -> int:
+    return int(_lib().or_mt64_nth(seed, n))
+
+
+def stream_raw(root: int, blk_start: int, v_start: int, n: int) -> np.ndarray:
+    out = np.empty(n, dtype=np.uint64)
+    _lib().or_stream_raw(root, blk_start, v_start, n, out.ctypes.data)
+    return out
+
+
+def probe_block(root: int, blk_start: int, v_start: int, L: int, V: int, dist: str = "rademacher"):
+    """Z (L, V), column-major fill order, float64."""
+    Z = np.empty((V, L), dtype=np.float64)
+    if dist == "rademacher":
+        _lib().or_rademacher_block(root, blk_start, v_start, L, V, Z.ctypes.data)
+    elif dist in ("gaussian", "normal"):
+        _lib().or_gaussian_block(root, blk_start, v_start, L, V, Z.ctypes.data)
+    else:
+        raise ValueError(f"unsupported rand_dist {dist!r}")
+    return Z.T  # (L, V), Fortran-contiguous
+
+
+# ----------------------------------------------------------------------------
+# Residual scale: src/native/gwldcore.cpp:3220-3333 (no covariates)
+# ----------------------------------------------------------------------------
+def inv_resvar(X, eps: float = 1e-10):
+    N = X.shape[0]
+    ss = np.einsum("ij,ij->j", X, X)
+    var = ss / float(N - 1)
+    safe = np.where(np.isnan(var), np.nan, np.maximum(var, eps))
+    return 1.0 / np.sqrt(safe)
+
+
+# ----------------------------------------------------------------------------
+# Phase 1: src/native/gwldcore.cpp:1002-1063 (bin CSR scale), :1582-1861
+# ----------------------------------------------------------------------------
+def bin_scales(annot_blk, inv):
+    """scale[i,k] = inv_i*sqrt(a_ik), 0 where the reference skips the entry."""
+    a = np.asarray(annot_blk, dtype=np.float64)
+    invd = np.asarray(inv, dtype=np.float64)[:, None]
+    with np.errstate(invalid="ignore", over="ignore"):
+        sc = invd * np.sqrt(np.where(a > 0, a, 0.0))
+    ok = (a != 0) & (a > 0) & np.isfinite(a) & np.isfinite(invd) & (invd > 0) & np.isfinite(sc)
+    return np.where(ok, sc, 0.0)
+
+
+def phase1_chunk(X, annot_blk, inv, Z):
+    """Xz contribution of one block: (N, B*V), column k*V+v."""
+    sc = bin_scales(annot_blk, inv)  # (L, B)
+    B = sc.shape[1]
+    V = Z.shape[1]
+    out = np.empty((X.shape[0], B * V), dtype=np.float64, order="F")
+    for k in range(B):
+        out[:, k * V:(k + 1) * V] = X @ (sc[:, k:k + 1] * Z)
+    return out
+
+
+# ----------------------------------------------------------------------------
+# Phase 2: src/native/gwldcore.cpp:2329-2600 (+ guards :295-417)
+# ----------------------------------------------------------------------------
+EXTREME_LD_WARN = 1.0e10
+QPANEL = 32768
+
+
+def phase2_accum(X, inv_left, tile_nvecs: int, Xz, meansq_accum, N_denom: int, blk_start: int,
+                 mc_aggregate_sumsq=None, mc_per_snp_sumsq=None, drop_extreme: bool = True):
+    L = X.shape[1]
+    Q = Xz.shape[1]
+    if tile_nvecs <= 0 or Q % tile_nvecs:
+        raise RuntimeError("Xz2d col count must be a multiple of tile_nvecs")
+    B = Q // tile_nvecs
+    denom = float(N_denom) - 1.0
+    if denom <= 0.0:
+        denom = 1.0
+    invd2 = 1.0 / (denom * denom)
+    inv = np.asarray(inv_left, dtype=np.float64)
+    left = np.where(np.isfinite(inv) & (inv > 0), inv * inv * invd2, 0.0)
+    qpanel = min(QPANEL, Q)
+    qpanel = (qpanel // 64) * 64
+    if qpanel < 64:
+        qpanel = min(Q, 64)
+    counts = {"nonfinite": 0, "extreme": 0}
+    for q0 in range(0, Q, qpanel):
+        q = min(qpanel, Q - q0)
+        work = X.T @ Xz[:, q0:q0 + q]
+        g = q0
+        while g < q0 + q:
+            k = g // tile_nvecs
+            v_in = g - k * tile_nvecs
+            ln = min(q0 + q - g, tile_nvecs - v_in)
+            w = work[:, g - q0:g - q0 + ln]
+            acc = np.sum(w * w, axis=1)
+            upd = acc * left
+            nonfin = ~np.isfinite(upd)
+            extreme = np.isfinite(upd) & (upd > EXTREME_LD_WARN * float(ln))
+            counts["nonfinite"] += int(nonfin.sum())
+            counts["extreme"] += int(extreme.sum())
+            keep = ~nonfin & ~(extreme & drop_extreme)
+            upd = np.where(upd < 0.0, 0.0, upd)
+            rows = blk_start + np.arange(L)
+            meansq_accum[rows[keep], k] += upd[keep]
+            if mc_aggregate_sumsq is not None or mc_per_snp_sumsq is not None:
+                w2 = w * w
+                sumsq = np.sum(w2 * w2, axis=1) * left * left
+                if mc_per_snp_sumsq is not None:
+                    mc_per_snp_sumsq[rows[keep], k] += sumsq[keep]
+                if mc_aggregate_sumsq is not None:
+                    mc_aggregate_sumsq[k] += float(np.sum(sumsq[keep]))
+            g += ln
+    return counts
+
+
+# ----------------------------------------------------------------------------
+# Driver: src/summit/ldscore/gw_ldscore.py:1753-2223 (BED, no covariates)
+# ----------------------------------------------------------------------------
+def bounded_vtile_sizes(nvecs: int, vmax: int, granularity: int = 64):
+    """src/summit/ldscore/gw_ldscore.py:687-697"""
+    width = min(nvecs, vmax)
+    if width >= granularity:
+        width = max(granularity, (width // granularity) * granularity)
+    return [min(width, nvecs - s) for s in range(0, nvecs, width)]
+
+
+@dataclass
+class GwResult:
+    ldscore: np.ndarray       # (M, B) float64, null-subtracted
+    meansq_accum: np.ndarray  # (M, B) raw accumulators
+    inv: np.ndarray           # (M,) residual scales
+    mc_per_snp_sumsq: np.ndarray | None = None
+
+
+def gw_ldscore(bed, n_total: int, m_total: int, annot=None, nvecs: int = 10, step_size: int = 1000,
+               seed: int = 1, ddof: int = 1, impute: str = "mean", vtiles=None, rows=None,
+               rand_dist: str = "rademacher", eps: float = 1e-10, mc: bool = False,
+               impute_seed: int | None = None) -> GwResult:
+    if annot is None:
+        annot = np.ones((m_total, 1), dtype=np.float64)
+    annot = np.asarray(annot, dtype=np.float64)
+    B = annot.shape[1]
+    if impute_seed is None:
+        impute_seed = int((np.uint64(seed) ^ np.uint64(0xA24BAED4963EE407)) & np.uint64(0xFFFFFFFFFFFFFFFF))
+    N = n_total if rows is None else len(rows)
+    blocks = [(s, min(m_total, s + step_size)) for s in range(0, m_total, step_size)]
+    if vtiles is None:
+        vtiles = [nvecs]
+    assert sum(vtiles) == nvecs
+    Xblk = [read_block_standardized(bed, n_total, s, e, rows, ddof, impute, impute_seed) for s, e in blocks]
+    inv_all = np.concatenate([inv_resvar(X, eps) for X in Xblk])
+    meansq = np.zeros((m_total, B), dtype=np.float64)
+    mcps = np.zeros((m_total, B), dtype=np.float64) if mc else None
+    N_denom = N - ddof + 1
+    v_start = 0
+    for Vt in vtiles:
+        Xz = np.zeros((N, B * Vt), dtype=np.float64, order="F")
+        for (s, e), X in zip(blocks, Xblk):
+            Z = probe_block(seed, s, v_start, e - s, Vt, rand_dist)
+            Xz += phase1_chunk(X, annot[s:e], inv_all[s:e], Z)
+        for (s, e), X in zip(blocks, Xblk):
+            phase2_accum(X, inv_all[s:e], Vt, Xz, meansq, N_denom, s, mc_per_snp_sumsq=mcps)
+        v_start += Vt
+    nsnps_bin = annot.sum(axis=0)
+    ld = meansq / float(nvecs) - (nsnps_bin / float(N - ddof))[None, :]
+    return GwResult(ld, meansq, inv_all, mcps)
+
+
+def exact_ld(bed, n_total: int, m_total: int, rows=None, ddof: int = 1, impute: str = "mean",
+             impute_seed: int = 0, eps: float = 1e-10, annot=None):
+    """Exact LD scores sum_i r_ij^2 a_ik - M_k/(N-1) from the dense correlation
+    (tests/test_pgen_gwld.py:732-739)."""
+    X = read_block_standardized(bed, n_total, 0, m_total, rows, ddof, impute, impute_seed)
+    N = X.shape[0]
+    d = float(N - 1)
+    x = X * inv_resvar(X, eps)[None, :]
+    corr = (x.T @ x) / d
+    r2 = corr * corr
+    if annot is None:
+        annot = np.ones((m_total, 1))
+    annot = np.asarray(annot, dtype=np.float64)
+    return r2 @ annot - annot.sum(axis=0)[None, :] / d, corr
+
+
+# ----------------------------------------------------------------------------
+# Exact integer products on masked genotype counts (missing -> 0)
+# ----------------------------------------------------------------------------
+def masked_geno(codes) -> np.ndarray:
+    return np.array([0, 0, 1, 2], dtype=np.int64)[codes]
+
+
+def gz_int(g_masked, Z) -> np.ndarray:
+    """U[n, b] = sum_i g[i, n] * Z[i, b] (int64)."""
+    return np.asarray(g_masked, dtype=np.int64).T @ np.asarray(Z, dtype=np.int64)
+
+
+def gtu_int(g_masked, U) -> np.ndarray:
+    """W[i, b] = sum_n g[i, n] * U[n, b] (int64)."""
+    return np.asarray(g_masked, dtype=np.int64) @ np.asarray(U, dtype=np.int64)

@@ -1,0 +1,819 @@
+// C ABI of libsummit_gpe.so (declared in include/summit_gpe.h).
+//
+// Host orchestration of the genotype-probe engine: device residency of the
+// genotype matrix, per-SNP statistics, probe streams, and the two LD-score
+// phases built from the packed-contraction kernel in gpe_kernels.hip.
+#include "../../include/summit_gpe.h"
+#include "gpe_internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <random>
+#include <string>
+#include <thread>
+#include <vector>
+
+using namespace gpe;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct Buf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct gpe_geno {
+    int device = 0;
+    int64_t n = 0, m = 0;
+    int64_t snp_stride = 0, smp_stride = 0;
+    int64_t m_alloc = 0, n_alloc = 0;
+    uint8_t* snp_raw = nullptr;
+    uint8_t* snp_work = nullptr;  // == snp_raw unless HWE-completed
+    uint8_t* smp_work = nullptr;
+    bool prepared = false;
+    int ddof = 1;
+    int impute_mode = GPE_IMPUTE_MEAN;
+    uint64_t impute_seed = 0;
+    int32_t* nobs = nullptr;
+    int64_t* sum = nullptr;
+    int64_t* sumsq = nullptr;
+    double* mean = nullptr;
+    double* inv_sd = nullptr;
+    int64_t n_missing = 0;
+    int64_t* miss_ptr = nullptr;
+    int32_t* miss_idx = nullptr;
+    std::map<std::string, Buf> ws;
+
+    void* scratch(const std::string& name, size_t bytes) {
+        Buf& b = ws[name];
+        if (b.bytes < bytes) {
+            if (b.p) GPE_HIP(hipFree(b.p));
+            b.p = nullptr;
+            b.bytes = 0;
+            GPE_HIP(hipMalloc(&b.p, std::max<size_t>(bytes, 256)));
+            b.bytes = std::max<size_t>(bytes, 256);
+        }
+        return b.p;
+    }
+};
+
+namespace {
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return GPE_OK;
+    } catch (const Error& e) {
+        g_err = e.what();
+        return std::strncmp(e.what(), "HIP error", 9) == 0 ? GPE_EDEVICE : GPE_EINVAL;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return GPE_EINTERNAL;
+    } catch (...) {
+        g_err = "unknown error";
+        return GPE_EINTERNAL;
+    }
+}
+
+inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+void require(bool ok, const std::string& msg) {
+    if (!ok) throw Error(msg);
+}
+
+int bitlen(uint64_t x) {
+    int b = 0;
+    while (x) {
+        ++b;
+        x >>= 1;
+    }
+    return b;
+}
+
+template <class T>
+T* dalloc(int64_t count) {
+    void* p = nullptr;
+    GPE_HIP(hipMalloc(&p, std::max<int64_t>(count, 1) * sizeof(T)));
+    return static_cast<T*>(p);
+}
+
+void dfree(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+// w[i][k] = inv_sd_i * inv_i * sqrt(a_ik), 0 where phase 1 skips the entry
+// (src/native/gwldcore.cpp:1002-1063: a == 0, a <= 0 or non-finite, inv <= 0
+// or non-finite, non-finite scale).
+__global__ void k_make_w(const double* annot, int n_bins, const double* inv, const double* inv_sd, int64_t k0,
+                         int64_t k1, double* w) {
+    const int64_t i = k0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= k1) return;
+    const double iv = inv[i - k0];
+    for (int k = 0; k < n_bins; ++k) {
+        const double a = annot[(i - k0) * n_bins + k];
+        double out = 0.0;
+        if (a != 0.0 && a > 0.0 && isfinite(a) && isfinite(iv) && iv > 0.0) {
+            const double sc = iv * sqrt(a);
+            if (isfinite(sc)) out = inv_sd[i] * sc;
+        }
+        w[i * n_bins + k] = out;
+    }
+}
+
+__global__ void k_absmax_i32(const int32_t* u, int64_t ldu, int64_t n, int n_cols, unsigned int* out) {
+    const int c = blockIdx.y;
+    unsigned int mx = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t v = u[(int64_t)c * ldu + i];
+        const unsigned int a = v < 0 ? (unsigned int)(-(int64_t)v) : (unsigned int)v;
+        mx = mx > a ? mx : a;
+    }
+    atomicMax(out, mx);
+}
+
+__global__ void k_scan_small(const int32_t* counts, int64_t n, int64_t* ptr) {
+    // single-thread exclusive scan; n is the sample count (done once per prepare)
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int64_t acc = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        ptr[i] = acc;
+        acc += counts[i];
+    }
+    ptr[n] = acc;
+}
+
+void free_prepared(gpe_geno* g) {
+    if (g->snp_work && g->snp_work != g->snp_raw) dfree(g->snp_work);
+    g->snp_work = g->snp_raw;
+    dfree(g->miss_ptr);
+    dfree(g->miss_idx);
+    g->miss_ptr = nullptr;
+    g->miss_idx = nullptr;
+    g->n_missing = 0;
+    g->prepared = false;
+}
+
+void require_prepared(const gpe_geno* g) {
+    require(g != nullptr, "null genotype handle");
+    require(g->prepared, "genotype handle is not prepared (call gpe_geno_prepare)");
+}
+
+struct LimbCfg {
+    int nl, ctc;
+};
+LimbCfg limb_cfg(int limbs) {
+    if (limbs == 5) return {5, 1};
+    if (limbs == 6) return {6, 1};
+    throw Error("limbs must be 5 or 6");
+}
+
+// Optional HIP-event timing of the engine's kernels, recorded on the stream
+// each kernel is launched on (gpe_profile_*).
+struct ProfEvent {
+    int kind;
+    hipEvent_t a, b;
+};
+struct Profiler {
+    bool on = false;
+    std::vector<ProfEvent> pending;
+    double ms[GPE_PROF_KINDS] = {};
+    int64_t count[GPE_PROF_KINDS] = {};
+};
+Profiler& prof() {
+    static Profiler p;
+    return p;
+}
+struct ProfScope {
+    int kind;
+    hipStream_t st;
+    hipEvent_t a = nullptr, b = nullptr;
+    ProfScope(int k, hipStream_t s) : kind(k), st(s) {
+        if (!prof().on) return;
+        GPE_HIP(hipEventCreate(&a));
+        GPE_HIP(hipEventCreate(&b));
+        GPE_HIP(hipEventRecord(a, st));
+    }
+    void stop() {
+        if (!a) return;
+        GPE_HIP(hipEventRecord(b, st));
+        prof().pending.push_back({kind, a, b});
+        a = nullptr;
+    }
+};
+
+int pick_splits(int64_t grid_xy, int64_t kcount, int64_t max_chunks_per_split) {
+    int64_t splits = 1;
+    while (grid_xy * splits < 1024 && splits * 2 <= kcount) splits *= 2;
+    while (ceil_div(kcount, splits) > max_chunks_per_split) ++splits;
+    return (int)splits;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* gpe_last_error(void) { return g_err.c_str(); }
+
+int gpe_profile_enable(int32_t on) {
+    return guarded([&] { prof().on = on != 0; });
+}
+
+int gpe_profile_read(int32_t kind, double* total_ms, int64_t* launches) {
+    return guarded([&] {
+        require(kind >= 0 && kind < GPE_PROF_KINDS, "unknown profile kind");
+        Profiler& p = prof();
+        for (auto& e : p.pending) {
+            GPE_HIP(hipEventSynchronize(e.b));
+            float ms = 0.f;
+            GPE_HIP(hipEventElapsedTime(&ms, e.a, e.b));
+            p.ms[e.kind] += ms;
+            p.count[e.kind] += 1;
+            (void)hipEventDestroy(e.a);
+            (void)hipEventDestroy(e.b);
+        }
+        p.pending.clear();
+        if (total_ms) *total_ms = p.ms[kind];
+        if (launches) *launches = p.count[kind];
+    });
+}
+
+int gpe_profile_reset(void) {
+    return guarded([&] {
+        double ms;
+        int64_t n;
+        if (gpe_profile_read(0, &ms, &n) != 0) throw Error(g_err);
+        Profiler& p = prof();
+        for (int k = 0; k < GPE_PROF_KINDS; ++k) {
+            p.ms[k] = 0.0;
+            p.count[k] = 0;
+        }
+    });
+}
+
+const char* gpe_version(void) { return "summit-gpe 0.1 (gfx950)"; }
+
+int gpe_set_device(int device) {
+    return guarded([&] { GPE_HIP(hipSetDevice(device)); });
+}
+
+int gpe_geno_create(int64_t n_samples, int64_t n_snps, gpe_geno** out) {
+    return guarded([&] {
+        require(out != nullptr, "null output handle");
+        require(n_samples > 1 && n_snps > 0, "genotype matrix needs n_samples > 1 and n_snps > 0");
+        require(n_samples < (int64_t)1 << 31 && n_snps < (int64_t)1 << 31, "genotype matrix too large");
+        auto* g = new gpe_geno();
+        try {
+            GPE_HIP(hipGetDevice(&g->device));
+            g->n = n_samples;
+            g->m = n_snps;
+            g->snp_stride = round_up(ceil_div(n_samples, 4), 64);
+            g->smp_stride = round_up(ceil_div(n_snps, 4), 64);
+            g->m_alloc = round_up(n_snps, kRowPad) + kRowPad;
+            g->n_alloc = round_up(n_samples, kRowPad) + kRowPad;
+            g->snp_raw = dalloc<uint8_t>(g->m_alloc * g->snp_stride);
+            g->snp_work = g->snp_raw;
+            g->smp_work = dalloc<uint8_t>(g->n_alloc * g->smp_stride);
+            launch_fill_u8(g->snp_raw, g->m_alloc * g->snp_stride, 0, nullptr);
+            launch_fill_u8(g->smp_work, g->n_alloc * g->smp_stride, 0, nullptr);
+            g->nobs = dalloc<int32_t>(g->m_alloc);
+            g->sum = dalloc<int64_t>(g->m_alloc);
+            g->sumsq = dalloc<int64_t>(g->m_alloc);
+            g->mean = dalloc<double>(g->m_alloc);
+            g->inv_sd = dalloc<double>(g->m_alloc);
+            launch_fill_u8(reinterpret_cast<uint8_t*>(g->mean), g->m_alloc * 8, 0, nullptr);
+            launch_fill_u8(reinterpret_cast<uint8_t*>(g->inv_sd), g->m_alloc * 8, 0, nullptr);
+            GPE_HIP(hipDeviceSynchronize());
+        } catch (...) {
+            gpe_geno_free(g);
+            throw;
+        }
+        *out = g;
+    });
+}
+
+int gpe_geno_free(gpe_geno* g) {
+    return guarded([&] {
+        if (!g) return;
+        free_prepared(g);
+        dfree(g->snp_raw);
+        dfree(g->smp_work);
+        dfree(g->nobs);
+        dfree(g->sum);
+        dfree(g->sumsq);
+        dfree(g->mean);
+        dfree(g->inv_sd);
+        for (auto& kv : g->ws) dfree(kv.second.p);
+        delete g;
+    });
+}
+
+int gpe_geno_shape(gpe_geno* g, int64_t* n_samples, int64_t* n_snps, int64_t* n_missing) {
+    return guarded([&] {
+        require(g != nullptr, "null genotype handle");
+        if (n_samples) *n_samples = g->n;
+        if (n_snps) *n_snps = g->m;
+        if (n_missing) *n_missing = g->n_missing;
+    });
+}
+
+int gpe_geno_upload_bed(gpe_geno* g, const uint8_t* bed_host, int64_t n_total, int64_t host_stride,
+                        int64_t snp_begin, int64_t snp_count, const int64_t* rows_host, void* stream) {
+    return guarded([&] {
+        require(g != nullptr && bed_host != nullptr, "null argument to gpe_geno_upload_bed");
+        require(host_stride >= ceil_div(n_total, 4), "host_stride smaller than ceil(n_total/4)");
+        require(snp_begin >= 0 && snp_count >= 0 && snp_begin + snp_count <= g->m, "SNP range out of bounds");
+        if (!rows_host) require(n_total == g->n, "without rows the BED sample count must equal n_samples");
+        hipStream_t st = S(stream);
+        int64_t* rows_dev = nullptr;
+        if (rows_host) {
+            for (int64_t i = 0; i < g->n; ++i) {
+                require(rows_host[i] >= 0 && rows_host[i] < n_total, "row_sel index out of range");
+                require(i == 0 || rows_host[i] > rows_host[i - 1], "row_sel must be sorted and unique");
+            }
+            rows_dev = static_cast<int64_t*>(g->scratch("upload_rows", g->n * sizeof(int64_t)));
+            GPE_HIP(hipMemcpyAsync(rows_dev, rows_host, g->n * sizeof(int64_t), hipMemcpyHostToDevice, st));
+        }
+        // stream through a pinned staging buffer in chunks of SNP rows
+        const int64_t chunk_rows = std::max<int64_t>(1, (int64_t)(64 << 20) / std::max<int64_t>(host_stride, 1));
+        uint8_t* pinned = nullptr;
+        GPE_HIP(hipHostMalloc(reinterpret_cast<void**>(&pinned), chunk_rows * host_stride, hipHostMallocDefault));
+        uint8_t* staging = static_cast<uint8_t*>(g->scratch("upload_staging", chunk_rows * host_stride));
+        try {
+            for (int64_t j0 = 0; j0 < snp_count; j0 += chunk_rows) {
+                const int64_t rows_now = std::min(chunk_rows, snp_count - j0);
+                GPE_HIP(hipStreamSynchronize(st));
+                std::memcpy(pinned, bed_host + j0 * host_stride, rows_now * host_stride);
+                GPE_HIP(hipMemcpyAsync(staging, pinned, rows_now * host_stride, hipMemcpyHostToDevice, st));
+                uint8_t* dst = g->snp_raw + (snp_begin + j0) * g->snp_stride;
+                if (rows_dev) {
+                    launch_select_rows(staging, host_stride, rows_now, rows_dev, g->n, dst, g->snp_stride, st);
+                } else {
+                    GPE_HIP(hipMemcpy2DAsync(dst, g->snp_stride, staging, host_stride, ceil_div(n_total, 4),
+                                             rows_now, hipMemcpyDeviceToDevice, st));
+                }
+                launch_clear_tail(dst, rows_now, g->snp_stride, g->n, st);
+            }
+            GPE_HIP(hipStreamSynchronize(st));
+        } catch (...) {
+            (void)hipHostFree(pinned);
+            throw;
+        }
+        GPE_HIP(hipHostFree(pinned));
+        free_prepared(g);
+    });
+}
+
+int gpe_geno_synthesize(gpe_geno* g, uint64_t seed, double maf_lo, double maf_hi, double miss_rate,
+                        int32_t block_len, double copy_prob, void* stream) {
+    return guarded([&] {
+        require(g != nullptr, "null genotype handle");
+        require(block_len > 0, "block_len must be positive");
+        require(maf_lo >= 0.0 && maf_hi <= 1.0 && maf_lo <= maf_hi, "MAF range must lie in [0, 1]");
+        require(miss_rate >= 0.0 && miss_rate < 1.0, "miss_rate must lie in [0, 1)");
+        launch_synth_markov(g->snp_raw, g->snp_stride, g->m, g->n, seed, maf_lo, maf_hi, miss_rate, block_len,
+                            copy_prob, copy_prob <= 0.0 ? 1 : 0, S(stream));
+        free_prepared(g);
+    });
+}
+
+int gpe_geno_prepare(gpe_geno* g, int32_t ddof, int32_t impute_mode, uint64_t impute_seed, int64_t snp_offset,
+                     const int64_t* global_rows_host, void* stream) {
+    return guarded([&] {
+        require(g != nullptr, "null genotype handle");
+        require(impute_mode == GPE_IMPUTE_MEAN || impute_mode == GPE_IMPUTE_HWE,
+                "impute_mode must be 'mean' or 'hwe'");
+        hipStream_t st = S(stream);
+        free_prepared(g);
+        g->ddof = ddof;
+        g->impute_mode = impute_mode;
+        g->impute_seed = impute_seed;
+        launch_snp_stats(g->snp_raw, g->snp_stride, g->m, g->n, g->nobs, g->sum, g->sumsq, st);
+        if (impute_mode == GPE_IMPUTE_HWE) {
+            g->snp_work = dalloc<uint8_t>(g->m_alloc * g->snp_stride);
+            GPE_HIP(hipMemcpyAsync(g->snp_work, g->snp_raw, g->m_alloc * g->snp_stride, hipMemcpyDeviceToDevice, st));
+            int64_t* rows_dev = nullptr;
+            if (global_rows_host) {
+                rows_dev = static_cast<int64_t*>(g->scratch("hwe_rows", g->n * sizeof(int64_t)));
+                GPE_HIP(hipMemcpyAsync(rows_dev, global_rows_host, g->n * sizeof(int64_t), hipMemcpyHostToDevice,
+                                       st));
+            }
+            launch_hwe_impute(g->snp_work, g->snp_stride, g->m, g->n, g->nobs, g->sum, impute_seed, snp_offset,
+                              rows_dev, st);
+            launch_snp_stats(g->snp_work, g->snp_stride, g->m, g->n, g->nobs, g->sum, g->sumsq, st);
+            launch_std_params(g->nobs, g->sum, g->sumsq, g->m, g->n, ddof, 1, g->mean, g->inv_sd, st);
+        } else {
+            launch_std_params(g->nobs, g->sum, g->sumsq, g->m, g->n, ddof, 0, g->mean, g->inv_sd, st);
+        }
+        launch_transpose(g->snp_work, g->snp_stride, g->m, g->smp_work, g->smp_stride, g->n, st);
+        // per-sample missing-call lists (mean mode only: HWE leaves none)
+        int32_t* counts = static_cast<int32_t*>(g->scratch("miss_counts", g->n * sizeof(int32_t)));
+        launch_miss_count(g->smp_work, g->smp_stride, g->n, counts, st);
+        g->miss_ptr = dalloc<int64_t>(g->n + 1);
+        hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(1), 0, st, counts, g->n, g->miss_ptr);
+        GPE_HIP(hipGetLastError());
+        int64_t total = 0;
+        GPE_HIP(hipMemcpyAsync(&total, g->miss_ptr + g->n, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        GPE_HIP(hipStreamSynchronize(st));
+        g->n_missing = total;
+        if (total > 0) {
+            g->miss_idx = dalloc<int32_t>(total);
+            launch_miss_fill(g->smp_work, g->smp_stride, g->n, g->miss_ptr, g->miss_idx, st);
+        }
+        GPE_HIP(hipStreamSynchronize(st));
+        g->prepared = true;
+    });
+}
+
+int gpe_geno_stats(gpe_geno* g, int32_t* nobs_host, int64_t* sum_host, int64_t* sumsq_host, double* mean_host,
+                   double* inv_sd_host) {
+    return guarded([&] {
+        require_prepared(g);
+        if (nobs_host) GPE_HIP(hipMemcpy(nobs_host, g->nobs, g->m * sizeof(int32_t), hipMemcpyDeviceToHost));
+        if (sum_host) GPE_HIP(hipMemcpy(sum_host, g->sum, g->m * sizeof(int64_t), hipMemcpyDeviceToHost));
+        if (sumsq_host) GPE_HIP(hipMemcpy(sumsq_host, g->sumsq, g->m * sizeof(int64_t), hipMemcpyDeviceToHost));
+        if (mean_host) GPE_HIP(hipMemcpy(mean_host, g->mean, g->m * sizeof(double), hipMemcpyDeviceToHost));
+        if (inv_sd_host) GPE_HIP(hipMemcpy(inv_sd_host, g->inv_sd, g->m * sizeof(double), hipMemcpyDeviceToHost));
+    });
+}
+
+int gpe_geno_download_rows(gpe_geno* g, int32_t layout, int64_t row_begin, int64_t row_count, uint8_t* host,
+                           int64_t host_stride) {
+    return guarded([&] {
+        require(g != nullptr && host != nullptr, "null argument to gpe_geno_download_rows");
+        const bool snp = layout == GPE_LAYOUT_SNP_RAW || layout == GPE_LAYOUT_SNP_WORK;
+        require(snp || (layout == GPE_LAYOUT_SAMPLE_WORK && g->prepared), "unknown or unprepared layout");
+        const int64_t rows = snp ? g->m : g->n;
+        const int64_t stride = snp ? g->snp_stride : g->smp_stride;
+        const int64_t used = snp ? ceil_div(g->n, 4) : ceil_div(g->m, 4);
+        const uint8_t* src = layout == GPE_LAYOUT_SNP_RAW ? g->snp_raw : (snp ? g->snp_work : g->smp_work);
+        require(row_begin >= 0 && row_count >= 0 && row_begin + row_count <= rows, "row range out of bounds");
+        require(host_stride >= used, "host_stride too small");
+        if (row_count == 0) return;
+        GPE_HIP(hipDeviceSynchronize());
+        GPE_HIP(hipMemcpy2D(host, host_stride, src + row_begin * stride, stride, used, row_count,
+                            hipMemcpyDeviceToHost));
+    });
+}
+
+int gpe_geno_download(gpe_geno* g, int32_t layout, uint8_t* host, int64_t host_stride) {
+    return guarded([&] {
+        require(g != nullptr && host != nullptr, "null argument to gpe_geno_download");
+        const uint8_t* src;
+        int64_t rows, stride, used;
+        if (layout == GPE_LAYOUT_SNP_RAW || layout == GPE_LAYOUT_SNP_WORK) {
+            src = layout == GPE_LAYOUT_SNP_RAW ? g->snp_raw : g->snp_work;
+            rows = g->m;
+            stride = g->snp_stride;
+            used = ceil_div(g->n, 4);
+        } else if (layout == GPE_LAYOUT_SAMPLE_WORK) {
+            require(g->prepared, "sample-major layout exists after gpe_geno_prepare");
+            src = g->smp_work;
+            rows = g->n;
+            stride = g->smp_stride;
+            used = ceil_div(g->m, 4);
+        } else {
+            throw Error("unknown layout");
+        }
+        require(host_stride >= used, "host_stride too small");
+        GPE_HIP(hipDeviceSynchronize());
+        GPE_HIP(hipMemcpy2D(host, host_stride, src, stride, used, rows, hipMemcpyDeviceToHost));
+    });
+}
+
+int gpe_resvar_inv(gpe_geno* g, int64_t snp_begin, int64_t snp_end, double eps, double* inv_out_dev,
+                   void* stream) {
+    return guarded([&] {
+        require_prepared(g);
+        require(snp_begin >= 0 && snp_end <= g->m && snp_begin <= snp_end, "SNP range out of bounds");
+        require(g->n > 1, "Invalid block dimensions in compute_inv_resvar_from_geno_block");
+        launch_resvar_inv(g->nobs, g->sum, g->sumsq, g->mean, g->inv_sd, snp_begin, snp_end, g->n,
+                          g->impute_mode == GPE_IMPUTE_HWE ? 1 : 0, eps, inv_out_dev, S(stream));
+    });
+}
+
+int gpe_gz_i32(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const int8_t* z_dev, int64_t ldz, int32_t n_cols,
+               int32_t* u_dev, int64_t ldu, int32_t accumulate, void* stream) {
+    return guarded([&] {
+        require_prepared(g);
+        require(snp_begin >= 0 && snp_end <= g->m && snp_begin < snp_end, "SNP range out of bounds");
+        require(n_cols > 0 && ldz >= n_cols && ldu >= g->n, "bad leading dimension");
+        const int64_t L = snp_end - snp_begin;
+        require(L * 2 * 128 < ((int64_t)1 << 31), "too many SNPs for one int32 product; split the range");
+        hipStream_t st = S(stream);
+        const int ctc = 2;
+        const int groups = (int)ceil_div(n_cols, 32 * ctc);
+        const int64_t coef_ctiles = (int64_t)groups * ctc;
+        const int64_t kc0 = snp_begin / kKChunk, kc1 = ceil_div(snp_end, kKChunk);
+        const int64_t kcount = kc1 - kc0;
+        int8_t* coef = static_cast<int8_t*>(g->scratch("coef", kcount * coef_ctiles * 4096));
+        launch_coef_i8(z_dev, ldz, snp_begin, snp_end, n_cols, coef_ctiles, coef, st);
+        if (!accumulate)
+            for (int c = 0; c < n_cols; ++c)
+                launch_fill_u8(reinterpret_cast<uint8_t*>(u_dev + (int64_t)c * ldu), g->n * 4, 0, st);
+        ContractArgs a{};
+        a.packed = g->smp_work;
+        a.stride = g->smp_stride;
+        a.r0 = 0;
+        a.r1 = g->n;
+        a.coef = coef;
+        a.coef_ctiles = coef_ctiles;
+        a.kc0 = kc0;
+        a.kc1 = kc1;
+        a.ctiles_used = (int)ceil_div(n_cols, 32);
+        a.out_i32 = u_dev;
+        a.ldo = ldu;
+        a.cols_out = n_cols;
+        a.out_row0 = 0;
+        const int64_t gx = ceil_div(g->n, 256);
+        const int splits = pick_splits(gx * groups, kcount, 1 << 30);
+        a.kc_per_split = ceil_div(kcount, splits);
+        ProfScope ps(GPE_PROF_GZ, st);
+        launch_contract(1, ctc, false, EPI_I32_ATOMIC, a, groups, splits, st);
+        ps.stop();
+    });
+}
+
+int gpe_gtu_i64(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const int32_t* u_dev, int64_t ldu,
+                int32_t n_cols, int64_t* w_dev, int64_t ldw, void* stream) {
+    return guarded([&] {
+        require_prepared(g);
+        require(snp_begin >= 0 && snp_end <= g->m && snp_begin < snp_end, "SNP range out of bounds");
+        require(n_cols > 0 && ldu >= g->n && ldw >= n_cols, "bad leading dimension");
+        hipStream_t st = S(stream);
+        // |U| must fit three balanced int8 limbs
+        unsigned int* mx = static_cast<unsigned int*>(g->scratch("absmax", sizeof(unsigned int)));
+        launch_fill_u8(reinterpret_cast<uint8_t*>(mx), 4, 0, st);
+        hipLaunchKernelGGL(k_absmax_i32, dim3(64, (unsigned)n_cols), dim3(256), 0, st, u_dev, ldu, g->n, n_cols, mx);
+        GPE_HIP(hipGetLastError());
+        unsigned int mx_h = 0;
+        GPE_HIP(hipMemcpyAsync(&mx_h, mx, 4, hipMemcpyDeviceToHost, st));
+        GPE_HIP(hipStreamSynchronize(st));
+        require(mx_h <= 8421504u, "|U| exceeds the three-limb range (8421504)");
+        const int nl = 3, ctc = 2;
+        const int groups = (int)ceil_div(n_cols, 32 * ctc);
+        const int64_t coef_ctiles = (int64_t)groups * nl * ctc;
+        const int64_t kcount = ceil_div(g->n, kKChunk);
+        int8_t* coef = static_cast<int8_t*>(g->scratch("coef", kcount * coef_ctiles * 4096));
+        launch_quant_i32(u_dev, ldu, g->n, n_cols, nl, ctc, coef_ctiles, coef, st);
+        ContractArgs a{};
+        a.packed = g->snp_work;
+        a.stride = g->snp_stride;
+        a.r0 = snp_begin;
+        a.r1 = snp_end;
+        a.coef = coef;
+        a.coef_ctiles = coef_ctiles;
+        a.kc0 = 0;
+        a.kc1 = kcount;
+        a.kc_per_split = kcount;
+        a.ctiles_used = (int)ceil_div(n_cols, 32);
+        a.out_i64 = w_dev;
+        a.ldo = ldw;
+        a.cols_out = n_cols;
+        a.out_row0 = snp_begin;
+        ProfScope ps(GPE_PROF_GTU, st);
+        launch_contract(nl, ctc, false, EPI_I64_ROWMAJOR, a, groups, 1, st);
+        ps.stop();
+    });
+}
+
+int gpe_probes(int32_t rand_dist, uint64_t root_seed, const int64_t* blk_starts_host, int32_t n_blocks,
+               int64_t blk_end, int32_t v_start, int32_t v_count, double* z_host) {
+    return guarded([&] {
+        require(n_blocks > 0 && v_count > 0 && blk_starts_host && z_host, "bad probe request");
+        const int64_t s0 = blk_starts_host[0];
+        const int64_t L = blk_end - s0;
+        require(L > 0, "empty probe range");
+        if (rand_dist == GPE_RAND_RADEMACHER) {
+            int64_t* starts = dalloc<int64_t>(n_blocks);
+            int8_t* z = dalloc<int8_t>((s0 + L) * (int64_t)v_count);
+            std::vector<int8_t> zh((size_t)(L * v_count));
+            try {
+                GPE_HIP(hipMemcpy(starts, blk_starts_host, n_blocks * sizeof(int64_t), hipMemcpyHostToDevice));
+                launch_mt_rademacher(root_seed, starts, n_blocks, blk_end, v_start, v_count, z, v_count, nullptr);
+                GPE_HIP(hipDeviceSynchronize());
+                GPE_HIP(hipMemcpy(zh.data(), z + s0 * v_count, zh.size(), hipMemcpyDeviceToHost));
+            } catch (...) {
+                dfree(starts);
+                dfree(z);
+                throw;
+            }
+            dfree(starts);
+            dfree(z);
+            for (size_t i = 0; i < zh.size(); ++i) z_host[i] = (double)zh[i];
+        } else if (rand_dist == GPE_RAND_GAUSSIAN) {
+            for (int b = 0; b < n_blocks; ++b) {
+                const int64_t s = blk_starts_host[b];
+                const int64_t e = b + 1 < n_blocks ? blk_starts_host[b + 1] : blk_end;
+                uint64_t seed = 0x1234abcdULL;
+                auto mix = [](uint64_t x) {
+                    x += 0x9e3779b97f4a7c15ULL;
+                    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
+                    x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
+                    return x ^ (x >> 31);
+                };
+                seed ^= mix(root_seed);
+                seed ^= mix((uint64_t)(int64_t)(int32_t)s + 0x9e37ULL);
+                seed ^= mix((uint64_t)(int64_t)v_start + 0x85ebULL);
+                std::mt19937_64 rng(seed);
+                std::normal_distribution<double> gN(0.0, 1.0);
+                for (int c = 0; c < v_count; ++c)
+                    for (int64_t r = 0; r < e - s; ++r) z_host[(s - s0 + r) * v_count + c] = gN(rng);
+            }
+        } else {
+            throw Error("rand_dist must be rademacher or gaussian");
+        }
+    });
+}
+
+int gpe_phase1_xz(gpe_geno* g, int64_t snp_begin, int64_t snp_end, int64_t step_size, const double* annot_dev,
+                  int32_t n_bins, const double* inv_dev, int32_t v_start, int32_t v_count, int32_t rand_dist,
+                  uint64_t root_seed, double* xz_dev, int64_t ldxz, const uint8_t* bin_mode_dev, int32_t limbs,
+                  void* stream) {
+    return guarded([&] {
+        require_prepared(g);
+        require(snp_begin >= 0 && snp_end <= g->m && snp_begin <= snp_end, "blk_end exceeds #SNPs in BIM");
+        require(step_size > 0, "step_size must be positive");
+        require(n_bins > 0 && v_count > 0, "annotation count and v_count must be positive");
+        require(ldxz >= g->n, "Xz2d_chunk shape must be (N, B*v_count)");
+        if (snp_end == snp_begin) return;
+        const LimbCfg lc = limb_cfg(limbs);
+        hipStream_t st = S(stream);
+        const int64_t L = snp_end - snp_begin;
+        const int v_pad = (int)round_up(v_count, 32);
+        const int ncols = n_bins * v_pad;
+        const int groups = (int)ceil_div(ncols, 32 * lc.ctc);
+        const int64_t coef_ctiles = (int64_t)groups * lc.nl * lc.ctc;
+        const int ncols_pad = groups * lc.ctc * 32;
+
+        // per-SNP weights w = inv_sd * inv * sqrt(a)
+        double* w = static_cast<double*>(g->scratch("p1_w", g->m_alloc * n_bins * sizeof(double)));
+        hipLaunchKernelGGL(k_make_w, dim3((unsigned)ceil_div(L, 256)), dim3(256), 0, st, annot_dev, n_bins, inv_dev,
+                           g->inv_sd, snp_begin, snp_end, w);
+        GPE_HIP(hipGetLastError());
+
+        // probes
+        std::vector<int64_t> starts;
+        for (int64_t s = snp_begin; s < snp_end; s += step_size) starts.push_back(s);
+        const int nb = (int)starts.size();
+        int8_t* z_i8 = nullptr;
+        double* z_f64 = nullptr;
+        const int64_t ldz = v_pad;
+        if (rand_dist == GPE_RAND_RADEMACHER) {
+            z_i8 = static_cast<int8_t*>(g->scratch("p1_z8", g->m_alloc * ldz));
+            int64_t* starts_dev = static_cast<int64_t*>(g->scratch("p1_starts", nb * sizeof(int64_t)));
+            GPE_HIP(hipMemcpyAsync(starts_dev, starts.data(), nb * sizeof(int64_t), hipMemcpyHostToDevice, st));
+            launch_mt_rademacher(root_seed, starts_dev, nb, snp_end, v_start, v_count, z_i8, ldz, st);
+        } else if (rand_dist == GPE_RAND_GAUSSIAN) {
+            z_f64 = static_cast<double*>(g->scratch("p1_z64", g->m_alloc * ldz * sizeof(double)));
+            std::vector<double> zh((size_t)(L * v_count));
+            require(gpe_probes(rand_dist, root_seed, starts.data(), nb, snp_end, v_start, v_count, zh.data()) == 0,
+                    g_err);
+            std::vector<double> zp((size_t)(L * ldz), 0.0);
+            for (int64_t r = 0; r < L; ++r)
+                for (int c = 0; c < v_count; ++c) zp[(size_t)(r * ldz + c)] = zh[(size_t)(r * v_count + c)];
+            GPE_HIP(hipMemcpyAsync(z_f64 + snp_begin * ldz, zp.data(), zp.size() * sizeof(double),
+                                   hipMemcpyHostToDevice, st));
+            GPE_HIP(hipStreamSynchronize(st));
+        } else {
+            throw Error("rand_dist must be 'rademacher' or 'gaussian'");
+        }
+
+        // quantization: shift per column, int8 limbs, centering constants
+        uint64_t* colmax = static_cast<uint64_t*>(g->scratch("colmax", ncols_pad * sizeof(uint64_t)));
+        int32_t* shift = static_cast<int32_t*>(g->scratch("shift", ncols_pad * sizeof(int32_t)));
+        double* col_scale = static_cast<double*>(g->scratch("col_scale", ncols_pad * sizeof(double)));
+        double* cst = static_cast<double*>(g->scratch("cst", ncols_pad * sizeof(double)));
+        launch_fill_u64(colmax, ncols_pad, 0, st);
+        launch_coef_p1_colmax(w, n_bins, z_i8, z_f64, ldz, snp_begin, snp_end, n_bins, v_count, v_pad, colmax, st);
+        const int prec = std::min(8 * lc.nl - 2, 62 - bitlen((uint64_t)(2 * L)));
+        launch_shift_from_colmax(colmax, ncols_pad, prec, shift, col_scale, st);
+        const int64_t kc0 = snp_begin / kKChunk, kc1 = ceil_div(snp_end, kKChunk);
+        const int64_t kcount = kc1 - kc0;
+        int8_t* coef = static_cast<int8_t*>(g->scratch("coef", kcount * coef_ctiles * 4096));
+        launch_fill_u8(reinterpret_cast<uint8_t*>(coef), kcount * coef_ctiles * 4096, 0, st);
+        launch_coef_p1_quant(w, n_bins, z_i8, z_f64, ldz, snp_begin, snp_end, n_bins, v_count, v_pad, shift, lc.nl,
+                             lc.ctc, coef_ctiles, kc0, coef, g->mean, cst, st);
+
+        // contraction over SNPs on the sample-major layout
+        const int64_t ldy = g->n_alloc;
+        int64_t* y = static_cast<int64_t*>(g->scratch("p1_y", (int64_t)ncols_pad * ldy * sizeof(int64_t)));
+        launch_fill_u64(reinterpret_cast<uint64_t*>(y), (int64_t)ncols_pad * ldy, 0, st);
+        ContractArgs a{};
+        a.packed = g->smp_work;
+        a.stride = g->smp_stride;
+        a.r0 = 0;
+        a.r1 = g->n;
+        a.coef = coef;
+        a.coef_ctiles = coef_ctiles;
+        a.kc0 = kc0;
+        a.kc1 = kc1;
+        a.ctiles_used = ncols / 32;
+        a.out_i64 = y;
+        a.ldo = ldy;
+        a.cols_out = ncols_pad;
+        a.out_row0 = 0;
+        const int64_t gx = ceil_div(g->n, 256);
+        const int splits = pick_splits(gx * groups, kcount, (int64_t)(((int64_t)1 << 31) / (256 * 128)) - 1);
+        a.kc_per_split = ceil_div(kcount, splits);
+        ProfScope ps(GPE_PROF_P1_CONTRACT, st);
+        launch_contract(lc.nl, lc.ctc, false, EPI_I64_ATOMIC, a, groups, splits, st);
+        ps.stop();
+
+        // centering, missing-call correction, write Xz
+        const bool miss = g->impute_mode == GPE_IMPUTE_MEAN && g->n_missing > 0;
+        launch_p1_finalize(y, ldy, g->n, n_bins, v_count, v_pad, col_scale, cst, miss ? g->miss_ptr : nullptr,
+                           miss ? g->miss_idx : nullptr, snp_begin, snp_end, w, n_bins, z_i8, z_f64, ldz, shift,
+                           g->mean, xz_dev, ldxz, bin_mode_dev, st);
+    });
+}
+
+int gpe_phase2_accum(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const double* inv_left_dev,
+                     int32_t tile_nvecs, const double* xz_dev, int64_t ldxz, int32_t n_bins, int32_t n_denom,
+                     double* meansq_dev, int64_t ld_meansq, double* mc_aggregate_sumsq_dev,
+                     double* mc_per_snp_sumsq_dev, double extreme_warn, int32_t drop_extreme, int32_t limbs,
+                     unsigned long long* guard_counts_host, void* stream) {
+    return guarded([&] {
+        require_prepared(g);
+        require(snp_begin >= 0 && snp_end <= g->m && snp_begin <= snp_end, "blk_end exceeds #SNPs in BIM");
+        require(tile_nvecs > 0 && n_bins > 0, "Xz2d col count must be a multiple of tile_nvecs");
+        require(ldxz >= g->n, "Xz2d shape mismatch in phase2_accum_XtXz_bed");
+        require(ld_meansq >= n_bins, "meansq_accum shape mismatch in phase2_accum_XtXz_bed");
+        hipStream_t st = S(stream);
+        unsigned long long* gc = static_cast<unsigned long long*>(g->scratch("guards", 2 * sizeof(unsigned long long)));
+        launch_fill_u64(reinterpret_cast<uint64_t*>(gc), 2, 0, st);
+        if (snp_end > snp_begin) {
+            const LimbCfg lc = limb_cfg(limbs);
+            const int v_pad = (int)round_up(tile_nvecs, 32);
+            const int ncols = n_bins * v_pad;
+            const int groups = (int)ceil_div(ncols, 32 * lc.ctc);
+            const int64_t coef_ctiles = (int64_t)groups * lc.nl * lc.ctc;
+            const int ncols_pad = groups * lc.ctc * 32;
+            const int64_t kcount = ceil_div(g->n, kKChunk);
+
+            uint64_t* colmax = static_cast<uint64_t*>(g->scratch("colmax", ncols_pad * sizeof(uint64_t)));
+            int32_t* shift = static_cast<int32_t*>(g->scratch("shift", ncols_pad * sizeof(int32_t)));
+            double* col_scale = static_cast<double*>(g->scratch("col_scale", ncols_pad * sizeof(double)));
+            int64_t* col_sum = static_cast<int64_t*>(g->scratch("col_sum", ncols_pad * sizeof(int64_t)));
+            launch_fill_u64(colmax, ncols_pad, 0, st);
+            launch_fill_u64(reinterpret_cast<uint64_t*>(col_sum), ncols_pad, 0, st);
+            launch_colmax_f64(xz_dev, ldxz, g->n, ncols, tile_nvecs, v_pad, colmax, st);
+            const int prec = std::min(8 * lc.nl - 2, 62 - bitlen((uint64_t)(2 * g->n)));
+            launch_shift_from_colmax(colmax, ncols_pad, prec, shift, col_scale, st);
+            int8_t* coef = static_cast<int8_t*>(g->scratch("coef", kcount * coef_ctiles * 4096));
+            launch_fill_u8(reinterpret_cast<uint8_t*>(coef), kcount * coef_ctiles * 4096, 0, st);
+            launch_quant_p2(xz_dev, ldxz, g->n, n_bins, tile_nvecs, v_pad, shift, lc.nl, lc.ctc, coef_ctiles, coef,
+                            col_sum, st);
+
+            const int ctiles_used = ncols / 32;
+            const bool mc = mc_aggregate_sumsq_dev || mc_per_snp_sumsq_dev;
+            double* s2 = static_cast<double*>(g->scratch("p2_s2", g->m_alloc * ctiles_used * sizeof(double)));
+            double* s4 = mc ? static_cast<double*>(g->scratch("p2_s4", g->m_alloc * ctiles_used * sizeof(double)))
+                            : nullptr;
+            ContractArgs a{};
+            a.packed = g->snp_work;
+            a.stride = g->snp_stride;
+            a.r0 = snp_begin;
+            a.r1 = snp_end;
+            a.coef = coef;
+            a.coef_ctiles = coef_ctiles;
+            a.kc0 = 0;
+            a.kc1 = kcount;
+            a.kc_per_split = kcount;
+            a.ctiles_used = ctiles_used;
+            a.mean = g->mean;
+            a.inv_sd = g->inv_sd;
+            a.col_scale = col_scale;
+            a.col_sum = col_sum;
+            a.s2 = s2;
+            a.s4 = s4;
+            a.ld_s = ctiles_used;
+            const bool miss = g->impute_mode == GPE_IMPUTE_MEAN && g->n_missing > 0;
+            a.has_miss = miss ? 1 : 0;
+            ProfScope ps(GPE_PROF_P2_CONTRACT, st);
+            launch_contract(lc.nl, lc.ctc, miss, EPI_P2, a, groups, 1, st);
+            ps.stop();
+
+            double denom = (double)n_denom - 1.0;
+            if (denom <= 0.0) denom = 1.0;
+            const double inv_denom2 = 1.0 / (denom * denom);
+            launch_p2_finalize(s2, s4, ctiles_used, snp_begin, snp_end, n_bins, v_pad / 32, tile_nvecs, inv_left_dev,
+                               inv_denom2, extreme_warn > 0.0 ? extreme_warn : INFINITY, drop_extreme, meansq_dev,
+                               ld_meansq, mc_aggregate_sumsq_dev, mc_per_snp_sumsq_dev, gc, st);
+        }
+        if (guard_counts_host) {
+            GPE_HIP(hipMemcpyAsync(guard_counts_host, gc, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+            GPE_HIP(hipStreamSynchronize(st));
+        }
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,134 @@
+// Internal declarations shared by the kernel and API translation units of
+// libsummit_gpe.so.  Not part of the public C ABI (include/summit_gpe.h).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace gpe {
+
+// ---------------------------------------------------------------------------
+// Device layout of a genotype matrix resident in HBM.
+//
+//   snp-major  : one row per SNP, 4 samples per byte (PLINK order inside a
+//                byte: sample 4b+t in bits 2t..2t+1), rows padded to a
+//                multiple of 64 bytes, row count padded to a multiple of 256.
+//   sample-major: the transpose, one row per sample, 4 SNPs per byte, same
+//                padding rules.  Both passes of the estimator contract along
+//                the contiguous (packed) axis of one of these two layouts.
+//
+// Codes: 00 -> 0, 01 -> missing, 10 -> 1, 11 -> 2.  Padding codes are 00 and
+// contribute nothing because every coefficient at a padded position is 0.
+// ---------------------------------------------------------------------------
+constexpr int kKChunk = 128;        // contraction-axis values per MFMA chunk
+constexpr int kRowPad = 256;        // row-count padding of both layouts
+
+__host__ __device__ inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+__host__ __device__ inline int64_t ceil_div(int64_t x, int64_t m) { return (x + m - 1) / m; }
+
+struct Error : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define GPE_HIP(expr)                                                                     \
+    do {                                                                                  \
+        hipError_t _e = (expr);                                                           \
+        if (_e != hipSuccess)                                                             \
+            throw ::gpe::Error(std::string("HIP error ") + hipGetErrorString(_e) + " at " + \
+                               __FILE__ + ":" + std::to_string(__LINE__) + " in " #expr);  \
+    } while (0)
+
+// Output modes of the packed-contraction kernel.
+enum EpiMode : int {
+    EPI_I32_ATOMIC = 0,  // NL == 1: atomically add int32 results into out_i32[c * ldo + r]
+    EPI_I64_ATOMIC = 1,  // combine limbs, atomically add int64 into out_i64[c * ldo + r]
+    EPI_I64_ROWMAJOR = 2,// combine limbs, store int64 to out_i64[r * ldo + c]
+    EPI_P2 = 3,          // phase-2 epilogue: standardized Work, per-tile sums of Work^2, Work^4
+};
+
+struct ContractArgs {
+    const uint8_t* packed;   // packed rows (either layout)
+    int64_t stride;          // bytes per packed row
+    int64_t r0, r1;          // rows [r0, r1) are produced
+    const int8_t* coef;      // swizzled coefficient limbs
+    int64_t coef_ctiles;     // c-tiles per K chunk in the swizzled layout
+    int64_t kc0, kc1;        // K chunks [kc0, kc1)
+    int64_t kc_per_split;    // K chunks per split (grid.z)
+    int32_t ctiles_used;     // number of real c-tiles (per limb) in the output
+    // outputs
+    int32_t* out_i32;
+    int64_t* out_i64;
+    int64_t ldo;
+    // phase-2 epilogue inputs
+    const double* mean;      // per row (SNP) mean
+    const double* inv_sd;    // per row inv sd
+    const double* col_scale; // per column 2^-shift
+    const int64_t* col_sum;  // per column exact sum of quantized values
+    double* s2;              // [row][ctile] sum of Work^2 over the tile's columns
+    double* s4;              // [row][ctile] sum of Work^4 (may be null)
+    int64_t ld_s;            // ctile count stride of s2/s4
+    int has_miss;
+    int32_t cols_out;        // I32/I64 epilogues: columns >= cols_out are not written
+    int64_t out_row0;        // I32/I64 epilogues: output row index = r - out_row0
+};
+
+// Launchers (gpe_kernels.hip)
+void launch_contract(int nl, int ctc, bool miss, EpiMode mode, const ContractArgs& a, int n_col_groups,
+                     int n_splits, hipStream_t st);
+void launch_clear_tail(uint8_t* snp, int64_t m, int64_t stride, int64_t n, hipStream_t st);
+void launch_select_rows(const uint8_t* src, int64_t src_stride, int64_t m, const int64_t* rows, int64_t n_sel,
+                        uint8_t* dst, int64_t dst_stride, hipStream_t st);
+void launch_snp_stats(const uint8_t* snp, int64_t stride, int64_t m, int64_t n, int32_t* nobs, int64_t* sum,
+                      int64_t* sumsq, hipStream_t st);
+void launch_std_params(const int32_t* nobs, const int64_t* sum, const int64_t* sumsq, int64_t m, int64_t n,
+                       int ddof, int completed, double* mean, double* inv_sd, hipStream_t st);
+void launch_hwe_impute(uint8_t* snp, int64_t stride, int64_t m, int64_t n, const int32_t* nobs,
+                       const int64_t* sum, uint64_t impute_seed, int64_t snp_offset, const int64_t* rows,
+                       hipStream_t st);
+void launch_transpose(const uint8_t* snp, int64_t snp_stride, int64_t m_pad, uint8_t* smp, int64_t smp_stride,
+                      int64_t n_pad, hipStream_t st);
+void launch_miss_count(const uint8_t* smp, int64_t stride, int64_t n, int32_t* counts, hipStream_t st);
+void launch_miss_fill(const uint8_t* smp, int64_t stride, int64_t n, const int64_t* ptr, int32_t* idx,
+                      hipStream_t st);
+void launch_fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t st);
+void launch_fill_u64(uint64_t* p, int64_t n, uint64_t v, hipStream_t st);
+void launch_mt_rademacher(uint64_t root, const int64_t* blk_starts, int n_blocks, int64_t blk_end_all,
+                          int v_start, int v_count, int8_t* z, int64_t ldz, hipStream_t st);
+void launch_coef_p1_colmax(const double* w, int64_t ldw, const int8_t* z_i8, const double* z_f64, int64_t ldz,
+                           int64_t k0, int64_t k1, int n_bins, int v_count, int v_pad, uint64_t* colmax_bits,
+                           hipStream_t st);
+void launch_coef_p1_quant(const double* w, int64_t ldw, const int8_t* z_i8, const double* z_f64, int64_t ldz,
+                          int64_t k0, int64_t k1, int n_bins, int v_count, int v_pad, const int32_t* shift,
+                          int nl, int ctc, int64_t coef_ctiles, int64_t kc0, int8_t* coef,
+                          const double* mean, double* cst, hipStream_t st);
+void launch_p1_finalize(const int64_t* y, int64_t ldy, int64_t n, int n_bins, int v_count, int v_pad,
+                        const double* col_scale, const double* cst, const int64_t* miss_ptr,
+                        const int32_t* miss_idx, int64_t k0, int64_t k1, const double* w, int64_t ldw,
+                        const int8_t* z_i8, const double* z_f64, int64_t ldz, const int32_t* shift,
+                        const double* mean, double* xz, int64_t ldxz, const uint8_t* col_init, hipStream_t st);
+void launch_colmax_f64(const double* x, int64_t ldx, int64_t n, int n_cols, int v_count, int v_pad,
+                       uint64_t* colmax_bits, hipStream_t st);
+void launch_quant_p2(const double* x, int64_t ldx, int64_t n, int n_bins, int v_count, int v_pad,
+                     const int32_t* shift, int nl, int ctc, int64_t coef_ctiles, int8_t* coef,
+                     int64_t* col_sum, hipStream_t st);
+void launch_shift_from_colmax(const uint64_t* colmax_bits, int n, int precision_bits, int32_t* shift,
+                              double* col_scale, hipStream_t st);
+void launch_p2_finalize(const double* s2, const double* s4, int64_t ld_s, int64_t r0, int64_t r1, int n_bins,
+                        int tiles_per_bin, int v_count, const double* inv_left, double inv_denom2,
+                        double extreme_warn, int drop_extreme, double* meansq, int64_t ld_meansq,
+                        double* mc_agg, double* mc_per_snp, unsigned long long* guard_counts,
+                        hipStream_t st);
+void launch_coef_i8(const int8_t* z, int64_t ldz, int64_t k0, int64_t k1, int n_cols, int64_t coef_ctiles,
+                    int8_t* coef, hipStream_t st);
+void launch_quant_i32(const int32_t* u, int64_t ldu, int64_t n, int n_cols, int nl, int ctc,
+                      int64_t coef_ctiles, int8_t* coef, hipStream_t st);
+void launch_synth_markov(uint8_t* snp, int64_t stride, int64_t m, int64_t n, uint64_t seed, double maf_lo,
+                         double maf_hi, double miss_rate, int block_len, double copy_prob, int iid,
+                         hipStream_t st);
+void launch_resvar_inv(const int32_t* nobs, const int64_t* sum, const int64_t* sumsq, const double* mean,
+                       const double* inv_sd, int64_t r0, int64_t r1, int64_t n, int completed, double eps,
+                       double* inv_out, hipStream_t st);
+
+}  // namespace gpe

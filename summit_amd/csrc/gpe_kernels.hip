@@ -1,0 +1,1141 @@
+// HIP kernels of the MI355X genotype-probe engine (gfx950 / CDNA4).
+//
+// The engine computes the two genotype traversals of SUMMIT's stochastic
+// LD-score estimator (src/native/gwldcore.cpp:1582-1861 phase 1 X*Z,
+// :2329-2600 phase 2 X^T(X*Z)) on a packed 2-bit genotype matrix resident in
+// HBM.  Every product is one kernel, k_contract: it streams packed rows with
+// 16-byte loads, expands 2-bit codes to int8 in registers (genotype count and
+// missing-call planes), and contracts them against int8 coefficient limbs on
+// v_mfma_i32_32x32x32_i8 with exact int32 accumulation.  Real-valued
+// coefficients (weighted probes, the FP64 X*Z fed to phase 2) are split into
+// balanced base-256 int8 limbs with a per-column power-of-two scale, so each
+// limb product is exact and the recombined int64 is the exact product of the
+// quantized operand; centering, scaling and missing-call corrections are
+// applied in FP64 in the epilogues.
+#include "gpe_internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+namespace gpe {
+
+typedef int32_t v4i __attribute__((ext_vector_type(4)));
+typedef int32_t v16i __attribute__((ext_vector_type(16)));
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+    x += 0x9e3779b97f4a7c15ULL;
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
+    return x ^ (x >> 31);
+}
+
+// Four 2-bit fields of b (bits 0..7) -> four bytes.
+__device__ __forceinline__ uint32_t spread4(uint32_t b) {
+    return (b | (b << 6) | (b << 12) | (b << 18)) & 0x03030303u;
+}
+
+// 16 codes -> 16 int8 genotype counts (missing -> 0) and 16 int8 missing flags.
+// Byte t of the result belongs to code t of w (bits 2t..2t+1).
+__device__ __forceinline__ void unpack16(uint32_t w, v4i& g, v4i& m) {
+    const uint32_t lo = w & 0x55555555u;
+    const uint32_t hi = (w >> 1) & 0x55555555u;
+    const uint32_t gv = hi + (hi & lo);  // 00->0 01->0 10->1 11->2, no carry out of a field
+    const uint32_t mv = lo & ~hi;        // 01 -> 1
+    g = v4i{(int32_t)spread4(gv & 0xffu), (int32_t)spread4((gv >> 8) & 0xffu),
+            (int32_t)spread4((gv >> 16) & 0xffu), (int32_t)spread4(gv >> 24)};
+    m = v4i{(int32_t)spread4(mv & 0xffu), (int32_t)spread4((mv >> 8) & 0xffu),
+            (int32_t)spread4((mv >> 16) & 0xffu), (int32_t)spread4(mv >> 24)};
+}
+
+__device__ __forceinline__ void unpack16_g(uint32_t w, v4i& g) {
+    const uint32_t lo = w & 0x55555555u;
+    const uint32_t hi = (w >> 1) & 0x55555555u;
+    const uint32_t gv = hi + (hi & lo);
+    g = v4i{(int32_t)spread4(gv & 0xffu), (int32_t)spread4((gv >> 8) & 0xffu),
+            (int32_t)spread4((gv >> 16) & 0xffu), (int32_t)spread4(gv >> 24)};
+}
+
+// Offset of coefficient byte (c-tile tg, value index kk in [0,128), lane row lr in [0,32))
+// inside one K chunk of the swizzled layout: [tile][step][lane][16 bytes].
+__device__ __forceinline__ int64_t swz_offset(int64_t kl, int64_t coef_ctiles, int64_t tg, int kk, int lr) {
+    const int h = kk >> 6;
+    const int s = (kk & 63) >> 4;
+    const int b = kk & 15;
+    const int lane = h * 32 + lr;
+    return (((kl * coef_ctiles + tg) * 4 + s) * 64 + lane) * 16 + b;
+}
+
+// Balanced base-256 digits of q, most significant first, into d[0..NL).
+template <int NL>
+__device__ __forceinline__ void balanced_digits(int64_t q, int8_t* d) {
+#pragma unroll
+    for (int l = NL - 1; l >= 0; --l) {
+        const int64_t r = ((q + 128) & 255) - 128;
+        d[l] = (int8_t)r;
+        q = (q - r) / 256;
+    }
+}
+
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ int64_t wave_sum_i64(int64_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Packed contraction on int8 MFMA.
+//
+//   out[c][r] = sum_k A[c][k] * plane(P[r][k])
+//
+// Workgroup: 4 waves.  Each wave owns 32 packed rows (MISS: genotype and
+// missing planes of the same rows) or 64 rows (two genotype tiles), and all
+// CT = NL*CTC coefficient tiles of one column group.  Per K chunk of 128
+// values the workgroup stages CT*4 KiB of swizzled coefficients in LDS
+// (double-buffered; next chunk prefetched into registers while the current
+// one is contracted), each lane loads 16 packed bytes (64 codes) of its row.
+// MFMA lane layout used here: operand lane l holds row (l & 31) and 16
+// consecutive K values of half (l >> 5); result register e of lane l holds
+// (row 8*(e>>2) + 4*(l>>5) + (e&3), column l & 31).
+// ---------------------------------------------------------------------------
+template <int NL, int CTC, bool MISS, int EPI>
+__global__ __launch_bounds__(256, 1) void k_contract(ContractArgs a) {
+    constexpr int CT = NL * CTC;
+    constexpr int RPW = MISS ? 32 : 64;
+    constexpr int RPB = 4 * RPW;
+    __shared__ __attribute__((aligned(16))) int8_t lds[2][CT * 4096];
+
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6;
+    const int lane = tid & 63;
+    const int half = lane >> 5;
+    const int lr = lane & 31;
+    // column groups of one row block are adjacent in launch order so the
+    // packed rows they share are served from L2
+    const int n_groups = (int)gridDim.y;
+    const int64_t lin = (int64_t)blockIdx.x * n_groups + blockIdx.y;
+    const int cg = (int)(lin % n_groups);
+    const int64_t rblk = lin / n_groups;
+    const int64_t rbase = a.r0 + rblk * RPB + (int64_t)wave * RPW;
+    const int64_t kc_begin = a.kc0 + (int64_t)blockIdx.z * a.kc_per_split;
+    const int64_t kc_end = kc_begin + a.kc_per_split < a.kc1 ? kc_begin + a.kc_per_split : a.kc1;
+
+    v16i acc[CT][2];
+#pragma unroll
+    for (int t = 0; t < CT; ++t) {
+        acc[t][0] = v16i{};
+        acc[t][1] = v16i{};
+    }
+
+    const uint8_t* prow0 = a.packed + (rbase + lr) * a.stride + half * 16;
+    const uint8_t* prow1 = MISS ? prow0 : prow0 + 32 * a.stride;
+
+    uint4 areg[CT];
+    uint4 breg0 = make_uint4(0, 0, 0, 0), breg1 = make_uint4(0, 0, 0, 0);
+
+    auto load_chunk = [&](int64_t kc) {
+        const uint4* src =
+            reinterpret_cast<const uint4*>(a.coef + ((kc - a.kc0) * a.coef_ctiles + (int64_t)cg * CT) * 4096);
+#pragma unroll
+        for (int t = 0; t < CT; ++t) areg[t] = src[t * 256 + tid];
+        breg0 = *reinterpret_cast<const uint4*>(prow0 + kc * 32);
+        if (!MISS) breg1 = *reinterpret_cast<const uint4*>(prow1 + kc * 32);
+    };
+
+    if (kc_begin < kc_end) load_chunk(kc_begin);
+    int buf = 0;
+    for (int64_t kc = kc_begin; kc < kc_end; ++kc) {
+        uint4* dst = reinterpret_cast<uint4*>(lds[buf]);
+#pragma unroll
+        for (int t = 0; t < CT; ++t) dst[t * 256 + tid] = areg[t];
+        const uint4 b0 = breg0;
+        const uint4 b1 = breg1;
+        __syncthreads();
+        if (kc + 1 < kc_end) load_chunk(kc + 1);
+        const int8_t* L = lds[buf];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const uint32_t w0 = s == 0 ? b0.x : s == 1 ? b0.y : s == 2 ? b0.z : b0.w;
+            v4i op0, op1;
+            if (MISS) {
+                unpack16(w0, op0, op1);
+            } else {
+                const uint32_t w1 = s == 0 ? b1.x : s == 1 ? b1.y : s == 2 ? b1.z : b1.w;
+                unpack16_g(w0, op0);
+                unpack16_g(w1, op1);
+            }
+#pragma unroll
+            for (int t = 0; t < CT; ++t) {
+                const v4i av = *reinterpret_cast<const v4i*>(L + ((t * 4 + s) * 64 + lane) * 16);
+                acc[t][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, op0, acc[t][0], 0, 0, 0);
+                acc[t][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, op1, acc[t][1], 0, 0, 0);
+            }
+        }
+        buf ^= 1;
+    }
+
+    // ------------------------------------------------------------- epilogues
+    if constexpr (EPI == EPI_I32_ATOMIC) {
+        static_assert(NL == 1, "int32 output needs one limb");
+#pragma unroll
+        for (int ctc = 0; ctc < CTC; ++ctc) {
+            const int ct = cg * CTC + ctc;
+            if (ct >= a.ctiles_used) continue;
+#pragma unroll
+            for (int bt = 0; bt < 2; ++bt) {
+                const int64_t r = rbase + (MISS ? 0 : bt * 32) + lr;
+                if (MISS && bt == 1) continue;
+                if (r >= a.r1) continue;
+#pragma unroll
+                for (int e = 0; e < 16; ++e) {
+                    const int c = ct * 32 + 8 * (e >> 2) + 4 * half + (e & 3);
+                    if (c < a.cols_out) atomicAdd(&a.out_i32[(int64_t)c * a.ldo + (r - a.out_row0)], acc[ctc][bt][e]);
+                }
+            }
+        }
+    } else if constexpr (EPI == EPI_I64_ATOMIC || EPI == EPI_I64_ROWMAJOR) {
+#pragma unroll
+        for (int ctc = 0; ctc < CTC; ++ctc) {
+            const int ct = cg * CTC + ctc;
+            if (ct >= a.ctiles_used) continue;
+#pragma unroll
+            for (int bt = 0; bt < 2; ++bt) {
+                if (MISS && bt == 1) continue;
+                const int64_t r = rbase + (MISS ? 0 : bt * 32) + lr;
+                if (r >= a.r1) continue;
+#pragma unroll
+                for (int e = 0; e < 16; ++e) {
+                    int64_t v = 0;
+#pragma unroll
+                    for (int l = 0; l < NL; ++l) v = v * 256 + (int64_t)acc[l * CTC + ctc][bt][e];
+                    const int c = ct * 32 + 8 * (e >> 2) + 4 * half + (e & 3);
+                    if (c >= a.cols_out) continue;
+                    if (EPI == EPI_I64_ATOMIC)
+                        atomicAdd(reinterpret_cast<unsigned long long*>(
+                                      &a.out_i64[(int64_t)c * a.ldo + (r - a.out_row0)]),
+                                  (unsigned long long)v);
+                    else
+                        a.out_i64[(r - a.out_row0) * a.ldo + c] = v;
+                }
+            }
+        }
+    } else {  // EPI_P2
+#pragma unroll
+        for (int ctc = 0; ctc < CTC; ++ctc) {
+            const int ct = cg * CTC + ctc;
+#pragma unroll
+            for (int bt = 0; bt < (MISS ? 1 : 2); ++bt) {
+                const int64_t r = rbase + (MISS ? 0 : bt * 32) + lr;
+                const bool live = r < a.r1 && ct < a.ctiles_used;
+                double s2 = 0.0, s4 = 0.0;
+                if (live) {
+                    const double mu = a.mean[r];
+                    const double isd = a.inv_sd[r];
+#pragma unroll
+                    for (int e = 0; e < 16; ++e) {
+                        int64_t ge = 0, me = 0;
+#pragma unroll
+                        for (int l = 0; l < NL; ++l) {
+                            ge = ge * 256 + (int64_t)acc[l * CTC + ctc][bt][e];
+                            if (MISS) me = me * 256 + (int64_t)acc[l * CTC + ctc][1][e];
+                        }
+                        const int c = ct * 32 + 8 * (e >> 2) + 4 * half + (e & 3);
+                        const double obs_sum = (double)(a.col_sum[c] - me);
+                        const double w = isd * a.col_scale[c] * ((double)ge - mu * obs_sum);
+                        const double w2 = w * w;
+                        s2 += w2;
+                        s4 += w2 * w2;
+                    }
+                }
+                s2 += __shfl_xor(s2, 32, 64);
+                s4 += __shfl_xor(s4, 32, 64);
+                if (live && half == 0) {
+                    a.s2[r * a.ld_s + ct] = s2;
+                    if (a.s4) a.s4[r * a.ld_s + ct] = s4;
+                }
+            }
+        }
+    }
+}
+
+template <int NL, int CTC, bool MISS, int EPI>
+static void launch_one(const ContractArgs& a, int n_col_groups, int n_splits, hipStream_t st) {
+    constexpr int RPB = 4 * (MISS ? 32 : 64);
+    const int64_t rows = a.r1 - a.r0;
+    if (rows <= 0 || a.kc1 <= a.kc0) return;
+    dim3 grid((unsigned)ceil_div(rows, RPB), (unsigned)n_col_groups, (unsigned)n_splits);
+    hipLaunchKernelGGL((k_contract<NL, CTC, MISS, EPI>), grid, dim3(256), 0, st, a);
+    GPE_HIP(hipGetLastError());
+}
+
+void launch_contract(int nl, int ctc, bool miss, EpiMode mode, const ContractArgs& a, int n_col_groups,
+                     int n_splits, hipStream_t st) {
+#define GPE_CASE(NL_, CTC_, MISS_, EPI_)                                                          \
+    if (nl == NL_ && ctc == CTC_ && miss == MISS_ && mode == EPI_) {                              \
+        launch_one<NL_, CTC_, MISS_, EPI_>(a, n_col_groups, n_splits, st);                        \
+        return;                                                                                   \
+    }
+    // exact integer products
+    GPE_CASE(1, 2, false, EPI_I32_ATOMIC)
+    GPE_CASE(3, 2, false, EPI_I64_ROWMAJOR)
+    // weighted phase 1 (sample-major rows, int64 partial sums)
+    GPE_CASE(5, 1, false, EPI_I64_ATOMIC)
+    GPE_CASE(6, 1, false, EPI_I64_ATOMIC)
+    // weighted phase 2
+    GPE_CASE(5, 1, false, EPI_P2)
+    GPE_CASE(5, 1, true, EPI_P2)
+    GPE_CASE(6, 1, false, EPI_P2)
+    GPE_CASE(6, 1, true, EPI_P2)
+#undef GPE_CASE
+    throw Error("no packed-contraction variant for nl=" + std::to_string(nl) + " ctc=" + std::to_string(ctc) +
+                " miss=" + std::to_string((int)miss) + " mode=" + std::to_string((int)mode));
+}
+
+// ---------------------------------------------------------------------------
+// Layout / statistics kernels
+// ---------------------------------------------------------------------------
+__global__ void k_fill_u8(uint8_t* p, int64_t n, uint8_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+__global__ void k_fill_u64(uint64_t* p, int64_t n, uint64_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+void launch_fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t st) {
+    if (n <= 0) return;
+    const int64_t blocks = ceil_div(n, 256) < 65536 ? ceil_div(n, 256) : 65536;
+    hipLaunchKernelGGL(k_fill_u8, dim3((unsigned)blocks), dim3(256), 0, st, p, n, v);
+    GPE_HIP(hipGetLastError());
+}
+void launch_fill_u64(uint64_t* p, int64_t n, uint64_t v, hipStream_t st) {
+    if (n <= 0) return;
+    const int64_t blocks = ceil_div(n, 256) < 65536 ? ceil_div(n, 256) : 65536;
+    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)blocks), dim3(256), 0, st, p, n, v);
+    GPE_HIP(hipGetLastError());
+}
+
+// Zero the codes of a SNP-major row past the last sample (padding must read as 00).
+__global__ void k_clear_tail(uint8_t* snp, int64_t m, int64_t stride, int64_t n) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= m) return;
+    uint8_t* row = snp + j * stride;
+    const int64_t full = n / 4;
+    const int rem = (int)(n % 4);
+    int64_t b = full;
+    if (rem) {
+        row[b] = (uint8_t)(row[b] & ((1u << (2 * rem)) - 1u));
+        ++b;
+    }
+    for (; b < stride; ++b) row[b] = 0;
+}
+void launch_clear_tail(uint8_t* snp, int64_t m, int64_t stride, int64_t n, hipStream_t st) {
+    if (m <= 0) return;
+    hipLaunchKernelGGL(k_clear_tail, dim3((unsigned)ceil_div(m, 256)), dim3(256), 0, st, snp, m, stride, n);
+    GPE_HIP(hipGetLastError());
+}
+
+// Gather selected samples (sorted global row ids) into a compact SNP-major matrix.
+__global__ void k_select_rows(const uint8_t* src, int64_t src_stride, int64_t m, const int64_t* rows,
+                              int64_t n_sel, uint8_t* dst, int64_t dst_stride) {
+    const int64_t nb = ceil_div(n_sel, 4);
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t j = blockIdx.y;
+    if (idx >= nb || j >= m) return;
+    const uint8_t* s = src + j * src_stride;
+    uint32_t out = 0;
+    for (int t = 0; t < 4; ++t) {
+        const int64_t n = idx * 4 + t;
+        if (n >= n_sel) break;
+        const int64_t g = rows[n];
+        const uint32_t code = (s[g >> 2] >> (2 * (g & 3))) & 3u;
+        out |= code << (2 * t);
+    }
+    dst[j * dst_stride + idx] = (uint8_t)out;
+}
+void launch_select_rows(const uint8_t* src, int64_t src_stride, int64_t m, const int64_t* rows, int64_t n_sel,
+                        uint8_t* dst, int64_t dst_stride, hipStream_t st) {
+    if (m <= 0 || n_sel <= 0) return;
+    for (int64_t j0 = 0; j0 < m; j0 += 65535) {
+        const int64_t mm = (m - j0) < 65535 ? (m - j0) : 65535;
+        hipLaunchKernelGGL(k_select_rows, dim3((unsigned)ceil_div(ceil_div(n_sel, 4), 256), (unsigned)mm), dim3(256),
+                           0, st, src + j0 * src_stride, src_stride, mm, rows, n_sel, dst + j0 * dst_stride,
+                           dst_stride);
+        GPE_HIP(hipGetLastError());
+    }
+}
+
+// Per-SNP observed count, allele sum and sum of squares (genotype.cpp:474-574).
+__global__ void k_snp_stats(const uint8_t* snp, int64_t stride, int64_t m, int64_t n, int32_t* nobs,
+                            int64_t* sum, int64_t* sumsq) {
+    const int64_t j = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (j >= m) return;
+    const uint32_t* row = reinterpret_cast<const uint32_t*>(snp + j * stride);
+    const int64_t words = stride / 4;
+    int64_t nmiss = 0, c1 = 0, c2 = 0;
+    for (int64_t w = lane; w < words; w += 64) {
+        const uint32_t x = row[w];
+        const uint32_t lo = x & 0x55555555u, hi = (x >> 1) & 0x55555555u;
+        nmiss += __popc(lo & ~hi);
+        c1 += __popc(hi & ~lo);
+        c2 += __popc(hi & lo);
+    }
+    nmiss = wave_sum_i64(nmiss);
+    c1 = wave_sum_i64(c1);
+    c2 = wave_sum_i64(c2);
+    if (lane == 0) {
+        nobs[j] = (int32_t)(n - nmiss);
+        sum[j] = c1 + 2 * c2;
+        sumsq[j] = c1 + 4 * c2;
+    }
+}
+void launch_snp_stats(const uint8_t* snp, int64_t stride, int64_t m, int64_t n, int32_t* nobs, int64_t* sum,
+                      int64_t* sumsq, hipStream_t st) {
+    if (m <= 0) return;
+    hipLaunchKernelGGL(k_snp_stats, dim3((unsigned)ceil_div(m, 4)), dim3(256), 0, st, snp, stride, m, n, nobs, sum,
+                       sumsq);
+    GPE_HIP(hipGetLastError());
+}
+
+// Mean / inverse sd: mean-imputed (genotype.cpp:652-676) or completed data
+// (genotype.cpp:629-650).
+__global__ void k_std_params(const int32_t* nobs, const int64_t* sum, const int64_t* sumsq, int64_t m, int64_t n,
+                             int ddof, int completed, double* mean, double* inv_sd) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= m) return;
+    const int64_t cnt = completed ? n : (int64_t)nobs[j];
+    const double dn = (double)cnt;
+    const double s = (double)sum[j];
+    const double ss = (double)sumsq[j];
+    double mu = 0.0, M2 = 0.0, inv = 1.0;
+    if (cnt > 0) {
+        mu = s / dn;
+        M2 = ss - (s * s) / dn;
+        if (M2 < 0.0 && M2 > -1e-12) M2 = 0.0;
+    }
+    const int64_t denom = cnt - (int64_t)ddof;
+    if (denom > 0 && M2 > 0.0) {
+        const double var = M2 / (double)denom;
+        if (var > 0.0 && isfinite(var)) inv = 1.0 / sqrt(var);
+    }
+    mean[j] = mu;
+    inv_sd[j] = inv;
+}
+void launch_std_params(const int32_t* nobs, const int64_t* sum, const int64_t* sumsq, int64_t m, int64_t n,
+                       int ddof, int completed, double* mean, double* inv_sd, hipStream_t st) {
+    if (m <= 0) return;
+    hipLaunchKernelGGL(k_std_params, dim3((unsigned)ceil_div(m, 256)), dim3(256), 0, st, nobs, sum, sumsq, m, n,
+                       ddof, completed, mean, inv_sd);
+    GPE_HIP(hipGetLastError());
+}
+
+// Residual scale 1/sqrt(max(||x_j||^2/(N-1), eps)) with ||x_j||^2 from the
+// integer moments (gwldcore.cpp:3220-3333, no covariates).
+__global__ void k_resvar_inv(const int32_t* nobs, const int64_t* sum, const int64_t* sumsq, const double* mean,
+                             const double* inv_sd, int64_t r0, int64_t r1, int64_t n, int completed, double eps,
+                             double* inv_out) {
+    const int64_t j = r0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= r1) return;
+    const double cnt = completed ? (double)n : (double)nobs[j];
+    const double mu = mean[j];
+    const double s = (double)sum[j];
+    const double ss = (double)sumsq[j];
+    double M2 = cnt > 0 ? ss - 2.0 * mu * s + mu * mu * cnt : 0.0;
+    if (M2 < 0.0) M2 = 0.0;
+    const double isd = inv_sd[j];
+    const double sq = M2 * isd * isd;
+    const double var = sq / (double)(n - 1);
+    double safe;
+    if (isnan(var)) safe = var;
+    else safe = var > eps ? var : eps;
+    inv_out[j - r0] = 1.0 / sqrt(safe);
+}
+void launch_resvar_inv(const int32_t* nobs, const int64_t* sum, const int64_t* sumsq, const double* mean,
+                       const double* inv_sd, int64_t r0, int64_t r1, int64_t n, int completed, double eps,
+                       double* inv_out, hipStream_t st) {
+    if (r1 <= r0) return;
+    hipLaunchKernelGGL(k_resvar_inv, dim3((unsigned)ceil_div(r1 - r0, 256)), dim3(256), 0, st, nobs, sum, sumsq,
+                       mean, inv_sd, r0, r1, n, completed, eps, inv_out);
+    GPE_HIP(hipGetLastError());
+}
+
+// Deterministic HWE completion of missing calls (genotype.cpp:122-184, :576-595):
+// per (SNP, selected row) a splitmix hash of (impute_seed, global SNP, global
+// row) is compared with the HWE genotype frequencies of the observed calls.
+__global__ void k_hwe_impute(uint8_t* snp, int64_t stride, int64_t m, int64_t n, const int32_t* nobs,
+                             const int64_t* sum, uint64_t impute_seed, int64_t snp_offset, const int64_t* rows) {
+    const int64_t words = stride / 4;
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t j = blockIdx.y + (int64_t)blockIdx.z * 65535;
+    if (w >= words || j >= m) return;
+    uint32_t* pw = reinterpret_cast<uint32_t*>(snp + j * stride) + w;
+    uint32_t x = *pw;
+    const uint32_t lo = x & 0x55555555u, hi = (x >> 1) & 0x55555555u;
+    uint32_t miss = lo & ~hi;
+    if (!miss) return;
+    const int64_t no = nobs[j];
+    double p = 0.0;
+    if (no > 0) {
+        p = (double)sum[j] / (2.0 * (double)no);
+        if (p < 0.0) p = 0.0;
+        if (p > 1.0) p = 1.0;
+    }
+    const double q0 = (1.0 - p) * (1.0 - p);
+    const double q01 = q0 + 2.0 * p * (1.0 - p);
+    uint64_t base = mix64(impute_seed);
+    base ^= mix64((uint64_t)(uint32_t)(int32_t)(snp_offset + j) + 0x243f6a8885a308d3ULL);
+    while (miss) {
+        const int bit = __ffs(miss) - 1;  // even bit position of the code's low bit
+        miss &= miss - 1;
+        const int t = bit >> 1;
+        const int64_t smp = w * 16 + t;
+        if (smp >= n) continue;
+        const int64_t grow = rows ? rows[smp] : smp;
+        uint32_t val;
+        if (q0 >= 1.0) {
+            val = 0;
+        } else if (q01 <= 0.0) {
+            val = 2;
+        } else {
+            uint64_t h = base ^ mix64((uint64_t)(uint32_t)(int32_t)grow + 0x9e3779b97f4a7c15ULL);
+            const uint64_t r = mix64(h);
+            const double u = (double)((r >> 11) & ((1ULL << 53) - 1)) * (1.0 / 9007199254740992.0);
+            val = u < q0 ? 0u : (u < q01 ? 1u : 2u);
+        }
+        const uint32_t code = val == 0 ? 0u : (val == 1 ? 2u : 3u);
+        x = (x & ~(3u << (2 * t))) | (code << (2 * t));
+    }
+    *pw = x;
+}
+void launch_hwe_impute(uint8_t* snp, int64_t stride, int64_t m, int64_t n, const int32_t* nobs,
+                       const int64_t* sum, uint64_t impute_seed, int64_t snp_offset, const int64_t* rows,
+                       hipStream_t st) {
+    if (m <= 0) return;
+    const int64_t words = stride / 4;
+    dim3 grid((unsigned)ceil_div(words, 256), (unsigned)(m < 65535 ? m : 65535), (unsigned)ceil_div(m, 65535));
+    hipLaunchKernelGGL(k_hwe_impute, grid, dim3(256), 0, st, snp, stride, m, n, nobs, sum, impute_seed, snp_offset,
+                       rows);
+    GPE_HIP(hipGetLastError());
+}
+
+// SNP-major -> sample-major transpose of 2-bit codes.  Tile: 64 SNPs x 256 samples.
+__global__ __launch_bounds__(256) void k_transpose(const uint8_t* snp, int64_t snp_stride, uint8_t* smp,
+                                                   int64_t smp_stride) {
+    __shared__ uint8_t codes[64][257];
+    const int tid = threadIdx.x;
+    const int64_t n0 = (int64_t)blockIdx.x * 256;
+    const int64_t j0 = (int64_t)blockIdx.y * 64;
+    {
+        const int row = tid >> 2;
+        const int part = tid & 3;
+        const uint4 v = *reinterpret_cast<const uint4*>(snp + (j0 + row) * snp_stride + n0 / 4 + part * 16);
+        const uint32_t ws[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int t = 0; t < 16; ++t) codes[row][part * 64 + q * 16 + t] = (uint8_t)((ws[q] >> (2 * t)) & 3u);
+    }
+    __syncthreads();
+    uint32_t out[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int r = 0; r < 64; ++r) out[r >> 4] |= (uint32_t)codes[r][tid] << (2 * (r & 15));
+    *reinterpret_cast<uint4*>(smp + (n0 + tid) * smp_stride + j0 / 4) = make_uint4(out[0], out[1], out[2], out[3]);
+}
+void launch_transpose(const uint8_t* snp, int64_t snp_stride, int64_t m, uint8_t* smp, int64_t smp_stride,
+                      int64_t n, hipStream_t st) {
+    if (m <= 0 || n <= 0) return;
+    const int64_t gx = ceil_div(n, 256), gy = ceil_div(m, 64);
+    for (int64_t y0 = 0; y0 < gy; y0 += 65535) {
+        const int64_t yy = (gy - y0) < 65535 ? (gy - y0) : 65535;
+        hipLaunchKernelGGL(k_transpose, dim3((unsigned)gx, (unsigned)yy), dim3(256), 0, st,
+                           snp + y0 * 64 * snp_stride, snp_stride, smp + y0 * 16, smp_stride);
+        GPE_HIP(hipGetLastError());
+    }
+}
+
+// Missing calls per sample row of the sample-major layout.
+__global__ void k_miss_count(const uint8_t* smp, int64_t stride, int64_t n, int32_t* counts) {
+    const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (r >= n) return;
+    const uint32_t* row = reinterpret_cast<const uint32_t*>(smp + r * stride);
+    int64_t c = 0;
+    for (int64_t w = lane; w < stride / 4; w += 64) {
+        const uint32_t x = row[w];
+        c += __popc((x & 0x55555555u) & ~((x >> 1) & 0x55555555u));
+    }
+    c = wave_sum_i64(c);
+    if (lane == 0) counts[r] = (int32_t)c;
+}
+void launch_miss_count(const uint8_t* smp, int64_t stride, int64_t n, int32_t* counts, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_miss_count, dim3((unsigned)ceil_div(n, 4)), dim3(256), 0, st, smp, stride, n, counts);
+    GPE_HIP(hipGetLastError());
+}
+
+// Sorted SNP indices of the missing calls of each sample row.
+__global__ void k_miss_fill(const uint8_t* smp, int64_t stride, int64_t n, const int64_t* ptr, int32_t* idx) {
+    const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (r >= n) return;
+    const uint32_t* row = reinterpret_cast<const uint32_t*>(smp + r * stride);
+    int64_t pos = ptr[r];
+    const int64_t words = stride / 4;
+    for (int64_t w0 = 0; w0 < words; w0 += 64) {
+        const int64_t w = w0 + lane;
+        uint32_t mm = 0;
+        if (w < words) {
+            const uint32_t x = row[w];
+            mm = (x & 0x55555555u) & ~((x >> 1) & 0x55555555u);
+        }
+        const int cnt = __popc(mm);
+        int incl = cnt;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += y;
+        }
+        int64_t p = pos + incl - cnt;
+        while (mm) {
+            const int bit = __ffs(mm) - 1;
+            mm &= mm - 1;
+            idx[p++] = (int32_t)(w * 16 + (bit >> 1));
+        }
+        pos += __shfl(incl, 63, 64);
+    }
+}
+void launch_miss_fill(const uint8_t* smp, int64_t stride, int64_t n, const int64_t* ptr, int32_t* idx,
+                      hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_miss_fill, dim3((unsigned)ceil_div(n, 4)), dim3(256), 0, st, smp, stride, n, ptr, idx);
+    GPE_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Probe streams on the GPU: std::mt19937_64 seeded with make_seed(root, block,
+// v_start) per SNP block (gwldcore.cpp:445-458, :1691-1716).  One wave per
+// block stream; the 312-word twist runs lane-parallel in three dependency
+// phases, and output t of the stream is probe t / L of SNP block_start + t % L.
+// ---------------------------------------------------------------------------
+constexpr int kMtN = 312;
+constexpr int kMtM = 156;
+constexpr uint64_t kMtA = 0xB5026F5AA96619E9ULL;
+constexpr uint64_t kMtUM = 0xFFFFFFFF80000000ULL;
+constexpr uint64_t kMtLM = 0x7FFFFFFFULL;
+
+__device__ __forceinline__ uint64_t mt_twist_one(uint64_t a, uint64_t b, uint64_t c) {
+    const uint64_t x = (a & kMtUM) | (b & kMtLM);
+    return c ^ (x >> 1) ^ ((x & 1ULL) ? kMtA : 0ULL);
+}
+
+__global__ __launch_bounds__(64) void k_mt_rademacher(uint64_t root, const int64_t* blk_starts, int n_blocks,
+                                                     int64_t blk_end_all, int v_start, int v_count, int8_t* z,
+                                                     int64_t ldz) {
+    __shared__ uint64_t mt[kMtN];
+    const int b = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (b >= n_blocks) return;
+    const int64_t s = blk_starts[b];
+    const int64_t e = (b + 1 < n_blocks) ? blk_starts[b + 1] : blk_end_all;
+    const int64_t L = e - s;
+    if (lane == 0) {
+        uint64_t seed = 0x1234abcdULL;
+        seed ^= mix64(root);
+        seed ^= mix64((uint64_t)(int64_t)(int32_t)s + 0x9e37ULL);
+        seed ^= mix64((uint64_t)(int64_t)v_start + 0x85ebULL);
+        mt[0] = seed;
+        for (int i = 1; i < kMtN; ++i) mt[i] = 6364136223846793005ULL * (mt[i - 1] ^ (mt[i - 1] >> 62)) + (uint64_t)i;
+    }
+    __syncthreads();
+    const int64_t total = L * (int64_t)v_count;
+    for (int64_t base = 0; base < total; base += kMtN) {
+        uint64_t nv[3];
+        // phase A: i in [0, 156) reads old values only
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const int i = lane + 64 * q;
+            if (i < kMtN - kMtM) nv[q] = mt_twist_one(mt[i], mt[i + 1], mt[i + kMtM]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const int i = lane + 64 * q;
+            if (i < kMtN - kMtM) mt[i] = nv[q];
+        }
+        __syncthreads();
+        // phase B: i in [156, 311) reads new mt[i-156] and old mt[i], mt[i+1]
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const int i = kMtN - kMtM + lane + 64 * q;
+            if (i < kMtN - 1) nv[q] = mt_twist_one(mt[i], mt[i + 1], mt[i + (kMtM - kMtN)]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const int i = kMtN - kMtM + lane + 64 * q;
+            if (i < kMtN - 1) mt[i] = nv[q];
+        }
+        __syncthreads();
+        if (lane == 0) mt[kMtN - 1] = mt_twist_one(mt[kMtN - 1], mt[0], mt[kMtM - 1]);
+        __syncthreads();
+        for (int i = lane; i < kMtN; i += 64) {
+            const int64_t t = base + i;
+            if (t >= total) break;
+            uint64_t y = mt[i];
+            y ^= (y >> 29) & 0x5555555555555555ULL;
+            y ^= (y << 17) & 0x71D67FFFEDA60000ULL;
+            y ^= (y << 37) & 0xFFF7EEE000000000ULL;
+            y ^= (y >> 43);
+            const int64_t v = t / L;
+            const int64_t r = t - v * L;
+            z[(s + r) * ldz + v] = (y & 1ULL) ? (int8_t)1 : (int8_t)-1;
+        }
+        __syncthreads();
+    }
+}
+void launch_mt_rademacher(uint64_t root, const int64_t* blk_starts, int n_blocks, int64_t blk_end_all,
+                          int v_start, int v_count, int8_t* z, int64_t ldz, hipStream_t st) {
+    if (n_blocks <= 0 || v_count <= 0) return;
+    hipLaunchKernelGGL(k_mt_rademacher, dim3((unsigned)n_blocks), dim3(64), 0, st, root, blk_starts, n_blocks,
+                       blk_end_all, v_start, v_count, z, ldz);
+    GPE_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Quantization of real coefficients into int8 limbs
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double p1_coef(const double* w, int64_t ldw, const int8_t* z_i8, const double* z_f64,
+                                          int64_t ldz, int64_t i, int k, int v) {
+    const double zz = z_i8 ? (double)z_i8[i * ldz + v] : z_f64[i * ldz + v];
+    return w[i * ldw + k] * zz;
+}
+
+// column max |coef| over SNPs [k0,k1); column c = bin * v_pad + v
+__global__ void k_coef_p1_colmax(const double* w, int64_t ldw, const int8_t* z_i8, const double* z_f64, int64_t ldz,
+                                 int64_t k0, int64_t k1, int v_count, int v_pad, uint64_t* colmax_bits) {
+    const int c = blockIdx.y;
+    const int k = c / v_pad, v = c % v_pad;
+    if (v >= v_count) return;
+    double mx = 0.0;
+    for (int64_t i = k0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < k1; i += (int64_t)gridDim.x * blockDim.x)
+        mx = fmax(mx, fabs(p1_coef(w, ldw, z_i8, z_f64, ldz, i, k, v)));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+    if ((threadIdx.x & 63) == 0 && mx > 0.0)
+        atomicMax(reinterpret_cast<unsigned long long*>(colmax_bits + c), (unsigned long long)__double_as_longlong(mx));
+}
+void launch_coef_p1_colmax(const double* w, int64_t ldw, const int8_t* z_i8, const double* z_f64, int64_t ldz,
+                           int64_t k0, int64_t k1, int n_bins, int v_count, int v_pad, uint64_t* colmax_bits,
+                           hipStream_t st) {
+    const int64_t L = k1 - k0;
+    if (L <= 0) return;
+    int64_t gx = ceil_div(L, 256);
+    if (gx > 64) gx = 64;
+    hipLaunchKernelGGL(k_coef_p1_colmax, dim3((unsigned)gx, (unsigned)(n_bins * v_pad)), dim3(256), 0, st, w, ldw,
+                       z_i8, z_f64, ldz, k0, k1, v_count, v_pad, colmax_bits);
+    GPE_HIP(hipGetLastError());
+}
+
+// shift_c = P - exponent(max|x_c|) so that |x * 2^shift| <= 2^P
+__global__ void k_shift_from_colmax(const uint64_t* colmax_bits, int n, int precision_bits, int32_t* shift,
+                                    double* col_scale) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const double mx = __longlong_as_double((long long)colmax_bits[c]);
+    int sh = 0;
+    if (mx > 0.0) {
+        int e = 0;
+        frexp(mx, &e);  // mx < 2^e
+        sh = precision_bits - e;
+    }
+    shift[c] = sh;
+    col_scale[c] = ldexp(1.0, -sh);
+}
+void launch_shift_from_colmax(const uint64_t* colmax_bits, int n, int precision_bits, int32_t* shift,
+                              double* col_scale, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_shift_from_colmax, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, colmax_bits, n,
+                       precision_bits, shift, col_scale);
+    GPE_HIP(hipGetLastError());
+}
+
+template <int NL>
+__global__ void k_coef_p1_quant(const double* w, int64_t ldw, const int8_t* z_i8, const double* z_f64, int64_t ldz,
+                                int64_t k0, int64_t k1, int v_count, int v_pad, const int32_t* shift, int ctc,
+                                int64_t coef_ctiles, int64_t kc0, int64_t kcount, int8_t* coef) {
+    // x: SNP offset inside the chunk range, y: column
+    const int64_t kk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (kk >= kcount * kKChunk) return;
+    const int64_t i = kc0 * kKChunk + kk;
+    const int k = c / v_pad, v = c % v_pad;
+    int64_t q = 0;
+    if (v < v_count && i >= k0 && i < k1) q = __double2ll_rn(ldexp(p1_coef(w, ldw, z_i8, z_f64, ldz, i, k, v), shift[c]));
+    int8_t d[NL];
+    balanced_digits<NL>(q, d);
+    const int ct = c >> 5, lr = c & 31;
+    const int cgp = ct / ctc, ctcl = ct % ctc;
+    const int64_t kl = kk / kKChunk;
+    const int kin = (int)(kk % kKChunk);
+#pragma unroll
+    for (int l = 0; l < NL; ++l) {
+        const int64_t tg = (int64_t)cgp * (NL * ctc) + l * ctc + ctcl;
+        coef[swz_offset(kl, coef_ctiles, tg, kin, lr)] = d[l];
+    }
+}
+
+// Cst_c = sum_i mean_i * deq(coef_ic), one deterministic block reduction per column
+__global__ void k_coef_p1_cst(const double* w, int64_t ldw, const int8_t* z_i8, const double* z_f64, int64_t ldz,
+                              int64_t k0, int64_t k1, int v_count, int v_pad, const int32_t* shift,
+                              const double* mean, double* cst) {
+    __shared__ double red[4];
+    const int c = blockIdx.x;
+    const int k = c / v_pad, v = c % v_pad;
+    double acc = 0.0;
+    if (v < v_count) {
+        const int sh = shift[c];
+        for (int64_t i = k0 + threadIdx.x; i < k1; i += blockDim.x) {
+            const double q = (double)__double2ll_rn(ldexp(p1_coef(w, ldw, z_i8, z_f64, ldz, i, k, v), sh));
+            acc += mean[i] * q;
+        }
+    }
+    acc = wave_sum_f64(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) cst[c] = v < v_count ? ldexp((red[0] + red[1]) + (red[2] + red[3]), -shift[c]) : 0.0;
+}
+
+void launch_coef_p1_quant(const double* w, int64_t ldw, const int8_t* z_i8, const double* z_f64, int64_t ldz,
+                          int64_t k0, int64_t k1, int n_bins, int v_count, int v_pad, const int32_t* shift, int nl,
+                          int ctc, int64_t coef_ctiles, int64_t kc0, int8_t* coef, const double* mean, double* cst,
+                          hipStream_t st) {
+    const int64_t kc1 = ceil_div(k1, kKChunk);
+    const int64_t kcount = kc1 - kc0;
+    const int ncols = n_bins * v_pad;
+    if (kcount > 0) {
+        dim3 grid((unsigned)ceil_div(kcount * kKChunk, 256), (unsigned)ncols);
+        if (nl == 5)
+            hipLaunchKernelGGL(k_coef_p1_quant<5>, grid, dim3(256), 0, st, w, ldw, z_i8, z_f64, ldz, k0, k1, v_count,
+                               v_pad, shift, ctc, coef_ctiles, kc0, kcount, coef);
+        else if (nl == 6)
+            hipLaunchKernelGGL(k_coef_p1_quant<6>, grid, dim3(256), 0, st, w, ldw, z_i8, z_f64, ldz, k0, k1, v_count,
+                               v_pad, shift, ctc, coef_ctiles, kc0, kcount, coef);
+        else
+            throw Error("phase-1 coefficient limbs must be 5 or 6");
+        GPE_HIP(hipGetLastError());
+    }
+    if (cst) {
+        hipLaunchKernelGGL(k_coef_p1_cst, dim3((unsigned)ncols), dim3(256), 0, st, w, ldw, z_i8, z_f64, ldz, k0, k1,
+                           v_count, v_pad, shift, mean, cst);
+        GPE_HIP(hipGetLastError());
+    }
+}
+
+// Phase-1 finalize: Xz[:, bin*V+v] (+)= Y*2^-shift - Cst + sum_{missing i} mean_i * coef_i.
+// One wave per sample, lanes over columns.
+__global__ void k_p1_finalize(const int64_t* y, int64_t ldy, int64_t n, int n_bins, int v_count, int v_pad,
+                              const double* col_scale, const double* cst, const int64_t* miss_ptr,
+                              const int32_t* miss_idx, int64_t k0, int64_t k1, const double* w, int64_t ldw,
+                              const int8_t* z_i8, const double* z_f64, int64_t ldz, const int32_t* shift,
+                              const double* mean, double* xz, int64_t ldxz, const uint8_t* col_mode) {
+    const int64_t smp = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (smp >= n) return;
+    int64_t mb = 0, me = 0;
+    if (miss_ptr) {
+        mb = miss_ptr[smp];
+        me = miss_ptr[smp + 1];
+        // first entry >= k0 (lists are sorted)
+        int64_t lo = mb, hi = me;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (miss_idx[mid] < k0) lo = mid + 1;
+            else hi = mid;
+        }
+        mb = lo;
+    }
+    const int ncols = n_bins * v_pad;
+    for (int c0 = 0; c0 < ncols; c0 += 64) {
+        const int c = c0 + lane;
+        if (c >= ncols) continue;
+        const int k = c / v_pad, v = c % v_pad;
+        if (v >= v_count) continue;
+        const int mode = col_mode ? col_mode[k] : 2;
+        double val = (double)y[(int64_t)c * ldy + smp] * col_scale[c] - cst[c];
+        if (miss_ptr) {
+            const int sh = shift[c];
+            const double sc = col_scale[c];
+            for (int64_t p = mb; p < me; ++p) {
+                const int64_t i = miss_idx[p];
+                if (i >= k1) break;
+                const double q = (double)__double2ll_rn(ldexp(p1_coef(w, ldw, z_i8, z_f64, ldz, i, k, v), sh));
+                val += mean[i] * (q * sc);
+            }
+        }
+        double* dst = xz + (int64_t)(k * v_count + v) * ldxz + smp;
+        if (mode == 1) *dst = val;
+        else *dst += val;
+    }
+}
+void launch_p1_finalize(const int64_t* y, int64_t ldy, int64_t n, int n_bins, int v_count, int v_pad,
+                        const double* col_scale, const double* cst, const int64_t* miss_ptr,
+                        const int32_t* miss_idx, int64_t k0, int64_t k1, const double* w, int64_t ldw,
+                        const int8_t* z_i8, const double* z_f64, int64_t ldz, const int32_t* shift,
+                        const double* mean, double* xz, int64_t ldxz, const uint8_t* col_mode, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_p1_finalize, dim3((unsigned)ceil_div(n, 4)), dim3(256), 0, st, y, ldy, n, n_bins, v_count,
+                       v_pad, col_scale, cst, miss_ptr, miss_idx, k0, k1, w, ldw, z_i8, z_f64, ldz, shift, mean, xz,
+                       ldxz, col_mode);
+    GPE_HIP(hipGetLastError());
+}
+
+// column max |x| of a column-major FP64 matrix whose column bin*V+v maps to padded column bin*v_pad+v
+__global__ void k_colmax_f64(const double* x, int64_t ldx, int64_t n, int v_count, int v_pad,
+                             uint64_t* colmax_bits) {
+    const int c = blockIdx.y;
+    const int k = c / v_pad, v = c % v_pad;
+    if (v >= v_count) return;
+    const double* col = x + (int64_t)(k * v_count + v) * ldx;
+    double mx = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        mx = fmax(mx, fabs(col[i]));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+    if ((threadIdx.x & 63) == 0 && mx > 0.0)
+        atomicMax(reinterpret_cast<unsigned long long*>(colmax_bits + c), (unsigned long long)__double_as_longlong(mx));
+}
+void launch_colmax_f64(const double* x, int64_t ldx, int64_t n, int n_cols, int v_count, int v_pad,
+                       uint64_t* colmax_bits, hipStream_t st) {
+    if (n <= 0 || n_cols <= 0) return;
+    int64_t gx = ceil_div(n, 256);
+    if (gx > 64) gx = 64;
+    hipLaunchKernelGGL(k_colmax_f64, dim3((unsigned)gx, (unsigned)n_cols), dim3(256), 0, st, x, ldx, n, v_count,
+                       v_pad, colmax_bits);
+    GPE_HIP(hipGetLastError());
+}
+
+// Phase-2 operand: limbs of round(Xz * 2^shift) over samples, plus exact column sums.
+template <int NL>
+__global__ void k_quant_p2(const double* x, int64_t ldx, int64_t n, int v_count, int v_pad, const int32_t* shift,
+                           int ctc, int64_t coef_ctiles, int64_t kcount, int8_t* coef, int64_t* col_sum) {
+    const int64_t kk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    const int k = c / v_pad, v = c % v_pad;
+    int64_t q = 0;
+    if (kk < kcount * kKChunk && v < v_count && kk < n)
+        q = __double2ll_rn(ldexp(x[(int64_t)(k * v_count + v) * ldx + kk], shift[c]));
+    const int64_t qs = wave_sum_i64(q);
+    if ((threadIdx.x & 63) == 0 && qs != 0)
+        atomicAdd(reinterpret_cast<unsigned long long*>(col_sum + c), (unsigned long long)qs);
+    if (kk >= kcount * kKChunk) return;
+    int8_t d[NL];
+    balanced_digits<NL>(q, d);
+    const int ct = c >> 5, lr = c & 31;
+    const int cgp = ct / ctc, ctcl = ct % ctc;
+    const int64_t kl = kk / kKChunk;
+    const int kin = (int)(kk % kKChunk);
+#pragma unroll
+    for (int l = 0; l < NL; ++l) {
+        const int64_t tg = (int64_t)cgp * (NL * ctc) + l * ctc + ctcl;
+        coef[swz_offset(kl, coef_ctiles, tg, kin, lr)] = d[l];
+    }
+}
+void launch_quant_p2(const double* x, int64_t ldx, int64_t n, int n_bins, int v_count, int v_pad,
+                     const int32_t* shift, int nl, int ctc, int64_t coef_ctiles, int8_t* coef, int64_t* col_sum,
+                     hipStream_t st) {
+    const int64_t kcount = ceil_div(n, kKChunk);
+    dim3 grid((unsigned)ceil_div(kcount * kKChunk, 256), (unsigned)(n_bins * v_pad));
+    if (nl == 5)
+        hipLaunchKernelGGL(k_quant_p2<5>, grid, dim3(256), 0, st, x, ldx, n, v_count, v_pad, shift, ctc, coef_ctiles,
+                           kcount, coef, col_sum);
+    else if (nl == 6)
+        hipLaunchKernelGGL(k_quant_p2<6>, grid, dim3(256), 0, st, x, ldx, n, v_count, v_pad, shift, ctc, coef_ctiles,
+                           kcount, coef, col_sum);
+    else
+        throw Error("phase-2 operand limbs must be 5 or 6");
+    GPE_HIP(hipGetLastError());
+}
+
+// Phase-2 finalize (gwldcore.cpp:2525-2593): per (SNP, bin) sum the per-tile
+// Work^2 sums, scale by inv^2/(N_denom-1)^2, guard, and accumulate.
+__global__ void k_p2_finalize(const double* s2, const double* s4, int64_t ld_s, int64_t r0, int64_t r1, int n_bins,
+                              int tiles_per_bin, int v_count, const double* inv_left, double inv_denom2,
+                              double extreme_warn, int drop_extreme, double* meansq, int64_t ld_meansq,
+                              double* mc_agg, double* mc_per_snp, unsigned long long* guard_counts) {
+    const int64_t j = r0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    if (j >= r1) return;
+    double acc = 0.0, acc4 = 0.0;
+    for (int t = 0; t < tiles_per_bin; ++t) {
+        acc += s2[j * ld_s + k * tiles_per_bin + t];
+        if (s4) acc4 += s4[j * ld_s + k * tiles_per_bin + t];
+    }
+    const double inv = inv_left[j - r0];
+    const double left = (isfinite(inv) && inv > 0.0) ? inv * inv * inv_denom2 : 0.0;
+    double upd = acc * left;
+    const double seg = (double)v_count;
+    if (!isfinite(upd)) {
+        atomicAdd(&guard_counts[0], 1ULL);
+        return;
+    }
+    if (isfinite(extreme_warn) && seg > 0.0 && upd > extreme_warn * seg) {
+        atomicAdd(&guard_counts[1], 1ULL);
+        if (drop_extreme) return;
+    }
+    if (upd < 0.0) upd = 0.0;
+    meansq[j * ld_meansq + k] += upd;
+    if (s4) {
+        const double sumsq = acc4 * left * left;
+        if (mc_per_snp) mc_per_snp[j * ld_meansq + k] += sumsq;
+        if (mc_agg) atomicAdd(&mc_agg[k], sumsq);
+    }
+}
+void launch_p2_finalize(const double* s2, const double* s4, int64_t ld_s, int64_t r0, int64_t r1, int n_bins,
+                        int tiles_per_bin, int v_count, const double* inv_left, double inv_denom2,
+                        double extreme_warn, int drop_extreme, double* meansq, int64_t ld_meansq, double* mc_agg,
+                        double* mc_per_snp, unsigned long long* guard_counts, hipStream_t st) {
+    if (r1 <= r0) return;
+    hipLaunchKernelGGL(k_p2_finalize, dim3((unsigned)ceil_div(r1 - r0, 256), (unsigned)n_bins), dim3(256), 0, st, s2,
+                       s4, ld_s, r0, r1, n_bins, tiles_per_bin, v_count, inv_left, inv_denom2, extreme_warn,
+                       drop_extreme, meansq, ld_meansq, mc_agg, mc_per_snp, guard_counts);
+    GPE_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Exact integer path operands
+// ---------------------------------------------------------------------------
+// int8 Z (row i, column b; row-major, ldz) -> one limb, two column tiles per group
+__global__ void k_coef_i8(const int8_t* z, int64_t ldz, int64_t k0, int64_t k1, int n_cols, int64_t coef_ctiles,
+                          int64_t kc0, int64_t kcount, int8_t* coef) {
+    const int64_t kk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (kk >= kcount * kKChunk) return;
+    const int64_t i = kc0 * kKChunk + kk;
+    int8_t val = 0;
+    if (c < n_cols && i >= k0 && i < k1) val = z[(i - k0) * ldz + c];
+    const int ct = c >> 5, lr = c & 31;
+    coef[swz_offset(kk / kKChunk, coef_ctiles, ct, (int)(kk % kKChunk), lr)] = val;
+}
+void launch_coef_i8(const int8_t* z, int64_t ldz, int64_t k0, int64_t k1, int n_cols, int64_t coef_ctiles,
+                    int8_t* coef, hipStream_t st) {
+    const int64_t kc0 = k0 / kKChunk;
+    const int64_t kcount = ceil_div(k1, kKChunk) - kc0;
+    if (kcount <= 0) return;
+    dim3 grid((unsigned)ceil_div(kcount * kKChunk, 256), (unsigned)(coef_ctiles * 32));
+    hipLaunchKernelGGL(k_coef_i8, grid, dim3(256), 0, st, z, ldz, k0, k1, n_cols, coef_ctiles, kc0, kcount, coef);
+    GPE_HIP(hipGetLastError());
+}
+
+// int32 U (column b, row n; column-major, ldu) -> NL balanced limbs over samples
+template <int NL>
+__global__ void k_quant_i32(const int32_t* u, int64_t ldu, int64_t n, int n_cols, int ctc, int64_t coef_ctiles,
+                            int64_t kcount, int8_t* coef) {
+    const int64_t kk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (kk >= kcount * kKChunk) return;
+    int64_t q = 0;
+    if (c < n_cols && kk < n) q = u[(int64_t)c * ldu + kk];
+    int8_t d[NL];
+    balanced_digits<NL>(q, d);
+    const int ct = c >> 5, lr = c & 31;
+    const int cgp = ct / ctc, ctcl = ct % ctc;
+#pragma unroll
+    for (int l = 0; l < NL; ++l) {
+        const int64_t tg = (int64_t)cgp * (NL * ctc) + l * ctc + ctcl;
+        coef[swz_offset(kk / kKChunk, coef_ctiles, tg, (int)(kk % kKChunk), lr)] = d[l];
+    }
+}
+void launch_quant_i32(const int32_t* u, int64_t ldu, int64_t n, int n_cols, int nl, int ctc, int64_t coef_ctiles,
+                      int8_t* coef, hipStream_t st) {
+    const int64_t kcount = ceil_div(n, kKChunk);
+    const int ncols_pad = (int)(coef_ctiles / nl) * 32;
+    dim3 grid((unsigned)ceil_div(kcount * kKChunk, 256), (unsigned)ncols_pad);
+    if (nl == 3)
+        hipLaunchKernelGGL(k_quant_i32<3>, grid, dim3(256), 0, st, u, ldu, n, n_cols, ctc, coef_ctiles, kcount, coef);
+    else
+        throw Error("int32 operand limbs must be 3");
+    GPE_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Synthetic genotypes written straight into the SNP-major layout.
+// Two haplotypes per sample follow a per-block Markov chain along SNPs: at a
+// block start each allele is a fresh Bernoulli(MAF_j) draw, inside a block it
+// copies the previous SNP's allele with probability copy_prob, else it is
+// redrawn (copy_prob = 0 gives i.i.d. Binomial(2, MAF_j) genotypes).  A call
+// is missing with probability miss_rate.  MAF_j = lo + (hi-lo) * U_j.
+// One thread owns 4 samples (one byte column) x one SNP block and draws from
+// a xorshift64* stream keyed by (seed, block, byte column).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t xs64(uint64_t& s) {
+    s ^= s >> 12;
+    s ^= s << 25;
+    s ^= s >> 27;
+    return s * 0x2545F4914F6CDD1DULL;
+}
+
+__global__ void k_synth_markov(uint8_t* snp, int64_t stride, int64_t m, int64_t n, uint64_t seed, double maf_lo,
+                               double maf_hi, uint64_t miss32, int block_len, uint32_t copy16) {
+    const int64_t bcol = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t blk = blockIdx.y + (int64_t)blockIdx.z * 65535;
+    const int64_t nbytes = ceil_div(n, 4);
+    if (bcol >= nbytes) return;
+    const int64_t j0 = blk * block_len;
+    if (j0 >= m) return;
+    uint64_t st = mix64(seed ^ mix64((uint64_t)blk * 0x9E3779B97F4A7C15ULL ^ mix64((uint64_t)bcol + 0x632BE59BD9B4E019ULL)));
+    if (st == 0) st = 0x9E3779B97F4A7C15ULL;
+    uint32_t a0 = 0, a1 = 0;  // bit t: allele of haplotype 0/1 of sample 4*bcol+t
+    const int64_t jend = (j0 + block_len < m) ? j0 + block_len : m;
+    const uint64_t maf_key = mix64(seed + 0xD1B54A32D192ED03ULL);
+    for (int64_t j = j0; j < jend; ++j) {
+        const uint64_t hj = mix64(maf_key ^ (uint64_t)j);
+        const double u = (double)(hj >> 11) * (1.0 / 9007199254740992.0);
+        const double p = maf_lo + (maf_hi - maf_lo) * u;
+        const uint32_t p16 = (uint32_t)(p * 65536.0);
+        uint32_t byte = 0;
+        for (int t = 0; t < 4; ++t) {
+            const uint64_t r1 = xs64(st);
+            const uint64_t r2 = xs64(st);
+            const uint32_t f0 = ((uint32_t)(r1 & 0xFFFF) < p16) ? 1u : 0u;
+            const uint32_t f1 = ((uint32_t)((r1 >> 16) & 0xFFFF) < p16) ? 1u : 0u;
+            const bool c0 = (j > j0) && ((uint32_t)((r1 >> 32) & 0xFFFF) < copy16);
+            const bool c1 = (j > j0) && ((uint32_t)(r1 >> 48) < copy16);
+            const uint32_t h0 = c0 ? ((a0 >> t) & 1u) : f0;
+            const uint32_t h1 = c1 ? ((a1 >> t) & 1u) : f1;
+            a0 = (a0 & ~(1u << t)) | (h0 << t);
+            a1 = (a1 & ~(1u << t)) | (h1 << t);
+            const uint32_t g = h0 + h1;
+            uint32_t code = g == 0 ? 0u : (g == 1 ? 2u : 3u);
+            if ((r2 >> 32) < miss32) code = 1u;
+            if (bcol * 4 + t >= n) code = 0u;
+            byte |= code << (2 * t);
+        }
+        snp[j * stride + bcol] = (uint8_t)byte;
+    }
+}
+void launch_synth_markov(uint8_t* snp, int64_t stride, int64_t m, int64_t n, uint64_t seed, double maf_lo,
+                         double maf_hi, double miss_rate, int block_len, double copy_prob, int iid,
+                         hipStream_t st) {
+    if (m <= 0 || n <= 0) return;
+    const int64_t nblk = ceil_div(m, block_len);
+    const uint64_t miss32 = (uint64_t)(miss_rate * 4294967296.0);
+    const uint32_t copy16 = iid ? 0u : (uint32_t)(copy_prob * 65536.0);
+    dim3 grid((unsigned)ceil_div(ceil_div(n, 4), 256), (unsigned)(nblk < 65535 ? nblk : 65535),
+              (unsigned)ceil_div(nblk, 65535));
+    hipLaunchKernelGGL(k_synth_markov, grid, dim3(256), 0, st, snp, stride, m, n, seed, maf_lo, maf_hi, miss32,
+                       block_len, copy16);
+    GPE_HIP(hipGetLastError());
+}
+
+}  // namespace gpe

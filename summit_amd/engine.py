@@ -1,0 +1,268 @@
+"""Device-level Python front end of libsummit_gpe.
+
+``Genotype`` owns one genotype matrix resident in HBM (both packed layouts and
+per-SNP statistics).  The functions below take torch tensors on the GPU and
+hand their device pointers to the C ABI; PyTorch is used only to allocate and
+move device buffers.  Shapes follow the reference's arrays:
+
+* ``xz``: the reference's Fortran (N, Q) ``Xz2d`` is held as a contiguous
+  torch tensor of shape (Q, N) (same memory layout).
+* ``meansq``: (M, B) row-major FP64.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+import torch
+
+from . import _native
+from ._native import c_i64, c_vp, call
+
+IMPUTE = {"mean": 0, "hwe": 1}
+RAND = {"rademacher": 0, "gaussian": 1, "normal": 1}
+LAYOUT = {"snp_raw": 0, "snp_work": 1, "sample_work": 2}
+
+
+def _ptr(t) -> int:
+    if t is None:
+        return 0
+    return int(t.data_ptr())
+
+
+def _stream(stream=None) -> int:
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return int(stream.cuda_stream)
+
+
+def _np_ptr(a) -> int:
+    return 0 if a is None else int(a.ctypes.data)
+
+
+class Genotype:
+    """A genotype matrix resident in HBM (n_samples x n_snps, 2-bit codes)."""
+
+    def __init__(self, n_samples: int, n_snps: int):
+        lib = _native.load()
+        h = c_vp()
+        _native.check(lib.gpe_geno_create(int(n_samples), int(n_snps), ctypes.byref(h)))
+        self._h = h
+        self.n = int(n_samples)
+        self.m = int(n_snps)
+        self.prepared_with = None
+
+    # ------------------------------------------------------------------ creation
+    @classmethod
+    def from_bed_bytes(cls, body: np.ndarray, n_total: int, n_snps: int, rows=None, stream=None) -> "Genotype":
+        """body: the BED bytes after the magic, (n_snps, ceil(n_total/4)) uint8 (may be a memmap)."""
+        body = np.asarray(body)
+        stride = (n_total + 3) // 4
+        if body.ndim != 2 or body.shape[0] < n_snps or body.shape[1] != stride:
+            raise RuntimeError("BED body must be (n_snps, ceil(n_total/4)) bytes")
+        rows_arr = None
+        n = n_total
+        if rows is not None:
+            rows_arr = np.ascontiguousarray(np.asarray(rows, dtype=np.int64))
+            n = int(rows_arr.size)
+        g = cls(n, n_snps)
+        chunk = max(1, (256 << 20) // max(stride, 1))
+        for j0 in range(0, n_snps, chunk):
+            j1 = min(n_snps, j0 + chunk)
+            part = np.ascontiguousarray(body[j0:j1])
+            call("gpe_geno_upload_bed", g._h, _np_ptr(part), n_total, stride, j0, j1 - j0, _np_ptr(rows_arr),
+                 _stream(stream))
+        return g
+
+    @classmethod
+    def synthetic(cls, n_samples: int, n_snps: int, seed: int, maf_lo: float = 0.05, maf_hi: float = 0.5,
+                  miss_rate: float = 0.01, block_len: int = 100, copy_prob: float = 0.0, stream=None):
+        g = cls(n_samples, n_snps)
+        call("gpe_geno_synthesize", g._h, int(seed) & 0xFFFFFFFFFFFFFFFF, float(maf_lo), float(maf_hi),
+             float(miss_rate), int(block_len), float(copy_prob), _stream(stream))
+        return g
+
+    # ------------------------------------------------------------------ state
+    def prepare(self, ddof: int = 1, impute: str = "mean", impute_seed: int = 0, snp_offset: int = 0,
+                global_rows=None, stream=None) -> "Genotype":
+        if impute not in IMPUTE:
+            raise RuntimeError("impute_mode must be 'mean' or 'hwe'")
+        rows = None if global_rows is None else np.ascontiguousarray(np.asarray(global_rows, dtype=np.int64))
+        call("gpe_geno_prepare", self._h, int(ddof), IMPUTE[impute], int(impute_seed) & 0xFFFFFFFFFFFFFFFF,
+             int(snp_offset), _np_ptr(rows), _stream(stream))
+        self.prepared_with = (int(ddof), impute, int(impute_seed))
+        return self
+
+    def stats(self) -> dict:
+        nobs = np.empty(self.m, dtype=np.int32)
+        s = np.empty(self.m, dtype=np.int64)
+        ss = np.empty(self.m, dtype=np.int64)
+        mean = np.empty(self.m, dtype=np.float64)
+        inv_sd = np.empty(self.m, dtype=np.float64)
+        call("gpe_geno_stats", self._h, _np_ptr(nobs), _np_ptr(s), _np_ptr(ss), _np_ptr(mean), _np_ptr(inv_sd))
+        return {"nobs": nobs, "sum": s, "sumsq": ss, "mean": mean, "inv_sd": inv_sd}
+
+    def n_missing(self) -> int:
+        n, m, miss = c_i64(), c_i64(), c_i64()
+        call("gpe_geno_shape", self._h, ctypes.byref(n), ctypes.byref(m), ctypes.byref(miss))
+        return int(miss.value)
+
+    def download(self, layout: str = "snp_raw") -> np.ndarray:
+        if layout == "sample_work":
+            rows, used = self.n, (self.m + 3) // 4
+        else:
+            rows, used = self.m, (self.n + 3) // 4
+        out = np.empty((rows, used), dtype=np.uint8)
+        call("gpe_geno_download", self._h, LAYOUT[layout], _np_ptr(out), used)
+        return out
+
+    def download_rows(self, layout: str, row_begin: int, row_count: int) -> np.ndarray:
+        used = (self.m + 3) // 4 if layout == "sample_work" else (self.n + 3) // 4
+        out = np.empty((row_count, used), dtype=np.uint8)
+        call("gpe_geno_download_rows", self._h, LAYOUT[layout], int(row_begin), int(row_count), _np_ptr(out), used)
+        return out
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _native.load().gpe_geno_free(self._h)
+            self._h = c_vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+
+# ---------------------------------------------------------------------- products
+def gz_i32(g: Genotype, z: torch.Tensor, snp_begin: int = 0, snp_end: int | None = None, out=None,
+           accumulate: bool = False, stream=None) -> torch.Tensor:
+    """U[b, n] = sum_i g[i, n] * z[i - snp_begin, b]; z int8 (L, B) on the GPU."""
+    snp_end = g.m if snp_end is None else snp_end
+    if z.dtype != torch.int8 or z.dim() != 2 or z.shape[0] != snp_end - snp_begin:
+        raise RuntimeError("z must be int8 of shape (snp_end - snp_begin, B)")
+    z = z.contiguous()
+    B = z.shape[1]
+    if out is None:
+        out = torch.zeros((B, g.n), dtype=torch.int32, device=z.device)
+        accumulate = True
+    call("gpe_gz_i32", g.handle, snp_begin, snp_end, _ptr(z), B, B, _ptr(out), out.stride(0), int(accumulate),
+         _stream(stream))
+    return out
+
+
+def gtu_i64(g: Genotype, u: torch.Tensor, snp_begin: int = 0, snp_end: int | None = None,
+            stream=None) -> torch.Tensor:
+    """W[i - snp_begin, b] = sum_n g[i, n] * u[b, n] (exact int64)."""
+    snp_end = g.m if snp_end is None else snp_end
+    if u.dtype != torch.int32 or u.dim() != 2 or u.shape[1] != g.n:
+        raise RuntimeError("u must be int32 of shape (B, n_samples)")
+    u = u.contiguous()
+    B = u.shape[0]
+    w = torch.empty((snp_end - snp_begin, B), dtype=torch.int64, device=u.device)
+    call("gpe_gtu_i64", g.handle, snp_begin, snp_end, _ptr(u), u.stride(0), B, _ptr(w), B, _stream(stream))
+    return w
+
+
+def resvar_inv(g: Genotype, snp_begin: int = 0, snp_end: int | None = None, eps: float = 1e-10,
+               out: torch.Tensor | None = None, stream=None) -> torch.Tensor:
+    snp_end = g.m if snp_end is None else snp_end
+    if out is None:
+        out = torch.empty(snp_end - snp_begin, dtype=torch.float64, device="cuda")
+    call("gpe_resvar_inv", g.handle, snp_begin, snp_end, float(eps), _ptr(out), _stream(stream))
+    return out
+
+
+def phase1_xz(g: Genotype, snp_begin: int, snp_end: int, step_size: int, annot: torch.Tensor,
+              inv: torch.Tensor, v_start: int, v_count: int, seed: int, xz: torch.Tensor,
+              rand_dist: str = "rademacher", bin_mode: torch.Tensor | None = None, limbs: int = 5,
+              stream=None) -> torch.Tensor:
+    """Xz[k*v_count+v, :] (+)= sum_i x_i inv_i sqrt(a_ik) z_iv over [snp_begin, snp_end)."""
+    L = snp_end - snp_begin
+    annot = annot.to(torch.float64).contiguous()
+    inv = inv.to(torch.float64).contiguous()
+    if annot.dim() != 2 or annot.shape[0] != L:
+        raise RuntimeError("L mismatch")
+    if inv.numel() != L:
+        raise RuntimeError("inv_right shape mismatch in phase1")
+    B = annot.shape[1]
+    if xz.dtype != torch.float64 or xz.dim() != 2 or xz.shape[0] != B * v_count or xz.shape[1] != g.n:
+        raise RuntimeError("Xz2d_chunk shape must be (N, B*v_count)")
+    if rand_dist not in RAND:
+        raise RuntimeError(f"unsupported rand_dist {rand_dist!r}")
+    call("gpe_phase1_xz", g.handle, snp_begin, snp_end, step_size, _ptr(annot), B, _ptr(inv), v_start, v_count,
+         RAND[rand_dist], int(seed) & 0xFFFFFFFFFFFFFFFF, _ptr(xz), xz.stride(0), _ptr(bin_mode), limbs,
+         _stream(stream))
+    return xz
+
+
+def phase2_accum(g: Genotype, snp_begin: int, snp_end: int, inv_left: torch.Tensor, tile_nvecs: int,
+                 xz: torch.Tensor, meansq: torch.Tensor, n_denom: int, mc_aggregate_sumsq=None,
+                 mc_per_snp_sumsq=None, extreme_warn: float = 1.0e10, drop_extreme: bool = True,
+                 limbs: int = 5, meansq_row0: int = 0, stream=None, want_guards: bool = False):
+    """meansq[j, k] += sum_v (x_j . Xz[k*V+v])^2 inv_j^2/(N_denom-1)^2 for j in [snp_begin, snp_end).
+
+    ``meansq`` may hold only rows [meansq_row0, ...); it is addressed by absolute SNP index."""
+    if xz.dtype != torch.float64 or xz.dim() != 2 or xz.shape[1] != g.n:
+        raise RuntimeError("Xz2d shape mismatch in phase2_accum_XtXz_bed")
+    Q = xz.shape[0]
+    if tile_nvecs <= 0 or Q % tile_nvecs:
+        raise RuntimeError("Xz2d col count must be a multiple of tile_nvecs")
+    B = Q // tile_nvecs
+    if meansq.dtype != torch.float64 or meansq.dim() != 2 or meansq.shape[1] != B or not meansq.is_contiguous():
+        raise RuntimeError("meansq_accum shape mismatch in phase2_accum_XtXz_bed")
+    inv_left = inv_left.to(torch.float64).contiguous()
+    if inv_left.numel() != snp_end - snp_begin:
+        raise RuntimeError("inv_left shape mismatch in phase2_accum_XtXz_bed")
+    shift = meansq_row0 * B * 8
+    base = _ptr(meansq) - shift
+    mcp = 0 if mc_per_snp_sumsq is None else _ptr(mc_per_snp_sumsq) - shift
+    counts = (ctypes.c_ulonglong * 2)()
+    call("gpe_phase2_accum", g.handle, snp_begin, snp_end, _ptr(inv_left), tile_nvecs, _ptr(xz), xz.stride(0), B,
+         int(n_denom), base, B, _ptr(mc_aggregate_sumsq), mcp, float(extreme_warn), int(bool(drop_extreme)), limbs,
+         ctypes.cast(counts, c_vp) if want_guards else None, _stream(stream))
+    if want_guards:
+        return int(counts[0]), int(counts[1])
+    return None
+
+
+def probes(root_seed: int, blk_starts, blk_end: int, v_start: int, v_count: int,
+           rand_dist: str = "rademacher") -> np.ndarray:
+    """Reference probe values for consecutive blocks, (blk_end - blk_starts[0], v_count) FP64."""
+    starts = np.ascontiguousarray(np.asarray(blk_starts, dtype=np.int64))
+    out = np.empty((blk_end - int(starts[0]), v_count), dtype=np.float64)
+    call("gpe_probes", RAND[rand_dist], int(root_seed) & 0xFFFFFFFFFFFFFFFF, _np_ptr(starts), len(starts),
+         int(blk_end), int(v_start), int(v_count), _np_ptr(out))
+    return out
+
+
+PROF_KINDS = {"p1_contract": 0, "p2_contract": 1, "gz": 2, "gtu": 3}
+
+
+def profile_enable(on: bool = True) -> None:
+    call("gpe_profile_enable", int(bool(on)))
+
+
+def profile_reset() -> None:
+    call("gpe_profile_reset")
+
+
+def profile_read(kind: str) -> tuple[float, int]:
+    """(total milliseconds, launches) of one kernel kind since the last reset."""
+    ms, n = ctypes.c_double(), c_i64()
+    call("gpe_profile_read", PROF_KINDS[kind], ctypes.byref(ms), ctypes.byref(n))
+    return float(ms.value), int(n.value)
+
+
+def default_device() -> torch.device:
+    if not torch.cuda.is_available():
+        raise RuntimeError("summit_amd needs a HIP device (torch.cuda.is_available() is False)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+os.environ.setdefault("HSA_ENABLE_IPC_MODE_LEGACY", "0")
