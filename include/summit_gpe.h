@@ -162,6 +162,33 @@ int gpe_phase2_accum(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const doub
                      double* mc_per_snp_sumsq_dev, double extreme_warn, int32_t drop_extreme, int32_t limbs,
                      unsigned long long* guard_counts_host, void* stream);
 
+/* Dense products with the standardized genotype matrix X (N x M, the handle's
+ * working codes, x_in = (g_in - mu_i) inv_sd_i, 0 where missing in mean mode):
+ *
+ *   gpe_xc_f64:  out[q][n] (+)= sum_{i in [snp_begin, snp_end)} x_in * c[i - snp_begin][q]
+ *                c row-major (L, n_cols) FP64; out column-major (N, n_cols) with ldo >= N.
+ *   gpe_xtv_f64: out[j - snp_begin][q] = sum_n x_jn * v[q][n]
+ *                v column-major (N, n_cols) (ldv >= N); out row-major (L, n_cols).
+ *
+ * The two GEMMs of apply_grm_bed_panel (src/native/gwldcore.cpp:1227-1252:
+ * coef = X_blk^T V, out += X_blk (diag(inv^2/M) coef)) and of the GxE
+ * source / target products; limbs 5 or 6 as for the phases. */
+int gpe_xc_f64(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const double* c_dev, int64_t ldc, int32_t n_cols,
+               double* out_dev, int64_t ldo, int32_t accumulate, int32_t limbs, void* stream);
+int gpe_xtv_f64(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const double* v_dev, int64_t ldv, int32_t n_cols,
+                double* out_dev, int64_t ldo, int32_t limbs, void* stream);
+
+/* Exact windowed LD scores for SNPs [snp_begin, snp_end) of one chromosome:
+ *   out[j][b] = sum_{i : |bp_i - bp_j| <= window_bp} r~2_ij a[i][b],
+ *   r~2 = r^2 - (1 - r^2)/(N - 2),  r the correlation of the ddof-0
+ * standardized, N-restandardized SNPs (exact integer cross products on int8
+ * MFMA).  window_snps > 0 replaces the BP window by |i - j| <= window_snps
+ * (bp_host may then be NULL).  annot_dev, out_dev: row-major (m, n_bins), m =
+ * snp_end - snp_begin.  Replaces compute_windowed_ld_chr without covariates
+ * (src/native/winldcore.cpp:783-931, r^2 and masks :409-594). Synchronizes. */
+int gpe_windowed_ld(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const int64_t* bp_host, double window_bp,
+                    int64_t window_snps, const double* annot_dev, int32_t n_bins, double* out_dev, void* stream);
+
 /* Probe values of the reference's genome-wide streams for blocks starting at
  * blk_starts_host[0..n_blocks) (last block ends at blk_end): Rademacher on the
  * GPU, Gaussian with the host std::normal_distribution.  z_host: row-major
@@ -180,7 +207,8 @@ enum {
     GPE_PROF_P2_CONTRACT = 1, /* phase-2 packed contraction with fused Work^2 epilogue */
     GPE_PROF_GZ = 2,          /* exact G.Z */
     GPE_PROF_GTU = 3,         /* exact G^T U */
-    GPE_PROF_KINDS = 4
+    GPE_PROF_GRAM = 4,        /* banded Gram of windowed LD */
+    GPE_PROF_KINDS = 5
 };
 int gpe_profile_enable(int32_t on);
 /* Accumulated milliseconds and launch count of one kernel kind (synchronizes

@@ -453,6 +453,50 @@ def exact_ld(bed, n_total: int, m_total: int, rows=None, ddof: int = 1, impute: 
 
 
 # ----------------------------------------------------------------------------
+# Exact windowed LD: src/native/winldcore.cpp:215-267 (restandardize),
+# :341-372 (panel preparation), :409-478 (unbiased r^2, BP mask), :480-594
+# (accumulation), :783-931 (driver); no covariates.
+# ----------------------------------------------------------------------------
+def restandardize_cols(X):
+    """Centre over all N rows, scale to population variance 1 (winldcore.cpp:215-267)."""
+    X = np.array(X, dtype=np.float64, order="F")
+    N = X.shape[0]
+    s = X.sum(axis=0)
+    ss = (X * X).sum(axis=0)
+    mean = s / N
+    m2 = ss - s * s / N
+    m2 = np.where((m2 < 0) & (m2 > -1e-12), 0.0, m2)
+    inv = np.ones_like(mean)
+    ok = (m2 > 0) & np.isfinite(m2)
+    var = np.where(ok, m2 / N, 0.0)
+    ok &= var > 0
+    inv[ok] = 1.0 / np.sqrt(var[ok])
+    return (X - mean[None, :]) * inv[None, :]
+
+
+def windowed_ld_chr(bed, n_total: int, chr_start: int, chr_end: int, bp, annot_chr, ld_wind_kb: float = None,
+                    rows=None, impute: str = "mean", impute_seed: int = 0, window_snps: int = 0):
+    """L[j, b] = sum_{i in window(j)} (r_ij^2 - (1 - r_ij^2)/(N - 2)) a[i, b]."""
+    X = read_block_standardized(bed, n_total, chr_start, chr_end, rows, 0, impute, impute_seed)
+    if impute == "mean":
+        X = restandardize_cols(X)
+    N = X.shape[0]
+    corr_dim = N - 1
+    r = (X.T @ X) / N
+    r2 = r * r
+    r2u = r2 - (1.0 - r2) / (corr_dim - 1)
+    m = chr_end - chr_start
+    if window_snps > 0:
+        idx = np.arange(m)
+        mask = np.abs(idx[:, None] - idx[None, :]) <= window_snps
+    else:
+        bpv = np.asarray(bp, dtype=np.float64)
+        mask = ~(np.abs(bpv[:, None] - bpv[None, :]) > ld_wind_kb * 1000.0)
+    r2u = np.where(mask, r2u, 0.0)
+    return r2u @ np.asarray(annot_chr, dtype=np.float64)
+
+
+# ----------------------------------------------------------------------------
 # Exact integer products on masked genotype counts (missing -> 0)
 # ----------------------------------------------------------------------------
 def masked_geno(codes) -> np.ndarray:

@@ -335,10 +335,9 @@ int gpe_geno_upload_bed(gpe_geno* g, const uint8_t* bed_host, int64_t n_total, i
         hipStream_t st = S(stream);
         int64_t* rows_dev = nullptr;
         if (rows_host) {
-            for (int64_t i = 0; i < g->n; ++i) {
-                require(rows_host[i] >= 0 && rows_host[i] < n_total, "row_sel index out of range");
-                require(i == 0 || rows_host[i] > rows_host[i - 1], "row_sel must be sorted and unique");
-            }
+            for (int64_t i = 0; i < g->n; ++i)
+                require(rows_host[i] >= 0 && rows_host[i] < n_total,
+                        "row_sel contains an out-of-range sample index");
             rows_dev = static_cast<int64_t*>(g->scratch("upload_rows", g->n * sizeof(int64_t)));
             GPE_HIP(hipMemcpyAsync(rows_dev, rows_host, g->n * sizeof(int64_t), hipMemcpyHostToDevice, st));
         }
@@ -634,6 +633,132 @@ int gpe_probes(int32_t rand_dist, uint64_t root_seed, const int64_t* blk_starts_
     });
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Shared drivers of the two traversals
+// ---------------------------------------------------------------------------
+namespace {
+
+// Phase-1 type product on the sample-major layout:
+//   out[(bin*v_count + v)][n] (=|+=) sum_{i in [s, e)} (g_in - mu_i m_in) * w[i][bin] * z[i][v]
+// (x_in = (g_in - mu_i) inv_sd_i m_in when w carries inv_sd).  w, z indexed by absolute SNP.
+void run_phase1(gpe_geno* g, int64_t s, int64_t e, const double* w, int n_bins, const int8_t* z_i8,
+                const double* z_f64, int64_t ldz, int v_count, double* out, int64_t ldo, const uint8_t* bin_mode,
+                int limbs, hipStream_t st) {
+    const LimbCfg lc = limb_cfg(limbs);
+    const int64_t L = e - s;
+    const int v_pad = (int)round_up(v_count, 32);
+    require(ldz >= v_pad, "probe leading dimension too small");
+    const int ncols = n_bins * v_pad;
+    const int groups = (int)ceil_div(ncols, 32 * lc.ctc);
+    const int64_t coef_ctiles = (int64_t)groups * lc.nl * lc.ctc;
+    const int ncols_pad = groups * lc.ctc * 32;
+    uint64_t* colmax = static_cast<uint64_t*>(g->scratch("colmax", ncols_pad * sizeof(uint64_t)));
+    int32_t* shift = static_cast<int32_t*>(g->scratch("shift", ncols_pad * sizeof(int32_t)));
+    double* col_scale = static_cast<double*>(g->scratch("col_scale", ncols_pad * sizeof(double)));
+    double* cst = static_cast<double*>(g->scratch("cst", ncols_pad * sizeof(double)));
+    launch_fill_u64(colmax, ncols_pad, 0, st);
+    launch_coef_p1_colmax(w, n_bins, z_i8, z_f64, ldz, s, e, n_bins, v_count, v_pad, colmax, st);
+    const int prec = std::min(8 * lc.nl - 2, 62 - bitlen((uint64_t)(2 * L)));
+    launch_shift_from_colmax(colmax, ncols_pad, prec, shift, col_scale, st);
+    const int64_t kc0 = s / kKChunk, kc1 = ceil_div(e, kKChunk);
+    const int64_t kcount = kc1 - kc0;
+    int8_t* coef = static_cast<int8_t*>(g->scratch("coef", kcount * coef_ctiles * 4096));
+    launch_fill_u8(reinterpret_cast<uint8_t*>(coef), kcount * coef_ctiles * 4096, 0, st);
+    launch_coef_p1_quant(w, n_bins, z_i8, z_f64, ldz, s, e, n_bins, v_count, v_pad, shift, lc.nl, lc.ctc,
+                         coef_ctiles, kc0, coef, g->mean, cst, st);
+    const int64_t ldy = g->n_alloc;
+    int64_t* y = static_cast<int64_t*>(g->scratch("p1_y", (int64_t)ncols_pad * ldy * sizeof(int64_t)));
+    launch_fill_u64(reinterpret_cast<uint64_t*>(y), (int64_t)ncols_pad * ldy, 0, st);
+    ContractArgs a{};
+    a.packed = g->smp_work;
+    a.stride = g->smp_stride;
+    a.r0 = 0;
+    a.r1 = g->n;
+    a.coef = coef;
+    a.coef_ctiles = coef_ctiles;
+    a.kc0 = kc0;
+    a.kc1 = kc1;
+    a.ctiles_used = ncols / 32;
+    a.out_i64 = y;
+    a.ldo = ldy;
+    a.cols_out = ncols_pad;
+    a.out_row0 = 0;
+    const int64_t gx = ceil_div(g->n, 256);
+    const int splits = pick_splits(gx * groups, kcount, (int64_t)(((int64_t)1 << 31) / (256 * 128)) - 1);
+    a.kc_per_split = ceil_div(kcount, splits);
+    ProfScope ps(GPE_PROF_P1_CONTRACT, st);
+    launch_contract(lc.nl, lc.ctc, false, EPI_I64_ATOMIC, a, groups, splits, st);
+    ps.stop();
+    const bool miss = g->impute_mode == GPE_IMPUTE_MEAN && g->n_missing > 0;
+    launch_p1_finalize(y, ldy, g->n, n_bins, v_count, v_pad, col_scale, cst, miss ? g->miss_ptr : nullptr,
+                       miss ? g->miss_idx : nullptr, s, e, w, n_bins, z_i8, z_f64, ldz, shift, g->mean, out, ldo,
+                       bin_mode, st);
+}
+
+// Phase-2 type product on the SNP-major layout against x (column-major (N, n_bins*v_count)):
+// quantize x into limbs and run the contraction with the given epilogue.
+void run_phase2(gpe_geno* g, int64_t s, int64_t e, const double* x, int64_t ldx, int n_bins, int v_count, int limbs,
+                EpiMode mode, ContractArgs& a, hipStream_t st, int* ctiles_used_out, int* v_pad_out) {
+    const LimbCfg lc = limb_cfg(limbs);
+    const int v_pad = (int)round_up(v_count, 32);
+    const int ncols = n_bins * v_pad;
+    const int groups = (int)ceil_div(ncols, 32 * lc.ctc);
+    const int64_t coef_ctiles = (int64_t)groups * lc.nl * lc.ctc;
+    const int ncols_pad = groups * lc.ctc * 32;
+    const int64_t kcount = ceil_div(g->n, kKChunk);
+    uint64_t* colmax = static_cast<uint64_t*>(g->scratch("colmax", ncols_pad * sizeof(uint64_t)));
+    int32_t* shift = static_cast<int32_t*>(g->scratch("shift", ncols_pad * sizeof(int32_t)));
+    double* col_scale = static_cast<double*>(g->scratch("col_scale", ncols_pad * sizeof(double)));
+    int64_t* col_sum = static_cast<int64_t*>(g->scratch("col_sum", ncols_pad * sizeof(int64_t)));
+    launch_fill_u64(colmax, ncols_pad, 0, st);
+    launch_fill_u64(reinterpret_cast<uint64_t*>(col_sum), ncols_pad, 0, st);
+    launch_colmax_f64(x, ldx, g->n, ncols, v_count, v_pad, colmax, st);
+    const int prec = std::min(8 * lc.nl - 2, 62 - bitlen((uint64_t)(2 * g->n)));
+    launch_shift_from_colmax(colmax, ncols_pad, prec, shift, col_scale, st);
+    int8_t* coef = static_cast<int8_t*>(g->scratch("coef", kcount * coef_ctiles * 4096));
+    launch_fill_u8(reinterpret_cast<uint8_t*>(coef), kcount * coef_ctiles * 4096, 0, st);
+    launch_quant_p2(x, ldx, g->n, n_bins, v_count, v_pad, shift, lc.nl, lc.ctc, coef_ctiles, coef, col_sum, st);
+    a.packed = g->snp_work;
+    a.stride = g->snp_stride;
+    a.r0 = s;
+    a.r1 = e;
+    a.coef = coef;
+    a.coef_ctiles = coef_ctiles;
+    a.kc0 = 0;
+    a.kc1 = kcount;
+    a.kc_per_split = kcount;
+    a.ctiles_used = ncols / 32;
+    a.mean = g->mean;
+    a.inv_sd = g->inv_sd;
+    a.col_scale = col_scale;
+    a.col_sum = col_sum;
+    const bool miss = g->impute_mode == GPE_IMPUTE_MEAN && g->n_missing > 0;
+    a.has_miss = miss ? 1 : 0;
+    ProfScope ps(GPE_PROF_P2_CONTRACT, st);
+    launch_contract(lc.nl, lc.ctc, miss, mode, a, groups, 1, st);
+    ps.stop();
+    if (ctiles_used_out) *ctiles_used_out = ncols / 32;
+    if (v_pad_out) *v_pad_out = v_pad;
+}
+
+__global__ void k_w_from_inv_sd(const double* inv_sd, int64_t s, int64_t e, double* w) {
+    const int64_t i = s + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < e) w[i] = inv_sd[i];
+}
+
+// copy a row-major (rows, cols) matrix into rows [row0, row0+rows) of a buffer with leading dimension ld
+void copy_rows(const double* src, int64_t lds, int64_t rows, int cols, double* dst, int64_t ld, int64_t row0,
+               hipStream_t st) {
+    GPE_HIP(hipMemcpy2DAsync(dst + row0 * ld, ld * sizeof(double), src, lds * sizeof(double), cols * sizeof(double),
+                             rows, hipMemcpyDeviceToDevice, st));
+}
+
+}  // namespace
+
+extern "C" {
+
 int gpe_phase1_xz(gpe_geno* g, int64_t snp_begin, int64_t snp_end, int64_t step_size, const double* annot_dev,
                   int32_t n_bins, const double* inv_dev, int32_t v_start, int32_t v_count, int32_t rand_dist,
                   uint64_t root_seed, double* xz_dev, int64_t ldxz, const uint8_t* bin_mode_dev, int32_t limbs,
@@ -645,22 +770,14 @@ int gpe_phase1_xz(gpe_geno* g, int64_t snp_begin, int64_t snp_end, int64_t step_
         require(n_bins > 0 && v_count > 0, "annotation count and v_count must be positive");
         require(ldxz >= g->n, "Xz2d_chunk shape must be (N, B*v_count)");
         if (snp_end == snp_begin) return;
-        const LimbCfg lc = limb_cfg(limbs);
+        (void)limb_cfg(limbs);
         hipStream_t st = S(stream);
         const int64_t L = snp_end - snp_begin;
         const int v_pad = (int)round_up(v_count, 32);
-        const int ncols = n_bins * v_pad;
-        const int groups = (int)ceil_div(ncols, 32 * lc.ctc);
-        const int64_t coef_ctiles = (int64_t)groups * lc.nl * lc.ctc;
-        const int ncols_pad = groups * lc.ctc * 32;
-
-        // per-SNP weights w = inv_sd * inv * sqrt(a)
         double* w = static_cast<double*>(g->scratch("p1_w", g->m_alloc * n_bins * sizeof(double)));
         hipLaunchKernelGGL(k_make_w, dim3((unsigned)ceil_div(L, 256)), dim3(256), 0, st, annot_dev, n_bins, inv_dev,
                            g->inv_sd, snp_begin, snp_end, w);
         GPE_HIP(hipGetLastError());
-
-        // probes
         std::vector<int64_t> starts;
         for (int64_t s = snp_begin; s < snp_end; s += step_size) starts.push_back(s);
         const int nb = (int)starts.size();
@@ -686,53 +803,124 @@ int gpe_phase1_xz(gpe_geno* g, int64_t snp_begin, int64_t snp_end, int64_t step_
         } else {
             throw Error("rand_dist must be 'rademacher' or 'gaussian'");
         }
+        run_phase1(g, snp_begin, snp_end, w, n_bins, z_i8, z_f64, ldz, v_count, xz_dev, ldxz, bin_mode_dev, limbs,
+                   st);
+    });
+}
 
-        // quantization: shift per column, int8 limbs, centering constants
-        uint64_t* colmax = static_cast<uint64_t*>(g->scratch("colmax", ncols_pad * sizeof(uint64_t)));
-        int32_t* shift = static_cast<int32_t*>(g->scratch("shift", ncols_pad * sizeof(int32_t)));
-        double* col_scale = static_cast<double*>(g->scratch("col_scale", ncols_pad * sizeof(double)));
-        double* cst = static_cast<double*>(g->scratch("cst", ncols_pad * sizeof(double)));
-        launch_fill_u64(colmax, ncols_pad, 0, st);
-        launch_coef_p1_colmax(w, n_bins, z_i8, z_f64, ldz, snp_begin, snp_end, n_bins, v_count, v_pad, colmax, st);
-        const int prec = std::min(8 * lc.nl - 2, 62 - bitlen((uint64_t)(2 * L)));
-        launch_shift_from_colmax(colmax, ncols_pad, prec, shift, col_scale, st);
-        const int64_t kc0 = snp_begin / kKChunk, kc1 = ceil_div(snp_end, kKChunk);
-        const int64_t kcount = kc1 - kc0;
-        int8_t* coef = static_cast<int8_t*>(g->scratch("coef", kcount * coef_ctiles * 4096));
-        launch_fill_u8(reinterpret_cast<uint8_t*>(coef), kcount * coef_ctiles * 4096, 0, st);
-        launch_coef_p1_quant(w, n_bins, z_i8, z_f64, ldz, snp_begin, snp_end, n_bins, v_count, v_pad, shift, lc.nl,
-                             lc.ctc, coef_ctiles, kc0, coef, g->mean, cst, st);
+int gpe_xc_f64(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const double* c_dev, int64_t ldc, int32_t n_cols,
+               double* out_dev, int64_t ldo, int32_t accumulate, int32_t limbs, void* stream) {
+    return guarded([&] {
+        require_prepared(g);
+        require(snp_begin >= 0 && snp_end <= g->m && snp_begin <= snp_end, "SNP range out of bounds");
+        require(n_cols > 0 && ldc >= n_cols && ldo >= g->n, "bad leading dimension");
+        if (snp_end == snp_begin) return;
+        (void)limb_cfg(limbs);
+        hipStream_t st = S(stream);
+        const int64_t L = snp_end - snp_begin;
+        const int v_pad = (int)round_up(n_cols, 32);
+        double* w = static_cast<double*>(g->scratch("xc_w", g->m_alloc * sizeof(double)));
+        hipLaunchKernelGGL(k_w_from_inv_sd, dim3((unsigned)ceil_div(L, 256)), dim3(256), 0, st, g->inv_sd,
+                           snp_begin, snp_end, w);
+        GPE_HIP(hipGetLastError());
+        double* z = static_cast<double*>(g->scratch("xc_z", g->m_alloc * v_pad * sizeof(double)));
+        copy_rows(c_dev, ldc, L, n_cols, z, v_pad, snp_begin, st);
+        uint8_t* mode = static_cast<uint8_t*>(g->scratch("xc_mode", 1));
+        launch_fill_u8(mode, 1, accumulate ? 2 : 1, st);
+        run_phase1(g, snp_begin, snp_end, w, 1, nullptr, z, v_pad, n_cols, out_dev, ldo, mode, limbs, st);
+    });
+}
 
-        // contraction over SNPs on the sample-major layout
-        const int64_t ldy = g->n_alloc;
-        int64_t* y = static_cast<int64_t*>(g->scratch("p1_y", (int64_t)ncols_pad * ldy * sizeof(int64_t)));
-        launch_fill_u64(reinterpret_cast<uint64_t*>(y), (int64_t)ncols_pad * ldy, 0, st);
+int gpe_xtv_f64(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const double* v_dev, int64_t ldv, int32_t n_cols,
+                double* out_dev, int64_t ldo, int32_t limbs, void* stream) {
+    return guarded([&] {
+        require_prepared(g);
+        require(snp_begin >= 0 && snp_end <= g->m && snp_begin <= snp_end, "SNP range out of bounds");
+        require(n_cols > 0 && ldv >= g->n && ldo >= n_cols, "bad leading dimension");
+        if (snp_end == snp_begin) return;
+        hipStream_t st = S(stream);
         ContractArgs a{};
-        a.packed = g->smp_work;
-        a.stride = g->smp_stride;
-        a.r0 = 0;
-        a.r1 = g->n;
-        a.coef = coef;
-        a.coef_ctiles = coef_ctiles;
-        a.kc0 = kc0;
-        a.kc1 = kc1;
-        a.ctiles_used = ncols / 32;
-        a.out_i64 = y;
-        a.ldo = ldy;
-        a.cols_out = ncols_pad;
-        a.out_row0 = 0;
-        const int64_t gx = ceil_div(g->n, 256);
-        const int splits = pick_splits(gx * groups, kcount, (int64_t)(((int64_t)1 << 31) / (256 * 128)) - 1);
-        a.kc_per_split = ceil_div(kcount, splits);
-        ProfScope ps(GPE_PROF_P1_CONTRACT, st);
-        launch_contract(lc.nl, lc.ctc, false, EPI_I64_ATOMIC, a, groups, splits, st);
-        ps.stop();
+        a.out_f64 = out_dev;
+        a.ldo = ldo;
+        a.cols_out = n_cols;
+        a.out_row0 = snp_begin;
+        run_phase2(g, snp_begin, snp_end, v_dev, ldv, 1, n_cols, limbs, EPI_WORK, a, st, nullptr, nullptr);
+    });
+}
 
-        // centering, missing-call correction, write Xz
+int gpe_windowed_ld(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const int64_t* bp_host, double window_bp,
+                    int64_t window_snps, const double* annot_dev, int32_t n_bins, double* out_dev, void* stream) {
+    return guarded([&] {
+        require_prepared(g);
+        require(snp_end >= snp_begin, "chr_end must be >= chr_start");
+        const int64_t m = snp_end - snp_begin;
+        require(m > 0, "Chromosome slice is empty");
+        require(snp_begin >= 0 && snp_end <= g->m, "SNP range out of bounds");
+        require(n_bins > 0, "annot_chr must have at least one column");
+        require(g->n > 2, "Too few rows for windowed LD score computation");
+        const double corr_dim = (double)g->n - 1.0;
+        require(corr_dim > 1.0, "Too few residual dimensions for unbiased windowed r^2 correction");
+        if (window_snps <= 0) {
+            require(bp_host != nullptr, "bp length mismatch in compute_windowed_ld_chr");
+            require(std::isfinite(window_bp) && window_bp > 0.0,
+                    "ld_wind_kb must define a finite positive BP window");
+            for (int64_t i = 1; i < m; ++i)
+                require(bp_host[i] >= bp_host[i - 1],
+                        "BP not sorted within chromosome block. Sort your .bim by CHR+BP.");
+        }
+        hipStream_t st = S(stream);
+        // tile pairs of the band
+        const int64_t nt = ceil_div(m, 64);
+        std::vector<int32_t> pairs;
+        for (int64_t ti = 0; ti < nt; ++ti) {
+            const int64_t i_last = std::min(m, (ti + 1) * 64) - 1;
+            for (int64_t tj = ti; tj < nt; ++tj) {
+                const int64_t j_first = tj * 64;
+                if (tj > ti) {
+                    const bool far = window_snps > 0 ? (j_first - i_last > window_snps)
+                                                     : ((double)(bp_host[j_first] - bp_host[i_last]) > window_bp);
+                    if (far) break;
+                }
+                pairs.push_back((int32_t)ti);
+                pairs.push_back((int32_t)tj);
+            }
+        }
+        const int64_t np = (int64_t)pairs.size() / 2;
+        int32_t* pairs_dev = static_cast<int32_t*>(g->scratch("win_pairs", pairs.size() * sizeof(int32_t)));
+        GPE_HIP(hipMemcpyAsync(pairs_dev, pairs.data(), pairs.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+        int64_t* bp_dev = nullptr;
+        if (window_snps <= 0) {
+            bp_dev = static_cast<int64_t*>(g->scratch("win_bp", m * sizeof(int64_t)));
+            GPE_HIP(hipMemcpyAsync(bp_dev, bp_host, m * sizeof(int64_t), hipMemcpyHostToDevice, st));
+        }
+        double* scale = static_cast<double*>(g->scratch("win_scale", g->m_alloc * sizeof(double)));
+        const int completed = g->impute_mode == GPE_IMPUTE_HWE ? 1 : 0;
+        launch_window_scale(g->nobs, g->sum, g->sumsq, g->m, g->n, completed, scale, st);
+        launch_fill_u8(reinterpret_cast<uint8_t*>(out_dev), m * n_bins * (int64_t)sizeof(double), 0, st);
+        GramArgs a{};
+        a.packed = g->snp_work;
+        a.stride = g->snp_stride;
+        a.n = g->n;
+        a.kc1 = ceil_div(g->n, kKChunk);
+        a.pairs = pairs_dev;
+        a.snp_begin = snp_begin;
+        a.snp_end = snp_end;
+        a.mean = g->mean;
+        a.scale = scale;
+        a.sum = g->sum;
+        a.nobs = g->nobs;
+        a.bp = bp_dev;
+        a.window_bp = window_bp;
+        a.window_snps = window_snps;
+        a.annot = annot_dev;
+        a.n_bins = n_bins;
+        a.out = out_dev;
+        a.corr_dim = corr_dim;
         const bool miss = g->impute_mode == GPE_IMPUTE_MEAN && g->n_missing > 0;
-        launch_p1_finalize(y, ldy, g->n, n_bins, v_count, v_pad, col_scale, cst, miss ? g->miss_ptr : nullptr,
-                           miss ? g->miss_idx : nullptr, snp_begin, snp_end, w, n_bins, z_i8, z_f64, ldz, shift,
-                           g->mean, xz_dev, ldxz, bin_mode_dev, st);
+        ProfScope ps(GPE_PROF_GRAM, st);
+        launch_gram_band(a, np, miss, st);
+        ps.stop();
+        GPE_HIP(hipStreamSynchronize(st));
     });
 }
 
@@ -751,63 +939,22 @@ int gpe_phase2_accum(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const doub
         unsigned long long* gc = static_cast<unsigned long long*>(g->scratch("guards", 2 * sizeof(unsigned long long)));
         launch_fill_u64(reinterpret_cast<uint64_t*>(gc), 2, 0, st);
         if (snp_end > snp_begin) {
-            const LimbCfg lc = limb_cfg(limbs);
+            (void)limb_cfg(limbs);
             const int v_pad = (int)round_up(tile_nvecs, 32);
-            const int ncols = n_bins * v_pad;
-            const int groups = (int)ceil_div(ncols, 32 * lc.ctc);
-            const int64_t coef_ctiles = (int64_t)groups * lc.nl * lc.ctc;
-            const int ncols_pad = groups * lc.ctc * 32;
-            const int64_t kcount = ceil_div(g->n, kKChunk);
-
-            uint64_t* colmax = static_cast<uint64_t*>(g->scratch("colmax", ncols_pad * sizeof(uint64_t)));
-            int32_t* shift = static_cast<int32_t*>(g->scratch("shift", ncols_pad * sizeof(int32_t)));
-            double* col_scale = static_cast<double*>(g->scratch("col_scale", ncols_pad * sizeof(double)));
-            int64_t* col_sum = static_cast<int64_t*>(g->scratch("col_sum", ncols_pad * sizeof(int64_t)));
-            launch_fill_u64(colmax, ncols_pad, 0, st);
-            launch_fill_u64(reinterpret_cast<uint64_t*>(col_sum), ncols_pad, 0, st);
-            launch_colmax_f64(xz_dev, ldxz, g->n, ncols, tile_nvecs, v_pad, colmax, st);
-            const int prec = std::min(8 * lc.nl - 2, 62 - bitlen((uint64_t)(2 * g->n)));
-            launch_shift_from_colmax(colmax, ncols_pad, prec, shift, col_scale, st);
-            int8_t* coef = static_cast<int8_t*>(g->scratch("coef", kcount * coef_ctiles * 4096));
-            launch_fill_u8(reinterpret_cast<uint8_t*>(coef), kcount * coef_ctiles * 4096, 0, st);
-            launch_quant_p2(xz_dev, ldxz, g->n, n_bins, tile_nvecs, v_pad, shift, lc.nl, lc.ctc, coef_ctiles, coef,
-                            col_sum, st);
-
-            const int ctiles_used = ncols / 32;
+            const int ctiles_used = n_bins * v_pad / 32;
             const bool mc = mc_aggregate_sumsq_dev || mc_per_snp_sumsq_dev;
-            double* s2 = static_cast<double*>(g->scratch("p2_s2", g->m_alloc * ctiles_used * sizeof(double)));
-            double* s4 = mc ? static_cast<double*>(g->scratch("p2_s4", g->m_alloc * ctiles_used * sizeof(double)))
-                            : nullptr;
             ContractArgs a{};
-            a.packed = g->snp_work;
-            a.stride = g->snp_stride;
-            a.r0 = snp_begin;
-            a.r1 = snp_end;
-            a.coef = coef;
-            a.coef_ctiles = coef_ctiles;
-            a.kc0 = 0;
-            a.kc1 = kcount;
-            a.kc_per_split = kcount;
-            a.ctiles_used = ctiles_used;
-            a.mean = g->mean;
-            a.inv_sd = g->inv_sd;
-            a.col_scale = col_scale;
-            a.col_sum = col_sum;
-            a.s2 = s2;
-            a.s4 = s4;
+            a.s2 = static_cast<double*>(g->scratch("p2_s2", g->m_alloc * ctiles_used * sizeof(double)));
+            a.s4 = mc ? static_cast<double*>(g->scratch("p2_s4", g->m_alloc * ctiles_used * sizeof(double))) : nullptr;
             a.ld_s = ctiles_used;
-            const bool miss = g->impute_mode == GPE_IMPUTE_MEAN && g->n_missing > 0;
-            a.has_miss = miss ? 1 : 0;
-            ProfScope ps(GPE_PROF_P2_CONTRACT, st);
-            launch_contract(lc.nl, lc.ctc, miss, EPI_P2, a, groups, 1, st);
-            ps.stop();
-
+            run_phase2(g, snp_begin, snp_end, xz_dev, ldxz, n_bins, tile_nvecs, limbs, EPI_P2, a, st, nullptr,
+                       nullptr);
             double denom = (double)n_denom - 1.0;
             if (denom <= 0.0) denom = 1.0;
             const double inv_denom2 = 1.0 / (denom * denom);
-            launch_p2_finalize(s2, s4, ctiles_used, snp_begin, snp_end, n_bins, v_pad / 32, tile_nvecs, inv_left_dev,
-                               inv_denom2, extreme_warn > 0.0 ? extreme_warn : INFINITY, drop_extreme, meansq_dev,
-                               ld_meansq, mc_aggregate_sumsq_dev, mc_per_snp_sumsq_dev, gc, st);
+            launch_p2_finalize(a.s2, a.s4, ctiles_used, snp_begin, snp_end, n_bins, v_pad / 32, tile_nvecs,
+                               inv_left_dev, inv_denom2, extreme_warn > 0.0 ? extreme_warn : INFINITY, drop_extreme,
+                               meansq_dev, ld_meansq, mc_aggregate_sumsq_dev, mc_per_snp_sumsq_dev, gc, st);
         }
         if (guard_counts_host) {
             GPE_HIP(hipMemcpyAsync(guard_counts_host, gc, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));

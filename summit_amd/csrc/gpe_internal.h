@@ -46,6 +46,7 @@ enum EpiMode : int {
     EPI_I64_ATOMIC = 1,  // combine limbs, atomically add int64 into out_i64[c * ldo + r]
     EPI_I64_ROWMAJOR = 2,// combine limbs, store int64 to out_i64[r * ldo + c]
     EPI_P2 = 3,          // phase-2 epilogue: standardized Work, per-tile sums of Work^2, Work^4
+    EPI_WORK = 4,        // phase-2 epilogue writing standardized Work to out_f64[r * ldo + c]
 };
 
 struct ContractArgs {
@@ -60,6 +61,7 @@ struct ContractArgs {
     // outputs
     int32_t* out_i32;
     int64_t* out_i64;
+    double* out_f64;
     int64_t ldo;
     // phase-2 epilogue inputs
     const double* mean;      // per row (SNP) mean
@@ -72,6 +74,27 @@ struct ContractArgs {
     int has_miss;
     int32_t cols_out;        // I32/I64 epilogues: columns >= cols_out are not written
     int64_t out_row0;        // I32/I64 epilogues: output row index = r - out_row0
+};
+
+// Banded Gram (windowed LD) arguments
+struct GramArgs {
+    const uint8_t* packed;
+    int64_t stride;
+    int64_t n;           // samples
+    int64_t kc1;         // K chunks
+    const int32_t* pairs;  // (ti, tj) tile pairs, ti <= tj, relative to snp_begin
+    int64_t snp_begin, snp_end;
+    const double* mean;
+    const double* scale;   // s_i
+    const int64_t* sum;
+    const int32_t* nobs;
+    const int64_t* bp;     // relative to snp_begin
+    double window_bp;      // used when window_snps <= 0
+    int64_t window_snps;
+    const double* annot;   // row-major (m, n_bins), relative
+    int n_bins;
+    double* out;           // row-major (m, n_bins), relative
+    double corr_dim;
 };
 
 // Launchers (gpe_kernels.hip)
@@ -127,6 +150,9 @@ void launch_quant_i32(const int32_t* u, int64_t ldu, int64_t n, int n_cols, int 
 void launch_synth_markov(uint8_t* snp, int64_t stride, int64_t m, int64_t n, uint64_t seed, double maf_lo,
                          double maf_hi, double miss_rate, int block_len, double copy_prob, int iid,
                          hipStream_t st);
+void launch_gram_band(const GramArgs& a, int64_t n_pairs, bool miss, hipStream_t st);
+void launch_window_scale(const int32_t* nobs, const int64_t* sum, const int64_t* sumsq, int64_t m, int64_t n,
+                         int completed, double* scale, hipStream_t st);
 void launch_resvar_inv(const int32_t* nobs, const int64_t* sum, const int64_t* sumsq, const double* mean,
                        const double* inv_sd, int64_t r0, int64_t r1, int64_t n, int completed, double eps,
                        double* inv_out, hipStream_t st);

@@ -109,7 +109,7 @@ __device__ __forceinline__ int64_t wave_sum_i64(int64_t v) {
 // (row 8*(e>>2) + 4*(l>>5) + (e&3), column l & 31).
 // ---------------------------------------------------------------------------
 template <int NL, int CTC, bool MISS, int EPI>
-__global__ __launch_bounds__(256, 1) void k_contract(ContractArgs a) {
+__global__ __launch_bounds__(256, (NL * CTC <= 2) ? 4 : 2) void k_contract(ContractArgs a) {
     constexpr int CT = NL * CTC;
     constexpr int RPW = MISS ? 32 : 64;
     constexpr int RPB = 4 * RPW;
@@ -140,28 +140,42 @@ __global__ __launch_bounds__(256, 1) void k_contract(ContractArgs a) {
     const uint8_t* prow0 = a.packed + (rbase + lr) * a.stride + half * 16;
     const uint8_t* prow1 = MISS ? prow0 : prow0 + 32 * a.stride;
 
+    // coefficient chunk kc+1 is prefetched into registers, packed rows two chunks ahead
     uint4 areg[CT];
-    uint4 breg0 = make_uint4(0, 0, 0, 0), breg1 = make_uint4(0, 0, 0, 0);
+    uint4 bq0 = make_uint4(0, 0, 0, 0), bq1 = make_uint4(0, 0, 0, 0);  // chunk kc+1, tile 0/1
+    uint4 bn0 = make_uint4(0, 0, 0, 0), bn1 = make_uint4(0, 0, 0, 0);  // chunk kc+2, tile 0/1
 
-    auto load_chunk = [&](int64_t kc) {
+    auto load_coef = [&](int64_t kc) {
         const uint4* src =
             reinterpret_cast<const uint4*>(a.coef + ((kc - a.kc0) * a.coef_ctiles + (int64_t)cg * CT) * 4096);
 #pragma unroll
         for (int t = 0; t < CT; ++t) areg[t] = src[t * 256 + tid];
-        breg0 = *reinterpret_cast<const uint4*>(prow0 + kc * 32);
-        if (!MISS) breg1 = *reinterpret_cast<const uint4*>(prow1 + kc * 32);
     };
 
-    if (kc_begin < kc_end) load_chunk(kc_begin);
+    if (kc_begin < kc_end) {
+        load_coef(kc_begin);
+        bq0 = *reinterpret_cast<const uint4*>(prow0 + kc_begin * 32);
+        if (!MISS) bq1 = *reinterpret_cast<const uint4*>(prow1 + kc_begin * 32);
+    }
+    if (kc_begin + 1 < kc_end) {
+        bn0 = *reinterpret_cast<const uint4*>(prow0 + (kc_begin + 1) * 32);
+        if (!MISS) bn1 = *reinterpret_cast<const uint4*>(prow1 + (kc_begin + 1) * 32);
+    }
     int buf = 0;
     for (int64_t kc = kc_begin; kc < kc_end; ++kc) {
         uint4* dst = reinterpret_cast<uint4*>(lds[buf]);
 #pragma unroll
         for (int t = 0; t < CT; ++t) dst[t * 256 + tid] = areg[t];
-        const uint4 b0 = breg0;
-        const uint4 b1 = breg1;
+        const uint4 b0 = bq0;
+        const uint4 b1 = bq1;
+        bq0 = bn0;
+        bq1 = bn1;
         __syncthreads();
-        if (kc + 1 < kc_end) load_chunk(kc + 1);
+        if (kc + 1 < kc_end) load_coef(kc + 1);
+        if (kc + 2 < kc_end) {
+            bn0 = *reinterpret_cast<const uint4*>(prow0 + (kc + 2) * 32);
+            if (!MISS) bn1 = *reinterpret_cast<const uint4*>(prow1 + (kc + 2) * 32);
+        }
         const int8_t* L = lds[buf];
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
@@ -226,6 +240,31 @@ __global__ __launch_bounds__(256, 1) void k_contract(ContractArgs a) {
                                   (unsigned long long)v);
                     else
                         a.out_i64[(r - a.out_row0) * a.ldo + c] = v;
+                }
+            }
+        }
+    } else if constexpr (EPI == EPI_WORK) {
+#pragma unroll
+        for (int ctc = 0; ctc < CTC; ++ctc) {
+            const int ct = cg * CTC + ctc;
+#pragma unroll
+            for (int bt = 0; bt < (MISS ? 1 : 2); ++bt) {
+                const int64_t r = rbase + (MISS ? 0 : bt * 32) + lr;
+                if (r >= a.r1 || ct >= a.ctiles_used) continue;
+                const double mu = a.mean[r];
+                const double isd = a.inv_sd[r];
+#pragma unroll
+                for (int e = 0; e < 16; ++e) {
+                    int64_t ge = 0, me = 0;
+#pragma unroll
+                    for (int l = 0; l < NL; ++l) {
+                        ge = ge * 256 + (int64_t)acc[l * CTC + ctc][bt][e];
+                        if (MISS) me = me * 256 + (int64_t)acc[l * CTC + ctc][1][e];
+                    }
+                    const int c = ct * 32 + 8 * (e >> 2) + 4 * half + (e & 3);
+                    if (c >= a.cols_out) continue;
+                    const double obs_sum = (double)(a.col_sum[c] - me);
+                    a.out_f64[(r - a.out_row0) * a.ldo + c] = isd * a.col_scale[c] * ((double)ge - mu * obs_sum);
                 }
             }
         }
@@ -296,6 +335,11 @@ void launch_contract(int nl, int ctc, bool miss, EpiMode mode, const ContractArg
     GPE_CASE(5, 1, true, EPI_P2)
     GPE_CASE(6, 1, false, EPI_P2)
     GPE_CASE(6, 1, true, EPI_P2)
+    // dense standardized X^T V
+    GPE_CASE(5, 1, false, EPI_WORK)
+    GPE_CASE(5, 1, true, EPI_WORK)
+    GPE_CASE(6, 1, false, EPI_WORK)
+    GPE_CASE(6, 1, true, EPI_WORK)
 #undef GPE_CASE
     throw Error("no packed-contraction variant for nl=" + std::to_string(nl) + " ctc=" + std::to_string(ctc) +
                 " miss=" + std::to_string((int)miss) + " mode=" + std::to_string((int)mode));
@@ -851,41 +895,59 @@ __global__ void k_p1_finalize(const int64_t* y, int64_t ldy, int64_t n, int n_bi
                               const double* mean, double* xz, int64_t ldxz, const uint8_t* col_mode) {
     const int64_t smp = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    if (smp >= n) return;
+    if (smp >= n) return;  // wave-uniform
     int64_t mb = 0, me = 0;
     if (miss_ptr) {
-        mb = miss_ptr[smp];
-        me = miss_ptr[smp + 1];
-        // first entry >= k0 (lists are sorted)
-        int64_t lo = mb, hi = me;
+        // entries of this sample's sorted list inside [k0, k1)
+        const int64_t b0 = miss_ptr[smp], b1 = miss_ptr[smp + 1];
+        int64_t lo = b0, hi = b1;
         while (lo < hi) {
             const int64_t mid = (lo + hi) >> 1;
             if (miss_idx[mid] < k0) lo = mid + 1;
             else hi = mid;
         }
         mb = lo;
+        hi = b1;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (miss_idx[mid] < k1) lo = mid + 1;
+            else hi = mid;
+        }
+        me = lo;
     }
     const int ncols = n_bins * v_pad;
     for (int c0 = 0; c0 < ncols; c0 += 64) {
         const int c = c0 + lane;
-        if (c >= ncols) continue;
         const int k = c / v_pad, v = c % v_pad;
-        if (v >= v_count) continue;
-        const int mode = col_mode ? col_mode[k] : 2;
-        double val = (double)y[(int64_t)c * ldy + smp] * col_scale[c] - cst[c];
+        const bool valid = c < ncols && v < v_count;
+        double val = valid ? (double)y[(int64_t)c * ldy + smp] * col_scale[c] - cst[c] : 0.0;
         if (miss_ptr) {
-            const int sh = shift[c];
-            const double sc = col_scale[c];
-            for (int64_t p = mb; p < me; ++p) {
-                const int64_t i = miss_idx[p];
-                if (i >= k1) break;
-                const double q = (double)__double2ll_rn(ldexp(p1_coef(w, ldw, z_i8, z_f64, ldz, i, k, v), sh));
-                val += mean[i] * (q * sc);
+            const int sh = valid ? shift[c] : 0;
+            const double sc = valid ? col_scale[c] : 0.0;
+            double acc = 0.0;
+            // indices are loaded 64 at a time and broadcast lane by lane, so the
+            // per-entry loads are independent and pipeline
+            for (int64_t p0 = mb; p0 < me; p0 += 64) {
+                const int cnt = (me - p0) < 64 ? (int)(me - p0) : 64;
+                const int32_t myidx = (p0 + lane < me) ? miss_idx[p0 + lane] : 0;
+#pragma unroll 8
+                for (int j = 0; j < cnt; ++j) {
+                    const int64_t i = __builtin_amdgcn_readlane(myidx, j);
+                    if (valid) {
+                        const double q =
+                            (double)__double2ll_rn(ldexp(p1_coef(w, ldw, z_i8, z_f64, ldz, i, k, v), sh));
+                        acc += mean[i] * (q * sc);
+                    }
+                }
             }
+            val += acc;
         }
-        double* dst = xz + (int64_t)(k * v_count + v) * ldxz + smp;
-        if (mode == 1) *dst = val;
-        else *dst += val;
+        if (valid) {
+            const int mode = col_mode ? col_mode[k] : 2;
+            double* dst = xz + (int64_t)(k * v_count + v) * ldxz + smp;
+            if (mode == 1) *dst = val;
+            else *dst += val;
+        }
     }
 }
 void launch_p1_finalize(const int64_t* y, int64_t ldy, int64_t n, int n_bins, int v_count, int v_pad,
@@ -1065,6 +1127,162 @@ void launch_quant_i32(const int32_t* u, int64_t ldu, int64_t n, int n_cols, int 
         hipLaunchKernelGGL(k_quant_i32<3>, grid, dim3(256), 0, st, u, ldu, n, n_cols, ctc, coef_ctiles, kcount, coef);
     else
         throw Error("int32 operand limbs must be 3");
+    GPE_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Banded exact Gram for windowed LD scores (winldcore.cpp:480-594).
+//
+// One workgroup per 64x64 SNP tile pair of the band; wave (wi, wj) computes a
+// 32x32 block of pairs.  Both MFMA operands are packed SNP rows expanded to
+// int8 genotype counts g and missing flags q, so the four exact integer
+// products GG, Gq, qG, qq give the centred cross product
+//   sum_n (g_i - mu_i o_i)(g_j - mu_j o_j),  o = 1 - q,
+// from which r_ij = s_i s_j dot / N with s = sqrt(N / M2) (the ddof-0
+// standardization followed by the restandardization over all N rows), the
+// unbiased r^2 - (1 - r^2)/(N - 2), and the window mask.  Row / column sums
+// against the annotations are reduced through LDS and added atomically.
+// ---------------------------------------------------------------------------
+
+
+template <bool MISS>
+__global__ __launch_bounds__(256) void k_gram_band(GramArgs a) {
+    __shared__ double r2t[4][32][33];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, half = lane >> 5, lr = lane & 31;
+    const int wi = wave >> 1, wj = wave & 1;
+    const int ti = a.pairs[2 * blockIdx.x], tj = a.pairs[2 * blockIdx.x + 1];
+    const int64_t i0 = a.snp_begin + (int64_t)ti * 64 + wi * 32;
+    const int64_t j0 = a.snp_begin + (int64_t)tj * 64 + wj * 32;
+    const uint8_t* pa = a.packed + (i0 + lr) * a.stride + half * 16;
+    const uint8_t* pb = a.packed + (j0 + lr) * a.stride + half * 16;
+    v16i gg = v16i{}, gq = v16i{}, qg = v16i{}, qq = v16i{};
+    uint4 na = *reinterpret_cast<const uint4*>(pa), nb = *reinterpret_cast<const uint4*>(pb);
+    for (int64_t kc = 0; kc < a.kc1; ++kc) {
+        const uint4 wa = na, wb = nb;
+        if (kc + 1 < a.kc1) {
+            na = *reinterpret_cast<const uint4*>(pa + (kc + 1) * 32);
+            nb = *reinterpret_cast<const uint4*>(pb + (kc + 1) * 32);
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const uint32_t xa = s == 0 ? wa.x : s == 1 ? wa.y : s == 2 ? wa.z : wa.w;
+            const uint32_t xb = s == 0 ? wb.x : s == 1 ? wb.y : s == 2 ? wb.z : wb.w;
+            v4i ga, qa, gb, qb;
+            unpack16(xa, ga, qa);
+            unpack16(xb, gb, qb);
+            gg = __builtin_amdgcn_mfma_i32_32x32x32_i8(ga, gb, gg, 0, 0, 0);
+            if (MISS) {
+                gq = __builtin_amdgcn_mfma_i32_32x32x32_i8(ga, qb, gq, 0, 0, 0);
+                qg = __builtin_amdgcn_mfma_i32_32x32x32_i8(qa, gb, qg, 0, 0, 0);
+                qq = __builtin_amdgcn_mfma_i32_32x32x32_i8(qa, qb, qq, 0, 0, 0);
+            }
+        }
+    }
+    const int64_t m = a.snp_end - a.snp_begin;
+    const double dn = (double)a.n;
+    const int64_t j = j0 + lr;
+    const int64_t jr = j - a.snp_begin;
+    const bool jv = jr < m;
+    const double mu_j = jv ? a.mean[j] : 0.0, s_j = jv ? a.scale[j] : 0.0;
+    const double sum_j = jv ? (double)a.sum[j] : 0.0;
+    const double nm_j = jv ? dn - (double)a.nobs[j] : 0.0;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        const int il = 8 * (e >> 2) + 4 * half + (e & 3);
+        const int64_t i = i0 + il;
+        const int64_t ir = i - a.snp_begin;
+        double v = 0.0;
+        if (jv && ir < m) {
+            bool inwin;
+            if (a.window_snps > 0) {
+                const int64_t d = jr > ir ? jr - ir : ir - jr;
+                inwin = d <= a.window_snps;
+            } else {
+                const double d = fabs((double)a.bp[ir] - (double)a.bp[jr]);
+                inwin = !(d > a.window_bp);
+            }
+            if (inwin) {
+                const double mu_i = a.mean[i];
+                const double sum_i = (double)a.sum[i];
+                double dot;
+                if (MISS) {
+                    const double nm_i = dn - (double)a.nobs[i];
+                    dot = (double)gg[e] - mu_j * (sum_i - (double)gq[e]) - mu_i * (sum_j - (double)qg[e]) +
+                          mu_i * mu_j * (dn - nm_i - nm_j + (double)qq[e]);
+                } else {
+                    dot = (double)gg[e] - mu_j * sum_i - mu_i * sum_j + mu_i * mu_j * dn;
+                }
+                const double r = a.scale[i] * s_j * dot / dn;
+                const double r2 = r * r;
+                v = r2 - (1.0 - r2) / (a.corr_dim - 1.0);
+            }
+        }
+        r2t[wave][il][lr] = v;
+    }
+    __syncthreads();
+    // rows: L_i += sum_j r2[i][j] a_j ; columns (off-diagonal tiles): L_j += sum_i r2[i][j] a_i
+    const bool diag = ti == tj;
+    if (half == 0) {
+        const int64_t i = i0 + lr;
+        const int64_t ir = i - a.snp_begin;
+        if (ir < m) {
+            for (int b = 0; b < a.n_bins; ++b) {
+                double acc = 0.0;
+                for (int c = 0; c < 32; ++c) {
+                    const int64_t jc = j0 + c - a.snp_begin;
+                    if (jc < m) acc += r2t[wave][lr][c] * a.annot[jc * a.n_bins + b];
+                }
+                atomicAdd(&a.out[ir * a.n_bins + b], acc);
+            }
+        }
+    } else if (!diag) {
+        const int64_t jc = j0 + lr - a.snp_begin;
+        if (jc < m) {
+            for (int b = 0; b < a.n_bins; ++b) {
+                double acc = 0.0;
+                for (int rr = 0; rr < 32; ++rr) {
+                    const int64_t irr = i0 + rr - a.snp_begin;
+                    if (irr < m) acc += r2t[wave][rr][lr] * a.annot[irr * a.n_bins + b];
+                }
+                atomicAdd(&a.out[jc * a.n_bins + b], acc);
+            }
+        }
+    }
+}
+
+void launch_gram_band(const GramArgs& a, int64_t n_pairs, bool miss, hipStream_t st) {
+    if (n_pairs <= 0) return;
+    for (int64_t p0 = 0; p0 < n_pairs; p0 += 1 << 30) {
+        GramArgs b = a;
+        b.pairs = a.pairs + 2 * p0;
+        const int64_t np = (n_pairs - p0) < (1 << 30) ? (n_pairs - p0) : (1 << 30);
+        if (miss)
+            hipLaunchKernelGGL(k_gram_band<true>, dim3((unsigned)np), dim3(256), 0, st, b);
+        else
+            hipLaunchKernelGGL(k_gram_band<false>, dim3((unsigned)np), dim3(256), 0, st, b);
+        GPE_HIP(hipGetLastError());
+    }
+}
+
+// s_i = sqrt(N / M2_i) (1 when M2_i == 0), M2 over the observed (mean) or completed calls
+__global__ void k_window_scale(const int32_t* nobs, const int64_t* sum, const int64_t* sumsq, int64_t m, int64_t n,
+                               int completed, double* scale) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= m) return;
+    const double cnt = completed ? (double)n : (double)nobs[j];
+    double M2 = 0.0;
+    if (cnt > 0) {
+        const double s = (double)sum[j];
+        M2 = (double)sumsq[j] - s * s / cnt;
+        if (M2 < 0.0 && M2 > -1e-12) M2 = 0.0;
+    }
+    scale[j] = (M2 > 0.0 && isfinite(M2)) ? sqrt((double)n / M2) : 1.0;
+}
+void launch_window_scale(const int32_t* nobs, const int64_t* sum, const int64_t* sumsq, int64_t m, int64_t n,
+                         int completed, double* scale, hipStream_t st) {
+    if (m <= 0) return;
+    hipLaunchKernelGGL(k_window_scale, dim3((unsigned)ceil_div(m, 256)), dim3(256), 0, st, nobs, sum, sumsq, m, n,
+                       completed, scale);
     GPE_HIP(hipGetLastError());
 }
 

@@ -231,6 +231,54 @@ def phase2_accum(g: Genotype, snp_begin: int, snp_end: int, inv_left: torch.Tens
     return None
 
 
+def xc(g: Genotype, c: torch.Tensor, snp_begin: int = 0, snp_end: int | None = None, out=None,
+       accumulate: bool = False, limbs: int = 5, stream=None) -> torch.Tensor:
+    """out[q, n] (+)= sum_i x[n, i] c[i - snp_begin, q] (standardized X; out (Q, N) = column-major (N, Q))."""
+    snp_end = g.m if snp_end is None else snp_end
+    c = c.to(torch.float64).contiguous()
+    if c.dim() != 2 or c.shape[0] != snp_end - snp_begin:
+        raise RuntimeError("c must be (snp_end - snp_begin, Q)")
+    Q = c.shape[1]
+    if out is None:
+        out = torch.zeros((Q, g.n), dtype=torch.float64, device=c.device)
+    call("gpe_xc_f64", g.handle, snp_begin, snp_end, _ptr(c), Q, Q, _ptr(out), out.stride(0), int(bool(accumulate)),
+         limbs, _stream(stream))
+    return out
+
+
+def xtv(g: Genotype, v: torch.Tensor, snp_begin: int = 0, snp_end: int | None = None, limbs: int = 5,
+        stream=None) -> torch.Tensor:
+    """out[j - snp_begin, q] = sum_n x[n, j] v[q, n]; v (Q, N) = column-major (N, Q)."""
+    snp_end = g.m if snp_end is None else snp_end
+    v = v.to(torch.float64).contiguous()
+    if v.dim() != 2 or v.shape[1] != g.n:
+        raise RuntimeError("v must be (Q, n_samples)")
+    Q = v.shape[0]
+    out = torch.empty((snp_end - snp_begin, Q), dtype=torch.float64, device=v.device)
+    call("gpe_xtv_f64", g.handle, snp_begin, snp_end, _ptr(v), v.stride(0), Q, _ptr(out), Q, limbs,
+         _stream(stream))
+    return out
+
+
+def apply_grm(g: Genotype, panel: torch.Tensor, inv_all: torch.Tensor, nsnps: int | None = None,
+              step_size: int | None = None, limbs: int = 5, stream=None) -> torch.Tensor:
+    """panel_out = (1/M) X diag(inv^2) X^T panel_in over the first nsnps SNPs
+    (apply_grm_bed_panel, src/native/gwldcore.cpp:1134-1257).  panel (Q, N) = column-major (N, Q).
+    The whole SNP range is one product pair; step_size only bounds the coefficient panel."""
+    M = g.m if nsnps is None else int(nsnps)
+    step = M if step_size is None else max(1, int(step_size))
+    inv_all = inv_all.to(torch.float64)
+    w = inv_all[:M] * inv_all[:M] / float(M)
+    w = torch.where(torch.isfinite(w) & (w > 0), w, torch.zeros_like(w))
+    out = torch.zeros_like(panel, dtype=torch.float64)
+    for s in range(0, M, step):
+        e = min(M, s + step)
+        coef = xtv(g, panel, s, e, limbs=limbs, stream=stream)
+        coef *= w[s:e, None]
+        xc(g, coef, s, e, out=out, accumulate=True, limbs=limbs, stream=stream)
+    return out
+
+
 def probes(root_seed: int, blk_starts, blk_end: int, v_start: int, v_count: int,
            rand_dist: str = "rademacher") -> np.ndarray:
     """Reference probe values for consecutive blocks, (blk_end - blk_starts[0], v_count) FP64."""

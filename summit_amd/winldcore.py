@@ -1,0 +1,83 @@
+"""Drop-in mirror of the reference's ``summit.winldcore`` (exact windowed LD scores).
+
+``compute_windowed_ld_chr`` keeps the keyword contract of
+src/native/winldcore.cpp:964-994 and returns an (m, B) Fortran FP64 array.
+The banded Gram runs on int8 MFMA with exact integer cross products
+(``gpe_windowed_ld``); panel caching (panel_cols, cache_mb) has no role on a
+device-resident matrix and those arguments are accepted and ignored.
+Covariate projection is not provided (it re-standardizes after a dense
+projection, src/native/winldcore.cpp:362-365).
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import engine
+from .gwldcore import _geno, _to_dev, get_max_threads, set_num_threads, set_verbose  # noqa: F401
+
+_PROGRESS = {"total": 0, "done": 0, "active": False}
+
+
+def get_progress_total() -> int:
+    return _PROGRESS["total"]
+
+
+def get_progress_done() -> int:
+    return _PROGRESS["done"]
+
+
+def progress_active() -> bool:
+    return _PROGRESS["active"]
+
+
+def windowed_ld(g: engine.Genotype, snp_begin: int, snp_end: int, bp=None, ld_wind_kb: float | None = None,
+                annot=None, window_snps: int = 0) -> np.ndarray:
+    """Device-level call: (m, B) FP64 windowed LD scores of SNPs [snp_begin, snp_end)."""
+    m = snp_end - snp_begin
+    if annot is None:
+        annot = np.ones((m, 1))
+    annot = np.ascontiguousarray(np.asarray(annot, dtype=np.float64))
+    if annot.ndim != 2 or annot.shape[0] != m:
+        raise RuntimeError("annot_chr shape mismatch in compute_windowed_ld_chr")
+    B = annot.shape[1]
+    bp_arr = None
+    if window_snps <= 0:
+        bp_arr = np.ascontiguousarray(np.asarray(bp, dtype=np.int64))
+        if bp_arr.shape[0] != m:
+            raise RuntimeError("bp length mismatch in compute_windowed_ld_chr")
+    a_d = torch.from_numpy(annot).to(engine.default_device())
+    out = torch.empty((m, B), dtype=torch.float64, device=a_d.device)
+    window_bp = float(ld_wind_kb) * 1000.0 if ld_wind_kb is not None else 0.0
+    engine.call("gpe_windowed_ld", g.handle, int(snp_begin), int(snp_end),
+                0 if bp_arr is None else int(bp_arr.ctypes.data), window_bp, int(window_snps), engine._ptr(a_d), B,
+                engine._ptr(out), engine._stream())
+    return out.cpu().numpy()
+
+
+def compute_windowed_ld_chr(bed_prefix, fam_path, chr_start, chr_end, bp, annot_chr, ld_wind_kb, chunk_size,
+                            row_sel=None, C=None, R=None, impute_mode="mean", impute_seed=None, panel_cols=0,
+                            cache_mb=-1):
+    """src/native/winldcore.cpp:783-931."""
+    chr_start, chr_end = int(chr_start), int(chr_end)
+    if chr_end < chr_start:
+        raise RuntimeError("chr_end must be >= chr_start")
+    m = chr_end - chr_start
+    if m <= 0:
+        raise RuntimeError("Chromosome slice is empty")
+    if int(chunk_size) <= 0:
+        raise RuntimeError("chunk_size must be > 0")
+    wbp = float(ld_wind_kb) * 1000.0
+    if not np.isfinite(wbp) or wbp <= 0.0:
+        raise RuntimeError("ld_wind_kb must define a finite positive BP window")
+    if C is not None or R is not None:
+        raise RuntimeError("covariate projection is not provided by the GPU windowed-LD engine")
+    g, _, m_total = _geno(bed_prefix, fam_path, row_sel, 0, impute_mode, impute_seed)
+    if chr_end > m_total:
+        raise RuntimeError("chr_end exceeds #SNPs in BIM")
+    _PROGRESS.update(total=1, done=0, active=True)
+    try:
+        out = windowed_ld(g, chr_start, chr_end, bp, ld_wind_kb, annot_chr)
+    finally:
+        _PROGRESS.update(done=1, active=False)
+    return np.asfortranarray(out)
