@@ -189,6 +189,28 @@ int gpe_xtv_f64(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const double* v
 int gpe_windowed_ld(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const int64_t* bp_host, double window_bp,
                     int64_t window_snps, const double* annot_dev, int32_t n_bins, double* out_dev, void* stream);
 
+/* GxE probe streams (src/summit/ldscore/gwe_ldscore.py:2484-2505, native
+ * restatement src/native/gxeldcore.cpp:10002-10138): block b (SNPs
+ * [blk_starts[b], next start or blk_end)) and probe v draw NumPy Philox4x64-10
+ * with key keys_host[(b*v_count + v)*2 .. +1] (NumPy's seeding of
+ * Philox(_make_seed(root, block start, probe id))); value = +-1 from the high
+ * bit of each generated byte.  z_dev: int8 row-major, row 0 = blk_starts[0]. */
+int gpe_philox_rademacher(const uint64_t* keys_host, const int64_t* blk_starts_host, int32_t n_blocks, int64_t blk_end,
+                          int32_t v_count, int8_t* z_dev, int64_t ldz, void* stream);
+
+/* Phase 1 with caller-provided int8 probes z_dev (row-major (L, v_count), row 0
+ * = snp_begin), otherwise as gpe_phase1_xz.  The GxE sources
+ * (src/native/gxeldcore.cpp:7418-7576) use it with their Philox probes. */
+int gpe_phase1_xz_z8(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const double* annot_dev, int32_t n_bins,
+                     const double* inv_dev, const int8_t* z_dev, int64_t ldz, int32_t v_count, double* xz_dev,
+                     int64_t ldxz, const uint8_t* bin_mode_dev, int32_t limbs, void* stream);
+
+/* out[j - snp_begin] = sum_n w_n x_jn^2 (standardized working codes): the
+ * ||e . x_j||^2 feature moment of the GxE interaction column
+ * (src/summit/ldscore/gwe_ldscore.py:2349-2353). */
+int gpe_weighted_sumsq(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const double* w_dev, double* out_dev,
+                       void* stream);
+
 /* Probe values of the reference's genome-wide streams for blocks starting at
  * blk_starts_host[0..n_blocks) (last block ends at blk_end): Rademacher on the
  * GPU, Gaussian with the host std::normal_distribution.  z_host: row-major

@@ -497,6 +497,76 @@ def windowed_ld_chr(bed, n_total: int, chr_start: int, chr_end: int, bp, annot_c
 
 
 # ----------------------------------------------------------------------------
+# SUMMIT-GxE directional scores, Python backend of GenomewideEnvLDScore:
+# probes src/summit/ldscore/gwe_ldscore.py:515-529, :2484-2505; features
+# :2164-2191, :2323-2375; sources :2561-2579, :3679-3709; targets and score
+# accumulation :2581-2586, :3738-3759; final division by nvecs :3826.
+# G is the mean-imputed standardized genotype block; C_int an orthonormal,
+# centred covariate basis (N x p); env the environment vector.
+# ----------------------------------------------------------------------------
+def gxe_probe_block(root: int, blk_start: int, probe_ids, L: int) -> np.ndarray:
+    """(L, V) +-1 probes from NumPy Philox(_make_seed(root, blk_start, probe_id))."""
+    Z = np.empty((L, len(probe_ids)), dtype=np.float64, order="F")
+    for c, pid in enumerate(probe_ids):
+        rng = np.random.Generator(np.random.Philox(make_seed(root, blk_start, int(pid))))
+        Z[:, c] = 2.0 * rng.integers(0, 2, size=L, dtype=np.int8).astype(np.float64) - 1.0
+    return Z
+
+
+def gxe_project_center(M, C_int):
+    M = np.array(M, dtype=np.float64, order="F")
+    if C_int is not None and C_int.shape[1] > 0:
+        M -= C_int @ (C_int.T @ M)
+    M -= M.mean(axis=0, keepdims=True)
+    return M
+
+
+def gxe_scores(bed, n_total: int, m_total: int, env, C_int, annot=None, nvecs: int = 8, seed: int = 1,
+               step_size: int = 1000, vtiles=None, ddof: int = 1, impute: str = "mean", rows=None,
+               probe_offset: int = 0, eps_var: float = 1e-10):
+    env = np.asarray(env, dtype=np.float64)
+    C_int = np.zeros((env.size, 0)) if C_int is None else np.asarray(C_int, dtype=np.float64)
+    if annot is None:
+        annot = np.ones((m_total, 1))
+    annot = np.asarray(annot, dtype=np.float64)
+    K = annot.shape[1]
+    G = read_block_standardized(bed, n_total, 0, m_total, rows, ddof, impute, 0)
+    N = G.shape[0]
+    df = float(N - C_int.shape[1] - 1)
+    X = gxe_project_center(G, C_int)
+    W = gxe_project_center(env[:, None] * G, C_int)
+    inv_x = 1.0 / np.sqrt(np.sum(X * X, axis=0) / df)
+    inv_w = 1.0 / np.sqrt(np.sum(W * W, axis=0) / df)
+    X *= inv_x[None, :]
+    W *= inv_w[None, :]
+    blocks = [(s, min(m_total, s + step_size)) for s in range(0, m_total, step_size)]
+    vtiles = [nvecs] if vtiles is None else list(vtiles)
+    acc = {k: np.zeros((m_total, K)) for k in ("xx", "xw", "wx", "ww")}
+    v0 = 0
+    for Vt in vtiles:
+        kt = K * Vt
+        src = np.zeros((N, 2 * kt))
+        for s, e in blocks:
+            Z = gxe_probe_block(seed, s, [probe_offset + v0 + v for v in range(Vt)], e - s)
+            sq = np.sqrt(np.maximum(annot[s:e], 0.0))
+            for k in range(K):
+                wz = sq[:, k:k + 1] * Z
+                src[:, k * Vt:(k + 1) * Vt] += X[:, s:e] @ wz
+                src[:, kt + k * Vt:kt + (k + 1) * Vt] += W[:, s:e] @ wz
+        wx_ = X.T @ src
+        ww_ = W.T @ src
+        for name, work, off in (("xx", wx_, 0), ("xw", wx_, kt), ("wx", ww_, 0), ("ww", ww_, kt)):
+            for k in range(K):
+                seg = work[:, off + k * Vt:off + (k + 1) * Vt]
+                acc[name][:, k] += np.sum(seg * seg, axis=1) / (df * df)
+        v0 += Vt
+    out = {k: v / float(nvecs) for k, v in acc.items()}
+    out["inv_x"] = inv_x
+    out["inv_w"] = inv_w
+    return out
+
+
+# ----------------------------------------------------------------------------
 # Exact integer products on masked genotype counts (missing -> 0)
 # ----------------------------------------------------------------------------
 def masked_geno(codes) -> np.ndarray:

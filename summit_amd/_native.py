@@ -40,7 +40,11 @@ SIGNATURES = {
     "gpe_phase2_accum": (ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_i32, c_vp, c_i64, c_i32, c_i32, c_vp, c_i64,
                                         c_vp, c_vp, c_dbl, c_i32, c_i32, c_vp, c_vp]),
     "gpe_probes": (ctypes.c_int, [c_i32, c_u64, c_vp, c_i32, c_i64, c_i32, c_i32, c_vp]),
-    "gpe_windowed_ld": (ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_dbl, c_i64, c_vp, c_i32, c_vp, c_vp]),
+    "gpe_philox_rademacher": (ctypes.c_int, [c_vp, c_vp, c_i32, c_i64, c_i32, c_vp, c_i64, c_vp]),
+    "gpe_phase1_xz_z8": (ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_i32, c_vp, c_vp, c_i64, c_i32, c_vp, c_i64, c_vp,
+                                        c_i32, c_vp]),
+    "gpe_weighted_sumsq": (ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_vp, c_vp]),
+    "gpe_windowed_ld":(ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_dbl, c_i64, c_vp, c_i32, c_vp, c_vp]),
     "gpe_xc_f64":(ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_i64, c_i32, c_vp, c_i64, c_i32, c_i32, c_vp]),
     "gpe_xtv_f64": (ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_i64, c_i32, c_vp, c_i64, c_i32, c_vp]),
     "gpe_profile_enable": (ctypes.c_int, [c_i32]),

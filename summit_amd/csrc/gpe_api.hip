@@ -557,7 +557,7 @@ int gpe_gtu_i64(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const int32_t* 
         GPE_HIP(hipMemcpyAsync(&mx_h, mx, 4, hipMemcpyDeviceToHost, st));
         GPE_HIP(hipStreamSynchronize(st));
         require(mx_h <= 8421504u, "|U| exceeds the three-limb range (8421504)");
-        const int nl = 3, ctc = 2;
+        const int nl = 3, ctc = 1;
         const int groups = (int)ceil_div(n_cols, 32 * ctc);
         const int64_t coef_ctiles = (int64_t)groups * nl * ctc;
         const int64_t kcount = ceil_div(g->n, kKChunk);
@@ -845,6 +845,80 @@ int gpe_xtv_f64(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const double* v
         a.cols_out = n_cols;
         a.out_row0 = snp_begin;
         run_phase2(g, snp_begin, snp_end, v_dev, ldv, 1, n_cols, limbs, EPI_WORK, a, st, nullptr, nullptr);
+    });
+}
+
+int gpe_philox_rademacher(const uint64_t* keys_host, const int64_t* blk_starts_host, int32_t n_blocks, int64_t blk_end,
+                          int32_t v_count, int8_t* z_dev, int64_t ldz, void* stream) {
+    return guarded([&] {
+        require(keys_host && blk_starts_host && z_dev && n_blocks > 0 && v_count > 0 && ldz >= v_count,
+                "bad Philox probe request");
+        hipStream_t st = S(stream);
+        std::vector<int64_t> lens((size_t)n_blocks);
+        int64_t max_len = 0;
+        for (int b = 0; b < n_blocks; ++b) {
+            const int64_t e = b + 1 < n_blocks ? blk_starts_host[b + 1] : blk_end;
+            lens[(size_t)b] = e - blk_starts_host[b];
+            require(lens[(size_t)b] > 0, "probe blocks must be non-empty and increasing");
+            max_len = std::max(max_len, lens[(size_t)b]);
+        }
+        const size_t nkeys = (size_t)n_blocks * (size_t)v_count * 2;
+        uint64_t* keys = nullptr;
+        int64_t* starts = nullptr;
+        int64_t* lens_d = nullptr;
+        GPE_HIP(hipMalloc(&keys, nkeys * sizeof(uint64_t)));
+        GPE_HIP(hipMalloc(&starts, n_blocks * sizeof(int64_t)));
+        GPE_HIP(hipMalloc(&lens_d, n_blocks * sizeof(int64_t)));
+        try {
+            GPE_HIP(hipMemcpyAsync(keys, keys_host, nkeys * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+            GPE_HIP(hipMemcpyAsync(starts, blk_starts_host, n_blocks * sizeof(int64_t), hipMemcpyHostToDevice, st));
+            GPE_HIP(hipMemcpyAsync(lens_d, lens.data(), n_blocks * sizeof(int64_t), hipMemcpyHostToDevice, st));
+            launch_philox_rademacher(keys, starts, lens_d, n_blocks, v_count, max_len, blk_starts_host[0], z_dev, ldz,
+                                     st);
+            GPE_HIP(hipStreamSynchronize(st));
+        } catch (...) {
+            dfree(keys);
+            dfree(starts);
+            dfree(lens_d);
+            throw;
+        }
+        dfree(keys);
+        dfree(starts);
+        dfree(lens_d);
+    });
+}
+
+int gpe_phase1_xz_z8(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const double* annot_dev, int32_t n_bins,
+                     const double* inv_dev, const int8_t* z_dev, int64_t ldz, int32_t v_count, double* xz_dev,
+                     int64_t ldxz, const uint8_t* bin_mode_dev, int32_t limbs, void* stream) {
+    return guarded([&] {
+        require_prepared(g);
+        require(snp_begin >= 0 && snp_end <= g->m && snp_begin <= snp_end, "blk_end exceeds #SNPs in BIM");
+        require(n_bins > 0 && v_count > 0 && ldz >= v_count, "bad probe matrix");
+        require(ldxz >= g->n, "Xz2d_chunk shape must be (N, B*v_count)");
+        if (snp_end == snp_begin) return;
+        (void)limb_cfg(limbs);
+        hipStream_t st = S(stream);
+        const int64_t L = snp_end - snp_begin;
+        const int v_pad = (int)round_up(v_count, 32);
+        double* w = static_cast<double*>(g->scratch("p1_w", g->m_alloc * n_bins * sizeof(double)));
+        hipLaunchKernelGGL(k_make_w, dim3((unsigned)ceil_div(L, 256)), dim3(256), 0, st, annot_dev, n_bins, inv_dev,
+                           g->inv_sd, snp_begin, snp_end, w);
+        GPE_HIP(hipGetLastError());
+        int8_t* z = static_cast<int8_t*>(g->scratch("p1_z8", g->m_alloc * v_pad));
+        GPE_HIP(hipMemcpy2DAsync(z + snp_begin * v_pad, v_pad, z_dev, ldz, v_count, L, hipMemcpyDeviceToDevice, st));
+        run_phase1(g, snp_begin, snp_end, w, n_bins, z, nullptr, v_pad, v_count, xz_dev, ldxz, bin_mode_dev, limbs,
+                   st);
+    });
+}
+
+int gpe_weighted_sumsq(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const double* w_dev, double* out_dev,
+                       void* stream) {
+    return guarded([&] {
+        require_prepared(g);
+        require(snp_begin >= 0 && snp_end <= g->m && snp_begin <= snp_end, "SNP range out of bounds");
+        launch_weighted_sumsq(g->snp_work, g->snp_stride, snp_begin, snp_end, g->n, g->mean, g->inv_sd, w_dev,
+                              out_dev, S(stream));
     });
 }
 

@@ -151,6 +151,10 @@ void launch_synth_markov(uint8_t* snp, int64_t stride, int64_t m, int64_t n, uin
                          double maf_hi, double miss_rate, int block_len, double copy_prob, int iid,
                          hipStream_t st);
 void launch_gram_band(const GramArgs& a, int64_t n_pairs, bool miss, hipStream_t st);
+void launch_philox_rademacher(const uint64_t* keys, const int64_t* starts, const int64_t* lens, int n_blocks,
+                              int v_count, int64_t max_len, int64_t row0, int8_t* z, int64_t ldz, hipStream_t st);
+void launch_weighted_sumsq(const uint8_t* snp, int64_t stride, int64_t r0, int64_t r1, int64_t n, const double* mean,
+                           const double* inv_sd, const double* w, double* out, hipStream_t st);
 void launch_window_scale(const int32_t* nobs, const int64_t* sum, const int64_t* sumsq, int64_t m, int64_t n,
                          int completed, double* scale, hipStream_t st);
 void launch_resvar_inv(const int32_t* nobs, const int64_t* sum, const int64_t* sumsq, const double* mean,

@@ -39,6 +39,24 @@ __device__ __forceinline__ uint32_t spread4(uint32_t b) {
     return (b | (b << 6) | (b << 12) | (b << 18)) & 0x03030303u;
 }
 
+// Sixteen 2-bit fields of x -> sixteen bytes (field t in byte t).  The fields
+// at the same position of every byte are isolated with one mask each (s0..s3)
+// and the resulting 4x4 byte matrix is transposed with v_perm_b32.
+__device__ __forceinline__ v4i spread16(uint32_t x) {
+    const uint32_t s0 = x & 0x03030303u;
+    const uint32_t s1 = (x >> 2) & 0x03030303u;
+    const uint32_t s2 = (x >> 4) & 0x03030303u;
+    const uint32_t s3 = (x >> 6) & 0x03030303u;
+    const uint32_t u01lo = __builtin_amdgcn_perm(s1, s0, 0x05010400u);
+    const uint32_t u23lo = __builtin_amdgcn_perm(s3, s2, 0x05010400u);
+    const uint32_t u01hi = __builtin_amdgcn_perm(s1, s0, 0x07030602u);
+    const uint32_t u23hi = __builtin_amdgcn_perm(s3, s2, 0x07030602u);
+    return v4i{(int32_t)__builtin_amdgcn_perm(u23lo, u01lo, 0x05040100u),
+               (int32_t)__builtin_amdgcn_perm(u23lo, u01lo, 0x07060302u),
+               (int32_t)__builtin_amdgcn_perm(u23hi, u01hi, 0x05040100u),
+               (int32_t)__builtin_amdgcn_perm(u23hi, u01hi, 0x07060302u)};
+}
+
 // 16 codes -> 16 int8 genotype counts (missing -> 0) and 16 int8 missing flags.
 // Byte t of the result belongs to code t of w (bits 2t..2t+1).
 __device__ __forceinline__ void unpack16(uint32_t w, v4i& g, v4i& m) {
@@ -46,18 +64,14 @@ __device__ __forceinline__ void unpack16(uint32_t w, v4i& g, v4i& m) {
     const uint32_t hi = (w >> 1) & 0x55555555u;
     const uint32_t gv = hi + (hi & lo);  // 00->0 01->0 10->1 11->2, no carry out of a field
     const uint32_t mv = lo & ~hi;        // 01 -> 1
-    g = v4i{(int32_t)spread4(gv & 0xffu), (int32_t)spread4((gv >> 8) & 0xffu),
-            (int32_t)spread4((gv >> 16) & 0xffu), (int32_t)spread4(gv >> 24)};
-    m = v4i{(int32_t)spread4(mv & 0xffu), (int32_t)spread4((mv >> 8) & 0xffu),
-            (int32_t)spread4((mv >> 16) & 0xffu), (int32_t)spread4(mv >> 24)};
+    g = spread16(gv);
+    m = spread16(mv);
 }
 
 __device__ __forceinline__ void unpack16_g(uint32_t w, v4i& g) {
     const uint32_t lo = w & 0x55555555u;
     const uint32_t hi = (w >> 1) & 0x55555555u;
-    const uint32_t gv = hi + (hi & lo);
-    g = v4i{(int32_t)spread4(gv & 0xffu), (int32_t)spread4((gv >> 8) & 0xffu),
-            (int32_t)spread4((gv >> 16) & 0xffu), (int32_t)spread4(gv >> 24)};
+    g = spread16(hi + (hi & lo));
 }
 
 // Offset of coefficient byte (c-tile tg, value index kk in [0,128), lane row lr in [0,32))
@@ -141,50 +155,54 @@ __global__ __launch_bounds__(256, (NL * CTC <= 2) ? 4 : 2) void k_contract(Contr
     const uint8_t* prow1 = MISS ? prow0 : prow0 + 32 * a.stride;
 
     // coefficient chunk kc+1 is prefetched into registers, packed rows two chunks ahead
-    uint4 areg[CT];
-    uint4 bq0 = make_uint4(0, 0, 0, 0), bq1 = make_uint4(0, 0, 0, 0);  // chunk kc+1, tile 0/1
-    uint4 bn0 = make_uint4(0, 0, 0, 0), bn1 = make_uint4(0, 0, 0, 0);  // chunk kc+2, tile 0/1
+    v4i areg[CT];
+    v4i bq0 = v4i{}, bq1 = v4i{};  // chunk kc+1, tile 0/1
+    v4i bn0 = v4i{}, bn1 = v4i{};  // chunk kc+2, tile 0/1
 
-    auto load_coef = [&](int64_t kc) {
-        const uint4* src =
-            reinterpret_cast<const uint4*>(a.coef + ((kc - a.kc0) * a.coef_ctiles + (int64_t)cg * CT) * 4096);
-#pragma unroll
-        for (int t = 0; t < CT; ++t) areg[t] = src[t * 256 + tid];
-    };
+    // (plain unrolled loops, no lambdas: the prefetch registers must stay in VGPRs)
+    const v4i* coef_base =
+        reinterpret_cast<const v4i*>(a.coef + ((int64_t)cg * CT) * 4096) + tid;
+    const int64_t coef_chunk_stride = a.coef_ctiles * 256;  // uint4 per K chunk
 
     if (kc_begin < kc_end) {
-        load_coef(kc_begin);
-        bq0 = *reinterpret_cast<const uint4*>(prow0 + kc_begin * 32);
-        if (!MISS) bq1 = *reinterpret_cast<const uint4*>(prow1 + kc_begin * 32);
+        const v4i* src = coef_base + (kc_begin - a.kc0) * coef_chunk_stride;
+#pragma unroll
+        for (int t = 0; t < CT; ++t) areg[t] = src[t * 256];
+        bq0 = *reinterpret_cast<const v4i*>(prow0 + kc_begin * 32);
+        if (!MISS) bq1 = *reinterpret_cast<const v4i*>(prow1 + kc_begin * 32);
     }
     if (kc_begin + 1 < kc_end) {
-        bn0 = *reinterpret_cast<const uint4*>(prow0 + (kc_begin + 1) * 32);
-        if (!MISS) bn1 = *reinterpret_cast<const uint4*>(prow1 + (kc_begin + 1) * 32);
+        bn0 = *reinterpret_cast<const v4i*>(prow0 + (kc_begin + 1) * 32);
+        if (!MISS) bn1 = *reinterpret_cast<const v4i*>(prow1 + (kc_begin + 1) * 32);
     }
     int buf = 0;
     for (int64_t kc = kc_begin; kc < kc_end; ++kc) {
-        uint4* dst = reinterpret_cast<uint4*>(lds[buf]);
+        v4i* dst = reinterpret_cast<v4i*>(lds[buf]);
 #pragma unroll
         for (int t = 0; t < CT; ++t) dst[t * 256 + tid] = areg[t];
-        const uint4 b0 = bq0;
-        const uint4 b1 = bq1;
+        const v4i b0 = bq0;
+        const v4i b1 = bq1;
         bq0 = bn0;
         bq1 = bn1;
         __syncthreads();
-        if (kc + 1 < kc_end) load_coef(kc + 1);
+        if (kc + 1 < kc_end) {
+            const v4i* src = coef_base + (kc + 1 - a.kc0) * coef_chunk_stride;
+#pragma unroll
+            for (int t = 0; t < CT; ++t) areg[t] = src[t * 256];
+        }
         if (kc + 2 < kc_end) {
-            bn0 = *reinterpret_cast<const uint4*>(prow0 + (kc + 2) * 32);
-            if (!MISS) bn1 = *reinterpret_cast<const uint4*>(prow1 + (kc + 2) * 32);
+            bn0 = *reinterpret_cast<const v4i*>(prow0 + (kc + 2) * 32);
+            if (!MISS) bn1 = *reinterpret_cast<const v4i*>(prow1 + (kc + 2) * 32);
         }
         const int8_t* L = lds[buf];
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-            const uint32_t w0 = s == 0 ? b0.x : s == 1 ? b0.y : s == 2 ? b0.z : b0.w;
+            const uint32_t w0 = (uint32_t)b0[s];
             v4i op0, op1;
             if (MISS) {
                 unpack16(w0, op0, op1);
             } else {
-                const uint32_t w1 = s == 0 ? b1.x : s == 1 ? b1.y : s == 2 ? b1.z : b1.w;
+                const uint32_t w1 = (uint32_t)b1[s];
                 unpack16_g(w0, op0);
                 unpack16_g(w1, op1);
             }
@@ -326,7 +344,7 @@ void launch_contract(int nl, int ctc, bool miss, EpiMode mode, const ContractArg
     }
     // exact integer products
     GPE_CASE(1, 2, false, EPI_I32_ATOMIC)
-    GPE_CASE(3, 2, false, EPI_I64_ROWMAJOR)
+    GPE_CASE(3, 1, false, EPI_I64_ROWMAJOR)
     // weighted phase 1 (sample-major rows, int64 partial sums)
     GPE_CASE(5, 1, false, EPI_I64_ATOMIC)
     GPE_CASE(6, 1, false, EPI_I64_ATOMIC)
@@ -1156,17 +1174,17 @@ __global__ __launch_bounds__(256) void k_gram_band(GramArgs a) {
     const uint8_t* pa = a.packed + (i0 + lr) * a.stride + half * 16;
     const uint8_t* pb = a.packed + (j0 + lr) * a.stride + half * 16;
     v16i gg = v16i{}, gq = v16i{}, qg = v16i{}, qq = v16i{};
-    uint4 na = *reinterpret_cast<const uint4*>(pa), nb = *reinterpret_cast<const uint4*>(pb);
+    v4i na = *reinterpret_cast<const v4i*>(pa), nb = *reinterpret_cast<const v4i*>(pb);
     for (int64_t kc = 0; kc < a.kc1; ++kc) {
-        const uint4 wa = na, wb = nb;
+        const v4i wa = na, wb = nb;
         if (kc + 1 < a.kc1) {
-            na = *reinterpret_cast<const uint4*>(pa + (kc + 1) * 32);
-            nb = *reinterpret_cast<const uint4*>(pb + (kc + 1) * 32);
+            na = *reinterpret_cast<const v4i*>(pa + (kc + 1) * 32);
+            nb = *reinterpret_cast<const v4i*>(pb + (kc + 1) * 32);
         }
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-            const uint32_t xa = s == 0 ? wa.x : s == 1 ? wa.y : s == 2 ? wa.z : wa.w;
-            const uint32_t xb = s == 0 ? wb.x : s == 1 ? wb.y : s == 2 ? wb.z : wb.w;
+            const uint32_t xa = (uint32_t)wa[s];
+            const uint32_t xb = (uint32_t)wb[s];
             v4i ga, qa, gb, qb;
             unpack16(xa, ga, qa);
             unpack16(xb, gb, qb);
@@ -1262,6 +1280,102 @@ void launch_gram_band(const GramArgs& a, int64_t n_pairs, bool miss, hipStream_t
             hipLaunchKernelGGL(k_gram_band<false>, dim3((unsigned)np), dim3(256), 0, st, b);
         GPE_HIP(hipGetLastError());
     }
+}
+
+// ---------------------------------------------------------------------------
+// GxE probe streams: NumPy Philox4x64-10 (counter-based; key per (SNP block,
+// probe id) supplied by the host from NumPy's own seeding), consumed as
+// Generator.integers(0, 2, int8): uint32 words are the low then high halves of
+// each 64-bit output, bytes little-endian, value = high bit of the byte
+// (src/summit/ldscore/gwe_ldscore.py:2484-2505, restated natively at
+// src/native/gxeldcore.cpp:10002-10138).  Thread per (stream, Philox block):
+// 32 probe values.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void mulhilo64(uint64_t a, uint64_t b, uint64_t& hi, uint64_t& lo) {
+    lo = a * b;
+    hi = __umul64hi(a, b);
+}
+
+__global__ void k_philox_rademacher(const uint64_t* keys, const int64_t* starts, const int64_t* lens, int n_streams,
+                                    int v_count, int64_t row0, int8_t* z, int64_t ldz, int64_t max_blocks) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stream = gid / max_blocks;
+    const int64_t t = gid % max_blocks;
+    if (stream >= n_streams) return;
+    const int blk = (int)(stream / v_count);
+    const int v = (int)(stream % v_count);
+    const int64_t L = lens[blk];
+    if (t * 32 >= L) return;
+    uint64_t c0 = (uint64_t)t + 1, c1 = 0, c2 = 0, c3 = 0;
+    uint64_t k0 = keys[2 * stream], k1 = keys[2 * stream + 1];
+#pragma unroll
+    for (int round = 0; round < 10; ++round) {
+        uint64_t hi0, lo0, hi1, lo1;
+        mulhilo64(0xD2E7470EE14C6C93ULL, c0, hi0, lo0);
+        mulhilo64(0xCA5A826395121157ULL, c2, hi1, lo1);
+        const uint64_t n0 = hi1 ^ c1 ^ k0, n1 = lo1, n2 = hi0 ^ c3 ^ k1, n3 = lo0;
+        c0 = n0;
+        c1 = n1;
+        c2 = n2;
+        c3 = n3;
+        k0 += 0x9E3779B97F4A7C15ULL;
+        k1 += 0xBB67AE8584CAA73BULL;
+    }
+    const uint64_t words[4] = {c0, c1, c2, c3};
+    const int64_t base = starts[blk] - row0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+#pragma unroll
+        for (int bt = 0; bt < 8; ++bt) {
+            const int64_t r = t * 32 + q * 8 + bt;
+            if (r < L) {
+                const uint32_t byte = (uint32_t)(words[q] >> (8 * bt)) & 0xFFu;
+                z[(base + r) * ldz + v] = (byte & 0x80u) ? (int8_t)1 : (int8_t)-1;
+            }
+        }
+    }
+}
+void launch_philox_rademacher(const uint64_t* keys, const int64_t* starts, const int64_t* lens, int n_blocks,
+                              int v_count, int64_t max_len, int64_t row0, int8_t* z, int64_t ldz, hipStream_t st) {
+    const int n_streams = n_blocks * v_count;
+    const int64_t max_blocks = ceil_div(max_len, 32);
+    const int64_t total = (int64_t)n_streams * max_blocks;
+    if (total <= 0) return;
+    hipLaunchKernelGGL(k_philox_rademacher, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, st, keys, starts, lens,
+                       n_streams, v_count, row0, z, ldz, max_blocks);
+    GPE_HIP(hipGetLastError());
+}
+
+// out[j] = sum_n w_n x_jn^2 for the standardized working codes (x = (g - mu) inv_sd, 0 where missing)
+__global__ void k_weighted_sumsq(const uint8_t* snp, int64_t stride, int64_t r0, int64_t r1, int64_t n,
+                                 const double* mean, const double* inv_sd, const double* w, double* out) {
+    const int64_t j = r0 + (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (j >= r1) return;
+    const uint8_t* row = snp + j * stride;
+    const double mu = mean[j];
+    const double v[4] = {(0.0 - mu), 0.0, (1.0 - mu), (2.0 - mu)};
+    double acc = 0.0;
+    for (int64_t b = lane; b < ceil_div(n, 4); b += 64) {
+        const uint32_t byte = row[b];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int64_t s = b * 4 + t;
+            if (s < n) {
+                const double x = v[(byte >> (2 * t)) & 3u];
+                acc += w[s] * x * x;
+            }
+        }
+    }
+    acc = wave_sum_f64(acc);
+    if (lane == 0) out[j - r0] = acc * inv_sd[j] * inv_sd[j];
+}
+void launch_weighted_sumsq(const uint8_t* snp, int64_t stride, int64_t r0, int64_t r1, int64_t n, const double* mean,
+                           const double* inv_sd, const double* w, double* out, hipStream_t st) {
+    if (r1 <= r0) return;
+    hipLaunchKernelGGL(k_weighted_sumsq, dim3((unsigned)ceil_div(r1 - r0, 4)), dim3(256), 0, st, snp, stride, r0, r1, n,
+                       mean, inv_sd, w, out);
+    GPE_HIP(hipGetLastError());
 }
 
 // s_i = sqrt(N / M2_i) (1 when M2_i == 0), M2 over the observed (mean) or completed calls

@@ -1,0 +1,165 @@
+"""SUMMIT-GxE directional LD scores on the GPU.
+
+Restates GenomewideEnvLDScore's computation (Python backend,
+src/summit/ldscore/gwe_ldscore.py:3354-3905) on the packed-contraction engine:
+
+* features: with G the mean-imputed standardized genotypes, P the projection
+  off an orthonormal centred covariate basis C plus the intercept,
+  X_j = P G_j inv_x_j and W_j = P(e . G_j) inv_w_j, inv = 1/sqrt(||P .||^2/df),
+  df = N - p - 1 (gwe_ldscore.py:2164-2191, :2323-2375).  The norms come from
+  one phase-2 traversal against [C, 1, C.e, e] plus two weighted sums of
+  squares -- no N x M matrix is formed.
+* sources S = [P(G D_x), P(e . (G D_w))] with D = inv sqrt(a) Z: one phase-1
+  traversal with 2K bins (gwe_ldscore.py:2561-2579).
+* targets X^T S and W^T S = inv_w G^T (e . S): two phase-2 traversals whose
+  fused epilogues give XX, XW, WX, WW (gwe_ldscore.py:2581-2586, :3738-3759).
+Probes are NumPy Philox streams per (SNP block, probe id), generated on the
+GPU from the keys NumPy's seeding yields.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import engine
+
+MASK64 = 0xFFFFFFFFFFFFFFFF
+
+
+def _mix64(x: int) -> int:
+    """src/summit/ldscore/gwe_ldscore.py:515-520"""
+    x = (x + 0x9E3779B97F4A7C15) & MASK64
+    x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+    x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & MASK64
+    return (x ^ (x >> 31)) & MASK64
+
+
+def make_seed(root: int, block: int, probe_id: int) -> int:
+    """src/summit/ldscore/gwe_ldscore.py:523-529"""
+    s = 0x1234ABCD
+    s ^= _mix64(int(root))
+    s ^= _mix64(int(block) + 0x9E37)
+    s ^= _mix64(int(probe_id) + 0x85EB)
+    return s & MASK64
+
+
+def philox_keys(root: int, blk_starts, probe_ids) -> np.ndarray:
+    """Philox keys NumPy derives for Philox(make_seed(root, block, probe)), (blocks*V*2,) uint64."""
+    keys = np.empty((len(blk_starts), len(probe_ids), 2), dtype=np.uint64)
+    for b, s in enumerate(blk_starts):
+        for v, pid in enumerate(probe_ids):
+            keys[b, v] = np.random.Philox(make_seed(root, int(s), int(pid))).state["state"]["key"]
+    return np.ascontiguousarray(keys.reshape(-1))
+
+
+def probes_device(root: int, snp_begin: int, snp_end: int, step_size: int, probe_ids,
+                  device=None) -> torch.Tensor:
+    """int8 (L, V) GxE probes of SNPs [snp_begin, snp_end) in blocks of step_size."""
+    starts = np.arange(snp_begin, snp_end, step_size, dtype=np.int64)
+    keys = philox_keys(root, starts, probe_ids)
+    V = len(probe_ids)
+    z = torch.empty((snp_end - snp_begin, V), dtype=torch.int8, device=device or engine.default_device())
+    engine.call("gpe_philox_rademacher", int(keys.ctypes.data), int(starts.ctypes.data), len(starts), int(snp_end), V,
+                engine._ptr(z), V, engine._stream())
+    return z
+
+
+def weighted_sumsq(g: engine.Genotype, w: torch.Tensor, snp_begin: int = 0, snp_end: int | None = None):
+    snp_end = g.m if snp_end is None else snp_end
+    w = w.to(torch.float64).contiguous()
+    out = torch.empty(snp_end - snp_begin, dtype=torch.float64, device=w.device)
+    engine.call("gpe_weighted_sumsq", g.handle, snp_begin, snp_end, engine._ptr(w), engine._ptr(out),
+                engine._stream())
+    return out
+
+
+def _project(M_qn: torch.Tensor, C: torch.Tensor | None) -> torch.Tensor:
+    """P applied to columns of a column-major (N, Q) matrix held as (Q, N)."""
+    if C is not None and C.shape[1] > 0:
+        M_qn = M_qn - (M_qn @ C) @ C.T
+    return M_qn - M_qn.mean(dim=1, keepdim=True)
+
+
+def features(g: engine.Genotype, env: torch.Tensor, C: torch.Tensor | None, eps_var: float = 1e-10,
+             limbs: int = 6):
+    """inv_x, inv_w (M,) and df for one environment."""
+    N = g.n
+    p = 0 if C is None else int(C.shape[1])
+    df = float(N - p - 1)
+    if df <= 0:
+        raise ValueError(f"Residual correlation degrees of freedom are non-positive: nsamp={N}, p_eff={p}, df={df}.")
+    ones = torch.ones(N, dtype=torch.float64, device=env.device)
+    rows = [ones[None, :]]
+    if p:
+        rows.insert(0, C.T)
+    rows_e = [env[None, :]]
+    if p:
+        rows_e.insert(0, (C * env[:, None]).T)
+    V = torch.cat(rows + rows_e, dim=0).contiguous()
+    T = engine.xtv(g, V, limbs=limbs)  # (M, 2p + 2)
+    ssg = weighted_sumsq(g, ones)
+    sseg = weighted_sumsq(g, env * env)
+    cg = T[:, :p]
+    one_g = T[:, p]
+    ceg = T[:, p + 1:2 * p + 1]
+    e_g = T[:, 2 * p + 1]
+    ssx = ssg - (cg * cg).sum(dim=1) - one_g * one_g / N
+    ssw = sseg - (ceg * ceg).sum(dim=1) - e_g * e_g / N
+    varx = ssx / df
+    varw = ssw / df
+    bad = ~(torch.isfinite(varx) & (varx > eps_var) & torch.isfinite(varw) & (varw > eps_var))
+    if bool(bad.any()):
+        n_bad = int(bad.sum())
+        raise ValueError(f"{n_bad} variant feature(s) have zero or invalid projected variance. GxE kernel "
+                         "normalization cannot retain zero columns; QC/remove these variants.")
+    return 1.0 / torch.sqrt(varx), 1.0 / torch.sqrt(varw), df
+
+
+def gxe_ldscore(g: engine.Genotype, env, C_int=None, annot=None, nvecs: int = 8, seed: int = 1,
+                step_size: int = 1000, vtiles=None, probe_offset: int = 0, limbs: int = 5, eps_var: float = 1e-10):
+    """XX, XW, WX, WW directional scores (M, K) for one environment (raw, divided by nvecs)."""
+    dev = engine.default_device()
+    M, N = g.m, g.n
+    env_d = torch.as_tensor(np.asarray(env, dtype=np.float64), device=dev)
+    C_d = None if C_int is None else torch.as_tensor(np.asarray(C_int, dtype=np.float64), device=dev)
+    if annot is None:
+        annot = np.ones((M, 1))
+    annot = np.asarray(annot, dtype=np.float64)
+    K = annot.shape[1]
+    inv_x, inv_w, df = features(g, env_d, C_d, eps_var=eps_var, limbs=max(limbs, 6))
+    a_pos = torch.as_tensor(np.maximum(annot, 0.0), device=dev)
+    annot2 = torch.cat([a_pos * (inv_x * inv_x)[:, None], a_pos * (inv_w * inv_w)[:, None]], dim=1).contiguous()
+    ones_inv = torch.ones(M, dtype=torch.float64, device=dev)
+    vtiles = [nvecs] if vtiles is None else [int(v) for v in vtiles]
+    acc_a = torch.zeros((M, 2 * K), dtype=torch.float64, device=dev)
+    acc_b = torch.zeros((M, 2 * K), dtype=torch.float64, device=dev)
+    v0 = 0
+    for Vt in vtiles:
+        kt = K * Vt
+        z = probes_device(seed, 0, M, step_size, [probe_offset + v0 + v for v in range(Vt)], dev)
+        y = torch.zeros((2 * kt, N), dtype=torch.float64, device=dev)
+        engine.call("gpe_phase1_xz_z8", g.handle, 0, M, engine._ptr(annot2), 2 * K, engine._ptr(ones_inv),
+                    engine._ptr(z), Vt, Vt, engine._ptr(y), N, None, limbs, engine._stream())
+        s = torch.empty_like(y)
+        s[:kt] = _project(y[:kt], C_d)
+        s[kt:] = _project(y[kt:] * env_d[None, :], C_d)
+        engine.phase2_accum(g, 0, M, inv_x, Vt, s, acc_a, int(df) + 1, extreme_warn=0.0, limbs=limbs)
+        es = (s * env_d[None, :]).contiguous()
+        engine.phase2_accum(g, 0, M, inv_w, Vt, es, acc_b, int(df) + 1, extreme_warn=0.0, limbs=limbs)
+        v0 += Vt
+    torch.cuda.synchronize()
+    a = acc_a.cpu().numpy() / float(nvecs)
+    b = acc_b.cpu().numpy() / float(nvecs)
+    return {"xx": a[:, :K], "xw": a[:, K:], "wx": b[:, :K], "ww": b[:, K:],
+            "inv_x": inv_x.cpu().numpy(), "inv_w": inv_w.cpu().numpy(), "df": df}
+
+
+def gxe_ldscore_multi(g: engine.Genotype, envs, C_ints=None, **kw):
+    """K environments: one directional score set per environment column (envs (N, K));
+    C_ints is one basis shared by every environment or a list of per-environment bases."""
+    envs = np.asarray(envs, dtype=np.float64)
+    out = []
+    for k in range(envs.shape[1]):
+        C = C_ints[k] if isinstance(C_ints, (list, tuple)) else C_ints
+        out.append(gxe_ldscore(g, envs[:, k], C, **kw))
+    return out

@@ -40,3 +40,31 @@ def test_windowed_mean_imputation_scale(oracle):
     idx = np.arange(12)
     expect = (unb * (np.abs(idx[:, None] - idx[None, :]) <= 3)) @ np.ones((12, 1))
     np.testing.assert_allclose(got, expect, rtol=1e-11, atol=1e-12)
+
+
+def test_gxe_oracle_within_five_se_of_kernel_expectation(oracle):
+    """E[XX_jk] = sum_i (X_j.X_i)^2 a_ik / df^2 (and likewise XW, WX, WW) for Rademacher
+    probes; the oracle's estimate must lie within 5 Monte Carlo SE."""
+    codes = random_codes(10, 40, seed=8, miss_rate=0.02)
+    bed = oracle.pack_codes(codes)
+    rng = np.random.default_rng(3)
+    env = rng.standard_normal(40)
+    raw = rng.standard_normal((40, 1))
+    raw -= raw.mean(axis=0)
+    C, _ = np.linalg.qr(raw)
+    nvecs = 3000
+    out = oracle.gxe_scores(bed, 40, 10, env, C, None, nvecs=nvecs, seed=5, step_size=4, vtiles=[1500, 1500])
+    G = oracle.read_block_standardized(bed, 40, 0, 10)
+    df = 40 - 1 - 1
+    X = oracle.gxe_project_center(G, C)
+    W = oracle.gxe_project_center(env[:, None] * G, C)
+    X /= np.sqrt((X * X).sum(0) / df)
+    W /= np.sqrt((W * W).sum(0) / df)
+    # the normalized features have norm^2 / df == 1 (gwe_ldscore.py:2440-2458)
+    np.testing.assert_allclose((X * X).sum(0) / df, 1.0, rtol=1e-12)
+    for name, L, R in (("xx", X, X), ("xw", X, W), ("wx", W, X), ("ww", W, W)):
+        c = (L.T @ R) / df
+        c2 = c * c
+        mean = c2.sum(axis=1)
+        var = 2.0 * (mean ** 2 - (c2 * c2).sum(axis=1)) / nvecs
+        assert np.all(np.abs(out[name][:, 0] - mean) <= 5.0 * np.sqrt(var) + 1e-12), name
