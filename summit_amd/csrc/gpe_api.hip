@@ -113,11 +113,23 @@ void dfree(void* p) {
 // w[i][k] = inv_sd_i * inv_i * sqrt(a_ik), 0 where phase 1 skips the entry
 // (src/native/gwldcore.cpp:1002-1063: a == 0, a <= 0 or non-finite, inv <= 0
 // or non-finite, non-finite scale).
-__global__ void k_make_w(const double* annot, int n_bins, const double* inv, const double* inv_sd, int64_t k0,
-                         int64_t k1, double* w) {
+// True when every standardized value of SNP j is 0: no calls, or every call
+// equals the mean (cnt * sumsq == sum^2, exact in integers).  Such a column
+// contributes nothing to any product, so its weight is set to 0; a huge weight
+// (e.g. the 1/sqrt(eps) residual scale of a monomorphic SNP) would otherwise
+// set the fixed-point scale of its whole coefficient column.
+__device__ __forceinline__ bool zero_column(const int32_t* nobs, const int64_t* sum, const int64_t* sumsq, int64_t j) {
+    const __int128 cnt = nobs[j];
+    const __int128 sm = sum[j];
+    return cnt == 0 || cnt * (__int128)sumsq[j] == sm * sm;
+}
+
+__global__ void k_make_w(const double* annot, int n_bins, const double* inv, const double* inv_sd,
+                         const int32_t* nobs, const int64_t* sum, const int64_t* sumsq, int64_t k0, int64_t k1,
+                         double* w) {
     const int64_t i = k0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= k1) return;
-    const double iv = inv[i - k0];
+    const double iv = zero_column(nobs, sum, sumsq, i) ? 0.0 : inv[i - k0];
     for (int k = 0; k < n_bins; ++k) {
         const double a = annot[(i - k0) * n_bins + k];
         double out = 0.0;
@@ -210,11 +222,25 @@ struct ProfScope {
     }
 };
 
-int pick_splits(int64_t grid_xy, int64_t kcount, int64_t max_chunks_per_split) {
-    int64_t splits = 1;
-    while (grid_xy * splits < 1024 && splits * 2 <= kcount) splits *= 2;
-    while (ceil_div(kcount, splits) > max_chunks_per_split) ++splits;
-    return (int)splits;
+// K splits of a contraction whose partial sums are added atomically.  Equal-work
+// workgroups finish in rounds of `slots` (the device's resident capacity), so the
+// split count is the smallest one whose last round is at least 90% full (the best
+// such fill up to 16 splits otherwise), then raised until no split exceeds
+// max_chunks_per_split.
+int pick_splits(int64_t grid_xy, int64_t kcount, int64_t max_chunks_per_split, int64_t slots) {
+    int64_t best = 1;
+    double best_fill = 0.0;
+    for (int64_t s = 1; s <= 16 && s <= kcount; ++s) {
+        const int64_t blocks = grid_xy * s;
+        const double fill = (double)blocks / (double)(ceil_div(blocks, slots) * slots);
+        if (fill > best_fill + 1e-9) {
+            best_fill = fill;
+            best = s;
+        }
+        if (fill >= 0.9) break;
+    }
+    while (ceil_div(kcount, best) > max_chunks_per_split) ++best;
+    return (int)best;
 }
 
 }  // namespace
@@ -533,7 +559,7 @@ int gpe_gz_i32(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const int8_t* z_
         a.cols_out = n_cols;
         a.out_row0 = 0;
         const int64_t gx = ceil_div(g->n, 256);
-        const int splits = pick_splits(gx * groups, kcount, 1 << 30);
+        const int splits = pick_splits(gx * groups, kcount, 1 << 30, contract_slots(1, ctc, false, EPI_I32_ATOMIC));
         a.kc_per_split = ceil_div(kcount, splits);
         ProfScope ps(GPE_PROF_GZ, st);
         launch_contract(1, ctc, false, EPI_I32_ATOMIC, a, groups, splits, st);
@@ -686,12 +712,23 @@ void run_phase1(gpe_geno* g, int64_t s, int64_t e, const double* w, int n_bins, 
     a.cols_out = ncols_pad;
     a.out_row0 = 0;
     const int64_t gx = ceil_div(g->n, 256);
-    const int splits = pick_splits(gx * groups, kcount, (int64_t)(((int64_t)1 << 31) / (256 * 128)) - 1);
+    const int splits = pick_splits(gx * groups, kcount, (int64_t)(((int64_t)1 << 31) / (256 * 128)) - 1,
+                                   contract_slots(lc.nl, lc.ctc, false, EPI_I64_ATOMIC));
     a.kc_per_split = ceil_div(kcount, splits);
     ProfScope ps(GPE_PROF_P1_CONTRACT, st);
     launch_contract(lc.nl, lc.ctc, false, EPI_I64_ATOMIC, a, groups, splits, st);
     ps.stop();
     const bool miss = g->impute_mode == GPE_IMPUTE_MEAN && g->n_missing > 0;
+    if (miss && z_i8) {
+        const int vw = (int)ceil_div(v_pad, 64);
+        const int rec_words = (int)round_up(2 * vw + n_bins, 2);
+        uint64_t* rec = static_cast<uint64_t*>(g->scratch("p1_mrec", L * rec_words * sizeof(uint64_t)));
+        launch_p1_miss_rec(w, n_bins, z_i8, ldz, s, e, n_bins, v_count, v_pad, shift, col_scale, g->mean, rec_words,
+                           rec, st);
+        launch_p1_finalize_z8(y, ldy, g->n, n_bins, v_count, v_pad, col_scale, cst, g->miss_ptr, g->miss_idx, s, e,
+                              rec, rec_words, out, ldo, bin_mode, st);
+        return;
+    }
     launch_p1_finalize(y, ldy, g->n, n_bins, v_count, v_pad, col_scale, cst, miss ? g->miss_ptr : nullptr,
                        miss ? g->miss_idx : nullptr, s, e, w, n_bins, z_i8, z_f64, ldz, shift, g->mean, out, ldo,
                        bin_mode, st);
@@ -743,9 +780,10 @@ void run_phase2(gpe_geno* g, int64_t s, int64_t e, const double* x, int64_t ldx,
     if (v_pad_out) *v_pad_out = v_pad;
 }
 
-__global__ void k_w_from_inv_sd(const double* inv_sd, int64_t s, int64_t e, double* w) {
+__global__ void k_w_from_inv_sd(const double* inv_sd, const int32_t* nobs, const int64_t* sum, const int64_t* sumsq,
+                                int64_t s, int64_t e, double* w) {
     const int64_t i = s + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < e) w[i] = inv_sd[i];
+    if (i < e) w[i] = zero_column(nobs, sum, sumsq, i) ? 0.0 : inv_sd[i];
 }
 
 // copy a row-major (rows, cols) matrix into rows [row0, row0+rows) of a buffer with leading dimension ld
@@ -776,7 +814,7 @@ int gpe_phase1_xz(gpe_geno* g, int64_t snp_begin, int64_t snp_end, int64_t step_
         const int v_pad = (int)round_up(v_count, 32);
         double* w = static_cast<double*>(g->scratch("p1_w", g->m_alloc * n_bins * sizeof(double)));
         hipLaunchKernelGGL(k_make_w, dim3((unsigned)ceil_div(L, 256)), dim3(256), 0, st, annot_dev, n_bins, inv_dev,
-                           g->inv_sd, snp_begin, snp_end, w);
+                           g->inv_sd, g->nobs, g->sum, g->sumsq, snp_begin, snp_end, w);
         GPE_HIP(hipGetLastError());
         std::vector<int64_t> starts;
         for (int64_t s = snp_begin; s < snp_end; s += step_size) starts.push_back(s);
@@ -820,8 +858,8 @@ int gpe_xc_f64(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const double* c_
         const int64_t L = snp_end - snp_begin;
         const int v_pad = (int)round_up(n_cols, 32);
         double* w = static_cast<double*>(g->scratch("xc_w", g->m_alloc * sizeof(double)));
-        hipLaunchKernelGGL(k_w_from_inv_sd, dim3((unsigned)ceil_div(L, 256)), dim3(256), 0, st, g->inv_sd,
-                           snp_begin, snp_end, w);
+        hipLaunchKernelGGL(k_w_from_inv_sd, dim3((unsigned)ceil_div(L, 256)), dim3(256), 0, st, g->inv_sd, g->nobs,
+                           g->sum, g->sumsq, snp_begin, snp_end, w);
         GPE_HIP(hipGetLastError());
         double* z = static_cast<double*>(g->scratch("xc_z", g->m_alloc * v_pad * sizeof(double)));
         copy_rows(c_dev, ldc, L, n_cols, z, v_pad, snp_begin, st);
@@ -903,7 +941,7 @@ int gpe_phase1_xz_z8(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const doub
         const int v_pad = (int)round_up(v_count, 32);
         double* w = static_cast<double*>(g->scratch("p1_w", g->m_alloc * n_bins * sizeof(double)));
         hipLaunchKernelGGL(k_make_w, dim3((unsigned)ceil_div(L, 256)), dim3(256), 0, st, annot_dev, n_bins, inv_dev,
-                           g->inv_sd, snp_begin, snp_end, w);
+                           g->inv_sd, g->nobs, g->sum, g->sumsq, snp_begin, snp_end, w);
         GPE_HIP(hipGetLastError());
         int8_t* z = static_cast<int8_t*>(g->scratch("p1_z8", g->m_alloc * v_pad));
         GPE_HIP(hipMemcpy2DAsync(z + snp_begin * v_pad, v_pad, z_dev, ldz, v_count, L, hipMemcpyDeviceToDevice, st));

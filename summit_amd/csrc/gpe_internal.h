@@ -24,6 +24,7 @@ namespace gpe {
 // ---------------------------------------------------------------------------
 constexpr int kKChunk = 128;        // contraction-axis values per MFMA chunk
 constexpr int kRowPad = 256;        // row-count padding of both layouts
+constexpr int kXcds = 8;            // MI355X: accelerator complex dies, one L2 each
 
 __host__ __device__ inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 __host__ __device__ inline int64_t ceil_div(int64_t x, int64_t m) { return (x + m - 1) / m; }
@@ -100,6 +101,8 @@ struct GramArgs {
 // Launchers (gpe_kernels.hip)
 void launch_contract(int nl, int ctc, bool miss, EpiMode mode, const ContractArgs& a, int n_col_groups,
                      int n_splits, hipStream_t st);
+// workgroups of one contraction variant resident on the whole device at once
+int contract_slots(int nl, int ctc, bool miss, EpiMode mode);
 void launch_clear_tail(uint8_t* snp, int64_t m, int64_t stride, int64_t n, hipStream_t st);
 void launch_select_rows(const uint8_t* src, int64_t src_stride, int64_t m, const int64_t* rows, int64_t n_sel,
                         uint8_t* dst, int64_t dst_stride, hipStream_t st);
@@ -131,6 +134,13 @@ void launch_p1_finalize(const int64_t* y, int64_t ldy, int64_t n, int n_bins, in
                         const int32_t* miss_idx, int64_t k0, int64_t k1, const double* w, int64_t ldw,
                         const int8_t* z_i8, const double* z_f64, int64_t ldz, const int32_t* shift,
                         const double* mean, double* xz, int64_t ldxz, const uint8_t* col_init, hipStream_t st);
+void launch_p1_miss_rec(const double* w, int64_t ldw, const int8_t* z, int64_t ldz, int64_t k0, int64_t k1,
+                        int n_bins, int v_count, int v_pad, const int32_t* shift, const double* col_scale,
+                        const double* mean, int rec_words, uint64_t* rec, hipStream_t st);
+void launch_p1_finalize_z8(const int64_t* y, int64_t ldy, int64_t n, int n_bins, int v_count, int v_pad,
+                           const double* col_scale, const double* cst, const int64_t* miss_ptr,
+                           const int32_t* miss_idx, int64_t k0, int64_t k1, const uint64_t* rec, int rec_words,
+                           double* xz, int64_t ldxz, const uint8_t* col_mode, hipStream_t st);
 void launch_colmax_f64(const double* x, int64_t ldx, int64_t n, int n_cols, int v_count, int v_pad,
                        uint64_t* colmax_bits, hipStream_t st);
 void launch_quant_p2(const double* x, int64_t ldx, int64_t n, int n_bins, int v_count, int v_pad,

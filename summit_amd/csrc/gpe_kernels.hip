@@ -18,6 +18,9 @@
 
 #include <cmath>
 #include <cstdint>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 namespace gpe {
 
@@ -134,14 +137,22 @@ __global__ __launch_bounds__(256, (NL * CTC <= 2) ? 4 : 2) void k_contract(Contr
     const int lane = tid & 63;
     const int half = lane >> 5;
     const int lr = lane & 31;
-    // column groups of one row block are adjacent in launch order so the
-    // packed rows they share are served from L2
+    // Workgroups are dispatched in flattened (x, y, z) order and dealt round-robin
+    // to the XCDs, each with its own L2.  The flattened index is remapped so that
+    // every XCD runs a contiguous range of (split, row block, column group) tiles:
+    // the column groups of a row block (sharing its packed rows) and neighbouring
+    // row blocks of a split (sharing coefficient chunks) then meet in one L2.
     const int n_groups = (int)gridDim.y;
-    const int64_t lin = (int64_t)blockIdx.x * n_groups + blockIdx.y;
+    const int64_t gx = gridDim.x;
+    const int64_t total = gx * n_groups * gridDim.z;
+    const int64_t d = blockIdx.x + gx * (blockIdx.y + (int64_t)n_groups * blockIdx.z);
+    const int64_t per_xcd = total / kXcds;
+    const int64_t lin = d < per_xcd * kXcds ? (d % kXcds) * per_xcd + d / kXcds : d;
     const int cg = (int)(lin % n_groups);
-    const int64_t rblk = lin / n_groups;
+    const int64_t rblk = (lin / n_groups) % gx;
+    const int64_t split = lin / n_groups / gx;
     const int64_t rbase = a.r0 + rblk * RPB + (int64_t)wave * RPW;
-    const int64_t kc_begin = a.kc0 + (int64_t)blockIdx.z * a.kc_per_split;
+    const int64_t kc_begin = a.kc0 + split * a.kc_per_split;
     const int64_t kc_end = kc_begin + a.kc_per_split < a.kc1 ? kc_begin + a.kc_per_split : a.kc1;
 
     v16i acc[CT][2];
@@ -361,6 +372,37 @@ void launch_contract(int nl, int ctc, bool miss, EpiMode mode, const ContractArg
 #undef GPE_CASE
     throw Error("no packed-contraction variant for nl=" + std::to_string(nl) + " ctc=" + std::to_string(ctc) +
                 " miss=" + std::to_string((int)miss) + " mode=" + std::to_string((int)mode));
+}
+
+template <int NL, int CTC, bool MISS, int EPI>
+static int blocks_per_cu_one() {
+    int nb = 0;
+    GPE_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)k_contract<NL, CTC, MISS, EPI>, 256, 0));
+    return nb;
+}
+
+int contract_slots(int nl, int ctc, bool miss, EpiMode mode) {
+    static std::mutex mu;
+    static std::map<std::tuple<int, int, int, bool, int>, int> cache;
+    int dev = 0;
+    GPE_HIP(hipGetDevice(&dev));
+    const auto key = std::make_tuple(dev, nl, ctc, miss, (int)mode);
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int cus = 0;
+    GPE_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    int nb = 0;
+#define GPE_CASE(NL_, CTC_, MISS_, EPI_) \
+    if (nl == NL_ && ctc == CTC_ && miss == MISS_ && mode == EPI_) nb = blocks_per_cu_one<NL_, CTC_, MISS_, EPI_>();
+    GPE_CASE(1, 2, false, EPI_I32_ATOMIC)
+    GPE_CASE(3, 1, false, EPI_I64_ROWMAJOR)
+    GPE_CASE(5, 1, false, EPI_I64_ATOMIC)
+    GPE_CASE(6, 1, false, EPI_I64_ATOMIC)
+#undef GPE_CASE
+    const int slots = cus * (nb > 0 ? nb : 1);
+    cache[key] = slots;
+    return slots;
 }
 
 // ---------------------------------------------------------------------------
@@ -783,7 +825,9 @@ __device__ __forceinline__ double p1_coef(const double* w, int64_t ldw, const in
     return w[i * ldw + k] * zz;
 }
 
-// column max |coef| over SNPs [k0,k1); column c = bin * v_pad + v
+// column max |coef| over SNPs [k0,k1); column c = bin * v_pad + v.  int8 probes
+// hold -1, 0 or +1, so the bound max|w_ik| is used for every column of bin k:
+// all columns of a bin then share one shift.
 __global__ void k_coef_p1_colmax(const double* w, int64_t ldw, const int8_t* z_i8, const double* z_f64, int64_t ldz,
                                  int64_t k0, int64_t k1, int v_count, int v_pad, uint64_t* colmax_bits) {
     const int c = blockIdx.y;
@@ -791,7 +835,7 @@ __global__ void k_coef_p1_colmax(const double* w, int64_t ldw, const int8_t* z_i
     if (v >= v_count) return;
     double mx = 0.0;
     for (int64_t i = k0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < k1; i += (int64_t)gridDim.x * blockDim.x)
-        mx = fmax(mx, fabs(p1_coef(w, ldw, z_i8, z_f64, ldz, i, k, v)));
+        mx = fmax(mx, z_i8 ? fabs(w[i * ldw + k]) : fabs(p1_coef(w, ldw, z_i8, z_f64, ldz, i, k, v)));
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
     if ((threadIdx.x & 63) == 0 && mx > 0.0)
@@ -977,6 +1021,137 @@ void launch_p1_finalize(const int64_t* y, int64_t ldy, int64_t n, int n_bins, in
     hipLaunchKernelGGL(k_p1_finalize, dim3((unsigned)ceil_div(n, 4)), dim3(256), 0, st, y, ldy, n, n_bins, v_count,
                        v_pad, col_scale, cst, miss_ptr, miss_idx, k0, k1, w, ldw, z_i8, z_f64, ldz, shift, mean, xz,
                        ldxz, col_mode);
+    GPE_HIP(hipGetLastError());
+}
+
+// Missing-call records for phase 1 with int8 probes (entries -1, 0, +1), one
+// per SNP i of [k0, k1), rec_words 8-byte words each:
+//   [pos_0, neg_0, ..., pos_{W-1}, neg_{W-1}, a_0, ..., a_{B-1}]
+// pos_w / neg_w: bit b set when z[i][64w + b] is > 0 / < 0;
+// a_k = mean_i * (round(w_ik 2^shift_k) * 2^-shift_k).
+// Rounding to nearest even commutes with negation, so the quantized
+// coefficient of column (k, v) is z_iv round(w_ik 2^shift_k) and
+// mean_i * deq(coef) is +-a_k bit for bit.
+__global__ void k_p1_miss_rec(const double* w, int64_t ldw, const int8_t* z, int64_t ldz, int64_t k0, int64_t k1,
+                              int n_bins, int v_count, int v_pad, const int32_t* shift, const double* col_scale,
+                              const double* mean, int rec_words, uint64_t* rec) {
+    const int64_t i = k0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= k1) return;
+    uint64_t* r = rec + (i - k0) * rec_words;
+    const int vw = (v_pad + 63) / 64;
+    for (int wd = 0; wd < vw; ++wd) {
+        uint64_t pos = 0, neg = 0;
+        for (int b = 0; b < 64; ++b) {
+            const int v = wd * 64 + b;
+            if (v >= v_count) break;
+            const int8_t zz = z[i * ldz + v];
+            if (zz > 0) pos |= 1ULL << b;
+            else if (zz < 0) neg |= 1ULL << b;
+        }
+        r[2 * wd] = pos;
+        r[2 * wd + 1] = neg;
+    }
+    for (int k = 0; k < n_bins; ++k) {
+        const int c = k * v_pad;
+        const double q = (double)__double2ll_rn(ldexp(w[i * ldw + k], shift[c]));
+        r[2 * vw + k] = (uint64_t)__double_as_longlong(mean[i] * (q * col_scale[c]));
+    }
+}
+void launch_p1_miss_rec(const double* w, int64_t ldw, const int8_t* z, int64_t ldz, int64_t k0, int64_t k1,
+                        int n_bins, int v_count, int v_pad, const int32_t* shift, const double* col_scale,
+                        const double* mean, int rec_words, uint64_t* rec, hipStream_t st) {
+    if (k1 <= k0) return;
+    hipLaunchKernelGGL(k_p1_miss_rec, dim3((unsigned)ceil_div(k1 - k0, 256)), dim3(256), 0, st, w, ldw, z, ldz, k0,
+                       k1, n_bins, v_count, v_pad, shift, col_scale, mean, rec_words, rec);
+    GPE_HIP(hipGetLastError());
+}
+
+// Phase-1 finalize for int8 probes with missing calls: as k_p1_finalize, the
+// missing-call term summed from the records of k_p1_miss_rec.  One wave per
+// sample; each pass covers 64 probe columns of one bin, so the record words a
+// pass reads are the same for every lane (scalar loads).
+__global__ void k_p1_finalize_z8(const int64_t* y, int64_t ldy, int64_t n, int n_bins, int v_count, int v_pad,
+                                 const double* col_scale, const double* cst, const int64_t* miss_ptr,
+                                 const int32_t* miss_idx, int64_t k0, int64_t k1, const uint64_t* rec, int rec_words,
+                                 double* xz, int64_t ldxz, const uint8_t* col_mode) {
+    // the sample, its missing-list range and the loop bounds below are wave-uniform (scalar registers)
+    const int64_t smp = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (smp >= n) return;
+    const int64_t b0 = miss_ptr[smp], b1 = miss_ptr[smp + 1];
+    int64_t lo = b0, hi = b1;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (miss_idx[mid] < k0) lo = mid + 1;
+        else hi = mid;
+    }
+    const int64_t mb = lo;
+    hi = b1;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (miss_idx[mid] < k1) lo = mid + 1;
+        else hi = mid;
+    }
+    const int64_t me = lo;
+    const int vw = (v_pad + 63) / 64;
+    for (int k = 0; k < n_bins; ++k) {
+        for (int wd = 0; wd < vw; ++wd) {
+            const int v = wd * 64 + lane;
+            const bool valid = v < v_count;
+            const int c = k * v_pad + v;
+            double val = valid ? (double)y[(int64_t)c * ldy + smp] * col_scale[c] - cst[c] : 0.0;
+            double acc = 0.0;
+            for (int64_t p0 = mb; p0 < me; p0 += 64) {
+                const int cnt = (me - p0) < 64 ? (int)(me - p0) : 64;
+                const int32_t myidx = (p0 + lane < me) ? miss_idx[p0 + lane] : (int32_t)k0;
+                int j = 0;
+                // groups of 8 entries: all record loads of a group are issued before the first is used
+                for (; j + 8 <= cnt; j += 8) {
+                    uint64_t pos[8], neg[8];
+                    double av[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const int64_t i = __builtin_amdgcn_readlane(myidx, j + u);
+                        const uint64_t* r = rec + (i - k0) * rec_words;
+                        pos[u] = r[2 * wd];
+                        neg[u] = r[2 * wd + 1];
+                        av[u] = __longlong_as_double((long long)r[2 * vw + k]);
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const bool bp = (pos[u] >> lane) & 1ULL;
+                        const bool bn = (neg[u] >> lane) & 1ULL;
+                        acc += bp ? av[u] : (bn ? -av[u] : 0.0);
+                    }
+                }
+                for (; j < cnt; ++j) {
+                    const int64_t i = __builtin_amdgcn_readlane(myidx, j);
+                    const uint64_t* r = rec + (i - k0) * rec_words;
+                    const uint64_t pos = r[2 * wd], neg = r[2 * wd + 1];
+                    const double a = __longlong_as_double((long long)r[2 * vw + k]);
+                    const bool bp = (pos >> lane) & 1ULL;
+                    const bool bn = (neg >> lane) & 1ULL;
+                    acc += bp ? a : (bn ? -a : 0.0);
+                }
+            }
+            val += acc;
+            if (valid) {
+                const int mode = col_mode ? col_mode[k] : 2;
+                double* dst = xz + (int64_t)(k * v_count + v) * ldxz + smp;
+                if (mode == 1) *dst = val;
+                else *dst += val;
+            }
+        }
+    }
+}
+void launch_p1_finalize_z8(const int64_t* y, int64_t ldy, int64_t n, int n_bins, int v_count, int v_pad,
+                           const double* col_scale, const double* cst, const int64_t* miss_ptr,
+                           const int32_t* miss_idx, int64_t k0, int64_t k1, const uint64_t* rec, int rec_words,
+                           double* xz, int64_t ldxz, const uint8_t* col_mode, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_p1_finalize_z8, dim3((unsigned)ceil_div(n, 4)), dim3(256), 0, st, y, ldy, n, n_bins,
+                       v_count, v_pad, col_scale, cst, miss_ptr, miss_idx, k0, k1, rec, rec_words, xz, ldxz,
+                       col_mode);
     GPE_HIP(hipGetLastError());
 }
 
