@@ -116,6 +116,9 @@ def main() -> int:
     ap.add_argument("--cpu-snps", type=int, default=4000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-exact", action="store_true")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="only run a plain 1 GiB read + 1 GiB write elementwise kernel three times (the reference "
+                         "point that scales FETCH_SIZE / WRITE_SIZE counter readings to bytes)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -128,6 +131,14 @@ def main() -> int:
     from summit_amd import engine, ldscore
 
     dev = engine.default_device()
+    if args.copy_calib:
+        x = torch.ones(1 << 28, dtype=torch.int32, device=dev)
+        y = torch.empty_like(x)
+        for _ in range(3):
+            torch.neg(x, out=y)
+        torch.cuda.synchronize()
+        log("copy calibration: 3 x (1 GiB read + 1 GiB write)")
+        return 0
     N, M, B = args.n, args.m, args.b
     seed = 1303
     t_gen = time.perf_counter()

@@ -720,13 +720,11 @@ void run_phase1(gpe_geno* g, int64_t s, int64_t e, const double* w, int n_bins, 
     ps.stop();
     const bool miss = g->impute_mode == GPE_IMPUTE_MEAN && g->n_missing > 0;
     if (miss && z_i8) {
-        const int vw = (int)ceil_div(v_pad, 64);
-        const int rec_words = (int)round_up(2 * vw + n_bins, 2);
-        uint64_t* rec = static_cast<uint64_t*>(g->scratch("p1_mrec", L * rec_words * sizeof(uint64_t)));
-        launch_p1_miss_rec(w, n_bins, z_i8, ldz, s, e, n_bins, v_count, v_pad, shift, col_scale, g->mean, rec_words,
-                           rec, st);
+        const int64_t vw = ceil_div(v_pad, 64);
+        void* rec = g->scratch("p1_mrec", L * n_bins * vw * 32);
+        launch_p1_miss_rec(w, n_bins, z_i8, ldz, s, e, n_bins, v_count, v_pad, shift, col_scale, g->mean, rec, st);
         launch_p1_finalize_z8(y, ldy, g->n, n_bins, v_count, v_pad, col_scale, cst, g->miss_ptr, g->miss_idx, s, e,
-                              rec, rec_words, out, ldo, bin_mode, st);
+                              rec, out, ldo, bin_mode, st);
         return;
     }
     launch_p1_finalize(y, ldy, g->n, n_bins, v_count, v_pad, col_scale, cst, miss ? g->miss_ptr : nullptr,

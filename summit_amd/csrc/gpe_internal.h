@@ -136,11 +136,12 @@ void launch_p1_finalize(const int64_t* y, int64_t ldy, int64_t n, int n_bins, in
                         const double* mean, double* xz, int64_t ldxz, const uint8_t* col_init, hipStream_t st);
 void launch_p1_miss_rec(const double* w, int64_t ldw, const int8_t* z, int64_t ldz, int64_t k0, int64_t k1,
                         int n_bins, int v_count, int v_pad, const int32_t* shift, const double* col_scale,
-                        const double* mean, int rec_words, uint64_t* rec, hipStream_t st);
+                        const double* mean, void* rec, hipStream_t st);
+// records of launch_p1_miss_rec: 32 bytes per (SNP, bin, 64 probe columns)
 void launch_p1_finalize_z8(const int64_t* y, int64_t ldy, int64_t n, int n_bins, int v_count, int v_pad,
                            const double* col_scale, const double* cst, const int64_t* miss_ptr,
-                           const int32_t* miss_idx, int64_t k0, int64_t k1, const uint64_t* rec, int rec_words,
-                           double* xz, int64_t ldxz, const uint8_t* col_mode, hipStream_t st);
+                           const int32_t* miss_idx, int64_t k0, int64_t k1, const void* rec, double* xz,
+                           int64_t ldxz, const uint8_t* col_mode, hipStream_t st);
 void launch_colmax_f64(const double* x, int64_t ldx, int64_t n, int n_cols, int v_count, int v_pad,
                        uint64_t* colmax_bits, hipStream_t st);
 void launch_quant_p2(const double* x, int64_t ldx, int64_t n, int n_bins, int v_count, int v_pad,

@@ -16,6 +16,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <map>
@@ -26,6 +27,7 @@ namespace gpe {
 
 typedef int32_t v4i __attribute__((ext_vector_type(4)));
 typedef int32_t v16i __attribute__((ext_vector_type(16)));
+typedef uint32_t v4u __attribute__((ext_vector_type(4)));
 
 // ---------------------------------------------------------------------------
 // helpers
@@ -37,52 +39,51 @@ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
     return x ^ (x >> 31);
 }
 
-// Four 2-bit fields of b (bits 0..7) -> four bytes.
-__device__ __forceinline__ uint32_t spread4(uint32_t b) {
-    return (b | (b << 6) | (b << 12) | (b << 18)) & 0x03030303u;
-}
+// Unpacking 16 codes (one 32-bit word, code t in bits 2t..2t+1) into MFMA
+// operand bytes.  Byte j of s[d] = (w >> 2d) & 0x03030303 holds code 4j + d,
+// so the operand's byte position 4d + j carries code 4j + d: the contraction
+// index is permuted (a 4x4 transpose) inside every group of 16.  The
+// coefficient layout (swz_offset) applies the same permutation, and products
+// of two unpacked operands see it on both sides, so every contraction sums the
+// same pairs.  The code -> value maps are byte-table lookups with v_perm_b32
+// (selector bytes 0..3 pick bytes of the table word):
+//   genotype count 00 -> 0, 01 (missing) -> 0, 10 -> 1, 11 -> 2;
+//   missing flag   01 -> 1, otherwise 0.
+constexpr uint32_t kGenoTable = 0x02010000u;
+constexpr uint32_t kMissTable = 0x00000100u;
 
-// Sixteen 2-bit fields of x -> sixteen bytes (field t in byte t).  The fields
-// at the same position of every byte are isolated with one mask each (s0..s3)
-// and the resulting 4x4 byte matrix is transposed with v_perm_b32.
-__device__ __forceinline__ v4i spread16(uint32_t x) {
-    const uint32_t s0 = x & 0x03030303u;
-    const uint32_t s1 = (x >> 2) & 0x03030303u;
-    const uint32_t s2 = (x >> 4) & 0x03030303u;
-    const uint32_t s3 = (x >> 6) & 0x03030303u;
-    const uint32_t u01lo = __builtin_amdgcn_perm(s1, s0, 0x05010400u);
-    const uint32_t u23lo = __builtin_amdgcn_perm(s3, s2, 0x05010400u);
-    const uint32_t u01hi = __builtin_amdgcn_perm(s1, s0, 0x07030602u);
-    const uint32_t u23hi = __builtin_amdgcn_perm(s3, s2, 0x07030602u);
-    return v4i{(int32_t)__builtin_amdgcn_perm(u23lo, u01lo, 0x05040100u),
-               (int32_t)__builtin_amdgcn_perm(u23lo, u01lo, 0x07060302u),
-               (int32_t)__builtin_amdgcn_perm(u23hi, u01hi, 0x05040100u),
-               (int32_t)__builtin_amdgcn_perm(u23hi, u01hi, 0x07060302u)};
+__device__ __forceinline__ void code_bytes(uint32_t w, uint32_t s[4]) {
+    s[0] = w & 0x03030303u;
+    s[1] = (w >> 2) & 0x03030303u;
+    s[2] = (w >> 4) & 0x03030303u;
+    s[3] = (w >> 6) & 0x03030303u;
 }
 
 // 16 codes -> 16 int8 genotype counts (missing -> 0) and 16 int8 missing flags.
-// Byte t of the result belongs to code t of w (bits 2t..2t+1).
 __device__ __forceinline__ void unpack16(uint32_t w, v4i& g, v4i& m) {
-    const uint32_t lo = w & 0x55555555u;
-    const uint32_t hi = (w >> 1) & 0x55555555u;
-    const uint32_t gv = hi + (hi & lo);  // 00->0 01->0 10->1 11->2, no carry out of a field
-    const uint32_t mv = lo & ~hi;        // 01 -> 1
-    g = spread16(gv);
-    m = spread16(mv);
+    uint32_t s[4];
+    code_bytes(w, s);
+    g = v4i{(int32_t)__builtin_amdgcn_perm(0u, kGenoTable, s[0]), (int32_t)__builtin_amdgcn_perm(0u, kGenoTable, s[1]),
+            (int32_t)__builtin_amdgcn_perm(0u, kGenoTable, s[2]), (int32_t)__builtin_amdgcn_perm(0u, kGenoTable, s[3])};
+    m = v4i{(int32_t)__builtin_amdgcn_perm(0u, kMissTable, s[0]), (int32_t)__builtin_amdgcn_perm(0u, kMissTable, s[1]),
+            (int32_t)__builtin_amdgcn_perm(0u, kMissTable, s[2]), (int32_t)__builtin_amdgcn_perm(0u, kMissTable, s[3])};
 }
 
 __device__ __forceinline__ void unpack16_g(uint32_t w, v4i& g) {
-    const uint32_t lo = w & 0x55555555u;
-    const uint32_t hi = (w >> 1) & 0x55555555u;
-    g = spread16(hi + (hi & lo));
+    uint32_t s[4];
+    code_bytes(w, s);
+    g = v4i{(int32_t)__builtin_amdgcn_perm(0u, kGenoTable, s[0]), (int32_t)__builtin_amdgcn_perm(0u, kGenoTable, s[1]),
+            (int32_t)__builtin_amdgcn_perm(0u, kGenoTable, s[2]), (int32_t)__builtin_amdgcn_perm(0u, kGenoTable, s[3])};
 }
 
 // Offset of coefficient byte (c-tile tg, value index kk in [0,128), lane row lr in [0,32))
-// inside one K chunk of the swizzled layout: [tile][step][lane][16 bytes].
+// inside one K chunk of the swizzled layout: [tile][step][lane][16 bytes].  Value kk
+// sits in lane half kk/64, step (kk%64)/16 and, within its group of 16, at the
+// transposed byte position the unpacked genotype operand uses.
 __device__ __forceinline__ int64_t swz_offset(int64_t kl, int64_t coef_ctiles, int64_t tg, int kk, int lr) {
     const int h = kk >> 6;
     const int s = (kk & 63) >> 4;
-    const int b = kk & 15;
+    const int b = ((kk & 3) << 2) | ((kk >> 2) & 3);
     const int lane = h * 32 + lr;
     return (((kl * coef_ctiles + tg) * 4 + s) * 64 + lane) * 16 + b;
 }
@@ -186,6 +187,14 @@ __global__ __launch_bounds__(256, (NL * CTC <= 2) ? 4 : 2) void k_contract(Contr
         bn0 = *reinterpret_cast<const v4i*>(prow0 + (kc_begin + 1) * 32);
         if (!MISS) bn1 = *reinterpret_cast<const v4i*>(prow1 + (kc_begin + 1) * 32);
     }
+    // operands of step 0 of the first chunk; later ones are unpacked one step ahead
+    v4i op0, op1;
+    if (MISS) {
+        unpack16((uint32_t)bq0[0], op0, op1);
+    } else {
+        unpack16_g((uint32_t)bq0[0], op0);
+        unpack16_g((uint32_t)bq1[0], op1);
+    }
     int buf = 0;
     for (int64_t kc = kc_begin; kc < kc_end; ++kc) {
         v4i* dst = reinterpret_cast<v4i*>(lds[buf]);
@@ -196,26 +205,30 @@ __global__ __launch_bounds__(256, (NL * CTC <= 2) ? 4 : 2) void k_contract(Contr
         bq0 = bn0;
         bq1 = bn1;
         __syncthreads();
-        if (kc + 1 < kc_end) {
-            const v4i* src = coef_base + (kc + 1 - a.kc0) * coef_chunk_stride;
+        // prefetches without branches (the loop body stays one scheduling region):
+        // past the end they re-read the last chunk, which is in bounds and unused
+        {
+            const int64_t kn = kc + 1 < kc_end ? kc + 1 : kc_end - 1;
+            const v4i* src = coef_base + (kn - a.kc0) * coef_chunk_stride;
 #pragma unroll
             for (int t = 0; t < CT; ++t) areg[t] = src[t * 256];
-        }
-        if (kc + 2 < kc_end) {
-            bn0 = *reinterpret_cast<const v4i*>(prow0 + (kc + 2) * 32);
-            if (!MISS) bn1 = *reinterpret_cast<const v4i*>(prow1 + (kc + 2) * 32);
+            const int64_t kr = kc + 2 < kc_end ? kc + 2 : kc_end - 1;
+            bn0 = *reinterpret_cast<const v4i*>(prow0 + kr * 32);
+            if (!MISS) bn1 = *reinterpret_cast<const v4i*>(prow1 + kr * 32);
         }
         const int8_t* L = lds[buf];
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-            const uint32_t w0 = (uint32_t)b0[s];
-            v4i op0, op1;
+            // unpack the operands of the next step (step 0 of the next chunk after
+            // step 3) before this step's MFMAs, so the two are independent and interleave
+            const uint32_t wn0 = s < 3 ? (uint32_t)b0[s + 1] : (uint32_t)bq0[0];
+            v4i nx0, nx1;
             if (MISS) {
-                unpack16(w0, op0, op1);
+                unpack16(wn0, nx0, nx1);
             } else {
-                const uint32_t w1 = (uint32_t)b1[s];
-                unpack16_g(w0, op0);
-                unpack16_g(w1, op1);
+                const uint32_t wn1 = s < 3 ? (uint32_t)b1[s + 1] : (uint32_t)bq1[0];
+                unpack16_g(wn0, nx0);
+                unpack16_g(wn1, nx1);
             }
 #pragma unroll
             for (int t = 0; t < CT; ++t) {
@@ -223,6 +236,19 @@ __global__ __launch_bounds__(256, (NL * CTC <= 2) ? 4 : 2) void k_contract(Contr
                 acc[t][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, op0, acc[t][0], 0, 0, 0);
                 acc[t][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, op1, acc[t][1], 0, 0, 0);
             }
+            op0 = nx0;
+            op1 = nx1;
+        }
+        // issue order: every MFMA followed by a couple of VALU (the next step's
+        // unpacking) and one LDS read per MFMA pair, so this wave keeps the VALU
+        // busy while its MFMAs run instead of alternating long bursts of each
+#pragma unroll
+        for (int i = 0; i < 4 * CT; ++i) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // LDS read
+            __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);  // VALU
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+            __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);  // VALU
         }
         buf ^= 1;
     }
@@ -1024,20 +1050,27 @@ void launch_p1_finalize(const int64_t* y, int64_t ldy, int64_t n, int n_bins, in
     GPE_HIP(hipGetLastError());
 }
 
-// Missing-call records for phase 1 with int8 probes (entries -1, 0, +1), one
-// per SNP i of [k0, k1), rec_words 8-byte words each:
-//   [pos_0, neg_0, ..., pos_{W-1}, neg_{W-1}, a_0, ..., a_{B-1}]
-// pos_w / neg_w: bit b set when z[i][64w + b] is > 0 / < 0;
+// Missing-call records for phase 1 with int8 probes (entries -1, 0, +1): for
+// SNP i of [k0, k1), bin k and probe word wd (columns 64wd .. 64wd+63) one
+// 32-byte record at rec[((i - k0) * n_bins + k) * vw + wd]:
+//   { pos bits 0-31, neg bits 0-31, pos bits 32-63, neg bits 32-63, a_k, 0 }
+// pos / neg: bit b set when z[i][64wd + b] is > 0 / < 0 (lane l of a wave
+// reads the pos/neg pair of its half, 8 bytes at offset 8 * (l / 32));
 // a_k = mean_i * (round(w_ik 2^shift_k) * 2^-shift_k).
 // Rounding to nearest even commutes with negation, so the quantized
 // coefficient of column (k, v) is z_iv round(w_ik 2^shift_k) and
 // mean_i * deq(coef) is +-a_k bit for bit.
+struct alignas(32) MissRec {
+    uint32_t pos_lo, neg_lo, pos_hi, neg_hi;
+    double a;
+    uint64_t pad;
+};
+
 __global__ void k_p1_miss_rec(const double* w, int64_t ldw, const int8_t* z, int64_t ldz, int64_t k0, int64_t k1,
                               int n_bins, int v_count, int v_pad, const int32_t* shift, const double* col_scale,
-                              const double* mean, int rec_words, uint64_t* rec) {
+                              const double* mean, MissRec* rec) {
     const int64_t i = k0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= k1) return;
-    uint64_t* r = rec + (i - k0) * rec_words;
     const int vw = (v_pad + 63) / 64;
     for (int wd = 0; wd < vw; ++wd) {
         uint64_t pos = 0, neg = 0;
@@ -1048,95 +1081,109 @@ __global__ void k_p1_miss_rec(const double* w, int64_t ldw, const int8_t* z, int
             if (zz > 0) pos |= 1ULL << b;
             else if (zz < 0) neg |= 1ULL << b;
         }
-        r[2 * wd] = pos;
-        r[2 * wd + 1] = neg;
-    }
-    for (int k = 0; k < n_bins; ++k) {
-        const int c = k * v_pad;
-        const double q = (double)__double2ll_rn(ldexp(w[i * ldw + k], shift[c]));
-        r[2 * vw + k] = (uint64_t)__double_as_longlong(mean[i] * (q * col_scale[c]));
+        for (int k = 0; k < n_bins; ++k) {
+            const int c = k * v_pad;
+            const double q = (double)__double2ll_rn(ldexp(w[i * ldw + k], shift[c]));
+            MissRec r;
+            r.pos_lo = (uint32_t)pos;
+            r.neg_lo = (uint32_t)neg;
+            r.pos_hi = (uint32_t)(pos >> 32);
+            r.neg_hi = (uint32_t)(neg >> 32);
+            r.a = mean[i] * (q * col_scale[c]);
+            r.pad = 0;
+            rec[((i - k0) * n_bins + k) * vw + wd] = r;
+        }
     }
 }
 void launch_p1_miss_rec(const double* w, int64_t ldw, const int8_t* z, int64_t ldz, int64_t k0, int64_t k1,
                         int n_bins, int v_count, int v_pad, const int32_t* shift, const double* col_scale,
-                        const double* mean, int rec_words, uint64_t* rec, hipStream_t st) {
+                        const double* mean, void* rec, hipStream_t st) {
     if (k1 <= k0) return;
     hipLaunchKernelGGL(k_p1_miss_rec, dim3((unsigned)ceil_div(k1 - k0, 256)), dim3(256), 0, st, w, ldw, z, ldz, k0,
-                       k1, n_bins, v_count, v_pad, shift, col_scale, mean, rec_words, rec);
+                       k1, n_bins, v_count, v_pad, shift, col_scale, mean, static_cast<MissRec*>(rec));
     GPE_HIP(hipGetLastError());
 }
 
-// Phase-1 finalize for int8 probes with missing calls: as k_p1_finalize, the
-// missing-call term summed from the records of k_p1_miss_rec.  One wave per
-// sample; each pass covers 64 probe columns of one bin, so the record words a
-// pass reads are the same for every lane (scalar loads).
-__global__ void k_p1_finalize_z8(const int64_t* y, int64_t ldy, int64_t n, int n_bins, int v_count, int v_pad,
-                                 const double* col_scale, const double* cst, const int64_t* miss_ptr,
-                                 const int32_t* miss_idx, int64_t k0, int64_t k1, const uint64_t* rec, int rec_words,
-                                 double* xz, int64_t ldxz, const uint8_t* col_mode) {
-    // the sample, its missing-list range and the loop bounds below are wave-uniform (scalar registers)
-    const int64_t smp = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+// Phase-1 finalize for int8 probes with missing calls: as k_p1_finalize, with
+// the missing-call term sum_{missing i} z_iv a_ik taken from the records of
+// k_p1_miss_rec.  One wave per sample, lanes over 64 probe columns of one bin
+// per pass.  The sample's entries are handled 64 at a time: lane j loads
+// entry j's index and record (the next group's loads are in flight while the
+// current group is summed), the records go to a per-wave LDS slot, and every
+// lane then reads each record back (its own half of the sign words plus the
+// shared a) and adds z * a with one FMA (z in {-1, 0, +1}, exact).
+constexpr int kMissGroup = 64;
+
+__global__ __launch_bounds__(256) void k_p1_finalize_z8(
+    const int64_t* __restrict__ y, int64_t ldy, int64_t n, int n_bins, int v_count, int v_pad,
+    const double* __restrict__ col_scale, const double* __restrict__ cst, const int64_t* __restrict__ miss_ptr,
+    const int32_t* __restrict__ miss_idx, int64_t k0, int64_t ra, int64_t rb, const MissRec* __restrict__ rec,
+    double* __restrict__ xz, int64_t ldxz, const uint8_t* __restrict__ col_mode) {
+    __shared__ MissRec stage[4][2][kMissGroup];
+    // this launch adds the missing calls at SNPs [ra, rb); the one with ra == k0
+    // also applies Y 2^-shift - Cst and the bins' overwrite/accumulate modes
+    const bool first = ra == k0;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t smp = (int64_t)blockIdx.x * 4 + wave;
     const int lane = threadIdx.x & 63;
     if (smp >= n) return;
     const int64_t b0 = miss_ptr[smp], b1 = miss_ptr[smp + 1];
     int64_t lo = b0, hi = b1;
     while (lo < hi) {
         const int64_t mid = (lo + hi) >> 1;
-        if (miss_idx[mid] < k0) lo = mid + 1;
+        if (miss_idx[mid] < ra) lo = mid + 1;
         else hi = mid;
     }
     const int64_t mb = lo;
     hi = b1;
     while (lo < hi) {
         const int64_t mid = (lo + hi) >> 1;
-        if (miss_idx[mid] < k1) lo = mid + 1;
+        if (miss_idx[mid] < rb) lo = mid + 1;
         else hi = mid;
     }
     const int64_t me = lo;
     const int vw = (v_pad + 63) / 64;
+    const int64_t recs_per_snp = (int64_t)n_bins * vw;
+    const int half = lane >> 5;
+    const uint32_t lbit = (uint32_t)(lane & 31);
     for (int k = 0; k < n_bins; ++k) {
         for (int wd = 0; wd < vw; ++wd) {
             const int v = wd * 64 + lane;
             const bool valid = v < v_count;
             const int c = k * v_pad + v;
-            double val = valid ? (double)y[(int64_t)c * ldy + smp] * col_scale[c] - cst[c] : 0.0;
+            double val = (valid && first) ? (double)y[(int64_t)c * ldy + smp] * col_scale[c] - cst[c] : 0.0;
+            const MissRec* rk = rec + (int64_t)k * vw + wd;
             double acc = 0.0;
-            for (int64_t p0 = mb; p0 < me; p0 += 64) {
-                const int cnt = (me - p0) < 64 ? (int)(me - p0) : 64;
-                const int32_t myidx = (p0 + lane < me) ? miss_idx[p0 + lane] : (int32_t)k0;
-                int j = 0;
-                // groups of 8 entries: all record loads of a group are issued before the first is used
-                for (; j + 8 <= cnt; j += 8) {
-                    uint64_t pos[8], neg[8];
-                    double av[8];
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) {
-                        const int64_t i = __builtin_amdgcn_readlane(myidx, j + u);
-                        const uint64_t* r = rec + (i - k0) * rec_words;
-                        pos[u] = r[2 * wd];
-                        neg[u] = r[2 * wd + 1];
-                        av[u] = __longlong_as_double((long long)r[2 * vw + k]);
-                    }
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) {
-                        const bool bp = (pos[u] >> lane) & 1ULL;
-                        const bool bn = (neg[u] >> lane) & 1ULL;
-                        acc += bp ? av[u] : (bn ? -av[u] : 0.0);
-                    }
+            // group g covers entries [mb + 64g, mb + 64g + 64); lane j holds entry j's index / record
+            // a lane past the end reads the record of SNP k0 (in bounds) and stages z = 0 for it
+            int32_t idx_next = (mb + lane < me) ? miss_idx[mb + lane] : (int32_t)k0;
+            v4u cur_w = *reinterpret_cast<const v4u*>(rk + ((int64_t)idx_next - k0) * recs_per_snp);
+            double cur_a = rk[((int64_t)idx_next - k0) * recs_per_snp].a;
+            idx_next = (mb + kMissGroup + lane < me) ? miss_idx[mb + kMissGroup + lane] : (int32_t)k0;
+            int buf = 0;
+            for (int64_t p0 = mb; p0 < me; p0 += kMissGroup) {
+                *reinterpret_cast<v4u*>(&stage[wave][buf][lane]) = cur_w;
+                stage[wave][buf][lane].a = cur_a;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                const int64_t pn = p0 + kMissGroup;
+                cur_w = *reinterpret_cast<const v4u*>(rk + ((int64_t)idx_next - k0) * recs_per_snp);
+                cur_a = rk[((int64_t)idx_next - k0) * recs_per_snp].a;
+                idx_next = (pn + kMissGroup + lane < me) ? miss_idx[pn + kMissGroup + lane] : (int32_t)k0;
+                const int cnt = (me - p0) < kMissGroup ? (int)(me - p0) : kMissGroup;
+                const MissRec* st = stage[wave][buf];
+#pragma unroll 8
+                for (int j = 0; j < cnt; ++j) {
+                    const uint2 pnw = *reinterpret_cast<const uint2*>(reinterpret_cast<const char*>(st + j) + 8 * half);
+                    const int zs = (int)((pnw.x >> lbit) & 1u) - (int)((pnw.y >> lbit) & 1u);
+                    acc = fma(st[j].a, (double)zs, acc);
                 }
-                for (; j < cnt; ++j) {
-                    const int64_t i = __builtin_amdgcn_readlane(myidx, j);
-                    const uint64_t* r = rec + (i - k0) * rec_words;
-                    const uint64_t pos = r[2 * wd], neg = r[2 * wd + 1];
-                    const double a = __longlong_as_double((long long)r[2 * vw + k]);
-                    const bool bp = (pos >> lane) & 1ULL;
-                    const bool bn = (neg >> lane) & 1ULL;
-                    acc += bp ? a : (bn ? -a : 0.0);
-                }
+                buf ^= 1;
             }
             val += acc;
             if (valid) {
-                const int mode = col_mode ? col_mode[k] : 2;
+                const int mode = (col_mode && first) ? col_mode[k] : 2;
                 double* dst = xz + (int64_t)(k * v_count + v) * ldxz + smp;
                 if (mode == 1) *dst = val;
                 else *dst += val;
@@ -1146,13 +1193,22 @@ __global__ void k_p1_finalize_z8(const int64_t* y, int64_t ldy, int64_t n, int n
 }
 void launch_p1_finalize_z8(const int64_t* y, int64_t ldy, int64_t n, int n_bins, int v_count, int v_pad,
                            const double* col_scale, const double* cst, const int64_t* miss_ptr,
-                           const int32_t* miss_idx, int64_t k0, int64_t k1, const uint64_t* rec, int rec_words,
-                           double* xz, int64_t ldxz, const uint8_t* col_mode, hipStream_t st) {
+                           const int32_t* miss_idx, int64_t k0, int64_t k1, const void* rec, double* xz,
+                           int64_t ldxz, const uint8_t* col_mode, hipStream_t st) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_p1_finalize_z8, dim3((unsigned)ceil_div(n, 4)), dim3(256), 0, st, y, ldy, n, n_bins,
-                       v_count, v_pad, col_scale, cst, miss_ptr, miss_idx, k0, k1, rec, rec_words, xz, ldxz,
-                       col_mode);
-    GPE_HIP(hipGetLastError());
+    const int vw = (int)ceil_div(v_pad, 64);
+    const int64_t rec_stride = (int64_t)n_bins * vw * (int64_t)sizeof(MissRec);
+    // SNP ranges whose records (about 2 MiB) stay resident in every XCD's L2
+    // while all samples walk them: one launch per range, in order
+    const int64_t range = std::max<int64_t>(4096, ((int64_t)2 << 20) / rec_stride);
+    for (int64_t ra = k0; ra < k1 || ra == k0; ra += range) {
+        const int64_t rb = std::min(k1, ra + range);
+        hipLaunchKernelGGL(k_p1_finalize_z8, dim3((unsigned)ceil_div(n, 4)), dim3(256), 0, st, y, ldy, n, n_bins,
+                           v_count, v_pad, col_scale, cst, miss_ptr, miss_idx, k0, ra, rb,
+                           static_cast<const MissRec*>(rec), xz, ldxz, col_mode);
+        GPE_HIP(hipGetLastError());
+        if (rb >= k1) break;
+    }
 }
 
 // column max |x| of a column-major FP64 matrix whose column bin*V+v maps to padded column bin*v_pad+v
