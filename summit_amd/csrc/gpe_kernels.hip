@@ -119,22 +119,37 @@ __device__ __forceinline__ int64_t wave_sum_i64(int64_t v) {
 // Workgroup: 4 waves.  Each wave owns 32 packed rows (MISS: genotype and
 // missing planes of the same rows) or 64 rows (two genotype tiles), and all
 // CT = NL*CTC coefficient tiles of one column group.  Per K chunk of 128
-// values the workgroup stages CT*4 KiB of swizzled coefficients in LDS
-// (double-buffered; next chunk prefetched into registers while the current
-// one is contracted), each lane loads 16 packed bytes (64 codes) of its row.
+// values the workgroup's CT*4 KiB of swizzled coefficients arrive in LDS by
+// direct global->LDS loads (three buffers, issued two chunks ahead, no
+// registers), and each lane loads 16 packed bytes (64 codes) of its row into
+// registers, also two chunks ahead.
 // MFMA lane layout used here: operand lane l holds row (l & 31) and 16
 // consecutive K values of half (l >> 5); result register e of lane l holds
 // (row 8*(e>>2) + 4*(l>>5) + (e&3), column l & 31).
 // ---------------------------------------------------------------------------
+constexpr uint32_t waitcnt_vm(uint32_t n) {  // s_waitcnt vmcnt(n), other counters unconstrained
+    return (n & 15u) | ((n >> 4) << 14) | (7u << 4) | (15u << 8);
+}
+
+template <int CT>
+__device__ __forceinline__ void coef_chunk_to_lds(const int8_t* src_chunk, int8_t* dst, int tid, int wave) {
+#pragma unroll
+    for (int t = 0; t < CT; ++t)
+        __builtin_amdgcn_global_load_lds(
+            (const __attribute__((address_space(1))) void*)(src_chunk + (t * 256 + tid) * 16),
+            (__attribute__((address_space(3))) void*)(dst + (t * 256 + wave * 64) * 16), 16, 0, 0);
+}
+
 template <int NL, int CTC, bool MISS, int EPI>
 __global__ __launch_bounds__(256, (NL * CTC <= 2) ? 4 : 2) void k_contract(ContractArgs a) {
     constexpr int CT = NL * CTC;
     constexpr int RPW = MISS ? 32 : 64;
     constexpr int RPB = 4 * RPW;
-    __shared__ __attribute__((aligned(16))) int8_t lds[2][CT * 4096];
+    constexpr int G = MISS ? 1 : 2;  // packed-row loads per lane per chunk
+    __shared__ __attribute__((aligned(16))) int8_t lds[3][CT * 4096];
 
     const int tid = threadIdx.x;
-    const int wave = tid >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int lane = tid & 63;
     const int half = lane >> 5;
     const int lr = lane & 31;
@@ -165,68 +180,58 @@ __global__ __launch_bounds__(256, (NL * CTC <= 2) ? 4 : 2) void k_contract(Contr
 
     const uint8_t* prow0 = a.packed + (rbase + lr) * a.stride + half * 16;
     const uint8_t* prow1 = MISS ? prow0 : prow0 + 32 * a.stride;
+    const int8_t* coef_grp = a.coef + (int64_t)cg * CT * 4096;
+    const int64_t coef_chunk_bytes = a.coef_ctiles * 4096;
 
-    // coefficient chunk kc+1 is prefetched into registers, packed rows two chunks ahead
-    v4i areg[CT];
-    v4i bq0 = v4i{}, bq1 = v4i{};  // chunk kc+1, tile 0/1
-    v4i bn0 = v4i{}, bn1 = v4i{};  // chunk kc+2, tile 0/1
-
-    // (plain unrolled loops, no lambdas: the prefetch registers must stay in VGPRs)
-    const v4i* coef_base =
-        reinterpret_cast<const v4i*>(a.coef + ((int64_t)cg * CT) * 4096) + tid;
-    const int64_t coef_chunk_stride = a.coef_ctiles * 256;  // uint4 per K chunk
-
+    // packed words of chunks kc (g0), kc+1 (g1); plain unrolled code, no lambdas,
+    // so these stay in VGPRs.  Prefetches past the end re-read the last chunk
+    // (in bounds, never used).
+    v4i g0a = v4i{}, g0b = v4i{}, g1a = v4i{}, g1b = v4i{};
     if (kc_begin < kc_end) {
-        const v4i* src = coef_base + (kc_begin - a.kc0) * coef_chunk_stride;
-#pragma unroll
-        for (int t = 0; t < CT; ++t) areg[t] = src[t * 256];
-        bq0 = *reinterpret_cast<const v4i*>(prow0 + kc_begin * 32);
-        if (!MISS) bq1 = *reinterpret_cast<const v4i*>(prow1 + kc_begin * 32);
-    }
-    if (kc_begin + 1 < kc_end) {
-        bn0 = *reinterpret_cast<const v4i*>(prow0 + (kc_begin + 1) * 32);
-        if (!MISS) bn1 = *reinterpret_cast<const v4i*>(prow1 + (kc_begin + 1) * 32);
+        const int64_t k1 = kc_begin + 1 < kc_end ? kc_begin + 1 : kc_end - 1;
+        coef_chunk_to_lds<CT>(coef_grp + (kc_begin - a.kc0) * coef_chunk_bytes, lds[0], tid, wave);
+        g0a = *reinterpret_cast<const v4i*>(prow0 + kc_begin * 32);
+        if (!MISS) g0b = *reinterpret_cast<const v4i*>(prow1 + kc_begin * 32);
+        coef_chunk_to_lds<CT>(coef_grp + (k1 - a.kc0) * coef_chunk_bytes, lds[1], tid, wave);
+        g1a = *reinterpret_cast<const v4i*>(prow0 + k1 * 32);
+        if (!MISS) g1b = *reinterpret_cast<const v4i*>(prow1 + k1 * 32);
     }
     // operands of step 0 of the first chunk; later ones are unpacked one step ahead
     v4i op0, op1;
     if (MISS) {
-        unpack16((uint32_t)bq0[0], op0, op1);
+        unpack16((uint32_t)g0a[0], op0, op1);
     } else {
-        unpack16_g((uint32_t)bq0[0], op0);
-        unpack16_g((uint32_t)bq1[0], op1);
+        unpack16_g((uint32_t)g0a[0], op0);
+        unpack16_g((uint32_t)g0b[0], op1);
     }
-    int buf = 0;
+    int buf = 0;  // LDS buffer of chunk kc
     for (int64_t kc = kc_begin; kc < kc_end; ++kc) {
-        v4i* dst = reinterpret_cast<v4i*>(lds[buf]);
-#pragma unroll
-        for (int t = 0; t < CT; ++t) dst[t * 256 + tid] = areg[t];
-        const v4i b0 = bq0;
-        const v4i b1 = bq1;
-        bq0 = bn0;
-        bq1 = bn1;
-        __syncthreads();
-        // prefetches without branches (the loop body stays one scheduling region):
-        // past the end they re-read the last chunk, which is in bounds and unused
-        {
-            const int64_t kn = kc + 1 < kc_end ? kc + 1 : kc_end - 1;
-            const v4i* src = coef_base + (kn - a.kc0) * coef_chunk_stride;
-#pragma unroll
-            for (int t = 0; t < CT; ++t) areg[t] = src[t * 256];
-            const int64_t kr = kc + 2 < kc_end ? kc + 2 : kc_end - 1;
-            bn0 = *reinterpret_cast<const v4i*>(prow0 + kr * 32);
-            if (!MISS) bn1 = *reinterpret_cast<const v4i*>(prow1 + kr * 32);
-        }
+        // chunk kc's coefficients (and packed words) have landed once only the
+        // CT + G loads issued for chunk kc+1 may still be in flight; the barrier
+        // then publishes every wave's part of the chunk
+        __builtin_amdgcn_s_waitcnt(waitcnt_vm(CT + G));
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        // chunk kc+2 into the buffer read two chunks ago (k2 passes through an empty
+        // asm so these loads cannot be hoisted above the wait and barrier)
+        int64_t k2 = kc + 2 < kc_end ? kc + 2 : kc_end - 1;
+        asm volatile("" : "+s"(k2));
+        const int buf2 = buf == 0 ? 2 : buf - 1;
+        coef_chunk_to_lds<CT>(coef_grp + (k2 - a.kc0) * coef_chunk_bytes, lds[buf2], tid, wave);
+        const v4i g2a = *reinterpret_cast<const v4i*>(prow0 + k2 * 32);
+        const v4i g2b = MISS ? v4i{} : *reinterpret_cast<const v4i*>(prow1 + k2 * 32);
         const int8_t* L = lds[buf];
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             // unpack the operands of the next step (step 0 of the next chunk after
             // step 3) before this step's MFMAs, so the two are independent and interleave
-            const uint32_t wn0 = s < 3 ? (uint32_t)b0[s + 1] : (uint32_t)bq0[0];
+            const uint32_t wn0 = s < 3 ? (uint32_t)g0a[s + 1] : (uint32_t)g1a[0];
             v4i nx0, nx1;
             if (MISS) {
                 unpack16(wn0, nx0, nx1);
             } else {
-                const uint32_t wn1 = s < 3 ? (uint32_t)b1[s + 1] : (uint32_t)bq1[0];
+                const uint32_t wn1 = s < 3 ? (uint32_t)g0b[s + 1] : (uint32_t)g1b[0];
                 unpack16_g(wn0, nx0);
                 unpack16_g(wn1, nx1);
             }
@@ -239,19 +244,14 @@ __global__ __launch_bounds__(256, (NL * CTC <= 2) ? 4 : 2) void k_contract(Contr
             op0 = nx0;
             op1 = nx1;
         }
-        // issue order: every MFMA followed by a couple of VALU (the next step's
-        // unpacking) and one LDS read per MFMA pair, so this wave keeps the VALU
-        // busy while its MFMAs run instead of alternating long bursts of each
-#pragma unroll
-        for (int i = 0; i < 4 * CT; ++i) {
-            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
-            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // LDS read
-            __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);  // VALU
-            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
-            __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);  // VALU
-        }
-        buf ^= 1;
+        g0a = g1a;
+        g0b = g1b;
+        g1a = g2a;
+        g1b = g2b;
+        buf = buf == 2 ? 0 : buf + 1;
     }
+    // no LDS-load may still be writing when the workgroup ends
+    __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
 
     // ------------------------------------------------------------- epilogues
     if constexpr (EPI == EPI_I32_ATOMIC) {
