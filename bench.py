@@ -103,6 +103,139 @@ def cpu_baseline(geno, n_snps_sample: int, b: int, seed: int, step: int = 1000) 
                       f"{dt:.1f} s"}
 
 
+def wall_to_one_percent(engine, ldscore, torch, dev) -> dict:
+    """Wall time for the plain GPU estimator to reach a mean relative LD-score
+    error <= 1% at N=5,000, M=10,000 (i.i.d. genotypes, MAF ~ U(0.05, 0.5), 1%
+    missing, seeds 101 / 1101).  The exact target L_j = sum_i r_ij^2 - M/(N-1)
+    (r with the residual scales of the estimator) comes from a dense FP64
+    X X^T on the GPU.  Probe counts double from 16,384 until the error bar is
+    met; the reported time is genotype preparation plus the run that met it."""
+    N, M, root = 5000, 10000, 1101
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g = engine.Genotype.synthetic(N, M, seed=101, maf_lo=0.05, maf_hi=0.5, miss_rate=0.01, block_len=100,
+                                  copy_prob=0.0)
+    g.prepare(ddof=1, impute="mean", impute_seed=ldscore.impute_seed_from_root(root))
+    torch.cuda.synchronize()
+    t_prep = time.perf_counter() - t0
+    # exact target from the raw codes and the engine's own standardization parameters
+    st = g.stats()
+    codes = torch.from_numpy(g.download("snp_raw")).to(dev)
+    sh = torch.tensor([0, 2, 4, 6], device=dev, dtype=torch.uint8)
+    c = ((codes[:, :, None] >> sh) & 3).reshape(M, -1)[:, :N]
+    val = torch.tensor([0.0, 0.0, 1.0, 2.0], device=dev, dtype=torch.float64)[c.long()]
+    obs = (c != 1).to(torch.float64)
+    mu = torch.from_numpy(st["mean"]).to(dev)[:, None]
+    isd = torch.from_numpy(st["inv_sd"]).to(dev)[:, None]
+    x = (val - mu) * isd * obs
+    inv = engine.resvar_inv(g, 0, M)
+    xs = x * inv[:, None]
+    r = (xs @ xs.T) / float(N - 1)
+    exact = ((r * r).sum(dim=1) - M / float(N - 1)).cpu().numpy()
+    del codes, c, val, obs, x, xs, r
+    tried = []
+    out = None
+    for B in (16384, 32768, 65536, 131072, 262144):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        res = ldscore.genomewide_ldscore(g, nvecs=B, seed=root, vtiles=ldscore.bounded_vtile_sizes(B, 8192))
+        t_run = time.perf_counter() - t1
+        err = float(np.mean(np.abs(res.ldscore[:, 0] - exact) / np.maximum(np.abs(exact), 1e-12)))
+        tried.append({"probes": B, "seconds": round(t_run, 4), "mean_rel_err": round(err, 5)})
+        log(f"C1 plain estimator B={B}: {t_run:.3f} s, mean rel err {err:.4f}")
+        if err <= 0.01:
+            out = {"probes": B, "seconds": round(t_prep + t_run, 4), "estimator_seconds": round(t_run, 4),
+                   "mean_rel_err": round(err, 5)}
+            break
+    g.close()
+    return {"workload": "N=5000 M=10000 i.i.d. genotypes, 1% missing, Rademacher, plain estimator, "
+                        "exact dense FP64 target", "reached": out, "tried": tried}
+
+
+def windowed_rate(engine, torch) -> dict:
+    """Exact windowed LD scores, N=50,000, M=300,000 on 22 chromosomes (block-Markov
+    genotypes, seed 404), SNP positions from a Poisson process with mean spacing
+    3 kb, +-1 Mb windows: one banded-Gram call per chromosome.  Pairs = N x sum of
+    window sizes (each SNP with every SNP of its window, itself included)."""
+    from summit_amd import winldcore
+    N, M, n_chr = 50_000, 300_000, 22
+    g = engine.Genotype.synthetic(N, M, seed=404, maf_lo=0.01, maf_hi=0.5, miss_rate=0.005, block_len=100,
+                                  copy_prob=0.9)
+    g.prepare(ddof=1, impute="mean")
+    rng = np.random.default_rng(1404)
+    bounds = np.linspace(0, M, n_chr + 1).astype(np.int64)
+    chrs = []
+    pairs = 0
+    for c in range(n_chr):
+        s0, s1 = int(bounds[c]), int(bounds[c + 1])
+        bp = np.cumsum(rng.exponential(3000.0, s1 - s0)).astype(np.int64) + 1
+        lo = np.searchsorted(bp, bp - 1_000_000, side="left")
+        hi = np.searchsorted(bp, bp + 1_000_000, side="right")
+        pairs += int((hi - lo).sum()) * N
+        chrs.append((s0, s1, bp))
+    winldcore.windowed_ld(g, chrs[0][0], chrs[0][1], bp=chrs[0][2], ld_wind_kb=1000.0)  # warm-up
+    torch.cuda.synchronize()
+    engine.profile_enable(True)
+    engine.profile_reset()
+    t0 = time.perf_counter()
+    for s0, s1, bp in chrs:
+        winldcore.windowed_ld(g, s0, s1, bp=bp, ld_wind_kb=1000.0)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    gram_ms, gram_n = engine.profile_read("gram")
+    engine.profile_enable(False)
+    g.close()
+    return {"workload": "N=50000 M=300000, 22 chromosomes, Poisson positions (3 kb), +-1 Mb windows",
+            "seconds": round(t, 4), "pairs": pairs, "gpairs_per_s": round(pairs / t / 1e9, 1),
+            "gram_kernel_ms": round(gram_ms, 3), "gram_launches": gram_n}
+
+
+def gxe_rate(engine, torch) -> dict:
+    """SUMMIT-GxE directional scores at N=100,000, M=500,000, B=64 for K=8
+    environments (6 standardised N(0,1), 2 Bernoulli(0.3)), block-Markov genotypes
+    (seed 303), environments and probes from root seed 1303.  Pairs per
+    environment: 2B sources (N*M*2B) plus two target traversals (N*M*4B)."""
+    from summit_amd import gxe
+    N, M, B, K = 100_000, 500_000, 64, 8
+    g = engine.Genotype.synthetic(N, M, seed=303, maf_lo=0.01, maf_hi=0.5, miss_rate=0.005, block_len=100,
+                                  copy_prob=0.9)
+    g.prepare(ddof=1, impute="mean")
+    rng = np.random.default_rng(1303)
+    envs = np.empty((N, K))
+    for k in range(6):
+        e = rng.standard_normal(N)
+        envs[:, k] = (e - e.mean()) / e.std()
+    envs[:, 6:] = (rng.random((N, 2)) < 0.3).astype(np.float64)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    gxe.gxe_ldscore_multi(g, envs, None, nvecs=B, seed=1303)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    g.close()
+    pairs = 6 * K * B * N * M
+    return {"workload": "N=100000 M=500000 B=64 K=8 environments, no covariates", "seconds": round(t, 3),
+            "pairs": pairs, "gpairs_per_s": round(pairs / t / 1e9, 1)}
+
+
+def ingest_rate(engine, torch, geno, n_snps: int) -> dict:
+    """Host -> HBM rate of the BED path: the first n_snps packed SNP rows of the
+    benchmark genotypes, copied to host memory, then uploaded (pinned,
+    double-buffered staging) and prepared (statistics, transpose, missing lists)."""
+    body = geno.download_rows("snp_raw", 0, n_snps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g = engine.Genotype.from_bed_bytes(body, geno.n, n_snps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    g.prepare(ddof=1, impute="mean")
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    g.close()
+    return {"bytes": int(body.nbytes), "upload_s": round(t1 - t0, 4), "prepare_s": round(t2 - t1, 4),
+            "upload_gbs": round(body.nbytes / (t1 - t0) / 1e9, 2),
+            "upload_prepare_gbs": round(body.nbytes / (t2 - t0) / 1e9, 2)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +249,8 @@ def main() -> int:
     ap.add_argument("--cpu-snps", type=int, default=4000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-exact", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the 1%%-error wall-time run and the host->HBM ingest measurement")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a plain 1 GiB read + 1 GiB write elementwise kernel three times (the reference "
                          "point that scales FETCH_SIZE / WRITE_SIZE counter readings to bytes)")
@@ -196,8 +331,13 @@ def main() -> int:
                 traffic = json.load(f).get(f"N={N},M={M},B={B},limbs={args.limbs},impute={args.impute}")
         except Exception:
             traffic = None
+    # int8 MFMA multiply-adds the kernel issues per algorithmic genotype-probe pair: one per limb of
+    # the real-valued operand, doubled in phase 2 when the missing-call plane is contracted too
+    miss_plane = args.impute == "mean" and geno.n_missing() > 0
+    per_pair = args.limbs * (2 if (kname.startswith("k_contract phase 2") and miss_plane) else 1)
     roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / INT8_PEAK_TOPS, 4), "traffic": traffic, "kernel": kname,
+                "int8_macs_per_pair": per_pair, "mfma_pipe_frac": round(achieved * per_pair / INT8_PEAK_TOPS, 4),
                 "avg_ms": round(avg_ms, 3),
                 "phase_ms": {"p1_contract": round(p1_ms / max(p1_n, 1), 3),
                              "p2_contract": round(p2_ms / max(p2_n, 1), 3)}}
@@ -228,14 +368,25 @@ def main() -> int:
             "gtu_tops": round(2.0 * N * M * B / (gtu_avg / 1000.0) / 1e12, 1)}
         del u, w, z
 
+    if not args.no_extras:
+        extra["ingest"] = ingest_rate(engine, torch, geno, min(M, 100_000))
+        log(f"ingest: {extra['ingest']}")
+
     cpu = None
     if not args.no_cpu:
         try:
             cpu = cpu_baseline(geno, min(args.cpu_snps, M), B, seed)
         except Exception as e:  # the baseline is reported, never the metric
             log(f"cpu baseline failed: {e}")
+    geno.close()
+    if not args.no_extras:
+        extra["wall_to_1pct"] = wall_to_one_percent(engine, ldscore, torch, dev)
+        extra["windowed"] = windowed_rate(engine, torch)
+        log(f"windowed: {extra['windowed']}")
+        extra["gxe"] = gxe_rate(engine, torch)
+        log(f"gxe: {extra['gxe']}")
     line = {
-        "metric": "genotype·probe Gop/s at N=100k,M=500k,B=64",
+        "metric": "genotype\u00b7probe Gop/s at N=100k,M=500k,B=64; wall time to 1% LD-score rel err",
         "value": round(value, 1),
         "unit": "Gop/s",
         "n_gpus": 1,
@@ -256,7 +407,6 @@ def main() -> int:
         **extra,
     }
     print(json.dumps(line), flush=True)
-    geno.close()
     return 0
 
 

@@ -222,6 +222,23 @@ struct ProfScope {
     }
 };
 
+// memcpy split over n_threads host threads (page-cache or mmap reads of a BED
+// body are far slower from one thread than the PCIe copy that follows)
+void parallel_memcpy(uint8_t* dst, const uint8_t* src, int64_t bytes, int n_threads) {
+    if (n_threads <= 1 || bytes < ((int64_t)4 << 20)) {
+        std::memcpy(dst, src, (size_t)bytes);
+        return;
+    }
+    std::vector<std::thread> pool;
+    const int64_t part = ceil_div(bytes, n_threads);
+    for (int t = 0; t < n_threads; ++t) {
+        const int64_t b0 = t * part, b1 = std::min(bytes, b0 + part);
+        if (b0 >= b1) break;
+        pool.emplace_back([=] { std::memcpy(dst + b0, src + b0, (size_t)(b1 - b0)); });
+    }
+    for (auto& th : pool) th.join();
+}
+
 // K splits of a contraction whose partial sums are added atomically.  Equal-work
 // workgroups finish in rounds of `slots` (the device's resident capacity), so the
 // split count is the smallest one whose last round is at least 90% full (the best
@@ -367,32 +384,52 @@ int gpe_geno_upload_bed(gpe_geno* g, const uint8_t* bed_host, int64_t n_total, i
             rows_dev = static_cast<int64_t*>(g->scratch("upload_rows", g->n * sizeof(int64_t)));
             GPE_HIP(hipMemcpyAsync(rows_dev, rows_host, g->n * sizeof(int64_t), hipMemcpyHostToDevice, st));
         }
-        // stream through a pinned staging buffer in chunks of SNP rows
+        // Stream through two pinned staging buffers in chunks of SNP rows: while the
+        // device copies and scatters chunk j, host threads fill the other buffer
+        // with chunk j+1.  An event per buffer marks when its last device use ended.
         const int64_t chunk_rows = std::max<int64_t>(1, (int64_t)(64 << 20) / std::max<int64_t>(host_stride, 1));
-        uint8_t* pinned = nullptr;
-        GPE_HIP(hipHostMalloc(reinterpret_cast<void**>(&pinned), chunk_rows * host_stride, hipHostMallocDefault));
-        uint8_t* staging = static_cast<uint8_t*>(g->scratch("upload_staging", chunk_rows * host_stride));
+        const int64_t chunk_bytes = chunk_rows * host_stride;
+        uint8_t* pinned[2] = {nullptr, nullptr};
+        hipEvent_t done[2] = {nullptr, nullptr};
+        auto release = [&] {
+            for (int b = 0; b < 2; ++b) {
+                if (done[b]) (void)hipEventDestroy(done[b]);
+                if (pinned[b]) (void)hipHostFree(pinned[b]);
+            }
+        };
         try {
-            for (int64_t j0 = 0; j0 < snp_count; j0 += chunk_rows) {
+            for (int b = 0; b < 2; ++b) {
+                GPE_HIP(hipHostMalloc(reinterpret_cast<void**>(&pinned[b]), chunk_bytes, hipHostMallocDefault));
+                GPE_HIP(hipEventCreateWithFlags(&done[b], hipEventDisableTiming));
+            }
+            uint8_t* staging[2] = {static_cast<uint8_t*>(g->scratch("upload_staging0", chunk_bytes)),
+                                   static_cast<uint8_t*>(g->scratch("upload_staging1", chunk_bytes))};
+            const int n_threads = (int)std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+            int64_t j = 0;
+            for (int64_t j0 = 0; j0 < snp_count; j0 += chunk_rows, ++j) {
+                const int b = (int)(j & 1);
                 const int64_t rows_now = std::min(chunk_rows, snp_count - j0);
-                GPE_HIP(hipStreamSynchronize(st));
-                std::memcpy(pinned, bed_host + j0 * host_stride, rows_now * host_stride);
-                GPE_HIP(hipMemcpyAsync(staging, pinned, rows_now * host_stride, hipMemcpyHostToDevice, st));
+                const int64_t bytes = rows_now * host_stride;
+                if (j >= 2) GPE_HIP(hipEventSynchronize(done[b]));
+                parallel_memcpy(pinned[b], bed_host + j0 * host_stride, bytes, n_threads);
+                GPE_HIP(hipMemcpyAsync(staging[b], pinned[b], bytes, hipMemcpyHostToDevice, st));
                 uint8_t* dst = g->snp_raw + (snp_begin + j0) * g->snp_stride;
                 if (rows_dev) {
-                    launch_select_rows(staging, host_stride, rows_now, rows_dev, g->n, dst, g->snp_stride, st);
+                    launch_select_rows(staging[b], host_stride, rows_now, rows_dev, g->n, dst, g->snp_stride, st);
                 } else {
-                    GPE_HIP(hipMemcpy2DAsync(dst, g->snp_stride, staging, host_stride, ceil_div(n_total, 4),
+                    GPE_HIP(hipMemcpy2DAsync(dst, g->snp_stride, staging[b], host_stride, ceil_div(n_total, 4),
                                              rows_now, hipMemcpyDeviceToDevice, st));
                 }
                 launch_clear_tail(dst, rows_now, g->snp_stride, g->n, st);
+                GPE_HIP(hipEventRecord(done[b], st));
             }
             GPE_HIP(hipStreamSynchronize(st));
         } catch (...) {
-            (void)hipHostFree(pinned);
+            (void)hipStreamSynchronize(st);
+            release();
             throw;
         }
-        GPE_HIP(hipHostFree(pinned));
+        release();
         free_prepared(g);
     });
 }

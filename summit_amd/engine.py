@@ -67,7 +67,10 @@ class Genotype:
             rows_arr = np.ascontiguousarray(np.asarray(rows, dtype=np.int64))
             n = int(rows_arr.size)
         g = cls(n, n_snps)
-        chunk = max(1, (256 << 20) // max(stride, 1))
+        # a C-contiguous body (file mapping or array) goes down in one call, which
+        # streams it through double-buffered pinned staging; other layouts are
+        # made contiguous 256 MB at a time
+        chunk = n_snps if body[:n_snps].flags.c_contiguous else max(1, (256 << 20) // max(stride, 1))
         for j0 in range(0, n_snps, chunk):
             j1 = min(n_snps, j0 + chunk)
             part = np.ascontiguousarray(body[j0:j1])
@@ -289,7 +292,7 @@ def probes(root_seed: int, blk_starts, blk_end: int, v_start: int, v_count: int,
     return out
 
 
-PROF_KINDS = {"p1_contract": 0, "p2_contract": 1, "gz": 2, "gtu": 3}
+PROF_KINDS = {"p1_contract": 0, "p2_contract": 1, "gz": 2, "gtu": 3, "gram": 4}
 
 
 def profile_enable(on: bool = True) -> None:

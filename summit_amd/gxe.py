@@ -154,12 +154,108 @@ def gxe_ldscore(g: engine.Genotype, env, C_int=None, annot=None, nvecs: int = 8,
             "inv_x": inv_x.cpu().numpy(), "inv_w": inv_w.cpu().numpy(), "df": df}
 
 
-def gxe_ldscore_multi(g: engine.Genotype, envs, C_ints=None, **kw):
-    """K environments: one directional score set per environment column (envs (N, K));
-    C_ints is one basis shared by every environment or a list of per-environment bases."""
+def _features_multi(g: engine.Genotype, envs_d: torch.Tensor, Cs, eps_var: float, limbs: int):
+    """inv_x, inv_w (E, M) and df (E,) for E environments from ONE phase-2 traversal
+    against every environment's [C_k, 1, C_k.e_k, e_k] rows plus one weighted sum
+    of squares per environment."""
+    N = g.n
+    ones = torch.ones(N, dtype=torch.float64, device=envs_d.device)
+    rows, layout = [], []
+    for k in range(envs_d.shape[1]):
+        e = envs_d[:, k]
+        C = Cs[k]
+        p = 0 if C is None else int(C.shape[1])
+        df = float(N - p - 1)
+        if df <= 0:
+            raise ValueError(f"Residual correlation degrees of freedom are non-positive: nsamp={N}, p_eff={p}, "
+                             f"df={df}.")
+        start = sum(r.shape[0] for r in rows)
+        if p:
+            rows.append(C.T)
+        rows.append(ones[None, :])
+        if p:
+            rows.append((C * e[:, None]).T)
+        rows.append(e[None, :])
+        layout.append((start, p, df))
+    T = engine.xtv(g, torch.cat(rows, dim=0).contiguous(), limbs=limbs)
+    ssg = weighted_sumsq(g, ones)
+    inv_x, inv_w, dfs = [], [], []
+    for k, (o, p, df) in enumerate(layout):
+        e = envs_d[:, k]
+        sseg = weighted_sumsq(g, e * e)
+        cg, one_g = T[:, o:o + p], T[:, o + p]
+        ceg, e_g = T[:, o + p + 1:o + 2 * p + 1], T[:, o + 2 * p + 1]
+        varx = (ssg - (cg * cg).sum(dim=1) - one_g * one_g / N) / df
+        varw = (sseg - (ceg * ceg).sum(dim=1) - e_g * e_g / N) / df
+        bad = ~(torch.isfinite(varx) & (varx > eps_var) & torch.isfinite(varw) & (varw > eps_var))
+        if bool(bad.any()):
+            raise ValueError(f"{int(bad.sum())} variant feature(s) have zero or invalid projected variance. GxE "
+                             "kernel normalization cannot retain zero columns; QC/remove these variants.")
+        inv_x.append(1.0 / torch.sqrt(varx))
+        inv_w.append(1.0 / torch.sqrt(varw))
+        dfs.append(df)
+    return torch.stack(inv_x), torch.stack(inv_w), dfs
+
+
+def gxe_ldscore_multi(g: engine.Genotype, envs, C_ints=None, annot=None, nvecs: int = 8, seed: int = 1,
+                      step_size: int = 1000, vtiles=None, probe_offset: int = 0, limbs: int = 5,
+                      eps_var: float = 1e-10):
+    """E environments (envs (N, E)) served together, as the reference's
+    MultiEnvironmentKernel does (src/native/gxeldcore.cpp:10281-13197): per probe
+    tile one phase-1 traversal produces the x- and w-sources of every
+    environment (2*K*E bins), and two phase-2 traversals score every
+    environment's targets (columns of all environments side by side; each
+    environment's residual scale and df^2 applied to its own bins afterwards).
+    C_ints is one basis shared by every environment or a list of per-environment
+    bases.  Returns one score dict per environment, as gxe_ldscore."""
+    dev = engine.default_device()
+    M, N = g.m, g.n
     envs = np.asarray(envs, dtype=np.float64)
+    if envs.ndim != 2 or envs.shape[0] != N:
+        raise ValueError("envs must be (N, E)")
+    E = envs.shape[1]
+    envs_d = torch.as_tensor(envs, device=dev)
+    Cs = [C_ints[k] if isinstance(C_ints, (list, tuple)) else C_ints for k in range(E)]
+    Cs = [None if C is None else torch.as_tensor(np.asarray(C, dtype=np.float64), device=dev) for C in Cs]
+    if annot is None:
+        annot = np.ones((M, 1))
+    annot = np.asarray(annot, dtype=np.float64)
+    K = annot.shape[1]
+    inv_x, inv_w, dfs = _features_multi(g, envs_d, Cs, eps_var, max(limbs, 6))
+    a_pos = torch.as_tensor(np.maximum(annot, 0.0), device=dev)
+    annot_all = torch.cat([torch.cat([a_pos * (inv_x[k] * inv_x[k])[:, None], a_pos * (inv_w[k] * inv_w[k])[:, None]],
+                                     dim=1) for k in range(E)], dim=1).contiguous()
+    ones_m = torch.ones(M, dtype=torch.float64, device=dev)
+    vtiles = [nvecs] if vtiles is None else [int(v) for v in vtiles]
+    acc_a = torch.zeros((M, 2 * K * E), dtype=torch.float64, device=dev)
+    acc_b = torch.zeros((M, 2 * K * E), dtype=torch.float64, device=dev)
+    v0 = 0
+    for Vt in vtiles:
+        kt = K * Vt
+        z = probes_device(seed, 0, M, step_size, [probe_offset + v0 + v for v in range(Vt)], dev)
+        y = torch.zeros((2 * kt * E, N), dtype=torch.float64, device=dev)
+        engine.call("gpe_phase1_xz_z8", g.handle, 0, M, engine._ptr(annot_all), 2 * K * E, engine._ptr(ones_m),
+                    engine._ptr(z), Vt, Vt, engine._ptr(y), N, None, limbs, engine._stream())
+        s = torch.empty_like(y)
+        es = torch.empty_like(y)
+        for k in range(E):
+            b = 2 * kt * k
+            e = envs_d[:, k][None, :]
+            s[b:b + kt] = _project(y[b:b + kt], Cs[k])
+            s[b + kt:b + 2 * kt] = _project(y[b + kt:b + 2 * kt] * e, Cs[k])
+            es[b:b + 2 * kt] = s[b:b + 2 * kt] * e
+        # n_denom 2: the (n_denom - 1)^2 divisor is 1; scales and df^2 are applied per environment below
+        engine.phase2_accum(g, 0, M, ones_m, Vt, s, acc_a, 2, extreme_warn=0.0, limbs=limbs)
+        engine.phase2_accum(g, 0, M, ones_m, Vt, es, acc_b, 2, extreme_warn=0.0, limbs=limbs)
+        v0 += Vt
+    torch.cuda.synchronize()
     out = []
-    for k in range(envs.shape[1]):
-        C = C_ints[k] if isinstance(C_ints, (list, tuple)) else C_ints
-        out.append(gxe_ldscore(g, envs[:, k], C, **kw))
+    for k in range(E):
+        cols = slice(2 * K * k, 2 * K * (k + 1))
+        sx = (inv_x[k] * inv_x[k])[:, None] / (dfs[k] * dfs[k] * float(nvecs))
+        sw = (inv_w[k] * inv_w[k])[:, None] / (dfs[k] * dfs[k] * float(nvecs))
+        a = (acc_a[:, cols] * sx).cpu().numpy()
+        b = (acc_b[:, cols] * sw).cpu().numpy()
+        out.append({"xx": a[:, :K], "xw": a[:, K:], "wx": b[:, :K], "ww": b[:, K:],
+                    "inv_x": inv_x[k].cpu().numpy(), "inv_w": inv_w[k].cpu().numpy(), "df": dfs[k]})
     return out

@@ -42,3 +42,31 @@ def test_gxe_scores_match_oracle(oracle, with_cov):
     for k in ("xx", "xw", "wx", "ww"):
         np.testing.assert_allclose(got[k], ref[k], rtol=1e-10, err_msg=k)
     g.close()
+
+
+@pytest.mark.parametrize("per_env_cov", [False, True])
+def test_gxe_multi_environment_matches_oracle(oracle, per_env_cov):
+    """Several environments served by one set of traversals (the reference's
+    MultiEnvironmentKernel) against the oracle run environment by environment."""
+    from summit_amd import engine, gxe
+    m, n, nvecs, step = 520, 380, 10, 200
+    codes = random_codes(m, n, seed=93, miss_rate=0.01)
+    rng = np.random.default_rng(8)
+    envs = np.column_stack([rng.standard_normal(n), rng.standard_normal(n), (rng.random(n) < 0.3).astype(float)])
+    envs[:, :2] = (envs[:, :2] - envs[:, :2].mean(axis=0)) / envs[:, :2].std(axis=0)
+    raw = np.column_stack([rng.standard_normal(n), rng.binomial(1, 0.4, n)]).astype(float)
+    raw -= raw.mean(axis=0)
+    C, _ = np.linalg.qr(raw)
+    Cs = [C, None, C[:, :1]] if per_env_cov else C
+    annot = np.column_stack([np.ones(m), (rng.random(m) < 0.25).astype(float)])
+    g = engine.Genotype.from_bed_bytes(oracle.pack_codes(codes), n, m).prepare(impute="mean")
+    got = gxe.gxe_ldscore_multi(g, envs, Cs, annot, nvecs=nvecs, seed=77, step_size=step, vtiles=[6, 4])
+    for k in range(envs.shape[1]):
+        Ck = Cs[k] if per_env_cov else C
+        ref = oracle.gxe_scores(oracle.pack_codes(codes), n, m, envs[:, k], Ck, annot, nvecs=nvecs, seed=77,
+                                step_size=step, vtiles=[6, 4])
+        np.testing.assert_allclose(got[k]["inv_x"], ref["inv_x"], rtol=1e-12)
+        np.testing.assert_allclose(got[k]["inv_w"], ref["inv_w"], rtol=1e-12)
+        for name in ("xx", "xw", "wx", "ww"):
+            np.testing.assert_allclose(got[k][name], ref[name], rtol=1e-10, err_msg=f"env {k} {name}")
+    g.close()
