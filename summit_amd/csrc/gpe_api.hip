@@ -728,9 +728,12 @@ void run_phase1(gpe_geno* g, int64_t s, int64_t e, const double* w, int n_bins, 
     const int64_t kc0 = s / kKChunk, kc1 = ceil_div(e, kKChunk);
     const int64_t kcount = kc1 - kc0;
     int8_t* coef = static_cast<int8_t*>(g->scratch("coef", kcount * coef_ctiles * 4096));
-    launch_fill_u8(reinterpret_cast<uint8_t*>(coef), kcount * coef_ctiles * 4096, 0, st);
+    // the quantizer writes every (value, column) of the first ncols columns; only
+    // padding column tiles need clearing
+    if (ncols_pad != ncols) launch_fill_u8(reinterpret_cast<uint8_t*>(coef), kcount * coef_ctiles * 4096, 0, st);
+    double* cst_part = static_cast<double*>(g->scratch("cst_part", (size_t)ncols_pad * kCstParts * sizeof(double)));
     launch_coef_p1_quant(w, n_bins, z_i8, z_f64, ldz, s, e, n_bins, v_count, v_pad, shift, lc.nl, lc.ctc,
-                         coef_ctiles, kc0, coef, g->mean, cst, st);
+                         coef_ctiles, kc0, coef, g->mean, cst, cst_part, st);
     const int64_t ldy = g->n_alloc;
     int64_t* y = static_cast<int64_t*>(g->scratch("p1_y", (int64_t)ncols_pad * ldy * sizeof(int64_t)));
     launch_fill_u64(reinterpret_cast<uint64_t*>(y), (int64_t)ncols_pad * ldy, 0, st);
@@ -790,7 +793,7 @@ void run_phase2(gpe_geno* g, int64_t s, int64_t e, const double* x, int64_t ldx,
     const int prec = std::min(8 * lc.nl - 2, 62 - bitlen((uint64_t)(2 * g->n)));
     launch_shift_from_colmax(colmax, ncols_pad, prec, shift, col_scale, st);
     int8_t* coef = static_cast<int8_t*>(g->scratch("coef", kcount * coef_ctiles * 4096));
-    launch_fill_u8(reinterpret_cast<uint8_t*>(coef), kcount * coef_ctiles * 4096, 0, st);
+    if (ncols_pad != ncols) launch_fill_u8(reinterpret_cast<uint8_t*>(coef), kcount * coef_ctiles * 4096, 0, st);
     launch_quant_p2(x, ldx, g->n, n_bins, v_count, v_pad, shift, lc.nl, lc.ctc, coef_ctiles, coef, col_sum, st);
     a.packed = g->snp_work;
     a.stride = g->snp_stride;

@@ -128,7 +128,9 @@ void launch_coef_p1_colmax(const double* w, int64_t ldw, const int8_t* z_i8, con
 void launch_coef_p1_quant(const double* w, int64_t ldw, const int8_t* z_i8, const double* z_f64, int64_t ldz,
                           int64_t k0, int64_t k1, int n_bins, int v_count, int v_pad, const int32_t* shift,
                           int nl, int ctc, int64_t coef_ctiles, int64_t kc0, int8_t* coef,
-                          const double* mean, double* cst, hipStream_t st);
+                          const double* mean, double* cst, double* cst_part, hipStream_t st);
+// SNP slices of the Cst reduction (cst_part holds n_cols * kCstParts doubles)
+constexpr int kCstParts = 128;
 void launch_p1_finalize(const int64_t* y, int64_t ldy, int64_t n, int n_bins, int v_count, int v_pad,
                         const double* col_scale, const double* cst, const int64_t* miss_ptr,
                         const int32_t* miss_idx, int64_t k0, int64_t k1, const double* w, int64_t ldw,

@@ -927,17 +927,22 @@ __global__ void k_coef_p1_quant(const double* w, int64_t ldw, const int8_t* z_i8
     }
 }
 
-// Cst_c = sum_i mean_i * deq(coef_ic), one deterministic block reduction per column
-__global__ void k_coef_p1_cst(const double* w, int64_t ldw, const int8_t* z_i8, const double* z_f64, int64_t ldz,
-                              int64_t k0, int64_t k1, int v_count, int v_pad, const int32_t* shift,
-                              const double* mean, double* cst) {
+// Cst_c = sum_i mean_i * deq(coef_ic) in two deterministic passes: blockIdx.y
+// = p sums its SNP slice into part[c * P + p], then k_coef_p1_cst_final adds
+// the P partials of each column in a fixed order.
+__global__ void k_coef_p1_cst_part(const double* w, int64_t ldw, const int8_t* z_i8, const double* z_f64, int64_t ldz,
+                                   int64_t k0, int64_t k1, int v_count, int v_pad, const int32_t* shift,
+                                   const double* mean, int64_t slice, double* part) {
     __shared__ double red[4];
     const int c = blockIdx.x;
+    const int P = gridDim.y;
     const int k = c / v_pad, v = c % v_pad;
+    const int64_t i0 = k0 + (int64_t)blockIdx.y * slice;
+    const int64_t i1 = i0 + slice < k1 ? i0 + slice : k1;
     double acc = 0.0;
     if (v < v_count) {
         const int sh = shift[c];
-        for (int64_t i = k0 + threadIdx.x; i < k1; i += blockDim.x) {
+        for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
             const double q = (double)__double2ll_rn(ldexp(p1_coef(w, ldw, z_i8, z_f64, ldz, i, k, v), sh));
             acc += mean[i] * q;
         }
@@ -945,13 +950,22 @@ __global__ void k_coef_p1_cst(const double* w, int64_t ldw, const int8_t* z_i8, 
     acc = wave_sum_f64(acc);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
     __syncthreads();
-    if (threadIdx.x == 0) cst[c] = v < v_count ? ldexp((red[0] + red[1]) + (red[2] + red[3]), -shift[c]) : 0.0;
+    if (threadIdx.x == 0) part[(int64_t)c * P + blockIdx.y] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+__global__ void k_coef_p1_cst_final(const double* part, int P, int v_count, int v_pad, const int32_t* shift,
+                                    double* cst) {
+    const int c = blockIdx.x;
+    const int v = c % v_pad;
+    double acc = 0.0;
+    for (int p = threadIdx.x; p < P; p += 64) acc += part[(int64_t)c * P + p];
+    acc = wave_sum_f64(acc);
+    if (threadIdx.x == 0) cst[c] = v < v_count ? ldexp(acc, -shift[c]) : 0.0;
 }
 
 void launch_coef_p1_quant(const double* w, int64_t ldw, const int8_t* z_i8, const double* z_f64, int64_t ldz,
                           int64_t k0, int64_t k1, int n_bins, int v_count, int v_pad, const int32_t* shift, int nl,
                           int ctc, int64_t coef_ctiles, int64_t kc0, int8_t* coef, const double* mean, double* cst,
-                          hipStream_t st) {
+                          double* cst_part, hipStream_t st) {
     const int64_t kc1 = ceil_div(k1, kKChunk);
     const int64_t kcount = kc1 - kc0;
     const int ncols = n_bins * v_pad;
@@ -968,8 +982,13 @@ void launch_coef_p1_quant(const double* w, int64_t ldw, const int8_t* z_i8, cons
         GPE_HIP(hipGetLastError());
     }
     if (cst) {
-        hipLaunchKernelGGL(k_coef_p1_cst, dim3((unsigned)ncols), dim3(256), 0, st, w, ldw, z_i8, z_f64, ldz, k0, k1,
-                           v_count, v_pad, shift, mean, cst);
+        const int64_t slice = std::max<int64_t>(4096, ceil_div(k1 - k0, kCstParts));
+        const int P = (int)std::max<int64_t>(1, ceil_div(k1 - k0, slice));
+        hipLaunchKernelGGL(k_coef_p1_cst_part, dim3((unsigned)ncols, (unsigned)P), dim3(256), 0, st, w, ldw, z_i8,
+                           z_f64, ldz, k0, k1, v_count, v_pad, shift, mean, slice, cst_part);
+        GPE_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_coef_p1_cst_final, dim3((unsigned)ncols), dim3(64), 0, st, cst_part, P, v_count, v_pad,
+                           shift, cst);
         GPE_HIP(hipGetLastError());
     }
 }

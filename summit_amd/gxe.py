@@ -43,13 +43,64 @@ def make_seed(root: int, block: int, probe_id: int) -> int:
     return s & MASK64
 
 
+_U32 = np.uint64(0xFFFFFFFF)
+
+
+def _mix64_vec(x: np.ndarray) -> np.ndarray:
+    """_mix64 on a uint64 array (wrap-around arithmetic)."""
+    x = x + np.uint64(0x9E3779B97F4A7C15)
+    x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return x ^ (x >> np.uint64(31))
+
+
+def seed_sequence_philox_keys(seeds: np.ndarray) -> np.ndarray:
+    """(n, 2) uint64: the key np.random.Philox(seed) takes from its SeedSequence
+    (entropy = the seed's little-endian 32-bit words, pool of 4 words hashed and
+    cross-mixed, then generate_state(2, uint64)) -- NumPy's published SeedSequence
+    algorithm, vectorized over seeds (numpy/random/bit_generator.pyx)."""
+    seeds = np.asarray(seeds, dtype=np.uint64).reshape(-1)
+    n = seeds.size
+    lo = seeds & _U32
+    hi = seeds >> np.uint64(32)
+    hc = np.full(n, 0x43B0D7E5, dtype=np.uint64)
+
+    def hashmix(v):
+        nonlocal hc
+        v = (v ^ hc) & _U32
+        hc = (hc * np.uint64(0x931E8875)) & _U32
+        v = (v * hc) & _U32
+        return v ^ (v >> np.uint64(16))
+
+    def mix(x, y):
+        r = (np.uint64(0xCA01F9DD) * x - np.uint64(0x4973F715) * y) & _U32
+        return r ^ (r >> np.uint64(16))
+
+    zero = np.zeros(n, dtype=np.uint64)
+    # a seed below 2^32 is one entropy word, otherwise two; the pool's other words hash zeros
+    pool = [hashmix(lo), hashmix(np.where(hi > 0, hi, zero)), hashmix(zero), hashmix(zero)]
+    for src in range(4):
+        for dst in range(4):
+            if src != dst:
+                pool[dst] = mix(pool[dst], hashmix(pool[src]))
+    hb = np.full(n, 0x8B51F9DD, dtype=np.uint64)
+    words = []
+    for i in range(4):
+        v = pool[i] ^ hb
+        hb = (hb * np.uint64(0x58F38DED)) & _U32
+        v = (v * hb) & _U32
+        words.append(v ^ (v >> np.uint64(16)))
+    return np.stack([words[0] | (words[1] << np.uint64(32)), words[2] | (words[3] << np.uint64(32))], axis=1)
+
+
 def philox_keys(root: int, blk_starts, probe_ids) -> np.ndarray:
     """Philox keys NumPy derives for Philox(make_seed(root, block, probe)), (blocks*V*2,) uint64."""
-    keys = np.empty((len(blk_starts), len(probe_ids), 2), dtype=np.uint64)
-    for b, s in enumerate(blk_starts):
-        for v, pid in enumerate(probe_ids):
-            keys[b, v] = np.random.Philox(make_seed(root, int(s), int(pid))).state["state"]["key"]
-    return np.ascontiguousarray(keys.reshape(-1))
+    starts = np.asarray(blk_starts, dtype=np.int64).astype(np.uint64)
+    pids = np.asarray(probe_ids, dtype=np.int64).astype(np.uint64)
+    root_mix = _mix64_vec(np.array([int(root) & MASK64], dtype=np.uint64))[0]
+    seeds = (np.uint64(0x1234ABCD) ^ root_mix ^ _mix64_vec(starts + np.uint64(0x9E37))[:, None]
+             ^ _mix64_vec(pids + np.uint64(0x85EB))[None, :])
+    return np.ascontiguousarray(seed_sequence_philox_keys(seeds).reshape(-1))
 
 
 def probes_device(root: int, snp_begin: int, snp_end: int, step_size: int, probe_ids,

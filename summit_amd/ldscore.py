@@ -86,3 +86,32 @@ def genomewide_ldscore(geno: engine.Genotype, nvecs: int, seed: int, annot: np.n
     ld = meansq_h / float(nvecs) - (nsnps_bin / float(N - ddof))[None, :]
     return LDScoreResult(ld, meansq_h, inv.cpu().numpy(), int(nvecs), nsnps_bin,
                          None if mc_ps is None else mc_ps.cpu().numpy(), tuple(guards))
+
+
+def write_gw_ldscore_table(outpath, snpdf, scores_df, *, snp_var=None, ld_rel=None) -> str:
+    """``<outpath>.gw.ldscore.gz``: the SNP columns (CHR SNP BP), V when given, one
+    column per annotation and LD_REL_<column> when given; tab-separated, six
+    decimals, gzip level 6 (src/summit/ldscore/gw_ldscore.py:1186-1200).
+    ``snpdf`` and ``scores_df`` share a row index."""
+    import pandas as pd
+
+    parts = [snpdf]
+    if snp_var is not None:
+        parts.append(pd.DataFrame({"V": snp_var}))
+    parts.append(scores_df)
+    if ld_rel is not None:
+        parts.append(pd.DataFrame(ld_rel, columns=[f"LD_REL_{c}" for c in scores_df.columns]))
+    table = pd.concat(parts, axis=1)
+    path = f"{outpath}.gw.ldscore.gz"
+    table.to_csv(path, index=False, compression={"method": "gzip", "compresslevel": 6}, sep="\t",
+                 float_format="%.6f")
+    return path
+
+
+def write_gw_annotation_masses(outpath, masses) -> str:
+    """``<outpath>.gw.M``: annotation masses on one tab-separated line, %.10g
+    (src/summit/ldscore/gw_ldscore.py:1203-1208)."""
+    path = f"{outpath}.gw.M"
+    with open(path, "w") as f:
+        f.write("\t".join(f"{float(x):.10g}" for x in masses) + "\n")
+    return path

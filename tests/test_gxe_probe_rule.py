@@ -28,3 +28,19 @@ def test_philox_keys_are_numpys():
     k = gxe.philox_keys(99, [0, 10], [3, 4])
     assert k.shape == (8,)
     np.testing.assert_array_equal(k[:2], np.random.Philox(gxe.make_seed(99, 0, 3)).state["state"]["key"])
+
+
+def test_vectorized_philox_keys_match_numpy():
+    """The vectorized SeedSequence key derivation against NumPy's own Philox seeding,
+    over make_seed(root, block, probe) seeds and edge-case seeds (0, 2^32-1, 2^32, 2^64-1)."""
+    import numpy as np
+    from summit_amd import gxe
+    starts, pids = [0, 1000, 7000, 123456], [0, 1, 5, 63, 1000]
+    got = gxe.philox_keys(1303, starts, pids).reshape(len(starts), len(pids), 2)
+    for b, s in enumerate(starts):
+        for v, p in enumerate(pids):
+            ref = np.random.Philox(gxe.make_seed(1303, s, p)).state["state"]["key"]
+            np.testing.assert_array_equal(got[b, v], ref)
+    edge = np.array([0, 1, 2**32 - 1, 2**32, 2**64 - 1, 12345678901234567], dtype=np.uint64)
+    ref = np.array([np.random.Philox(int(x)).state["state"]["key"] for x in edge])
+    np.testing.assert_array_equal(gxe.seed_sequence_philox_keys(edge), ref)
