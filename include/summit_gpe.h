@@ -205,11 +205,13 @@ int gpe_phase1_xz_z8(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const doub
                      const double* inv_dev, const int8_t* z_dev, int64_t ldz, int32_t v_count, double* xz_dev,
                      int64_t ldxz, const uint8_t* bin_mode_dev, int32_t limbs, void* stream);
 
-/* out[j - snp_begin] = sum_n w_n x_jn^2 (standardized working codes): the
- * ||e . x_j||^2 feature moment of the GxE interaction column
+/* out[e * ldo + (j - snp_begin)] = sum_n w[e * ldw + n] x_jn^2 for each of
+ * n_weights weight vectors (standardized working codes): the ||e . x_j||^2
+ * feature moments of the GxE interaction columns, every environment in one
+ * pass over the genotypes per 16 weight vectors
  * (src/summit/ldscore/gwe_ldscore.py:2349-2353). */
-int gpe_weighted_sumsq(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const double* w_dev, double* out_dev,
-                       void* stream);
+int gpe_weighted_sumsq(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const double* w_dev, int32_t n_weights,
+                       int64_t ldw, double* out_dev, int64_t ldo, void* stream);
 
 /* Probe values of the reference's genome-wide streams for blocks starting at
  * blk_starts_host[0..n_blocks) (last block ends at blk_end): Rademacher on the

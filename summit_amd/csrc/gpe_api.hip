@@ -760,8 +760,7 @@ void run_phase1(gpe_geno* g, int64_t s, int64_t e, const double* w, int n_bins, 
     ps.stop();
     const bool miss = g->impute_mode == GPE_IMPUTE_MEAN && g->n_missing > 0;
     if (miss && z_i8) {
-        const int64_t vw = ceil_div(v_pad, 64);
-        void* rec = g->scratch("p1_mrec", L * n_bins * vw * 32);
+        void* rec = g->scratch("p1_mrec", p1_miss_rec_bytes(L, n_bins, v_pad));
         launch_p1_miss_rec(w, n_bins, z_i8, ldz, s, e, n_bins, v_count, v_pad, shift, col_scale, g->mean, rec, st);
         launch_p1_finalize_z8(y, ldy, g->n, n_bins, v_count, v_pad, col_scale, cst, g->miss_ptr, g->miss_idx, s, e,
                               rec, out, ldo, bin_mode, st);
@@ -816,6 +815,16 @@ void run_phase2(gpe_geno* g, int64_t s, int64_t e, const double* x, int64_t ldx,
     ps.stop();
     if (ctiles_used_out) *ctiles_used_out = ncols / 32;
     if (v_pad_out) *v_pad_out = v_pad;
+}
+
+// W_sm[s][e] = w[e][s] for s < n and e < n_w, 0 elsewhere: (n_pad, ec) sample-major
+__global__ void k_weights_sample_major(const double* w, int64_t ldw, int n_w, int64_t n, int64_t n_pad, int ec,
+                                       double* wsm) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n_pad * ec) return;
+    const int64_t s = idx / ec;
+    const int e = (int)(idx % ec);
+    wsm[idx] = (s < n && e < n_w) ? w[(int64_t)e * ldw + s] : 0.0;
 }
 
 __global__ void k_w_from_inv_sd(const double* inv_sd, const int32_t* nobs, const int64_t* sum, const int64_t* sumsq,
@@ -988,13 +997,25 @@ int gpe_phase1_xz_z8(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const doub
     });
 }
 
-int gpe_weighted_sumsq(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const double* w_dev, double* out_dev,
-                       void* stream) {
+int gpe_weighted_sumsq(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const double* w_dev, int32_t n_weights,
+                       int64_t ldw, double* out_dev, int64_t ldo, void* stream) {
     return guarded([&] {
         require_prepared(g);
         require(snp_begin >= 0 && snp_end <= g->m && snp_begin <= snp_end, "SNP range out of bounds");
-        launch_weighted_sumsq(g->snp_work, g->snp_stride, snp_begin, snp_end, g->n, g->mean, g->inv_sd, w_dev,
-                              out_dev, S(stream));
+        require(n_weights >= 0 && ldw >= g->n && ldo >= snp_end - snp_begin, "bad weighted-sum-of-squares shape");
+        if (n_weights == 0 || snp_end == snp_begin) return;
+        hipStream_t st = S(stream);
+        const int64_t n_pad = round_up(ceil_div(g->n, 4), 16) * 4;
+        for (int e0 = 0; e0 < n_weights; e0 += 16) {
+            const int nw = std::min(16, n_weights - e0);
+            const int ec = nw <= 8 ? 8 : 16;
+            double* wsm = static_cast<double*>(g->scratch("wss_w", (size_t)n_pad * 16 * sizeof(double)));
+            hipLaunchKernelGGL(k_weights_sample_major, dim3((unsigned)ceil_div(n_pad * ec, 256)), dim3(256), 0, st,
+                               w_dev + (int64_t)e0 * ldw, ldw, nw, g->n, n_pad, ec, wsm);
+            GPE_HIP(hipGetLastError());
+            launch_weighted_sumsq(g->snp_work, g->snp_stride, snp_begin, snp_end, g->n, g->mean, g->inv_sd, wsm, ec,
+                                  nw, out_dev + (int64_t)e0 * ldo, ldo, st);
+        }
     });
 }
 

@@ -139,7 +139,8 @@ void launch_p1_finalize(const int64_t* y, int64_t ldy, int64_t n, int n_bins, in
 void launch_p1_miss_rec(const double* w, int64_t ldw, const int8_t* z, int64_t ldz, int64_t k0, int64_t k1,
                         int n_bins, int v_count, int v_pad, const int32_t* shift, const double* col_scale,
                         const double* mean, void* rec, hipStream_t st);
-// records of launch_p1_miss_rec: 32 bytes per (SNP, bin, 64 probe columns)
+// bytes of launch_p1_miss_rec records for n_snps SNPs
+int64_t p1_miss_rec_bytes(int64_t n_snps, int n_bins, int v_pad);
 void launch_p1_finalize_z8(const int64_t* y, int64_t ldy, int64_t n, int n_bins, int v_count, int v_pad,
                            const double* col_scale, const double* cst, const int64_t* miss_ptr,
                            const int32_t* miss_idx, int64_t k0, int64_t k1, const void* rec, double* xz,
@@ -167,7 +168,8 @@ void launch_gram_band(const GramArgs& a, int64_t n_pairs, bool miss, hipStream_t
 void launch_philox_rademacher(const uint64_t* keys, const int64_t* starts, const int64_t* lens, int n_blocks,
                               int v_count, int64_t max_len, int64_t row0, int8_t* z, int64_t ldz, hipStream_t st);
 void launch_weighted_sumsq(const uint8_t* snp, int64_t stride, int64_t r0, int64_t r1, int64_t n, const double* mean,
-                           const double* inv_sd, const double* w, double* out, hipStream_t st);
+                           const double* inv_sd, const double* W_sample_major, int ec, int n_w, double* out,
+                           int64_t ldo, hipStream_t st);
 void launch_window_scale(const int32_t* nobs, const int64_t* sum, const int64_t* sumsq, int64_t m, int64_t n,
                          int completed, double* scale, hipStream_t st);
 void launch_resvar_inv(const int32_t* nobs, const int64_t* sum, const int64_t* sumsq, const double* mean,

@@ -1070,27 +1070,25 @@ void launch_p1_finalize(const int64_t* y, int64_t ldy, int64_t n, int n_bins, in
 }
 
 // Missing-call records for phase 1 with int8 probes (entries -1, 0, +1): for
-// SNP i of [k0, k1), bin k and probe word wd (columns 64wd .. 64wd+63) one
-// 32-byte record at rec[((i - k0) * n_bins + k) * vw + wd]:
-//   { pos bits 0-31, neg bits 0-31, pos bits 32-63, neg bits 32-63, a_k, 0 }
-// pos / neg: bit b set when z[i][64wd + b] is > 0 / < 0 (lane l of a wave
-// reads the pos/neg pair of its half, 8 bytes at offset 8 * (l / 32));
+// SNP i of [k0, k1) and probe word wd (columns 64wd .. 64wd+63) one record of
+// rec_bytes = round_up(16 + 8 n_bins, 16) bytes at byte ((i - k0) vw + wd) rec_bytes:
+//   uint32 { pos bits 0-31, neg bits 0-31, pos bits 32-63, neg bits 32-63 },
+//   then double a_k for every bin k,
+// pos / neg: bit b set when z[i][64wd + b] is > 0 / < 0 (lane l of a wave reads
+// the pos/neg pair of its half, 8 bytes at offset 8 (l / 32));
 // a_k = mean_i * (round(w_ik 2^shift_k) * 2^-shift_k).
 // Rounding to nearest even commutes with negation, so the quantized
 // coefficient of column (k, v) is z_iv round(w_ik 2^shift_k) and
-// mean_i * deq(coef) is +-a_k bit for bit.
-struct alignas(32) MissRec {
-    uint32_t pos_lo, neg_lo, pos_hi, neg_hi;
-    double a;
-    uint64_t pad;
-};
+// mean_i * deq(coef) is z_iv a_k bit for bit.
+__host__ __device__ constexpr int64_t miss_rec_bytes(int n_bins) { return ((16 + 8 * (int64_t)n_bins + 15) / 16) * 16; }
 
 __global__ void k_p1_miss_rec(const double* w, int64_t ldw, const int8_t* z, int64_t ldz, int64_t k0, int64_t k1,
                               int n_bins, int v_count, int v_pad, const int32_t* shift, const double* col_scale,
-                              const double* mean, MissRec* rec) {
+                              const double* mean, uint8_t* rec) {
     const int64_t i = k0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= k1) return;
     const int vw = (v_pad + 63) / 64;
+    const int64_t rb = miss_rec_bytes(n_bins);
     for (int wd = 0; wd < vw; ++wd) {
         uint64_t pos = 0, neg = 0;
         for (int b = 0; b < 64; ++b) {
@@ -1100,18 +1098,19 @@ __global__ void k_p1_miss_rec(const double* w, int64_t ldw, const int8_t* z, int
             if (zz > 0) pos |= 1ULL << b;
             else if (zz < 0) neg |= 1ULL << b;
         }
+        uint8_t* r = rec + ((i - k0) * vw + wd) * rb;
+        uint32_t* hdr = reinterpret_cast<uint32_t*>(r);
+        hdr[0] = (uint32_t)pos;
+        hdr[1] = (uint32_t)neg;
+        hdr[2] = (uint32_t)(pos >> 32);
+        hdr[3] = (uint32_t)(neg >> 32);
+        double* a = reinterpret_cast<double*>(r + 16);
         for (int k = 0; k < n_bins; ++k) {
             const int c = k * v_pad;
             const double q = (double)__double2ll_rn(ldexp(w[i * ldw + k], shift[c]));
-            MissRec r;
-            r.pos_lo = (uint32_t)pos;
-            r.neg_lo = (uint32_t)neg;
-            r.pos_hi = (uint32_t)(pos >> 32);
-            r.neg_hi = (uint32_t)(neg >> 32);
-            r.a = mean[i] * (q * col_scale[c]);
-            r.pad = 0;
-            rec[((i - k0) * n_bins + k) * vw + wd] = r;
+            a[k] = mean[i] * (q * col_scale[c]);
         }
+        for (int64_t b = 16 + 8 * (int64_t)n_bins; b < rb; ++b) r[b] = 0;
     }
 }
 void launch_p1_miss_rec(const double* w, int64_t ldw, const int8_t* z, int64_t ldz, int64_t k0, int64_t k1,
@@ -1119,26 +1118,32 @@ void launch_p1_miss_rec(const double* w, int64_t ldw, const int8_t* z, int64_t l
                         const double* mean, void* rec, hipStream_t st) {
     if (k1 <= k0) return;
     hipLaunchKernelGGL(k_p1_miss_rec, dim3((unsigned)ceil_div(k1 - k0, 256)), dim3(256), 0, st, w, ldw, z, ldz, k0,
-                       k1, n_bins, v_count, v_pad, shift, col_scale, mean, static_cast<MissRec*>(rec));
+                       k1, n_bins, v_count, v_pad, shift, col_scale, mean, static_cast<uint8_t*>(rec));
     GPE_HIP(hipGetLastError());
+}
+int64_t p1_miss_rec_bytes(int64_t n_snps, int n_bins, int v_pad) {
+    return n_snps * ceil_div(v_pad, 64) * miss_rec_bytes(n_bins);
 }
 
 // Phase-1 finalize for int8 probes with missing calls: as k_p1_finalize, with
 // the missing-call term sum_{missing i} z_iv a_ik taken from the records of
-// k_p1_miss_rec.  One wave per sample, lanes over 64 probe columns of one bin
-// per pass.  The sample's entries are handled 64 at a time: lane j loads
-// entry j's index and record (the next group's loads are in flight while the
-// current group is summed), the records go to a per-wave LDS slot, and every
-// lane then reads each record back (its own half of the sign words plus the
-// shared a) and adds z * a with one FMA (z in {-1, 0, +1}, exact).
+// k_p1_miss_rec for KB bins [kb0, kb0 + KB) at once.  One wave per sample, lanes
+// over 64 probe columns per pass.  The sample's entries are handled 64 at a
+// time: lane j loads entry j's record (the next group's loads are in flight
+// while the current group is summed) and stages it in the wave's LDS slot;
+// every lane then reads each record back (its own half of the sign words once,
+// the KB shared a_k) and adds z * a_k with one FMA per bin (z in {-1, 0, +1}, exact).
 constexpr int kMissGroup = 64;
 
+template <int KB>
 __global__ __launch_bounds__(256) void k_p1_finalize_z8(
-    const int64_t* __restrict__ y, int64_t ldy, int64_t n, int n_bins, int v_count, int v_pad,
+    const int64_t* __restrict__ y, int64_t ldy, int64_t n, int n_bins, int kb0, int v_count, int v_pad,
     const double* __restrict__ col_scale, const double* __restrict__ cst, const int64_t* __restrict__ miss_ptr,
-    const int32_t* __restrict__ miss_idx, int64_t k0, int64_t ra, int64_t rb, const MissRec* __restrict__ rec,
+    const int32_t* __restrict__ miss_idx, int64_t k0, int64_t ra, int64_t rb, const uint8_t* __restrict__ rec,
     double* __restrict__ xz, int64_t ldxz, const uint8_t* __restrict__ col_mode) {
-    __shared__ MissRec stage[4][2][kMissGroup];
+    constexpr int kStage = ((16 + 8 * KB + 15) / 16) * 16;  // staged bytes per entry
+    constexpr int kPieces = kStage / 16;
+    __shared__ __attribute__((aligned(16))) uint8_t stage[4][kMissGroup * kStage];
     // this launch adds the missing calls at SNPs [ra, rb); the one with ra == k0
     // also applies Y 2^-shift - Cst and the bins' overwrite/accumulate modes
     const bool first = ra == k0;
@@ -1162,70 +1167,113 @@ __global__ __launch_bounds__(256) void k_p1_finalize_z8(
     }
     const int64_t me = lo;
     const int vw = (v_pad + 63) / 64;
-    const int64_t recs_per_snp = (int64_t)n_bins * vw;
+    const int64_t rbytes = miss_rec_bytes(n_bins);
     const int half = lane >> 5;
     const uint32_t lbit = (uint32_t)(lane & 31);
-    for (int k = 0; k < n_bins; ++k) {
-        for (int wd = 0; wd < vw; ++wd) {
-            const int v = wd * 64 + lane;
-            const bool valid = v < v_count;
-            const int c = k * v_pad + v;
-            double val = (valid && first) ? (double)y[(int64_t)c * ldy + smp] * col_scale[c] - cst[c] : 0.0;
-            const MissRec* rk = rec + (int64_t)k * vw + wd;
-            double acc = 0.0;
-            // group g covers entries [mb + 64g, mb + 64g + 64); lane j holds entry j's index / record
-            // a lane past the end reads the record of SNP k0 (in bounds) and stages z = 0 for it
-            int32_t idx_next = (mb + lane < me) ? miss_idx[mb + lane] : (int32_t)k0;
-            v4u cur_w = *reinterpret_cast<const v4u*>(rk + ((int64_t)idx_next - k0) * recs_per_snp);
-            double cur_a = rk[((int64_t)idx_next - k0) * recs_per_snp].a;
-            idx_next = (mb + kMissGroup + lane < me) ? miss_idx[mb + kMissGroup + lane] : (int32_t)k0;
-            int buf = 0;
-            for (int64_t p0 = mb; p0 < me; p0 += kMissGroup) {
-                *reinterpret_cast<v4u*>(&stage[wave][buf][lane]) = cur_w;
-                stage[wave][buf][lane].a = cur_a;
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                const int64_t pn = p0 + kMissGroup;
-                cur_w = *reinterpret_cast<const v4u*>(rk + ((int64_t)idx_next - k0) * recs_per_snp);
-                cur_a = rk[((int64_t)idx_next - k0) * recs_per_snp].a;
-                idx_next = (pn + kMissGroup + lane < me) ? miss_idx[pn + kMissGroup + lane] : (int32_t)k0;
-                const int cnt = (me - p0) < kMissGroup ? (int)(me - p0) : kMissGroup;
-                const MissRec* st = stage[wave][buf];
-#pragma unroll 8
-                for (int j = 0; j < cnt; ++j) {
-                    const uint2 pnw = *reinterpret_cast<const uint2*>(reinterpret_cast<const char*>(st + j) + 8 * half);
-                    const int zs = (int)((pnw.x >> lbit) & 1u) - (int)((pnw.y >> lbit) & 1u);
-                    acc = fma(st[j].a, (double)zs, acc);
-                }
-                buf ^= 1;
+    uint8_t* st = stage[wave];
+    for (int wd = 0; wd < vw; ++wd) {
+        const int v = wd * 64 + lane;
+        const bool valid = v < v_count;
+        // byte offset of this pass's part of SNP i's record: header + a[kb0 .. kb0+KB)
+        const uint8_t* rbase = rec + (int64_t)wd * rbytes;
+        double acc[KB];
+#pragma unroll
+        for (int k = 0; k < KB; ++k) acc[k] = 0.0;
+        // a lane past the end reads the record of SNP k0 (in bounds) and stages nothing used
+        auto entry_ptr = [&](int64_t p) -> const uint8_t* {
+            const int32_t i = (p < me) ? miss_idx[p] : (int32_t)k0;
+            return rbase + ((int64_t)i - k0) * vw * rbytes;
+        };
+        v4u cur[kPieces];
+        {
+            const uint8_t* e = entry_ptr(mb + lane);
+            cur[0] = *reinterpret_cast<const v4u*>(e);
+#pragma unroll
+            for (int q = 1; q < kPieces; ++q) {
+                // a values kb0.. of this pass; bytes past the record's bins are never read back
+                const int64_t off = 16 + 8 * (int64_t)kb0 + 16 * (q - 1);
+                cur[q] = off + 16 <= rbytes ? *reinterpret_cast<const v4u*>(e + off) : v4u{};
             }
-            val += acc;
-            if (valid) {
-                const int mode = (col_mode && first) ? col_mode[k] : 2;
-                double* dst = xz + (int64_t)(k * v_count + v) * ldxz + smp;
+        }
+        for (int64_t p0 = mb; p0 < me; p0 += kMissGroup) {
+#pragma unroll
+            for (int q = 0; q < kPieces; ++q) *reinterpret_cast<v4u*>(st + lane * kStage + q * 16) = cur[q];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            {
+                const uint8_t* e = entry_ptr(p0 + kMissGroup + lane);
+                cur[0] = *reinterpret_cast<const v4u*>(e);
+#pragma unroll
+                for (int q = 1; q < kPieces; ++q) {
+                    const int64_t off = 16 + 8 * (int64_t)kb0 + 16 * (q - 1);
+                    cur[q] = off + 16 <= rbytes ? *reinterpret_cast<const v4u*>(e + off) : v4u{};
+                }
+            }
+            const int cnt = (me - p0) < kMissGroup ? (int)(me - p0) : kMissGroup;
+            for (int jx = 0; jx < cnt; ++jx) {
+                const uint8_t* se = st + jx * kStage;
+                const uint2 pn = *reinterpret_cast<const uint2*>(se + 8 * half);
+                const double zs = (double)((int)((pn.x >> lbit) & 1u) - (int)((pn.y >> lbit) & 1u));
+                const double* av = reinterpret_cast<const double*>(se + 16);
+#pragma unroll
+                for (int k = 0; k < KB; ++k) acc[k] = fma(av[k], zs, acc[k]);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        if (valid) {
+#pragma unroll
+            for (int k = 0; k < KB; ++k) {
+                const int bin = kb0 + k;
+                const int c = bin * v_pad + v;
+                double val = acc[k];
+                if (first) val += (double)y[(int64_t)c * ldy + smp] * col_scale[c] - cst[c];
+                const int mode = (col_mode && first) ? col_mode[bin] : 2;
+                double* dst = xz + (int64_t)(bin * v_count + v) * ldxz + smp;
                 if (mode == 1) *dst = val;
                 else *dst += val;
             }
         }
     }
 }
+
 void launch_p1_finalize_z8(const int64_t* y, int64_t ldy, int64_t n, int n_bins, int v_count, int v_pad,
                            const double* col_scale, const double* cst, const int64_t* miss_ptr,
                            const int32_t* miss_idx, int64_t k0, int64_t k1, const void* rec, double* xz,
                            int64_t ldxz, const uint8_t* col_mode, hipStream_t st) {
     if (n <= 0) return;
     const int vw = (int)ceil_div(v_pad, 64);
-    const int64_t rec_stride = (int64_t)n_bins * vw * (int64_t)sizeof(MissRec);
+    const int64_t rec_stride = (int64_t)vw * miss_rec_bytes(n_bins);
     // SNP ranges whose records (about 2 MiB) stay resident in every XCD's L2
-    // while all samples walk them: one launch per range, in order
+    // while all samples walk them: one launch per range (and bin group), in order
     const int64_t range = std::max<int64_t>(4096, ((int64_t)2 << 20) / rec_stride);
+    const uint8_t* r = static_cast<const uint8_t*>(rec);
     for (int64_t ra = k0; ra < k1 || ra == k0; ra += range) {
         const int64_t rb = std::min(k1, ra + range);
-        hipLaunchKernelGGL(k_p1_finalize_z8, dim3((unsigned)ceil_div(n, 4)), dim3(256), 0, st, y, ldy, n, n_bins,
-                           v_count, v_pad, col_scale, cst, miss_ptr, miss_idx, k0, ra, rb,
-                           static_cast<const MissRec*>(rec), xz, ldxz, col_mode);
-        GPE_HIP(hipGetLastError());
+        // bins in groups of 16, 8, 4, 2, 1
+        for (int kb0 = 0; kb0 < n_bins;) {
+            const int left = n_bins - kb0;
+            const dim3 grid((unsigned)ceil_div(n, 4));
+#define GPE_FZ8(KB_)                                                                                             \
+    hipLaunchKernelGGL(k_p1_finalize_z8<KB_>, grid, dim3(256), 0, st, y, ldy, n, n_bins, kb0, v_count, v_pad,    \
+                       col_scale, cst, miss_ptr, miss_idx, k0, ra, rb, r, xz, ldxz, col_mode);                    \
+    kb0 += KB_
+            if (left >= 16) {
+                GPE_FZ8(16);
+            } else if (left >= 8) {
+                GPE_FZ8(8);
+            } else if (left >= 4) {
+                GPE_FZ8(4);
+            } else if (left >= 2) {
+                GPE_FZ8(2);
+            } else {
+                GPE_FZ8(1);
+            }
+#undef GPE_FZ8
+            GPE_HIP(hipGetLastError());
+        }
         if (rb >= k1) break;
     }
 }
@@ -1596,35 +1644,61 @@ void launch_philox_rademacher(const uint64_t* keys, const int64_t* starts, const
     GPE_HIP(hipGetLastError());
 }
 
-// out[j] = sum_n w_n x_jn^2 for the standardized working codes (x = (g - mu) inv_sd, 0 where missing)
-__global__ void k_weighted_sumsq(const uint8_t* snp, int64_t stride, int64_t r0, int64_t r1, int64_t n,
-                                 const double* mean, const double* inv_sd, const double* w, double* out) {
-    const int64_t j = r0 + (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (j >= r1) return;
-    const uint8_t* row = snp + j * stride;
-    const double mu = mean[j];
-    const double v[4] = {(0.0 - mu), 0.0, (1.0 - mu), (2.0 - mu)};
-    double acc = 0.0;
-    for (int64_t b = lane; b < ceil_div(n, 4); b += 64) {
-        const uint32_t byte = row[b];
+// out[e][j - r0] = sum_n W[n][e] x_jn^2 for e < n_w (n_w <= EC) and the
+// standardized working codes (x = (g - mu) inv_sd, 0 where missing).  One lane
+// per SNP; each lane reads 64 codes of its row at a time and, per sample, picks
+// x^2 from its four-entry table and adds EC weighted terms.  W is sample-major
+// (n_pad, EC), zero beyond n and beyond n_w, so the padded code bytes of a
+// row (code 00) add nothing; a sample's EC weights are the same for every lane
+// and arrive by scalar loads.
+template <int EC>
+__global__ __launch_bounds__(256) void k_weighted_sumsq(const uint8_t* __restrict__ snp, int64_t stride, int64_t r0,
+                                                        int64_t r1, int64_t n_bytes, const double* __restrict__ mean,
+                                                        const double* __restrict__ inv_sd,
+                                                        const double* __restrict__ W, int n_w,
+                                                        double* __restrict__ out, int64_t ldo) {
+    const int64_t j = r0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = j < r1;
+    const int64_t jj = live ? j : r0;  // idle lanes read a valid row
+    const double mu = mean[jj];
+    const double x2[4] = {mu * mu, 0.0, (1.0 - mu) * (1.0 - mu), (2.0 - mu) * (2.0 - mu)};
+    double acc[EC];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const int64_t s = b * 4 + t;
-            if (s < n) {
-                const double x = v[(byte >> (2 * t)) & 3u];
-                acc += w[s] * x * x;
+    for (int e = 0; e < EC; ++e) acc[e] = 0.0;
+    const uint8_t* row = snp + jj * stride;
+    for (int64_t b0 = 0; b0 < n_bytes; b0 += 16) {
+        const v4u wv = *reinterpret_cast<const v4u*>(row + b0);
+        const double* wr = W + b0 * 4 * EC;  // weights of sample 4 * b0
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+#pragma unroll 4
+            for (int t = 0; t < 16; ++t) {
+                const uint32_t c = (wv[d] >> (2 * t)) & 3u;
+                const double v = c == 0 ? x2[0] : (c == 2 ? x2[2] : (c == 3 ? x2[3] : 0.0));
+                const double* ws = wr + (d * 16 + t) * EC;
+#pragma unroll
+                for (int e = 0; e < EC; ++e) acc[e] = fma(ws[e], v, acc[e]);
             }
         }
     }
-    acc = wave_sum_f64(acc);
-    if (lane == 0) out[j - r0] = acc * inv_sd[j] * inv_sd[j];
+    if (!live) return;
+    const double s2 = inv_sd[j] * inv_sd[j];
+    for (int e = 0; e < n_w; ++e) out[(int64_t)e * ldo + (j - r0)] = acc[e] * s2;
 }
 void launch_weighted_sumsq(const uint8_t* snp, int64_t stride, int64_t r0, int64_t r1, int64_t n, const double* mean,
-                           const double* inv_sd, const double* w, double* out, hipStream_t st) {
-    if (r1 <= r0) return;
-    hipLaunchKernelGGL(k_weighted_sumsq, dim3((unsigned)ceil_div(r1 - r0, 4)), dim3(256), 0, st, snp, stride, r0, r1, n,
-                       mean, inv_sd, w, out);
+                           const double* inv_sd, const double* W_sample_major, int ec, int n_w, double* out,
+                           int64_t ldo, hipStream_t st) {
+    if (r1 <= r0 || n_w <= 0) return;
+    const int64_t n_bytes = round_up(ceil_div(n, 4), 16);
+    const dim3 grid((unsigned)ceil_div(r1 - r0, 256));
+    if (ec == 8)
+        hipLaunchKernelGGL(k_weighted_sumsq<8>, grid, dim3(256), 0, st, snp, stride, r0, r1, n_bytes, mean, inv_sd,
+                           W_sample_major, n_w, out, ldo);
+    else if (ec == 16)
+        hipLaunchKernelGGL(k_weighted_sumsq<16>, grid, dim3(256), 0, st, snp, stride, r0, r1, n_bytes, mean, inv_sd,
+                           W_sample_major, n_w, out, ldo);
+    else
+        throw Error("weighted sums of squares take 8 or 16 weights per pass");
     GPE_HIP(hipGetLastError());
 }
 

@@ -116,12 +116,16 @@ def probes_device(root: int, snp_begin: int, snp_end: int, step_size: int, probe
 
 
 def weighted_sumsq(g: engine.Genotype, w: torch.Tensor, snp_begin: int = 0, snp_end: int | None = None):
+    """sum_n w_n x_jn^2 over SNPs [snp_begin, snp_end): w (N,) -> (L,), or (E, N) -> (E, L)
+    with every weight vector served by the same pass over the genotypes."""
     snp_end = g.m if snp_end is None else snp_end
-    w = w.to(torch.float64).contiguous()
-    out = torch.empty(snp_end - snp_begin, dtype=torch.float64, device=w.device)
-    engine.call("gpe_weighted_sumsq", g.handle, snp_begin, snp_end, engine._ptr(w), engine._ptr(out),
-                engine._stream())
-    return out
+    single = w.dim() == 1
+    w2 = (w[None, :] if single else w).to(torch.float64).contiguous()
+    L = snp_end - snp_begin
+    out = torch.empty((w2.shape[0], L), dtype=torch.float64, device=w2.device)
+    engine.call("gpe_weighted_sumsq", g.handle, snp_begin, snp_end, engine._ptr(w2), w2.shape[0], w2.stride(0),
+                engine._ptr(out), L, engine._stream())
+    return out[0] if single else out
 
 
 def _project(M_qn: torch.Tensor, C: torch.Tensor | None) -> torch.Tensor:
@@ -148,8 +152,7 @@ def features(g: engine.Genotype, env: torch.Tensor, C: torch.Tensor | None, eps_
         rows_e.insert(0, (C * env[:, None]).T)
     V = torch.cat(rows + rows_e, dim=0).contiguous()
     T = engine.xtv(g, V, limbs=limbs)  # (M, 2p + 2)
-    ssg = weighted_sumsq(g, ones)
-    sseg = weighted_sumsq(g, env * env)
+    ssg, sseg = weighted_sumsq(g, torch.stack([ones, env * env]))
     cg = T[:, :p]
     one_g = T[:, p]
     ceg = T[:, p + 1:2 * p + 1]
@@ -229,11 +232,12 @@ def _features_multi(g: engine.Genotype, envs_d: torch.Tensor, Cs, eps_var: float
         rows.append(e[None, :])
         layout.append((start, p, df))
     T = engine.xtv(g, torch.cat(rows, dim=0).contiguous(), limbs=limbs)
-    ssg = weighted_sumsq(g, ones)
+    # ||x_j||^2 and every environment's ||e_k . x_j||^2 from one pass
+    ss_all = weighted_sumsq(g, torch.cat([ones[None, :], (envs_d * envs_d).T], dim=0))
+    ssg = ss_all[0]
     inv_x, inv_w, dfs = [], [], []
     for k, (o, p, df) in enumerate(layout):
-        e = envs_d[:, k]
-        sseg = weighted_sumsq(g, e * e)
+        sseg = ss_all[1 + k]
         cg, one_g = T[:, o:o + p], T[:, o + p]
         ceg, e_g = T[:, o + p + 1:o + 2 * p + 1], T[:, o + 2 * p + 1]
         varx = (ssg - (cg * cg).sum(dim=1) - one_g * one_g / N) / df

@@ -126,9 +126,9 @@ def _phase1_ref(oracle, codes, annot, inv, seed, v_start, v, step, mode="mean", 
 
 
 @pytest.mark.parametrize("mode", ["mean", "hwe"])
-def test_phase1_matches_oracle(engine, oracle, mode):
-    _say(f"phase1 {mode}")
-    m, n, K, V, step = 700, 333, 3, 7, 100
+def test_phase1_matches_oracle(engine, oracle, mode, K=3):
+    _say(f"phase1 {mode} bins={K}")
+    m, n, V, step = 700, 333, 7, 100
     codes = random_codes(m, n, seed=11, miss_rate=0.03, monomorphic=2)
     rng = np.random.default_rng(3)
     annot = (rng.random((m, K)) < 0.6).astype(np.float64) * rng.uniform(0.5, 2.0, size=(m, K))
@@ -145,6 +145,12 @@ def test_phase1_matches_oracle(engine, oracle, mode):
     err = np.abs(got - ref) / scale
     assert err.max() < 1e-11, err.max()
     g.close()
+
+
+@pytest.mark.parametrize("K", [7, 17])
+def test_phase1_many_bins_match_oracle(engine, oracle, K):
+    """Bin counts the missing-call finalize splits into groups (7 = 4 + 2 + 1, 17 = 16 + 1)."""
+    test_phase1_matches_oracle(engine, oracle, "mean", K)
 
 
 @pytest.mark.parametrize("mode,limbs,rtol", [("mean", 5, 5e-11), ("hwe", 5, 5e-11), ("mean", 6, 1e-12)])

@@ -70,3 +70,20 @@ def test_gxe_multi_environment_matches_oracle(oracle, per_env_cov):
         for name in ("xx", "xw", "wx", "ww"):
             np.testing.assert_allclose(got[k][name], ref[name], rtol=1e-10, err_msg=f"env {k} {name}")
     g.close()
+
+
+@pytest.mark.parametrize("n_w", [1, 3, 17])
+def test_weighted_sumsq_matches_dense(oracle, n_w):
+    """sum_n w_n x_jn^2 for several weight vectors in one call (16 per pass) against the
+    oracle's standardized matrix; a SNP range that starts mid-matrix."""
+    from summit_amd import engine, gxe
+    m, n = 300, 1037
+    codes = random_codes(m, n, seed=41, miss_rate=0.03, monomorphic=1)
+    g = engine.Genotype.from_bed_bytes(oracle.pack_codes(codes), n, m).prepare(impute="mean")
+    rng = np.random.default_rng(n_w)
+    W = rng.standard_normal((n_w, n)) ** 2
+    X = oracle.read_block_standardized(oracle.pack_codes(codes), n, 0, m, None, 1, "mean", 0)
+    got = gxe.weighted_sumsq(g, torch.as_tensor(W, device=engine.default_device()), 20, 290).cpu().numpy()
+    ref = W @ (X[:, 20:290] ** 2)
+    np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-9)
+    g.close()
