@@ -38,7 +38,16 @@ enum {
 };
 
 enum { GPE_IMPUTE_MEAN = 0, GPE_IMPUTE_HWE = 1 };      /* src/native/common/genotype.hpp:33-36 */
-enum { GPE_RAND_RADEMACHER = 0, GPE_RAND_GAUSSIAN = 1 }; /* src/native/gwldcore.cpp:1681-1682 */
+/* Probe distributions (src/native/gwldcore.cpp:1681-1716): the reference draws
+ * in the precision of the caller's arrays, so Gaussian / spherical streams come
+ * in a double and a float version (float variates, stored as double). */
+enum {
+    GPE_RAND_RADEMACHER = 0,
+    GPE_RAND_GAUSSIAN = 1,
+    GPE_RAND_SPHERICAL = 2,
+    GPE_RAND_GAUSSIAN_F32 = 3,
+    GPE_RAND_SPHERICAL_F32 = 4
+};
 enum { GPE_LAYOUT_SNP_RAW = 0, GPE_LAYOUT_SNP_WORK = 1, GPE_LAYOUT_SAMPLE_WORK = 2 };
 
 /* Message of the calling thread's last failure ("" if none). */
@@ -215,7 +224,7 @@ int gpe_weighted_sumsq(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const do
 
 /* Probe values of the reference's genome-wide streams for blocks starting at
  * blk_starts_host[0..n_blocks) (last block ends at blk_end): Rademacher on the
- * GPU, Gaussian with the host std::normal_distribution.  z_host: row-major
+ * GPU; Gaussian and spherical with the host std::normal_distribution<T>.  z_host: row-major
  * (blk_end - blk_starts_host[0]) x v_count FP64.  Follows
  * src/native/gwldcore.cpp:445-458, :1679-1716. */
 int gpe_probes(int32_t rand_dist, uint64_t root_seed, const int64_t* blk_starts_host, int32_t n_blocks,

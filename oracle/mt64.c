@@ -16,7 +16,11 @@
  *     src/native/gwldcore.cpp:1691-1716.
  *   - Gaussian fill: libstdc++'s std::normal_distribution (Marsaglia polar
  *     method fed by std::generate_canonical), the documented algorithm of the
- *     GNU implementation the reference is built with.
+ *     GNU implementation the reference is built with, for T = double and
+ *     T = float (the reference draws in the precision of the caller's arrays).
+ *   - Spherical fill: the Gaussian fill with each probe column rescaled to
+ *     norm sqrt(L), the sum of squares in long double:
+ *     src/native/gwldcore.cpp:1702-1715.
  */
 #include <math.h>
 #include <stdint.h>
@@ -146,6 +150,75 @@ void or_gaussian_block(uint64_t root, int blk_start, int v_start, int L, int V, 
     for (int c = 0; c < V; ++c)
         for (int r = 0; r < L; ++r)
             Z[(size_t)c * (size_t)L + (size_t)r] = or_normal_next(&nd, &s);
+}
+
+/* std::generate_canonical<float, 24>(mt19937_64): one 64-bit draw converted to
+ * float (round to nearest) and divided by 2^64 in float. */
+static float or_canonical_float(or_mt64* s) {
+    const float r = 18446744073709551616.0f; /* 2^64 */
+    float ret = (float)or_mt64_next(s) / r;
+    if (ret >= 1.0f) ret = nextafterf(1.0f, 0.0f);
+    return ret;
+}
+
+typedef struct {
+    int saved_available;
+    float saved;
+} or_normalf;
+
+/* normal_distribution<float>: 2 * U is a float product, the "- 1.0" is a double
+ * subtraction rounded back to float, the rest is float arithmetic (logf, sqrtf). */
+static float or_normalf_next(or_normalf* nd, or_mt64* s) {
+    if (nd->saved_available) {
+        nd->saved_available = 0;
+        return nd->saved;
+    }
+    float x, y, r2;
+    do {
+        x = (float)((double)(2.0f * or_canonical_float(s)) - 1.0);
+        y = (float)((double)(2.0f * or_canonical_float(s)) - 1.0);
+        r2 = x * x + y * y;
+    } while (r2 > 1.0 || r2 == 0.0);
+    const float mult = sqrtf(-2.0f * logf(r2) / r2);
+    nd->saved = x * mult;
+    nd->saved_available = 1;
+    return y * mult;
+}
+
+/* dist: 1 gaussian, 2 spherical (double); 3 gaussian, 4 spherical (float, stored
+ * as double).  Column-major L x V; one distribution object for the whole fill. */
+void or_real_probe_block(uint64_t root, int blk_start, int v_start, int L, int V, int dist, double* Z) {
+    or_mt64 s;
+    or_normal nd = {0, 0.0};
+    or_normalf ndf = {0, 0.0f};
+    or_mt64_seed(&s, or_make_seed(root, blk_start, v_start));
+    const int f32 = dist >= 3;
+    const int spherical = dist == 2 || dist == 4;
+    for (int c = 0; c < V; ++c) {
+        double* zc = Z + (size_t)c * (size_t)L;
+        long double ss = 0.0L;
+        if (f32) {
+            for (int r = 0; r < L; ++r) {
+                const float z = or_normalf_next(&ndf, &s);
+                zc[r] = (double)z;
+                if (spherical) ss += (long double)z * (long double)z;
+            }
+            if (spherical) {
+                const float sc = ss > 0.0L ? (float)sqrtl((long double)L / ss) : 1.0f;
+                for (int r = 0; r < L; ++r) zc[r] = (double)((float)zc[r] * sc);
+            }
+        } else {
+            for (int r = 0; r < L; ++r) {
+                const double z = or_normal_next(&nd, &s);
+                zc[r] = z;
+                if (spherical) ss += (long double)z * (long double)z;
+            }
+            if (spherical) {
+                const double sc = ss > 0.0L ? (double)sqrtl((long double)L / ss) : 1.0;
+                for (int r = 0; r < L; ++r) zc[r] = zc[r] * sc;
+            }
+        }
+    }
 }
 
 /* Raw engine outputs for a (root, block, v0) stream, for stream tests. */

@@ -58,6 +58,9 @@ def _lib():
             fn.restype = None
             fn.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                            ctypes.c_void_p]
+        lib.or_real_probe_block.restype = None
+        lib.or_real_probe_block.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_int, ctypes.c_void_p]
         lib.or_stream_raw.restype = None
         lib.or_stream_raw.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int64,
                                       ctypes.c_void_p]
@@ -281,15 +284,17 @@ def stream_raw(root: int, blk_start: int, v_start: int, n: int) -> np.ndarray:
     return out
 
 
-def probe_block(root: int, blk_start: int, v_start: int, L: int, V: int, dist: str = "rademacher"):
-    """Z (L, V), column-major fill order, float64."""
+def probe_block(root: int, blk_start: int, v_start: int, L: int, V: int, dist: str = "rademacher",
+                fp32: bool = False):
+    """Z (L, V), column-major fill order, float64 values.  As the reference
+    (src/native/gwldcore.cpp:1691-1716): "rademacher", "spherical", anything else
+    Gaussian; fp32 draws the float-precision streams of float32 callers."""
     Z = np.empty((V, L), dtype=np.float64)
     if dist == "rademacher":
         _lib().or_rademacher_block(root, blk_start, v_start, L, V, Z.ctypes.data)
-    elif dist in ("gaussian", "normal"):
-        _lib().or_gaussian_block(root, blk_start, v_start, L, V, Z.ctypes.data)
     else:
-        raise ValueError(f"unsupported rand_dist {dist!r}")
+        code = (2 if dist == "spherical" else 1) + (2 if fp32 else 0)
+        _lib().or_real_probe_block(root, blk_start, v_start, L, V, code, Z.ctypes.data)
     return Z.T  # (L, V), Fortran-contiguous
 
 

@@ -77,13 +77,34 @@ static inline uint64_t h_make_seed(uint64_t root, int block, int v0) {
     return s;
 }
 
-// dist: 0 rademacher, 1 gaussian (double)
+// The reference's probe fill (src/native/gwldcore.cpp:1691-1716) with the
+// standard library it is built against, for T = double and T = float.
+template <class T>
+static void h_probe_fill(uint64_t root, int blk_start, int v_start, int L, int V, bool rademacher, bool spherical,
+                         double* Z) {
+    std::mt19937_64 rng(h_make_seed(root, blk_start, v_start));
+    std::normal_distribution<T> gN(0, (T)1);
+    std::vector<T> zc((size_t)L);
+    for (int c = 0; c < V; ++c) {
+        long double ss = 0.0L;
+        for (int r = 0; r < L; ++r) {
+            T z = rademacher ? ((rng() & 1) ? T(+1) : T(-1)) : gN(rng);
+            zc[(size_t)r] = z;
+            if (spherical) ss += (long double)z * (long double)z;
+        }
+        if (spherical) {
+            const T zscale = ss > 0.0L ? (T)std::sqrt((long double)L / ss) : T(1);
+            for (int r = 0; r < L; ++r) zc[(size_t)r] *= zscale;
+        }
+        for (int r = 0; r < L; ++r) Z[(size_t)c * (size_t)L + (size_t)r] = (double)zc[(size_t)r];
+    }
+}
+
+// dist: 0 rademacher, 1 gaussian, 2 spherical (double); 3 gaussian, 4 spherical (float)
 extern "C" void ref_std_probe_block(uint64_t root, int blk_start, int v_start, int L, int V,
                                     int dist, double* Z) {
-    std::mt19937_64 rng(h_make_seed(root, blk_start, v_start));
-    std::normal_distribution<double> gN(0, 1.0);
-    for (int c = 0; c < V; ++c)
-        for (int r = 0; r < L; ++r)
-            Z[(size_t)c * (size_t)L + (size_t)r] =
-                dist == 0 ? ((rng() & 1) ? 1.0 : -1.0) : gN(rng);
+    if (dist >= 3)
+        h_probe_fill<float>(root, blk_start, v_start, L, V, false, dist == 4, Z);
+    else
+        h_probe_fill<double>(root, blk_start, v_start, L, V, dist == 0, dist == 2, Z);
 }

@@ -222,6 +222,48 @@ struct ProfScope {
     }
 };
 
+bool is_real_probe_dist(int d) {
+    return d == GPE_RAND_GAUSSIAN || d == GPE_RAND_SPHERICAL || d == GPE_RAND_GAUSSIAN_F32 ||
+           d == GPE_RAND_SPHERICAL_F32;
+}
+
+// The reference's real-valued probe fill for one SNP block
+// (src/native/gwldcore.cpp:1679-1716): std::mt19937_64 seeded by
+// make_seed(root, block start, v_start), one std::normal_distribution<T> for the
+// whole block, column by column; spherical probes rescale each column to norm
+// sqrt(L) with the sum of squares in long double.  T is the precision of the
+// caller's arrays (float32 callers draw float variates).  Writes z[r * ldz + c].
+template <class T>
+void real_probe_fill(uint64_t root, int64_t blk_start, int v_start, int64_t L, int V, bool spherical, double* z,
+                     int64_t ldz) {
+    auto mix = [](uint64_t x) {
+        x += 0x9e3779b97f4a7c15ULL;
+        x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
+        x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
+        return x ^ (x >> 31);
+    };
+    uint64_t seed = 0x1234abcdULL;
+    seed ^= mix(root);
+    seed ^= mix((uint64_t)(int64_t)(int32_t)blk_start + 0x9e37ULL);
+    seed ^= mix((uint64_t)(int64_t)v_start + 0x85ebULL);
+    std::mt19937_64 rng(seed);
+    std::normal_distribution<T> gN(0, (T)1);
+    std::vector<T> col((size_t)L);
+    for (int c = 0; c < V; ++c) {
+        long double ss = 0.0L;
+        for (int64_t r = 0; r < L; ++r) {
+            const T v = gN(rng);
+            col[(size_t)r] = v;
+            if (spherical) ss += (long double)v * (long double)v;
+        }
+        if (spherical) {
+            const T sc = ss > 0.0L ? (T)std::sqrt((long double)L / ss) : T(1);
+            for (int64_t r = 0; r < L; ++r) col[(size_t)r] *= sc;
+        }
+        for (int64_t r = 0; r < L; ++r) z[r * ldz + c] = (double)col[(size_t)r];
+    }
+}
+
 // memcpy split over n_threads host threads (page-cache or mmap reads of a BED
 // body are far slower from one thread than the PCIe copy that follows)
 void parallel_memcpy(uint8_t* dst, const uint8_t* src, int64_t bytes, int n_threads) {
@@ -671,27 +713,19 @@ int gpe_probes(int32_t rand_dist, uint64_t root_seed, const int64_t* blk_starts_
             dfree(starts);
             dfree(z);
             for (size_t i = 0; i < zh.size(); ++i) z_host[i] = (double)zh[i];
-        } else if (rand_dist == GPE_RAND_GAUSSIAN) {
+        } else if (is_real_probe_dist(rand_dist)) {
             for (int b = 0; b < n_blocks; ++b) {
                 const int64_t s = blk_starts_host[b];
                 const int64_t e = b + 1 < n_blocks ? blk_starts_host[b + 1] : blk_end;
-                uint64_t seed = 0x1234abcdULL;
-                auto mix = [](uint64_t x) {
-                    x += 0x9e3779b97f4a7c15ULL;
-                    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
-                    x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
-                    return x ^ (x >> 31);
-                };
-                seed ^= mix(root_seed);
-                seed ^= mix((uint64_t)(int64_t)(int32_t)s + 0x9e37ULL);
-                seed ^= mix((uint64_t)(int64_t)v_start + 0x85ebULL);
-                std::mt19937_64 rng(seed);
-                std::normal_distribution<double> gN(0.0, 1.0);
-                for (int c = 0; c < v_count; ++c)
-                    for (int64_t r = 0; r < e - s; ++r) z_host[(s - s0 + r) * v_count + c] = gN(rng);
+                if (rand_dist >= GPE_RAND_GAUSSIAN_F32)
+                    real_probe_fill<float>(root_seed, s, v_start, e - s, v_count, rand_dist == GPE_RAND_SPHERICAL_F32,
+                                           z_host + (s - s0) * v_count, v_count);
+                else
+                    real_probe_fill<double>(root_seed, s, v_start, e - s, v_count, rand_dist == GPE_RAND_SPHERICAL,
+                                            z_host + (s - s0) * v_count, v_count);
             }
         } else {
-            throw Error("rand_dist must be rademacher or gaussian");
+            throw Error("rand_dist must be rademacher, gaussian or spherical");
         }
     });
 }
@@ -713,6 +747,8 @@ void run_phase1(gpe_geno* g, int64_t s, int64_t e, const double* w, int n_bins, 
     const int64_t L = e - s;
     const int v_pad = (int)round_up(v_count, 32);
     require(ldz >= v_pad, "probe leading dimension too small");
+    require((int64_t)n_bins * v_pad <= 65535,
+            "bins x padded probes exceeds 65535 columns per traversal; split the probes into V-tiles");
     const int ncols = n_bins * v_pad;
     const int groups = (int)ceil_div(ncols, 32 * lc.ctc);
     const int64_t coef_ctiles = (int64_t)groups * lc.nl * lc.ctc;
@@ -777,6 +813,8 @@ void run_phase2(gpe_geno* g, int64_t s, int64_t e, const double* x, int64_t ldx,
                 EpiMode mode, ContractArgs& a, hipStream_t st, int* ctiles_used_out, int* v_pad_out) {
     const LimbCfg lc = limb_cfg(limbs);
     const int v_pad = (int)round_up(v_count, 32);
+    require((int64_t)n_bins * v_pad <= 65535,
+            "bins x padded probes exceeds 65535 columns per traversal; split the probes into V-tiles");
     const int ncols = n_bins * v_pad;
     const int groups = (int)ceil_div(ncols, 32 * lc.ctc);
     const int64_t coef_ctiles = (int64_t)groups * lc.nl * lc.ctc;
@@ -874,7 +912,7 @@ int gpe_phase1_xz(gpe_geno* g, int64_t snp_begin, int64_t snp_end, int64_t step_
             int64_t* starts_dev = static_cast<int64_t*>(g->scratch("p1_starts", nb * sizeof(int64_t)));
             GPE_HIP(hipMemcpyAsync(starts_dev, starts.data(), nb * sizeof(int64_t), hipMemcpyHostToDevice, st));
             launch_mt_rademacher(root_seed, starts_dev, nb, snp_end, v_start, v_count, z_i8, ldz, st);
-        } else if (rand_dist == GPE_RAND_GAUSSIAN) {
+        } else if (is_real_probe_dist(rand_dist)) {
             z_f64 = static_cast<double*>(g->scratch("p1_z64", g->m_alloc * ldz * sizeof(double)));
             std::vector<double> zh((size_t)(L * v_count));
             require(gpe_probes(rand_dist, root_seed, starts.data(), nb, snp_end, v_start, v_count, zh.data()) == 0,
@@ -886,7 +924,7 @@ int gpe_phase1_xz(gpe_geno* g, int64_t snp_begin, int64_t snp_end, int64_t step_
                                    hipMemcpyHostToDevice, st));
             GPE_HIP(hipStreamSynchronize(st));
         } else {
-            throw Error("rand_dist must be 'rademacher' or 'gaussian'");
+            throw Error("rand_dist must be 'rademacher', 'gaussian' or 'spherical'");
         }
         run_phase1(g, snp_begin, snp_end, w, n_bins, z_i8, z_f64, ldz, v_count, xz_dev, ldxz, bin_mode_dev, limbs,
                    st);

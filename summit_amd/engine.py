@@ -21,7 +21,17 @@ from . import _native
 from ._native import c_i64, c_vp, call
 
 IMPUTE = {"mean": 0, "hwe": 1}
-RAND = {"rademacher": 0, "gaussian": 1, "normal": 1}
+RAND = {"rademacher": 0, "gaussian": 1, "normal": 1, "spherical": 2}
+
+
+def rand_code(rand_dist: str, fp32: bool = False) -> int:
+    """The reference's rule (src/native/gwldcore.cpp:1695-1696): "rademacher",
+    "spherical", anything else Gaussian; float32 callers draw float variates."""
+    if rand_dist == "rademacher":
+        return 0
+    if rand_dist == "spherical":
+        return 4 if fp32 else 2
+    return 3 if fp32 else 1
 LAYOUT = {"snp_raw": 0, "snp_work": 1, "sample_work": 2}
 
 
@@ -184,7 +194,7 @@ def resvar_inv(g: Genotype, snp_begin: int = 0, snp_end: int | None = None, eps:
 def phase1_xz(g: Genotype, snp_begin: int, snp_end: int, step_size: int, annot: torch.Tensor,
               inv: torch.Tensor, v_start: int, v_count: int, seed: int, xz: torch.Tensor,
               rand_dist: str = "rademacher", bin_mode: torch.Tensor | None = None, limbs: int = 5,
-              stream=None) -> torch.Tensor:
+              stream=None, probe_fp32: bool = False) -> torch.Tensor:
     """Xz[k*v_count+v, :] (+)= sum_i x_i inv_i sqrt(a_ik) z_iv over [snp_begin, snp_end)."""
     L = snp_end - snp_begin
     annot = annot.to(torch.float64).contiguous()
@@ -196,11 +206,9 @@ def phase1_xz(g: Genotype, snp_begin: int, snp_end: int, step_size: int, annot: 
     B = annot.shape[1]
     if xz.dtype != torch.float64 or xz.dim() != 2 or xz.shape[0] != B * v_count or xz.shape[1] != g.n:
         raise RuntimeError("Xz2d_chunk shape must be (N, B*v_count)")
-    if rand_dist not in RAND:
-        raise RuntimeError(f"unsupported rand_dist {rand_dist!r}")
     call("gpe_phase1_xz", g.handle, snp_begin, snp_end, step_size, _ptr(annot), B, _ptr(inv), v_start, v_count,
-         RAND[rand_dist], int(seed) & 0xFFFFFFFFFFFFFFFF, _ptr(xz), xz.stride(0), _ptr(bin_mode), limbs,
-         _stream(stream))
+         rand_code(rand_dist, probe_fp32), int(seed) & 0xFFFFFFFFFFFFFFFF, _ptr(xz), xz.stride(0), _ptr(bin_mode),
+         limbs, _stream(stream))
     return xz
 
 
@@ -283,11 +291,11 @@ def apply_grm(g: Genotype, panel: torch.Tensor, inv_all: torch.Tensor, nsnps: in
 
 
 def probes(root_seed: int, blk_starts, blk_end: int, v_start: int, v_count: int,
-           rand_dist: str = "rademacher") -> np.ndarray:
+           rand_dist: str = "rademacher", fp32: bool = False) -> np.ndarray:
     """Reference probe values for consecutive blocks, (blk_end - blk_starts[0], v_count) FP64."""
     starts = np.ascontiguousarray(np.asarray(blk_starts, dtype=np.int64))
     out = np.empty((blk_end - int(starts[0]), v_count), dtype=np.float64)
-    call("gpe_probes", RAND[rand_dist], int(root_seed) & 0xFFFFFFFFFFFFFFFF, _np_ptr(starts), len(starts),
+    call("gpe_probes", rand_code(rand_dist, fp32), int(root_seed) & 0xFFFFFFFFFFFFFFFF, _np_ptr(starts), len(starts),
          int(blk_end), int(v_start), int(v_count), _np_ptr(out))
     return out
 

@@ -8,7 +8,8 @@ libsummit_gpe on the GPU.  Differences a caller can see:
   file identity and sample selection replaces the mmap cache of
   src/native/common/genotype.cpp:350-410); ``prefetch_bed_block`` loads it.
 * Arithmetic is FP64 whatever the array dtype; float32 arrays receive the
-  FP64 results rounded to float32.
+  FP64 results rounded to float32.  Gaussian / spherical probes are drawn in
+  the arrays' precision, as the reference's std::normal_distribution<T> does.
 * Arrays may be NumPy arrays (host, copied to and from the GPU per call) or
   torch tensors already on the GPU (no copies: ``Xz2d`` as a contiguous (Q, N)
   tensor for the column-major layout, ``meansq_accum`` as (M, B)).
@@ -290,10 +291,13 @@ def _phase1(bed_prefix, fam_path, blk_start, blk_end, row_sel, ddof, annot_blk, 
                 bin_init_mask[k] = 1
     projecting = bool(project_right) and C is not None and R is not None
     annot_d = _to_dev(annot)
+    # the reference draws its probes in the precision of the caller's arrays
+    # (std::normal_distribution<T>, src/native/gwldcore.cpp:1694)
+    fp32 = str(getattr(Xz2d_chunk, "dtype", "")).endswith("float32")
     if rowmajor or projecting:
         contrib = torch.zeros((shape_q, N), dtype=torch.float64, device=_dev())
         engine.phase1_xz(g, blk_start, blk_end, L, annot_d, inv_d, int(v_start), int(v_count), int(seed), contrib,
-                         rand_dist=rand_dist, limbs=LIMBS)
+                         rand_dist=rand_dist, limbs=LIMBS, probe_fp32=fp32)
         if projecting:
             contrib = _project_dev(contrib, C, R)
         if rowmajor:
@@ -314,7 +318,7 @@ def _phase1(bed_prefix, fam_path, blk_start, blk_end, row_sel, ddof, annot_blk, 
     xz, _ = _xz_colmajor_dev(Xz2d_chunk)
     mode_d = torch.from_numpy(modes).to(_dev())
     engine.phase1_xz(g, blk_start, blk_end, L, annot_d, inv_d, int(v_start), int(v_count), int(seed), xz,
-                     rand_dist=rand_dist, bin_mode=mode_d, limbs=LIMBS)
+                     rand_dist=rand_dist, bin_mode=mode_d, limbs=LIMBS, probe_fp32=fp32)
     _write_back_colmajor(Xz2d_chunk, xz)
 
 

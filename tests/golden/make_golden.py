@@ -66,7 +66,8 @@ def main() -> None:
     # --- probe streams (std::mt19937_64 + std::normal_distribution<double>)
     pr = {}
     for root, blk, v0, L, V in ((1101, 0, 0, 50, 10), (1101, 1000, 64, 33, 7), (2**63 + 12345, 7, 3, 19, 5)):
-        for dist, di in (("rademacher", 0), ("gaussian", 1)):
+        for dist, di in (("rademacher", 0), ("gaussian", 1), ("spherical", 2), ("gaussian32", 3),
+                         ("spherical32", 4)):
             Z = np.empty((V, L))
             ref.ref_std_probe_block(root, blk, v0, L, V, di, Z.ctypes.data)
             pr[f"{dist}_{root}_{blk}_{v0}_{L}_{V}"] = Z.T.copy()

@@ -102,15 +102,40 @@ def test_exact_integer_products(engine, oracle, m, n, b):
     g.close()
 
 
-@pytest.mark.parametrize("dist", ["rademacher", "gaussian"])
-def test_probe_streams(engine, oracle, dist):
-    _say(f"probes {dist}")
+@pytest.mark.parametrize("dist,fp32", [("rademacher", False), ("gaussian", False), ("spherical", False),
+                                       ("gaussian", True), ("spherical", True)])
+def test_probe_streams(engine, oracle, dist, fp32):
+    _say(f"probes {dist} fp32={fp32}")
     starts = [0, 1000, 2000, 2500]
     end = 3333
-    z = engine.probes(1101, starts, end, v_start=64, v_count=13, rand_dist=dist)
-    ref = np.concatenate([oracle.probe_block(1101, s, 64, e - s, 13, dist)
+    z = engine.probes(1101, starts, end, v_start=64, v_count=13, rand_dist=dist, fp32=fp32)
+    ref = np.concatenate([oracle.probe_block(1101, s, 64, e - s, 13, dist, fp32=fp32)
                           for s, e in zip(starts, starts[1:] + [end])])
     np.testing.assert_array_equal(z, ref)
+
+
+def test_phase1_spherical_probes_match_oracle(engine, oracle):
+    """Phase 1 with spherical probes (FP64 probe operand, per-block column rescale)."""
+    _say("phase1 spherical")
+    m, n, K, V, step = 450, 290, 2, 5, 100
+    codes = random_codes(m, n, seed=13, miss_rate=0.02)
+    rng = np.random.default_rng(6)
+    annot = np.ones((m, K))
+    annot[:, 1] = rng.random(m) < 0.4
+    inv = rng.uniform(0.8, 1.2, size=m)
+    g = _geno(engine, oracle, codes).prepare(impute="mean")
+    xz = torch.zeros((K * V, n), dtype=torch.float64, device="cuda")
+    engine.phase1_xz(g, 0, m, step, torch.from_numpy(annot).cuda(), torch.from_numpy(inv).cuda(), 3, V, 4242, xz,
+                     rand_dist="spherical")
+    bed = oracle.pack_codes(codes)
+    ref = np.zeros((n, K * V))
+    for s in range(0, m, step):
+        e = min(m, s + step)
+        X = oracle.read_block_standardized(bed, n, s, e, None, 1, "mean", 0)
+        ref += oracle.phase1_chunk(X, annot[s:e], inv[s:e], oracle.probe_block(4242, s, 3, e - s, V, "spherical"))
+    err = np.abs(xz.cpu().numpy().T - ref) / np.abs(ref).max(axis=0, keepdims=True)
+    assert err.max() < 1e-11, err.max()
+    g.close()
 
 
 def _phase1_ref(oracle, codes, annot, inv, seed, v_start, v, step, mode="mean", impute_seed=0):
@@ -126,9 +151,9 @@ def _phase1_ref(oracle, codes, annot, inv, seed, v_start, v, step, mode="mean", 
 
 
 @pytest.mark.parametrize("mode", ["mean", "hwe"])
-def test_phase1_matches_oracle(engine, oracle, mode, K=3):
-    _say(f"phase1 {mode} bins={K}")
-    m, n, V, step = 700, 333, 7, 100
+def test_phase1_matches_oracle(engine, oracle, mode, K=3, V=7):
+    _say(f"phase1 {mode} bins={K} probes={V}")
+    m, n, step = 700, 333, 100
     codes = random_codes(m, n, seed=11, miss_rate=0.03, monomorphic=2)
     rng = np.random.default_rng(3)
     annot = (rng.random((m, K)) < 0.6).astype(np.float64) * rng.uniform(0.5, 2.0, size=(m, K))
@@ -151,6 +176,11 @@ def test_phase1_matches_oracle(engine, oracle, mode, K=3):
 def test_phase1_many_bins_match_oracle(engine, oracle, K):
     """Bin counts the missing-call finalize splits into groups (7 = 4 + 2 + 1, 17 = 16 + 1)."""
     test_phase1_matches_oracle(engine, oracle, "mean", K)
+
+
+def test_phase1_wide_probe_tile_matches_oracle(engine, oracle):
+    """100 probes: two 64-column sign words per missing-call record, padded columns."""
+    test_phase1_matches_oracle(engine, oracle, "mean", 2, 100)
 
 
 @pytest.mark.parametrize("mode,limbs,rtol", [("mean", 5, 5e-11), ("hwe", 5, 5e-11), ("mean", 6, 1e-12)])

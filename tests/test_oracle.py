@@ -32,12 +32,23 @@ def test_make_seed_mixes_every_argument(oracle):
 
 
 def test_probe_streams_match_reference_golden(oracle):
+    """Rademacher, Gaussian and spherical streams in double and float precision against
+    libstdc++'s own std::mt19937_64 / std::normal_distribution<T> driven by the
+    reference's fill (vectors written by tests/golden/make_golden.py)."""
     g = np.load(os.path.join(GOLDEN, "probe_streams.npz"))
-    assert len(g.files) == 6
+    assert len(g.files) == 15
     for key in g.files:
         dist, root, blk, v0, L, V = key.split("_")
-        Z = oracle.probe_block(int(root), int(blk), int(v0), int(L), int(V), dist)
+        fp32 = dist.endswith("32")
+        Z = oracle.probe_block(int(root), int(blk), int(v0), int(L), int(V), dist.rstrip("32"), fp32=fp32)
         np.testing.assert_array_equal(Z, g[key], err_msg=key)
+
+
+def test_spherical_columns_have_norm_sqrt_l(oracle):
+    """Spherical probes: each column of a block has squared norm L (to rounding)."""
+    for fp32 in (False, True):
+        Z = oracle.probe_block(9, 3000, 5, 41, 6, "spherical", fp32=fp32)
+        np.testing.assert_allclose((Z * Z).sum(axis=0), 41.0, rtol=1e-5 if fp32 else 1e-13)
 
 
 def test_rademacher_uses_low_bit_column_major(oracle):
