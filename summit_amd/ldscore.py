@@ -42,6 +42,7 @@ class LDScoreResult:
     nsnps_bin: np.ndarray
     mc_per_snp_sumsq: np.ndarray | None = None
     guard_counts: tuple = (0, 0)
+    mc_var: np.ndarray | None = None  # per-SNP MC variance of ldscore (mc=True)
 
 
 def genomewide_ldscore(geno: engine.Genotype, nvecs: int, seed: int, annot: np.ndarray | None = None,
@@ -84,8 +85,9 @@ def genomewide_ldscore(geno: engine.Genotype, nvecs: int, seed: int, annot: np.n
     meansq_h = meansq.cpu().numpy()
     nsnps_bin = annot.sum(axis=0, dtype=np.float64)
     ld = meansq_h / float(nvecs) - (nsnps_bin / float(N - ddof))[None, :]
-    return LDScoreResult(ld, meansq_h, inv.cpu().numpy(), int(nvecs), nsnps_bin,
-                         None if mc_ps is None else mc_ps.cpu().numpy(), tuple(guards))
+    mc_ps_h = None if mc_ps is None else mc_ps.cpu().numpy()
+    mc_var = None if mc_ps_h is None else mc_variance_from_probe_sums(meansq_h, mc_ps_h, int(nvecs))
+    return LDScoreResult(ld, meansq_h, inv.cpu().numpy(), int(nvecs), nsnps_bin, mc_ps_h, tuple(guards), mc_var)
 
 
 def write_gw_ldscore_table(outpath, snpdf, scores_df, *, snp_var=None, ld_rel=None) -> str:
@@ -115,3 +117,67 @@ def write_gw_annotation_masses(outpath, masses) -> str:
     with open(path, "w") as f:
         f.write("\t".join(f"{float(x):.10g}" for x in masses) + "\n")
     return path
+
+
+def mc_variance_from_probe_sums(probe_sum, probe_sumsq, nvecs: int, return_info: bool = False):
+    """Monte Carlo variance of a probe mean, sample_var(Y_v) / V, from S1 = sum_v Y_v
+    and S2 = sum_v Y_v^2 per element: (S2 - S1^2 / V) / (V (V - 1))
+    (src/summit/ldscore/gw_ldscore.py:874-944).  A negative centred sum within
+    512 eps max(|S2|, |S1^2/V|, 1) is round-off and becomes 0; beyond that, or
+    non-finite, the element is NaN.  V < 2 gives NaN everywhere."""
+    first = np.asarray(probe_sum)
+    second = np.asarray(probe_sumsq)
+    if first.shape != second.shape:
+        raise ValueError("probe_sum and probe_sumsq must have identical shapes")
+    if first.ndim != 2:
+        raise ValueError("probe_sum and probe_sumsq must be two-dimensional")
+    v = int(nvecs)
+    if v < 2:
+        res = np.full(first.shape, np.nan)
+        info = {"n_roundoff_clipped": np.zeros(first.shape[1], dtype=np.int64),
+                "n_numerical_failures": np.full(first.shape[1], first.shape[0], dtype=np.int64)}
+        return (res, info) if return_info else res
+    eps = np.finfo(first.dtype).eps if np.issubdtype(first.dtype, np.floating) else np.finfo(np.float64).eps
+    f = first.astype(np.float64)
+    s = second.astype(np.float64)
+    corr = f * f / float(v)
+    centred = s - corr
+    tol = 512.0 * eps * np.maximum(np.maximum(np.abs(s), np.abs(corr)), 1.0)
+    finite = np.isfinite(centred) & np.isfinite(tol)
+    tiny = finite & (centred < 0.0) & (centred >= -tol)
+    bad = ~finite | (centred < -tol)
+    centred = np.where(tiny, 0.0, centred)
+    centred = np.where(bad, np.nan, centred)
+    res = centred / float(v * (v - 1))
+    info = {"n_roundoff_clipped": tiny.sum(axis=0).astype(np.int64),
+            "n_numerical_failures": bad.sum(axis=0).astype(np.int64)}
+    return (res, info) if return_info else res
+
+
+def integrated_mc_variance_from_probe_sums(per_snp_probe_sum, aggregate_probe_sumsq, nvecs: int,
+                                           return_info: bool = False):
+    """Sum over SNPs of the per-SNP MC variances from the per-SNP first sums and the
+    per-bin aggregate second sums (src/summit/ldscore/gw_ldscore.py:947-995)."""
+    first = np.asarray(per_snp_probe_sum)
+    second = np.asarray(aggregate_probe_sumsq, dtype=np.float64)
+    if first.ndim != 2 or second.shape != (first.shape[1],):
+        raise ValueError("aggregate MC sufficient-statistic shapes are inconsistent")
+    v = int(nvecs)
+    if v < 2:
+        res = np.full(second.shape, np.nan)
+        info = {"n_roundoff_clipped": np.zeros(second.shape, dtype=np.int64),
+                "n_numerical_failures": np.ones(second.shape, dtype=np.int64)}
+        return (res, info) if return_info else res
+    f = first.astype(np.float64)
+    fsq = np.einsum("ij,ij->j", f, f)
+    centred = second - fsq / float(v)
+    eps = np.finfo(first.dtype).eps if np.issubdtype(first.dtype, np.floating) else np.finfo(np.float64).eps
+    tol = 512.0 * eps * np.maximum(np.maximum(np.abs(second), np.abs(fsq / float(v))), 1.0)
+    finite = np.isfinite(centred) & np.isfinite(tol)
+    tiny = finite & (centred < 0.0) & (centred >= -tol)
+    bad = ~finite | (centred < -tol)
+    centred = np.where(tiny, 0.0, centred)
+    centred = np.where(bad, np.nan, centred)
+    res = centred / float(v * (v - 1))
+    info = {"n_roundoff_clipped": tiny.astype(np.int64), "n_numerical_failures": bad.astype(np.int64)}
+    return (res, info) if return_info else res

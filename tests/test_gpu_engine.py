@@ -133,8 +133,10 @@ def test_phase1_spherical_probes_match_oracle(engine, oracle):
         e = min(m, s + step)
         X = oracle.read_block_standardized(bed, n, s, e, None, 1, "mean", 0)
         ref += oracle.phase1_chunk(X, annot[s:e], inv[s:e], oracle.probe_block(4242, s, 3, e - s, V, "spherical"))
+    # real-valued probes: the 38-bit column quantization is relative to the largest
+    # |w z| of the column, which Gaussian tails push to several times the typical one
     err = np.abs(xz.cpu().numpy().T - ref) / np.abs(ref).max(axis=0, keepdims=True)
-    assert err.max() < 1e-11, err.max()
+    assert err.max() < 3e-11, err.max()
     g.close()
 
 
