@@ -238,6 +238,106 @@ def ingest_rate(engine, torch, geno, n_snps: int) -> dict:
             "upload_prepare_gbs": round(body.nbytes / (t2 - t0) / 1e9, 2)}
 
 
+def learned_cv(engine, ldscore, torch, dev, plain_max: int = 98304, plain_tile: int = 2048) -> dict:
+    """Learned control variate at N=50,000, M=200,000 (block-Markov genotypes,
+    MAF ~ U(0.01, 0.5), 0.5% missing, 8 random 10% annotations).  The network
+    is trained on seed 505 against a 1,000-probe plain run (root 1505) and
+    evaluated on seed 506 (root 1506): control variate with B=8 probes vs plain
+    B=8, plain B=1,000 and the exact LD scores (full-window int8 Gram).  The
+    plain estimator's time to <= 1% mean relative error comes from one run in
+    tiles of ``plain_tile`` probes, the error of the running mean checked after
+    each tile outside the timed region."""
+    from summit_amd import cv, mlp
+    N, M = 50_000, 200_000
+
+    def data(seed):
+        g = engine.Genotype.synthetic(N, M, seed=seed, maf_lo=0.01, maf_hi=0.5, miss_rate=0.005, block_len=100,
+                                      copy_prob=0.9)
+        g.prepare(ddof=1, impute="mean")
+        annot = (np.random.default_rng(seed + 1000).random((M, 8)) < 0.1).astype(np.float64)
+        return g, annot
+
+    out = {"workload": "N=50000 M=200000 block-Markov genotypes, 8 annotations; train seed 505 (1000-probe "
+                       "labels), evaluate seed 506; error = mean |L_hat - L| / L against exact LD"}
+    ga, annot_a = data(505)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fa = cv.snp_features(ga, annot_a)
+    t1 = time.perf_counter()
+    lab = ldscore.genomewide_ldscore(ga, 1000, seed=1505, vtiles=ldscore.bounded_vtile_sizes(1000, 512), mc=True)
+    t2 = time.perf_counter()
+    model = cv.fit(fa, lab.ldscore[:, 0], lab.mc_var[:, 0], epochs=40, batch=4096, seed=0)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    ga.close()
+    out["training"] = {"features_s": round(t1 - t0, 4), "labels_1000_probes_s": round(t2 - t1, 4),
+                       "train_s": round(t3 - t2, 4), "epochs": model.info["epochs"],
+                       "best_epoch": model.info["best_epoch"], "steps": model.info["steps"],
+                       "tau2": model.tau2, "tau2_raw": model.tau2_raw}
+    log(f"cv training: {out['training']}")
+
+    gb, annot_b = data(506)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    exact = cv.exact_ldscore(gb)
+    out["exact_s"] = round(time.perf_counter() - t0, 4)
+
+    def err(est):
+        return float(np.mean(np.abs(est - exact) / np.abs(exact)))
+
+    p1000 = ldscore.genomewide_ldscore(gb, 1000, seed=1506, vtiles=ldscore.bounded_vtile_sizes(1000, 512))
+    net = mlp.Net(model.in_dim, max_rows=M)
+    cv.cv_ldscore(gb, model, annot_b, nvecs=8, seed=1507, net=net)  # warm-up (kernel loads, workspaces)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = cv.cv_ldscore(gb, model, annot_b, nvecs=8, seed=1506, net=net)
+    torch.cuda.synchronize()
+    t_cv = time.perf_counter() - t0
+    feats_b = cv.snp_features(gb, annot_b)
+    base = feats_b[:, 0] + cv.remainder_scale(feats_b[:, 0]) * model.resid_mean  # same features, no network
+    out["errors"] = {"cv_b8": round(err(res.ldscore), 5), "plain_b8": round(err(res.plain), 5),
+                     "prediction": round(err(res.prediction), 5), "inwindow_plus_mean_resid": round(err(base), 5),
+                     "plain_b1000": round(err(p1000.ldscore[:, 0]), 5),
+                     "cv_b8_vs_plain_b1000": round(float(np.mean(np.abs(res.ldscore - p1000.ldscore[:, 0]) /
+                                                                np.abs(p1000.ldscore[:, 0]))), 5)}
+    out["cv_seconds"] = round(t_cv, 4)
+    out["cv_parts_s"] = {k: round(v, 4) for k, v in res.seconds.items()}
+    out["cv_fallback_snps"] = res.fallback
+    net.close()
+    log(f"cv errors: {out['errors']}, cv {t_cv:.3f} s")
+
+    # plain estimator until <= 1% mean relative error
+    inv = engine.resvar_inv(gb, 0, M)
+    annot1 = torch.ones((M, 1), dtype=torch.float64, device=dev)
+    meansq = torch.zeros((M, 1), dtype=torch.float64, device=dev)
+    exact_d = torch.from_numpy(exact).to(dev)
+    xz = torch.empty((plain_tile, N), dtype=torch.float64, device=dev)
+    elapsed, v0, reached, trace = 0.0, 0, None, []
+    while v0 < plain_max:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        xz.zero_()
+        engine.phase1_xz(gb, 0, M, 1000, annot1, inv, v0, plain_tile, 1506, xz)
+        engine.phase2_accum(gb, 0, M, inv, plain_tile, xz, meansq, N)
+        torch.cuda.synchronize()
+        elapsed += time.perf_counter() - t0
+        v0 += plain_tile
+        est = meansq[:, 0] / float(v0) - M / float(N - 1)
+        e = float(torch.mean(torch.abs(est - exact_d) / torch.abs(exact_d)).item())
+        if v0 % (4 * plain_tile) == 0:
+            trace.append({"probes": v0, "seconds": round(elapsed, 3), "err": round(e, 5)})
+        if e <= 0.01:
+            reached = {"probes": v0, "seconds": round(elapsed, 4), "err": round(e, 5)}
+            break
+    gb.close()
+    out["plain_to_1pct"] = reached
+    out["plain_trace"] = trace
+    if reached is not None and out["errors"]["cv_b8"] <= 0.01:
+        out["speedup_to_1pct"] = round(reached["seconds"] / t_cv, 1)
+    log(f"plain to 1%: {reached}")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -251,6 +351,8 @@ def main() -> int:
     ap.add_argument("--cpu-snps", type=int, default=4000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-exact", action="store_true")
+    ap.add_argument("--no-cv", action="store_true", help="skip the learned control-variate measurement")
+    ap.add_argument("--only-cv", action="store_true", help="run only the learned control-variate measurement")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the 1%%-error wall-time run and the host->HBM ingest measurement")
     ap.add_argument("--copy-calib", action="store_true",
@@ -275,6 +377,9 @@ def main() -> int:
             torch.neg(x, out=y)
         torch.cuda.synchronize()
         log("copy calibration: 3 x (1 GiB read + 1 GiB write)")
+        return 0
+    if args.only_cv:
+        print(json.dumps({"learned_cv": learned_cv(engine, ldscore, torch, dev)}), flush=True)
         return 0
     N, M, B = args.n, args.m, args.b
     seed = 1303
@@ -387,6 +492,8 @@ def main() -> int:
         log(f"windowed: {extra['windowed']}")
         extra["gxe"] = gxe_rate(engine, torch)
         log(f"gxe: {extra['gxe']}")
+    if not args.no_extras and not args.no_cv:
+        extra["learned_cv"] = learned_cv(engine, ldscore, torch, dev)
     line = {
         "metric": "genotype\u00b7probe Gop/s at N=100k,M=500k,B=64; wall time to 1% LD-score rel err",
         "value": round(value, 1),

@@ -231,6 +231,45 @@ int gpe_probes(int32_t rand_dist, uint64_t root_seed, const int64_t* blk_starts_
                int64_t blk_end, int32_t v_start, int32_t v_count, double* z_host);
 
 /* ------------------------------------------------------------------------
+ * Learned LD-score predictor (no reference counterpart: the control variate
+ * of the stochastic estimator).  A three-layer MLP in_dim -> 128 -> 128 -> 1
+ * with GELU (erf form) after the first two layers; FP32 master weights used
+ * as BF16, inputs and hidden activations as BF16, FP32 accumulation on bf16
+ * MFMA, FP32 biases, Adam(W) with FP32 moments.
+ *
+ * Flat FP32 parameter vector (gpe_mlp_n_params elements): W1 (128 x 32,
+ * row-major; columns >= in_dim are zero), b1 (128), W2 (128 x 128), b2 (128),
+ * w3 (128), b3 (1).  Features are bf16 bit patterns (uint16): x_dev
+ * row-major (rows, >= 32 columns, ldx a multiple of 8, zero beyond in_dim),
+ * and for training also x^T (32 rows of ldxT >= rows).  Calls enqueue work on
+ * `stream` and allocate nothing after gpe_mlp_create, so gpe_mlp_train_step
+ * may be captured in a HIP graph.
+ * ------------------------------------------------------------------------ */
+typedef struct gpe_mlp gpe_mlp;
+
+int gpe_mlp_n_params(int64_t* n);
+/* Network on the current device with workspace for max_rows rows per step;
+ * parameters start at zero (set them with gpe_mlp_set_params). */
+int gpe_mlp_create(int32_t in_dim, int64_t max_rows, gpe_mlp** out);
+int gpe_mlp_free(gpe_mlp* net);
+/* Replace the parameters (synchronous); resets the Adam moments and step. */
+int gpe_mlp_set_params(gpe_mlp* net, const float* params_host, int64_t n);
+int gpe_mlp_get_params(gpe_mlp* net, float* params_host, int64_t n);
+/* Gradient of the last gpe_mlp_train_step (before its update). */
+int gpe_mlp_get_grad(gpe_mlp* net, float* grad_host, int64_t n);
+/* Adam steps taken so far. */
+int gpe_mlp_step_count(gpe_mlp* net, int32_t* step_host);
+/* y_dev[r] = network(x_r) for r < rows (FP32); any row count (processed in
+ * chunks of max_rows). */
+int gpe_mlp_forward(gpe_mlp* net, const uint16_t* x_dev, int64_t ldx, int64_t rows, float* y_dev, void* stream);
+/* One step on rows <= max_rows: loss = mean_r (y_r - t_r)^2 (written to
+ * loss_dev[0] when non-NULL), its gradient, and when update != 0 one Adam(W)
+ * update (decoupled weight decay) with bias correction by the step count. */
+int gpe_mlp_train_step(gpe_mlp* net, const uint16_t* x_dev, int64_t ldx, const uint16_t* xT_dev, int64_t ldxT,
+                       const float* target_dev, int64_t rows, float lr, float beta1, float beta2, float eps,
+                       float weight_decay, int32_t update, float* loss_dev, void* stream);
+
+/* ------------------------------------------------------------------------
  * Kernel timing (no reference counterpart: the reference's per-block phase
  * timers, src/native/gwldcore.cpp:225-258, print wall time to stderr; these
  * record HIP events around the contraction kernels on their own stream).

@@ -47,6 +47,17 @@ SIGNATURES = {
     "gpe_windowed_ld":(ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_dbl, c_i64, c_vp, c_i32, c_vp, c_vp]),
     "gpe_xc_f64":(ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_i64, c_i32, c_vp, c_i64, c_i32, c_i32, c_vp]),
     "gpe_xtv_f64": (ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_i64, c_i32, c_vp, c_i64, c_i32, c_vp]),
+    "gpe_mlp_n_params": (ctypes.c_int, [ctypes.POINTER(c_i64)]),
+    "gpe_mlp_create": (ctypes.c_int, [c_i32, c_i64, ctypes.POINTER(c_vp)]),
+    "gpe_mlp_free": (ctypes.c_int, [c_vp]),
+    "gpe_mlp_set_params": (ctypes.c_int, [c_vp, c_vp, c_i64]),
+    "gpe_mlp_get_params": (ctypes.c_int, [c_vp, c_vp, c_i64]),
+    "gpe_mlp_get_grad": (ctypes.c_int, [c_vp, c_vp, c_i64]),
+    "gpe_mlp_step_count": (ctypes.c_int, [c_vp, ctypes.POINTER(c_i32)]),
+    "gpe_mlp_forward": (ctypes.c_int, [c_vp, c_vp, c_i64, c_i64, c_vp, c_vp]),
+    "gpe_mlp_train_step": (ctypes.c_int, [c_vp, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, ctypes.c_float,
+                                          ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, c_i32,
+                                          c_vp, c_vp]),
     "gpe_profile_enable": (ctypes.c_int, [c_i32]),
     "gpe_profile_read": (ctypes.c_int, [c_i32, ctypes.POINTER(c_dbl), ctypes.POINTER(c_i64)]),
     "gpe_profile_reset": (ctypes.c_int, []),

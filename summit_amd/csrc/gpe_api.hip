@@ -86,6 +86,12 @@ int guarded(F&& f) {
 
 inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+}  // namespace
+
+void gpe::set_last_error(const char* msg) { g_err = msg ? msg : ""; }
+
+namespace {
+
 void require(bool ok, const std::string& msg) {
     if (!ok) throw Error(msg);
 }

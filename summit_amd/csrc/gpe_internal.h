@@ -176,4 +176,8 @@ void launch_resvar_inv(const int32_t* nobs, const int64_t* sum, const int64_t* s
                        const double* inv_sd, int64_t r0, int64_t r1, int64_t n, int completed, double eps,
                        double* inv_out, hipStream_t st);
 
+// Records the calling thread's failure message (returned by gpe_last_error);
+// entry points of other translation units report through it.
+void set_last_error(const char* msg);
+
 }  // namespace gpe
