@@ -238,6 +238,49 @@ def ingest_rate(engine, torch, geno, n_snps: int) -> dict:
             "upload_prepare_gbs": round(body.nbytes / (t2 - t0) / 1e9, 2)}
 
 
+def partitioned(engine, ldscore, torch) -> dict:
+    """Partitioned LD scores, N=50,000, M=200,000 (block-Markov genotypes, seed
+    202, MAF ~ U(0.01, 0.5), 0.5% missing), 8 binary annotations each a random
+    10% of SNPs, B=100 Rademacher probes (root 1202), Monte Carlo variances on:
+    the 5-limb and 6-limb int8 paths (their difference bounds the fixed-point
+    error) and the 5-limb error against the exact partitioned LD scores
+    (full-window Gram) in Monte Carlo standard errors.  Pairs per run: N*M*B*K
+    in each traversal (every bin's columns run over all SNPs)."""
+    from summit_amd import cv
+    N, M, B, K = 50_000, 200_000, 100, 8
+    g = engine.Genotype.synthetic(N, M, seed=202, maf_lo=0.01, maf_hi=0.5, miss_rate=0.005, block_len=100,
+                                  copy_prob=0.9)
+    g.prepare(ddof=1, impute="mean")
+    annot = (np.random.default_rng(1202).random((M, K)) < 0.1).astype(np.float64)
+    out = {"workload": "N=50000 M=200000, 8 random 10% annotations, B=100 Rademacher, mean imputation"}
+    res = {}
+    for limbs in (5, 6):
+        ldscore.genomewide_ldscore(g, B, seed=1, annot=annot, limbs=limbs, mc=True)  # warm-up, same shapes
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res[limbs] = ldscore.genomewide_ldscore(g, B, seed=1202, annot=annot, limbs=limbs, mc=True)
+        torch.cuda.synchronize()
+        out[f"seconds_{limbs}_limbs"] = round(time.perf_counter() - t0, 4)
+    pairs = 2.0 * N * M * B * K
+    out["pairs"] = pairs
+    out["gpairs_per_s_5_limbs"] = round(pairs / out["seconds_5_limbs"] / 1e9, 1)
+    l5, l6 = res[5].ldscore, res[6].ldscore
+    # relative to each bin's mean absolute score (single partitioned scores can be ~0)
+    out["max_rel_diff_5_vs_6_limbs"] = float(np.max(np.abs(l5 - l6) / np.mean(np.abs(l6), axis=0)[None, :]))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    exact = cv.exact_ldscore(g, annot)
+    out["exact_s"] = round(time.perf_counter() - t0, 4)
+    g.close()
+    # statistical check of the reference's own tests: |error| within 5 Monte Carlo standard errors
+    se = np.sqrt(np.maximum(res[5].mc_var, 0.0))
+    zs = np.abs(l5 - exact) / np.maximum(se, 1e-300)
+    out["frac_within_5_mc_se"] = round(float(np.mean(zs <= 5.0)), 6)
+    out["mean_abs_z"] = round(float(np.mean(zs)), 4)
+    out["mean_abs_err_vs_exact"] = round(float(np.mean(np.abs(l5 - exact))), 5)
+    return out
+
+
 def learned_cv(engine, ldscore, torch, dev, plain_max: int = 98304, plain_tile: int = 2048) -> dict:
     """Learned control variate at N=50,000, M=200,000 (block-Markov genotypes,
     MAF ~ U(0.01, 0.5), 0.5% missing, 8 random 10% annotations).  The network
@@ -352,7 +395,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-exact", action="store_true")
     ap.add_argument("--no-cv", action="store_true", help="skip the learned control-variate measurement")
-    ap.add_argument("--only-cv", action="store_true", help="run only the learned control-variate measurement")
+    ap.add_argument("--only", choices=["learned_cv", "partitioned"],
+                    help="run only one extra measurement and print it")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the 1%%-error wall-time run and the host->HBM ingest measurement")
     ap.add_argument("--copy-calib", action="store_true",
@@ -378,8 +422,11 @@ def main() -> int:
         torch.cuda.synchronize()
         log("copy calibration: 3 x (1 GiB read + 1 GiB write)")
         return 0
-    if args.only_cv:
+    if args.only == "learned_cv":
         print(json.dumps({"learned_cv": learned_cv(engine, ldscore, torch, dev)}), flush=True)
+        return 0
+    if args.only == "partitioned":
+        print(json.dumps({"partitioned": partitioned(engine, ldscore, torch)}), flush=True)
         return 0
     N, M, B = args.n, args.m, args.b
     seed = 1303
@@ -492,6 +539,9 @@ def main() -> int:
         log(f"windowed: {extra['windowed']}")
         extra["gxe"] = gxe_rate(engine, torch)
         log(f"gxe: {extra['gxe']}")
+    if not args.no_extras:
+        extra["partitioned"] = partitioned(engine, ldscore, torch)
+        log(f"partitioned: {extra['partitioned']}")
     if not args.no_extras and not args.no_cv:
         extra["learned_cv"] = learned_cv(engine, ldscore, torch, dev)
     line = {

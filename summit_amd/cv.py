@@ -67,17 +67,22 @@ def snp_features(geno: engine.Genotype, annot_bits: np.ndarray | None = None,
     return np.ascontiguousarray(np.concatenate(parts, axis=1))
 
 
-def exact_ldscore(geno: engine.Genotype, ddof: int = 1) -> np.ndarray:
-    """Exact LD scores L_j = sum_i r_ij^2 - M/(N - ddof) over every SNP pair:
-    the banded int8-MFMA Gram with a window spanning all SNPs gives
-    sum_i r~_ij^2 with r~^2 = r^2 - (1 - r^2)/(N - 2), from which
-    sum_i r_ij^2 = (sum_i r~_ij^2 + M/(N - 2)) (N - 2)/(N - 1).  r is the
-    correlation of the mean-imputed genotypes, the one the stochastic
-    estimator targets (its residual scales normalize each column)."""
+def exact_ldscore(geno: engine.Genotype, annot: np.ndarray | None = None, ddof: int = 1) -> np.ndarray:
+    """Exact (partitioned) LD scores L_jk = sum_i r_ij^2 a_ik - M_k/(N - ddof),
+    M_k = sum_i a_ik, over every SNP pair: the banded int8-MFMA Gram with a
+    window spanning all SNPs gives sum_i r~_ij^2 a_ik with
+    r~^2 = r^2 - (1 - r^2)/(N - 2), from which
+    sum_i r_ij^2 a_ik = (sum_i r~_ij^2 a_ik + M_k/(N - 2)) (N - 2)/(N - 1).
+    r is the correlation of the mean-imputed genotypes, the one the stochastic
+    estimator targets (its residual scales normalize each column).  Returns
+    (M,) without annotations, else (M, K)."""
     M, N = geno.m, geno.n
-    w = winldcore.windowed_ld(geno, 0, M, window_snps=max(1, M))[:, 0]
-    s = (w + M / (N - 2.0)) * (N - 2.0) / (N - 1.0)
-    return s - M / float(N - ddof)
+    a = np.ones((M, 1)) if annot is None else np.asarray(annot, dtype=np.float64)
+    mk = a.sum(axis=0)
+    w = winldcore.windowed_ld(geno, 0, M, annot=a, window_snps=max(1, M))
+    s = (w + mk[None, :] / (N - 2.0)) * (N - 2.0) / (N - 1.0)
+    out = s - mk[None, :] / float(N - ddof)
+    return out[:, 0] if annot is None else out
 
 
 def remainder_scale(w: np.ndarray) -> np.ndarray:
