@@ -395,6 +395,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-exact", action="store_true")
     ap.add_argument("--no-cv", action="store_true", help="skip the learned control-variate measurement")
+    ap.add_argument("--no-tune", action="store_true", help="keep the built-in schedule heuristics")
     ap.add_argument("--only", choices=["learned_cv", "partitioned"],
                     help="run only one extra measurement and print it")
     ap.add_argument("--no-extras", action="store_true",
@@ -437,6 +438,15 @@ def main() -> int:
     torch.cuda.synchronize()
     log(f"genotypes N={N} M={M} generated+prepared in {time.perf_counter() - t_gen:.1f} s, "
         f"missing calls {geno.n_missing()}")
+
+    # schedule: a cached choice for this device and shape, else tune now (outside the timed region)
+    from summit_amd import autotune
+    tuned = None
+    if not args.no_tune:
+        tuned = autotune.apply_cached(N, M, 1, B, args.limbs)
+        if tuned is None:
+            tuned = autotune.tune_phase1(geno, B, 1, args.limbs, seed=seed)
+        log(f"autotune: {tuned}")
 
     annot = torch.ones((M, 1), dtype=torch.float64, device=dev)
     inv = engine.resvar_inv(geno, 0, M)
@@ -563,6 +573,7 @@ def main() -> int:
                    "n": N, "m": M, "b": B, "limbs": args.limbs, "impute": args.impute},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "autotune": tuned,
         **extra,
     }
     print(json.dumps(line), flush=True)

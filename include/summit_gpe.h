@@ -280,13 +280,32 @@ enum {
     GPE_PROF_GZ = 2,          /* exact G.Z */
     GPE_PROF_GTU = 3,         /* exact G^T U */
     GPE_PROF_GRAM = 4,        /* banded Gram of windowed LD */
-    GPE_PROF_KINDS = 5
+    GPE_PROF_P1_FINALIZE = 5, /* phase-1 missing-call finalize (all SNP-range launches of one traversal) */
+    GPE_PROF_KINDS = 6
 };
 int gpe_profile_enable(int32_t on);
 /* Accumulated milliseconds and launch count of one kernel kind (synchronizes
  * on the recorded events). */
 int gpe_profile_read(int32_t kind, double* total_ms, int64_t* launches);
 int gpe_profile_reset(void);
+
+/* ------------------------------------------------------------------------
+ * Schedule table (no reference counterpart: the reference's OpenMP/BLAS
+ * blocking is fixed at build time).  Per shape key (rows of the contraction,
+ * contraction length, padded columns, limbs) and knob, a value that overrides
+ * the built-in heuristic; filled by the autotuner (summit_amd/autotune.py),
+ * which times the candidates on the device.  value <= 0 removes the entry.
+ *   GPE_SCHED_P1_SPLITS:        K splits of the phase-1 contraction (1..64)
+ *   GPE_SCHED_P1_FINALIZE_KIB:  SNP-range size of the phase-1 missing-call
+ *                               finalize, in KiB of records (64..65536)
+ * Phase-1 keys: rows = samples, length = SNPs of the call, columns = bins x
+ * probes rounded up to 32 per bin.
+ * ------------------------------------------------------------------------ */
+enum { GPE_SCHED_P1_SPLITS = 0, GPE_SCHED_P1_FINALIZE_KIB = 1, GPE_SCHED_KNOBS = 2 };
+int gpe_schedule_set(int32_t knob, int64_t rows, int64_t length, int32_t columns, int32_t limbs, int32_t value);
+/* The value in force for a key (0 = the heuristic decides). */
+int gpe_schedule_get(int32_t knob, int64_t rows, int64_t length, int32_t columns, int32_t limbs, int32_t* value);
+int gpe_schedule_clear(void);
 
 #ifdef __cplusplus
 }

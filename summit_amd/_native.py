@@ -58,6 +58,9 @@ SIGNATURES = {
     "gpe_mlp_train_step": (ctypes.c_int, [c_vp, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, ctypes.c_float,
                                           ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, c_i32,
                                           c_vp, c_vp]),
+    "gpe_schedule_set": (ctypes.c_int, [c_i32, c_i64, c_i64, c_i32, c_i32, c_i32]),
+    "gpe_schedule_get": (ctypes.c_int, [c_i32, c_i64, c_i64, c_i32, c_i32, ctypes.POINTER(c_i32)]),
+    "gpe_schedule_clear": (ctypes.c_int, []),
     "gpe_profile_enable": (ctypes.c_int, [c_i32]),
     "gpe_profile_read": (ctypes.c_int, [c_i32, ctypes.POINTER(c_dbl), ctypes.POINTER(c_i64)]),
     "gpe_profile_reset": (ctypes.c_int, []),

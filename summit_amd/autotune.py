@@ -1,0 +1,159 @@
+"""Autotuner of the traversal schedule.
+
+The phase-1 contraction splits its K range (SNPs) over ``splits`` workgroup
+layers whose int64 partial sums meet atomically, and the phase-1 missing-call
+finalize walks the missing-call records in SNP ranges sized to stay resident
+in each XCD's L2.  The library picks both by heuristics (the smallest split
+count whose last wave of workgroups fills >= 90% of the device's resident
+slots; 2 MiB ranges).  ``tune_phase1`` times the candidates on the device
+with HIP events around the kernels themselves (``gpe_profile_*``), keeps the
+fastest in the library's schedule table for that shape key, and records the
+choice in a JSON cache so later processes on the same device and shape start
+from it (``apply_cached``).
+
+Shape key: (rows = samples, length = SNPs of the traversal, columns = bins x
+probes rounded up to 32 per bin, limbs), as the library looks it up.
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+import time
+
+import torch
+
+from . import engine
+from ._native import c_i32, call
+
+SCHED_P1_SPLITS = 0
+SCHED_P1_FINALIZE_KIB = 1
+_KNOBS = {"p1_splits": SCHED_P1_SPLITS, "p1_finalize_kib": SCHED_P1_FINALIZE_KIB}
+
+
+def cache_path() -> str:
+    return os.environ.get("SUMMIT_AMD_SCHEDULE_CACHE",
+                          os.path.join(os.path.expanduser("~"), ".cache", "summit_amd", "schedule.json"))
+
+
+def _device_key() -> str:
+    p = torch.cuda.get_device_properties(torch.cuda.current_device())
+    return f"{p.name}|{getattr(p, 'gcnArchName', '')}|cu{p.multi_processor_count}"
+
+
+def shape_key(n: int, length: int, n_bins: int, v_count: int, limbs: int) -> tuple:
+    return (int(n), int(length), int(n_bins) * ((int(v_count) + 31) // 32 * 32), int(limbs))
+
+
+def set_value(knob: str, key: tuple, value: int) -> None:
+    call("gpe_schedule_set", _KNOBS[knob], key[0], key[1], key[2], key[3], int(value))
+
+
+def get_value(knob: str, key: tuple) -> int:
+    v = c_i32()
+    call("gpe_schedule_get", _KNOBS[knob], key[0], key[1], key[2], key[3], ctypes.byref(v))
+    return int(v.value)
+
+
+def clear() -> None:
+    call("gpe_schedule_clear")
+
+
+def _load_cache(path: str) -> dict:
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store_cache(path: str, entry_key: str, entry: dict) -> None:
+    data = _load_cache(path)
+    data[entry_key] = entry
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        tmp = path + f".tmp{os.getpid()}"
+        with open(tmp, "w") as f:
+            json.dump(data, f, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+    except OSError:
+        pass  # a read-only home: the choice still holds for this process
+
+
+def _entry_key(key: tuple) -> str:
+    return f"{_device_key()}|n={key[0]}|len={key[1]}|cols={key[2]}|limbs={key[3]}"
+
+
+def apply_cached(n: int, length: int, n_bins: int, v_count: int, limbs: int = 5, path: str | None = None) -> dict | None:
+    """Load a cached choice for this device and shape into the library; returns it (or None)."""
+    key = shape_key(n, length, n_bins, v_count, limbs)
+    entry = _load_cache(path or cache_path()).get(_entry_key(key))
+    if not entry:
+        return None
+    for knob, value in entry.get("choice", {}).items():
+        if knob in _KNOBS:
+            set_value(knob, key, value)
+    return entry
+
+
+def _time_phase1(g, L, annot, inv, v_count, seed, xz, limbs, reps, kind) -> float:
+    """Median milliseconds of one kernel kind over reps phase-1 traversals."""
+    times = []
+    for r in range(reps):
+        engine.profile_enable(True)
+        engine.profile_reset()
+        engine.phase1_xz(g, 0, L, 1000, annot, inv, r * v_count, v_count, seed, xz, limbs=limbs)
+        ms, n = engine.profile_read(kind)
+        engine.profile_enable(False)
+        times.append(ms)
+    times.sort()
+    return times[len(times) // 2]
+
+
+def tune_phase1(g: engine.Genotype, v_count: int, n_bins: int = 1, limbs: int = 5, seed: int = 1, reps: int = 3,
+                split_candidates=(2, 3, 4, 6, 8, 12, 16, 24, 32),
+                range_kib_candidates=(1024, 2048, 4096, 8192, 16384, 32768, 65536),
+                path: str | None = None, store: bool = True) -> dict:
+    """Time phase-1 schedules over every SNP of the prepared handle ``g`` for
+    ``n_bins`` bins of ``v_count`` probes; keep and cache the fastest."""
+    if g.prepared_with is None:
+        raise RuntimeError("genotype handle is not prepared")
+    dev = engine.default_device()
+    N, L = g.n, g.m
+    key = shape_key(N, L, n_bins, v_count, limbs)
+    annot = torch.ones((L, n_bins), dtype=torch.float64, device=dev)
+    inv = engine.resvar_inv(g, 0, L)
+    xz = torch.zeros((n_bins * v_count, N), dtype=torch.float64, device=dev)
+    t0 = time.perf_counter()
+    for knob in _KNOBS:
+        set_value(knob, key, 0)
+    engine.phase1_xz(g, 0, L, 1000, annot, inv, 0, v_count, seed, xz, limbs=limbs)  # warm-up, workspaces
+    torch.cuda.synchronize()
+    heuristic = {"p1_contract": round(_time_phase1(g, L, annot, inv, v_count, seed, xz, limbs, reps, "p1_contract"), 4)}
+    if g.n_missing() > 0:
+        heuristic["p1_finalize"] = round(_time_phase1(g, L, annot, inv, v_count, seed, xz, limbs, reps,
+                                                      "p1_finalize"), 4)
+    split_ms = {}
+    for s in split_candidates:
+        set_value("p1_splits", key, s)
+        split_ms[int(s)] = round(_time_phase1(g, L, annot, inv, v_count, seed, xz, limbs, reps, "p1_contract"), 4)
+    best_split = min(split_ms, key=split_ms.get)
+    set_value("p1_splits", key, best_split)
+    range_ms = {}
+    if g.n_missing() > 0:
+        for kib in range_kib_candidates:
+            set_value("p1_finalize_kib", key, kib)
+            range_ms[int(kib)] = round(_time_phase1(g, L, annot, inv, v_count, seed, xz, limbs, reps, "p1_finalize"), 4)
+        best_range = min(range_ms, key=range_ms.get)
+        set_value("p1_finalize_kib", key, best_range)
+    else:
+        best_range = None
+    torch.cuda.synchronize()
+    choice = {"p1_splits": best_split}
+    if best_range is not None:
+        choice["p1_finalize_kib"] = best_range
+    entry = {"choice": choice, "heuristic_ms": heuristic, "p1_contract_ms": split_ms, "p1_finalize_ms": range_ms,
+             "tune_seconds": round(time.perf_counter() - t0, 3)}
+    if store:
+        _store_cache(path or cache_path(), _entry_key(key), entry)
+    return entry

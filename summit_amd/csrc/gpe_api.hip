@@ -12,6 +12,8 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
+#include <tuple>
 #include <random>
 #include <string>
 #include <thread>
@@ -292,6 +294,31 @@ void parallel_memcpy(uint8_t* dst, const uint8_t* src, int64_t bytes, int n_thre
 // split count is the smallest one whose last round is at least 90% full (the best
 // such fill up to 16 splits otherwise), then raised until no split exceeds
 // max_chunks_per_split.
+// Schedule overrides written by the autotuner: (knob, rows, length, columns,
+// limbs) -> value.  Read on every traversal; guarded by a mutex.
+struct SchedKey {
+    int32_t knob;
+    int64_t rows, length;
+    int32_t columns, limbs;
+    bool operator<(const SchedKey& o) const {
+        return std::tie(knob, rows, length, columns, limbs) < std::tie(o.knob, o.rows, o.length, o.columns, o.limbs);
+    }
+};
+std::mutex& sched_mu() {
+    static std::mutex m;
+    return m;
+}
+std::map<SchedKey, int32_t>& sched_table() {
+    static std::map<SchedKey, int32_t> t;
+    return t;
+}
+int32_t sched_lookup(int32_t knob, int64_t rows, int64_t length, int32_t columns, int32_t limbs) {
+    std::lock_guard<std::mutex> lock(sched_mu());
+    auto& t = sched_table();
+    auto it = t.find(SchedKey{knob, rows, length, columns, limbs});
+    return it == t.end() ? 0 : it->second;
+}
+
 int pick_splits(int64_t grid_xy, int64_t kcount, int64_t max_chunks_per_split, int64_t slots) {
     int64_t best = 1;
     double best_fill = 0.0;
@@ -334,6 +361,37 @@ int gpe_profile_read(int32_t kind, double* total_ms, int64_t* launches) {
         p.pending.clear();
         if (total_ms) *total_ms = p.ms[kind];
         if (launches) *launches = p.count[kind];
+    });
+}
+
+int gpe_schedule_set(int32_t knob, int64_t rows, int64_t length, int32_t columns, int32_t limbs, int32_t value) {
+    return guarded([&] {
+        require(knob >= 0 && knob < GPE_SCHED_KNOBS, "unknown schedule knob");
+        require(rows > 0 && length > 0 && columns > 0 && limbs > 0, "schedule keys must be positive");
+        if (knob == GPE_SCHED_P1_SPLITS) require(value <= 64, "phase-1 splits must be in [1, 64]");
+        if (knob == GPE_SCHED_P1_FINALIZE_KIB)
+            require(value <= 0 || (value >= 64 && value <= 65536), "finalize range must be in [64, 65536] KiB");
+        std::lock_guard<std::mutex> lock(sched_mu());
+        const SchedKey k{knob, rows, length, columns, limbs};
+        if (value <= 0)
+            sched_table().erase(k);
+        else
+            sched_table()[k] = value;
+    });
+}
+
+int gpe_schedule_get(int32_t knob, int64_t rows, int64_t length, int32_t columns, int32_t limbs, int32_t* value) {
+    return guarded([&] {
+        require(knob >= 0 && knob < GPE_SCHED_KNOBS, "unknown schedule knob");
+        require(value != nullptr, "value is null");
+        *value = sched_lookup(knob, rows, length, columns, limbs);
+    });
+}
+
+int gpe_schedule_clear(void) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lock(sched_mu());
+        sched_table().clear();
     });
 }
 
@@ -794,9 +852,14 @@ void run_phase1(gpe_geno* g, int64_t s, int64_t e, const double* w, int n_bins, 
     a.cols_out = ncols_pad;
     a.out_row0 = 0;
     const int64_t gx = ceil_div(g->n, 256);
-    const int splits = pick_splits(gx * groups, kcount, (int64_t)(((int64_t)1 << 31) / (256 * 128)) - 1,
-                                   contract_slots(lc.nl, lc.ctc, false, EPI_I64_ATOMIC));
+    // int32 per-limb partial sums over a split's K range: at most 2^31 / (256 * 128) chunks per split
+    const int64_t max_chunks = (int64_t)(((int64_t)1 << 31) / (256 * 128)) - 1;
+    int splits = sched_lookup(GPE_SCHED_P1_SPLITS, g->n, L, ncols_pad, lc.nl);
+    if (splits <= 0)
+        splits = pick_splits(gx * groups, kcount, max_chunks, contract_slots(lc.nl, lc.ctc, false, EPI_I64_ATOMIC));
+    splits = (int)std::min<int64_t>(std::max<int64_t>(splits, ceil_div(kcount, max_chunks)), kcount);
     a.kc_per_split = ceil_div(kcount, splits);
+    splits = (int)ceil_div(kcount, a.kc_per_split);
     ProfScope ps(GPE_PROF_P1_CONTRACT, st);
     launch_contract(lc.nl, lc.ctc, false, EPI_I64_ATOMIC, a, groups, splits, st);
     ps.stop();
@@ -804,8 +867,11 @@ void run_phase1(gpe_geno* g, int64_t s, int64_t e, const double* w, int n_bins, 
     if (miss && z_i8) {
         void* rec = g->scratch("p1_mrec", p1_miss_rec_bytes(L, n_bins, v_pad));
         launch_p1_miss_rec(w, n_bins, z_i8, ldz, s, e, n_bins, v_count, v_pad, shift, col_scale, g->mean, rec, st);
+        const int32_t kib = sched_lookup(GPE_SCHED_P1_FINALIZE_KIB, g->n, L, ncols_pad, lc.nl);
+        ProfScope pf(GPE_PROF_P1_FINALIZE, st);
         launch_p1_finalize_z8(y, ldy, g->n, n_bins, v_count, v_pad, col_scale, cst, g->miss_ptr, g->miss_idx, s, e,
-                              rec, out, ldo, bin_mode, st);
+                              rec, out, ldo, bin_mode, (int64_t)kib << 10, st);
+        pf.stop();
         return;
     }
     launch_p1_finalize(y, ldy, g->n, n_bins, v_count, v_pad, col_scale, cst, miss ? g->miss_ptr : nullptr,

@@ -144,7 +144,7 @@ int64_t p1_miss_rec_bytes(int64_t n_snps, int n_bins, int v_pad);
 void launch_p1_finalize_z8(const int64_t* y, int64_t ldy, int64_t n, int n_bins, int v_count, int v_pad,
                            const double* col_scale, const double* cst, const int64_t* miss_ptr,
                            const int32_t* miss_idx, int64_t k0, int64_t k1, const void* rec, double* xz,
-                           int64_t ldxz, const uint8_t* col_mode, hipStream_t st);
+                           int64_t ldxz, const uint8_t* col_mode, int64_t range_bytes, hipStream_t st);
 void launch_colmax_f64(const double* x, int64_t ldx, int64_t n, int n_cols, int v_count, int v_pad,
                        uint64_t* colmax_bits, hipStream_t st);
 void launch_quant_p2(const double* x, int64_t ldx, int64_t n, int n_bins, int v_count, int v_pad,

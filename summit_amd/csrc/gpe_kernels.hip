@@ -1242,13 +1242,15 @@ __global__ __launch_bounds__(256) void k_p1_finalize_z8(
 void launch_p1_finalize_z8(const int64_t* y, int64_t ldy, int64_t n, int n_bins, int v_count, int v_pad,
                            const double* col_scale, const double* cst, const int64_t* miss_ptr,
                            const int32_t* miss_idx, int64_t k0, int64_t k1, const void* rec, double* xz,
-                           int64_t ldxz, const uint8_t* col_mode, hipStream_t st) {
+                           int64_t ldxz, const uint8_t* col_mode, int64_t range_bytes, hipStream_t st) {
     if (n <= 0) return;
     const int vw = (int)ceil_div(v_pad, 64);
     const int64_t rec_stride = (int64_t)vw * miss_rec_bytes(n_bins);
-    // SNP ranges whose records (about 2 MiB) stay resident in every XCD's L2
-    // while all samples walk them: one launch per range (and bin group), in order
-    const int64_t range = std::max<int64_t>(4096, ((int64_t)2 << 20) / rec_stride);
+    // SNP ranges whose records (range_bytes, default about 2 MiB) stay resident
+    // in every XCD's L2 while all samples walk them: one launch per range (and
+    // bin group), in order
+    if (range_bytes <= 0) range_bytes = (int64_t)2 << 20;
+    const int64_t range = std::max<int64_t>(4096, range_bytes / rec_stride);
     const uint8_t* r = static_cast<const uint8_t*>(rec);
     for (int64_t ra = k0; ra < k1 || ra == k0; ra += range) {
         const int64_t rb = std::min(k1, ra + range);

@@ -49,3 +49,26 @@ def test_error_path_reports_message_without_device_work():
 def test_version_string():
     from summit_amd import _native
     assert b"gfx950" in _native.load().gpe_version()
+
+
+def test_schedule_table_set_get_clear_and_validation():
+    from summit_amd import _native, autotune
+    key = autotune.shape_key(1000, 5000, 2, 40, 5)
+    assert key == (1000, 5000, 128, 5)
+    autotune.clear()
+    assert autotune.get_value("p1_splits", key) == 0
+    autotune.set_value("p1_splits", key, 4)
+    autotune.set_value("p1_finalize_kib", key, 1024)
+    assert autotune.get_value("p1_splits", key) == 4
+    assert autotune.get_value("p1_finalize_kib", key) == 1024
+    assert autotune.get_value("p1_splits", (1000, 5000, 128, 6)) == 0
+    autotune.set_value("p1_splits", key, 0)  # removes the entry
+    assert autotune.get_value("p1_splits", key) == 0
+    with pytest.raises(_native.GpeError, match="splits"):
+        autotune.set_value("p1_splits", key, 65)
+    with pytest.raises(_native.GpeError, match="finalize range"):
+        autotune.set_value("p1_finalize_kib", key, 8)
+    with pytest.raises(_native.GpeError, match="unknown schedule knob"):
+        _native.call("gpe_schedule_set", 7, 1, 1, 1, 1, 1)
+    autotune.clear()
+    assert autotune.get_value("p1_finalize_kib", key) == 0

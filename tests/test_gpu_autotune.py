@@ -1,0 +1,48 @@
+"""The schedule autotuner on the GPU: it times candidates, keeps the fastest in
+the library's table and cache, and a tuned schedule gives the same phase-1
+products as the heuristic one."""
+from __future__ import annotations
+
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def test_tune_phase1_caches_and_keeps_results():
+    from summit_amd import autotune, engine
+    dev = engine.default_device()
+    N, M, V = 2000, 5000, 40
+    print(f"autotune phase 1 N={N} M={M} V={V}", flush=True)
+    g = engine.Genotype.synthetic(N, M, seed=9, maf_lo=0.05, maf_hi=0.5, miss_rate=0.01, block_len=100, copy_prob=0.5)
+    g.prepare(ddof=1, impute="mean")
+    annot = torch.ones((M, 1), dtype=torch.float64, device=dev)
+    inv = engine.resvar_inv(g, 0, M)
+    key = autotune.shape_key(N, M, 1, V, 5)
+    autotune.clear()
+    ref = torch.zeros((V, N), dtype=torch.float64, device=dev)
+    engine.phase1_xz(g, 0, M, 1000, annot, inv, 0, V, 77, ref)
+    path = os.path.join(tempfile.mkdtemp(), "sched.json")
+    entry = autotune.tune_phase1(g, V, reps=1, split_candidates=(1, 2, 7), range_kib_candidates=(64, 4096),
+                                 path=path)
+    assert entry["choice"]["p1_splits"] in (1, 2, 7)
+    assert entry["choice"]["p1_finalize_kib"] in (64, 4096)
+    assert autotune.get_value("p1_splits", key) == entry["choice"]["p1_splits"]
+    for s in (1, 2, 7):
+        for kib in (64, 4096):
+            autotune.set_value("p1_splits", key, s)
+            autotune.set_value("p1_finalize_kib", key, kib)
+            got = torch.zeros((V, N), dtype=torch.float64, device=dev)
+            engine.phase1_xz(g, 0, M, 1000, annot, inv, 0, V, 77, got)
+            scale = float(ref.abs().max())
+            assert float((got - ref).abs().max()) <= 1e-13 * scale, (s, kib)
+    autotune.clear()
+    loaded = autotune.apply_cached(N, M, 1, V, 5, path=path)
+    assert loaded is not None and loaded["choice"] == entry["choice"]
+    assert autotune.get_value("p1_splits", key) == entry["choice"]["p1_splits"]
+    autotune.clear()
+    g.close()
