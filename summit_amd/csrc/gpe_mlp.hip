@@ -23,7 +23,7 @@
 
 using namespace gpe;
 
-namespace {
+namespace gpe_mlpk {
 
 constexpr int kH = 128;        // hidden width
 constexpr int kInPad = 32;     // input width after zero padding
@@ -410,7 +410,9 @@ void require(bool ok, const std::string& msg) {
     if (!ok) throw Error(msg);
 }
 
-}  // namespace
+}  // namespace gpe_mlpk
+
+using namespace gpe_mlpk;
 
 struct gpe_mlp {
     int device = 0;
