@@ -157,7 +157,8 @@ def wall_to_one_percent(engine, ldscore, torch, dev) -> dict:
 def windowed_rate(engine, torch) -> dict:
     """Exact windowed LD scores, N=50,000, M=300,000 on 22 chromosomes (block-Markov
     genotypes, seed 404), SNP positions from a Poisson process with mean spacing
-    3 kb, +-1 Mb windows: one banded-Gram call per chromosome.  Pairs = N x sum of
+    3 kb, +-1 Mb windows: every chromosome's windows in one banded-Gram launch,
+    against the baseline of one launch per chromosome.  Pairs = N x sum of
     window sizes (each SNP with every SNP of its window, itself included)."""
     from summit_amd import winldcore
     N, M, n_chr = 50_000, 300_000, 22
@@ -176,20 +177,33 @@ def windowed_rate(engine, torch) -> dict:
         pairs += int((hi - lo).sum()) * N
         chrs.append((s0, s1, bp))
     winldcore.windowed_ld(g, chrs[0][0], chrs[0][1], bp=chrs[0][2], ld_wind_kb=1000.0)  # warm-up
+    winldcore.windowed_ld_multi(g, chrs, ld_wind_kb=1000.0)
+    # baseline: one launch per chromosome
     torch.cuda.synchronize()
     engine.profile_enable(True)
     engine.profile_reset()
     t0 = time.perf_counter()
-    for s0, s1, bp in chrs:
-        winldcore.windowed_ld(g, s0, s1, bp=bp, ld_wind_kb=1000.0)
+    per_chr = [winldcore.windowed_ld(g, s0, s1, bp=bp, ld_wind_kb=1000.0) for s0, s1, bp in chrs]
+    torch.cuda.synchronize()
+    t_chr = time.perf_counter() - t0
+    gram_chr_ms, gram_chr_n = engine.profile_read("gram")
+    # every chromosome's windows in one launch
+    engine.profile_reset()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    allc = winldcore.windowed_ld_multi(g, chrs, ld_wind_kb=1000.0)
     torch.cuda.synchronize()
     t = time.perf_counter() - t0
     gram_ms, gram_n = engine.profile_read("gram")
     engine.profile_enable(False)
     g.close()
+    ref = np.concatenate(per_chr, axis=0)
     return {"workload": "N=50000 M=300000, 22 chromosomes, Poisson positions (3 kb), +-1 Mb windows",
             "seconds": round(t, 4), "pairs": pairs, "gpairs_per_s": round(pairs / t / 1e9, 1),
-            "gram_kernel_ms": round(gram_ms, 3), "gram_launches": gram_n}
+            "gram_kernel_ms": round(gram_ms, 3), "gram_launches": gram_n,
+            "per_chromosome_launches": {"seconds": round(t_chr, 4), "gram_kernel_ms": round(gram_chr_ms, 3),
+                                        "gram_launches": gram_chr_n},
+            "max_abs_diff_vs_per_chromosome": float(np.max(np.abs(allc - ref)))}
 
 
 def gxe_rate(engine, torch) -> dict:
@@ -396,7 +410,7 @@ def main() -> int:
     ap.add_argument("--no-exact", action="store_true")
     ap.add_argument("--no-cv", action="store_true", help="skip the learned control-variate measurement")
     ap.add_argument("--no-tune", action="store_true", help="keep the built-in schedule heuristics")
-    ap.add_argument("--only", choices=["learned_cv", "partitioned"],
+    ap.add_argument("--only", choices=["learned_cv", "partitioned", "gxe", "windowed"],
                     help="run only one extra measurement and print it")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the 1%%-error wall-time run and the host->HBM ingest measurement")
@@ -428,6 +442,12 @@ def main() -> int:
         return 0
     if args.only == "partitioned":
         print(json.dumps({"partitioned": partitioned(engine, ldscore, torch)}), flush=True)
+        return 0
+    if args.only == "gxe":
+        print(json.dumps({"gxe": gxe_rate(engine, torch)}), flush=True)
+        return 0
+    if args.only == "windowed":
+        print(json.dumps({"windowed": windowed_rate(engine, torch)}), flush=True)
         return 0
     N, M, B = args.n, args.m, args.b
     seed = 1303

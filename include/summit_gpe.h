@@ -198,6 +198,17 @@ int gpe_xtv_f64(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const double* v
 int gpe_windowed_ld(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const int64_t* bp_host, double window_bp,
                     int64_t window_snps, const double* annot_dev, int32_t n_bins, double* out_dev, void* stream);
 
+/* The same for n_chr chromosomes in ONE banded-Gram launch (every
+ * chromosome's tile pairs in one work list): chromosome c is SNPs
+ * [chr_bounds_host[2c], chr_bounds_host[2c+1]) (sorted, non-overlapping);
+ * rows of bp_host, annot_dev and out_dev are SNP ids minus chr_bounds_host[0]
+ * (rows between chromosomes are left 0).  The reference runs
+ * compute_windowed_ld_chr once per chromosome (src/summit/ldscore/win_ldscore.py:974);
+ * results equal those per-chromosome calls.  Synchronizes. */
+int gpe_windowed_ld_multi(gpe_geno* g, int32_t n_chr, const int64_t* chr_bounds_host, const int64_t* bp_host,
+                          double window_bp, int64_t window_snps, const double* annot_dev, int32_t n_bins,
+                          double* out_dev, void* stream);
+
 /* GxE probe streams (src/summit/ldscore/gwe_ldscore.py:2484-2505, native
  * restatement src/native/gxeldcore.cpp:10002-10138): block b (SNPs
  * [blk_starts[b], next start or blk_end)) and probe v draw NumPy Philox4x64-10

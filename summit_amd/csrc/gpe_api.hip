@@ -950,6 +950,96 @@ void copy_rows(const double* src, int64_t lds, int64_t rows, int cols, double* d
                              rows, hipMemcpyDeviceToDevice, st));
 }
 
+// Exact windowed LD of n_chr chromosomes [bounds[2c], bounds[2c+1]) (sorted,
+// non-overlapping, absolute SNP ids) in ONE banded-Gram launch: every
+// chromosome's 64x64 tile pairs within the window go into one work list.
+// Rows of bp (when used), annot and out are SNP ids relative to bounds[0].
+void windowed_ld_impl(gpe_geno* g, int n_chr, const int64_t* bounds, const int64_t* bp_host, double window_bp,
+                      int64_t window_snps, const double* annot_dev, int32_t n_bins, double* out_dev,
+                      hipStream_t st) {
+    require(n_bins > 0, "annot_chr must have at least one column");
+    require(g->n > 2, "Too few rows for windowed LD score computation");
+    const double corr_dim = (double)g->n - 1.0;
+    require(corr_dim > 1.0, "Too few residual dimensions for unbiased windowed r^2 correction");
+    const int64_t row0 = bounds[0];
+    for (int c = 0; c < n_chr; ++c) {
+        const int64_t b = bounds[2 * c], e = bounds[2 * c + 1];
+        require(e > b, "Chromosome slice is empty");
+        require(b >= 0 && e <= g->m, "SNP range out of bounds");
+        require(c == 0 || b >= bounds[2 * c - 1], "chromosome SNP ranges must be sorted and must not overlap");
+    }
+    const int64_t rows = bounds[2 * n_chr - 1] - row0;
+    if (window_snps <= 0) {
+        require(bp_host != nullptr, "bp length mismatch in compute_windowed_ld_chr");
+        require(std::isfinite(window_bp) && window_bp > 0.0, "ld_wind_kb must define a finite positive BP window");
+        for (int c = 0; c < n_chr; ++c)
+            for (int64_t i = bounds[2 * c] + 1; i < bounds[2 * c + 1]; ++i)
+                require(bp_host[i - row0] >= bp_host[i - 1 - row0],
+                        "BP not sorted within chromosome block. Sort your .bim by CHR+BP.");
+    }
+    // (ti, tj, chromosome) tile pairs of every band
+    std::vector<int32_t> pairs;
+    for (int c = 0; c < n_chr; ++c) {
+        const int64_t b = bounds[2 * c], m = bounds[2 * c + 1] - b;
+        const int64_t nt = ceil_div(m, 64);
+        for (int64_t ti = 0; ti < nt; ++ti) {
+            const int64_t i_last = std::min(m, (ti + 1) * 64) - 1;
+            for (int64_t tj = ti; tj < nt; ++tj) {
+                const int64_t j_first = tj * 64;
+                if (tj > ti) {
+                    const bool far = window_snps > 0
+                                         ? (j_first - i_last > window_snps)
+                                         : ((double)(bp_host[b - row0 + j_first] - bp_host[b - row0 + i_last]) >
+                                            window_bp);
+                    if (far) break;
+                }
+                pairs.push_back((int32_t)ti);
+                pairs.push_back((int32_t)tj);
+                pairs.push_back((int32_t)c);
+            }
+        }
+    }
+    const int64_t np = (int64_t)pairs.size() / 3;
+    int32_t* pairs_dev = static_cast<int32_t*>(g->scratch("win_pairs", pairs.size() * sizeof(int32_t)));
+    GPE_HIP(hipMemcpyAsync(pairs_dev, pairs.data(), pairs.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    int64_t* chr_dev = static_cast<int64_t*>(g->scratch("win_chr", 2 * (size_t)n_chr * sizeof(int64_t)));
+    GPE_HIP(hipMemcpyAsync(chr_dev, bounds, 2 * (size_t)n_chr * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    int64_t* bp_dev = nullptr;
+    if (window_snps <= 0) {
+        bp_dev = static_cast<int64_t*>(g->scratch("win_bp", rows * sizeof(int64_t)));
+        GPE_HIP(hipMemcpyAsync(bp_dev, bp_host, rows * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    }
+    double* scale = static_cast<double*>(g->scratch("win_scale", g->m_alloc * sizeof(double)));
+    const int completed = g->impute_mode == GPE_IMPUTE_HWE ? 1 : 0;
+    launch_window_scale(g->nobs, g->sum, g->sumsq, g->m, g->n, completed, scale, st);
+    launch_fill_u8(reinterpret_cast<uint8_t*>(out_dev), rows * n_bins * (int64_t)sizeof(double), 0, st);
+    GramArgs a{};
+    a.packed = g->snp_work;
+    a.stride = g->snp_stride;
+    a.n = g->n;
+    a.kc1 = ceil_div(g->n, kKChunk);
+    a.pairs = pairs_dev;
+    a.chr = chr_dev;
+    a.row0 = row0;
+    a.mean = g->mean;
+    a.scale = scale;
+    a.sum = g->sum;
+    a.nobs = g->nobs;
+    a.bp = bp_dev;
+    a.window_bp = window_bp;
+    a.window_snps = window_snps;
+    a.annot = annot_dev;
+    a.n_bins = n_bins;
+    a.out = out_dev;
+    a.corr_dim = corr_dim;
+    const bool miss = g->impute_mode == GPE_IMPUTE_MEAN && g->n_missing > 0;
+    ProfScope ps(GPE_PROF_GRAM, st);
+    launch_gram_band(a, np, miss, st);
+    ps.stop();
+    // the host pair list and bounds must outlive the copies
+    GPE_HIP(hipStreamSynchronize(st));
+}
+
 }  // namespace
 
 extern "C" {
@@ -1134,74 +1224,20 @@ int gpe_windowed_ld(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const int64
     return guarded([&] {
         require_prepared(g);
         require(snp_end >= snp_begin, "chr_end must be >= chr_start");
-        const int64_t m = snp_end - snp_begin;
-        require(m > 0, "Chromosome slice is empty");
-        require(snp_begin >= 0 && snp_end <= g->m, "SNP range out of bounds");
-        require(n_bins > 0, "annot_chr must have at least one column");
-        require(g->n > 2, "Too few rows for windowed LD score computation");
-        const double corr_dim = (double)g->n - 1.0;
-        require(corr_dim > 1.0, "Too few residual dimensions for unbiased windowed r^2 correction");
-        if (window_snps <= 0) {
-            require(bp_host != nullptr, "bp length mismatch in compute_windowed_ld_chr");
-            require(std::isfinite(window_bp) && window_bp > 0.0,
-                    "ld_wind_kb must define a finite positive BP window");
-            for (int64_t i = 1; i < m; ++i)
-                require(bp_host[i] >= bp_host[i - 1],
-                        "BP not sorted within chromosome block. Sort your .bim by CHR+BP.");
-        }
-        hipStream_t st = S(stream);
-        // tile pairs of the band
-        const int64_t nt = ceil_div(m, 64);
-        std::vector<int32_t> pairs;
-        for (int64_t ti = 0; ti < nt; ++ti) {
-            const int64_t i_last = std::min(m, (ti + 1) * 64) - 1;
-            for (int64_t tj = ti; tj < nt; ++tj) {
-                const int64_t j_first = tj * 64;
-                if (tj > ti) {
-                    const bool far = window_snps > 0 ? (j_first - i_last > window_snps)
-                                                     : ((double)(bp_host[j_first] - bp_host[i_last]) > window_bp);
-                    if (far) break;
-                }
-                pairs.push_back((int32_t)ti);
-                pairs.push_back((int32_t)tj);
-            }
-        }
-        const int64_t np = (int64_t)pairs.size() / 2;
-        int32_t* pairs_dev = static_cast<int32_t*>(g->scratch("win_pairs", pairs.size() * sizeof(int32_t)));
-        GPE_HIP(hipMemcpyAsync(pairs_dev, pairs.data(), pairs.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
-        int64_t* bp_dev = nullptr;
-        if (window_snps <= 0) {
-            bp_dev = static_cast<int64_t*>(g->scratch("win_bp", m * sizeof(int64_t)));
-            GPE_HIP(hipMemcpyAsync(bp_dev, bp_host, m * sizeof(int64_t), hipMemcpyHostToDevice, st));
-        }
-        double* scale = static_cast<double*>(g->scratch("win_scale", g->m_alloc * sizeof(double)));
-        const int completed = g->impute_mode == GPE_IMPUTE_HWE ? 1 : 0;
-        launch_window_scale(g->nobs, g->sum, g->sumsq, g->m, g->n, completed, scale, st);
-        launch_fill_u8(reinterpret_cast<uint8_t*>(out_dev), m * n_bins * (int64_t)sizeof(double), 0, st);
-        GramArgs a{};
-        a.packed = g->snp_work;
-        a.stride = g->snp_stride;
-        a.n = g->n;
-        a.kc1 = ceil_div(g->n, kKChunk);
-        a.pairs = pairs_dev;
-        a.snp_begin = snp_begin;
-        a.snp_end = snp_end;
-        a.mean = g->mean;
-        a.scale = scale;
-        a.sum = g->sum;
-        a.nobs = g->nobs;
-        a.bp = bp_dev;
-        a.window_bp = window_bp;
-        a.window_snps = window_snps;
-        a.annot = annot_dev;
-        a.n_bins = n_bins;
-        a.out = out_dev;
-        a.corr_dim = corr_dim;
-        const bool miss = g->impute_mode == GPE_IMPUTE_MEAN && g->n_missing > 0;
-        ProfScope ps(GPE_PROF_GRAM, st);
-        launch_gram_band(a, np, miss, st);
-        ps.stop();
-        GPE_HIP(hipStreamSynchronize(st));
+        require(snp_end - snp_begin > 0, "Chromosome slice is empty");
+        const int64_t bounds[2] = {snp_begin, snp_end};
+        windowed_ld_impl(g, 1, bounds, bp_host, window_bp, window_snps, annot_dev, n_bins, out_dev, S(stream));
+    });
+}
+
+int gpe_windowed_ld_multi(gpe_geno* g, int32_t n_chr, const int64_t* chr_bounds_host, const int64_t* bp_host,
+                          double window_bp, int64_t window_snps, const double* annot_dev, int32_t n_bins,
+                          double* out_dev, void* stream) {
+    return guarded([&] {
+        require_prepared(g);
+        require(n_chr > 0 && chr_bounds_host != nullptr, "at least one chromosome is required");
+        windowed_ld_impl(g, n_chr, chr_bounds_host, bp_host, window_bp, window_snps, annot_dev, n_bins, out_dev,
+                         S(stream));
     });
 }
 

@@ -83,18 +83,20 @@ struct GramArgs {
     int64_t stride;
     int64_t n;           // samples
     int64_t kc1;         // K chunks
-    const int32_t* pairs;  // (ti, tj) tile pairs, ti <= tj, relative to snp_begin
-    int64_t snp_begin, snp_end;
+    const int32_t* pairs;  // (ti, tj, chr) triples: 64-SNP tiles of chromosome chr, ti <= tj,
+                           // counted from the chromosome's first SNP
+    const int64_t* chr;    // (begin, end) absolute SNP bounds per chromosome
+    int64_t row0;          // absolute SNP index of row 0 of bp, annot and out
     const double* mean;
     const double* scale;   // s_i
     const int64_t* sum;
     const int32_t* nobs;
-    const int64_t* bp;     // relative to snp_begin
+    const int64_t* bp;     // relative to row0
     double window_bp;      // used when window_snps <= 0
     int64_t window_snps;
-    const double* annot;   // row-major (m, n_bins), relative
+    const double* annot;   // row-major (rows, n_bins), relative to row0
     int n_bins;
-    double* out;           // row-major (m, n_bins), relative
+    double* out;           // row-major (rows, n_bins), relative to row0
     double corr_dim;
 };
 

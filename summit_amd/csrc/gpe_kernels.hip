@@ -1469,9 +1469,10 @@ __global__ __launch_bounds__(256) void k_gram_band(GramArgs a) {
     __shared__ double r2t[4][32][33];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, half = lane >> 5, lr = lane & 31;
     const int wi = wave >> 1, wj = wave & 1;
-    const int ti = a.pairs[2 * blockIdx.x], tj = a.pairs[2 * blockIdx.x + 1];
-    const int64_t i0 = a.snp_begin + (int64_t)ti * 64 + wi * 32;
-    const int64_t j0 = a.snp_begin + (int64_t)tj * 64 + wj * 32;
+    const int ti = a.pairs[3 * blockIdx.x], tj = a.pairs[3 * blockIdx.x + 1], ch = a.pairs[3 * blockIdx.x + 2];
+    const int64_t cb = a.chr[2 * ch], ce = a.chr[2 * ch + 1];
+    const int64_t i0 = cb + (int64_t)ti * 64 + wi * 32;
+    const int64_t j0 = cb + (int64_t)tj * 64 + wj * 32;
     const uint8_t* pa = a.packed + (i0 + lr) * a.stride + half * 16;
     const uint8_t* pb = a.packed + (j0 + lr) * a.stride + half * 16;
     v16i gg = v16i{}, gq = v16i{}, qg = v16i{}, qq = v16i{};
@@ -1497,11 +1498,10 @@ __global__ __launch_bounds__(256) void k_gram_band(GramArgs a) {
             }
         }
     }
-    const int64_t m = a.snp_end - a.snp_begin;
     const double dn = (double)a.n;
     const int64_t j = j0 + lr;
-    const int64_t jr = j - a.snp_begin;
-    const bool jv = jr < m;
+    const int64_t jr = j - a.row0;
+    const bool jv = j < ce;
     const double mu_j = jv ? a.mean[j] : 0.0, s_j = jv ? a.scale[j] : 0.0;
     const double sum_j = jv ? (double)a.sum[j] : 0.0;
     const double nm_j = jv ? dn - (double)a.nobs[j] : 0.0;
@@ -1509,12 +1509,12 @@ __global__ __launch_bounds__(256) void k_gram_band(GramArgs a) {
     for (int e = 0; e < 16; ++e) {
         const int il = 8 * (e >> 2) + 4 * half + (e & 3);
         const int64_t i = i0 + il;
-        const int64_t ir = i - a.snp_begin;
+        const int64_t ir = i - a.row0;
         double v = 0.0;
-        if (jv && ir < m) {
+        if (jv && i < ce) {
             bool inwin;
             if (a.window_snps > 0) {
-                const int64_t d = jr > ir ? jr - ir : ir - jr;
+                const int64_t d = j > i ? j - i : i - j;
                 inwin = d <= a.window_snps;
             } else {
                 const double d = fabs((double)a.bp[ir] - (double)a.bp[jr]);
@@ -1543,27 +1543,26 @@ __global__ __launch_bounds__(256) void k_gram_band(GramArgs a) {
     const bool diag = ti == tj;
     if (half == 0) {
         const int64_t i = i0 + lr;
-        const int64_t ir = i - a.snp_begin;
-        if (ir < m) {
+        if (i < ce) {
             for (int b = 0; b < a.n_bins; ++b) {
                 double acc = 0.0;
                 for (int c = 0; c < 32; ++c) {
-                    const int64_t jc = j0 + c - a.snp_begin;
-                    if (jc < m) acc += r2t[wave][lr][c] * a.annot[jc * a.n_bins + b];
+                    const int64_t jc = j0 + c;
+                    if (jc < ce) acc += r2t[wave][lr][c] * a.annot[(jc - a.row0) * a.n_bins + b];
                 }
-                atomicAdd(&a.out[ir * a.n_bins + b], acc);
+                atomicAdd(&a.out[(i - a.row0) * a.n_bins + b], acc);
             }
         }
     } else if (!diag) {
-        const int64_t jc = j0 + lr - a.snp_begin;
-        if (jc < m) {
+        const int64_t jc = j0 + lr;
+        if (jc < ce) {
             for (int b = 0; b < a.n_bins; ++b) {
                 double acc = 0.0;
                 for (int rr = 0; rr < 32; ++rr) {
-                    const int64_t irr = i0 + rr - a.snp_begin;
-                    if (irr < m) acc += r2t[wave][rr][lr] * a.annot[irr * a.n_bins + b];
+                    const int64_t irr = i0 + rr;
+                    if (irr < ce) acc += r2t[wave][rr][lr] * a.annot[(irr - a.row0) * a.n_bins + b];
                 }
-                atomicAdd(&a.out[jc * a.n_bins + b], acc);
+                atomicAdd(&a.out[(jc - a.row0) * a.n_bins + b], acc);
             }
         }
     }
@@ -1573,7 +1572,7 @@ void launch_gram_band(const GramArgs& a, int64_t n_pairs, bool miss, hipStream_t
     if (n_pairs <= 0) return;
     for (int64_t p0 = 0; p0 < n_pairs; p0 += 1 << 30) {
         GramArgs b = a;
-        b.pairs = a.pairs + 2 * p0;
+        b.pairs = a.pairs + 3 * p0;
         const int64_t np = (n_pairs - p0) < (1 << 30) ? (n_pairs - p0) : (1 << 30);
         if (miss)
             hipLaunchKernelGGL(k_gram_band<true>, dim3((unsigned)np), dim3(256), 0, st, b);

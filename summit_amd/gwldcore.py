@@ -145,7 +145,11 @@ def _xz_colmajor_dev(x):
             return x, False
         return x.to(device=_dev(), dtype=torch.float64).contiguous(), True
     a = np.asarray(x)
-    return torch.from_numpy(np.ascontiguousarray(a.T, dtype=np.float64)).to(_dev()), True
+    # an Xz2d the caller allocated with np.empty may hold signalling-NaN bit
+    # patterns where phase 1 is about to overwrite (bin_init_mask); converting
+    # them is harmless and not worth a warning
+    with np.errstate(invalid="ignore"):
+        return torch.from_numpy(np.ascontiguousarray(a.T, dtype=np.float64)).to(_dev()), True
 
 
 def _write_back_colmajor(dst, dev):

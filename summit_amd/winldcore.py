@@ -55,6 +55,38 @@ def windowed_ld(g: engine.Genotype, snp_begin: int, snp_end: int, bp=None, ld_wi
     return out.cpu().numpy()
 
 
+def windowed_ld_multi(g: engine.Genotype, chroms, ld_wind_kb: float | None = None, annot=None,
+                      window_snps: int = 0) -> np.ndarray:
+    """Every chromosome in one banded-Gram launch.  chroms: [(snp_begin,
+    snp_end, bp or None), ...] sorted and non-overlapping; returns (rows, B)
+    FP64 for SNPs [chroms[0][0], chroms[-1][1]) (rows between chromosomes 0),
+    equal to one ``windowed_ld`` call per chromosome."""
+    bounds = np.ascontiguousarray(np.asarray([[int(c[0]), int(c[1])] for c in chroms], dtype=np.int64).reshape(-1))
+    if bounds.size == 0:
+        raise RuntimeError("at least one chromosome is required")
+    row0, rows = int(bounds[0]), int(bounds[-1] - bounds[0])
+    if annot is None:
+        annot = np.ones((rows, 1))
+    annot = np.ascontiguousarray(np.asarray(annot, dtype=np.float64))
+    if annot.ndim != 2 or annot.shape[0] != rows:
+        raise RuntimeError("annot_chr shape mismatch in compute_windowed_ld_chr")
+    bp_all = None
+    if window_snps <= 0:
+        bp_all = np.zeros(rows, dtype=np.int64)
+        for s0, s1, bp in chroms:
+            bp = np.asarray(bp, dtype=np.int64)
+            if bp.shape[0] != s1 - s0:
+                raise RuntimeError("bp length mismatch in compute_windowed_ld_chr")
+            bp_all[s0 - row0:s1 - row0] = bp
+    a_d = torch.from_numpy(annot).to(engine.default_device())
+    out = torch.empty((rows, annot.shape[1]), dtype=torch.float64, device=a_d.device)
+    window_bp = float(ld_wind_kb) * 1000.0 if ld_wind_kb is not None else 0.0
+    engine.call("gpe_windowed_ld_multi", g.handle, int(bounds.size // 2), int(bounds.ctypes.data),
+                0 if bp_all is None else int(bp_all.ctypes.data), window_bp, int(window_snps), engine._ptr(a_d),
+                annot.shape[1], engine._ptr(out), engine._stream())
+    return out.cpu().numpy()
+
+
 def compute_windowed_ld_chr(bed_prefix, fam_path, chr_start, chr_end, bp, annot_chr, ld_wind_kb, chunk_size,
                             row_sel=None, C=None, R=None, impute_mode="mean", impute_seed=None, panel_cols=0,
                             cache_mb=-1):
