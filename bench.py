@@ -213,6 +213,14 @@ def gxe_rate(engine, torch) -> dict:
     environment: 2B sources (N*M*2B) plus two target traversals (N*M*4B)."""
     from summit_amd import gxe
     N, M, B, K = 100_000, 500_000, 64, 8
+    # warm-up on a small matrix: first launches of every kernel the path uses
+    # (module loads are a once-per-process cost, not part of the workload)
+    gw = engine.Genotype.synthetic(2000, 3000, seed=1, maf_lo=0.05, maf_hi=0.5, miss_rate=0.005, block_len=100,
+                                   copy_prob=0.9)
+    gw.prepare(ddof=1, impute="mean")
+    ew = np.random.default_rng(0).standard_normal((2000, K))
+    gxe.gxe_ldscore_multi(gw, (ew - ew.mean(axis=0)) / ew.std(axis=0), None, nvecs=B, seed=1)
+    gw.close()
     g = engine.Genotype.synthetic(N, M, seed=303, maf_lo=0.01, maf_hi=0.5, miss_rate=0.005, block_len=100,
                                   copy_prob=0.9)
     g.prepare(ddof=1, impute="mean")

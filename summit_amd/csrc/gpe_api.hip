@@ -1204,17 +1204,59 @@ int gpe_weighted_sumsq(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const do
         require(snp_begin >= 0 && snp_end <= g->m && snp_begin <= snp_end, "SNP range out of bounds");
         require(n_weights >= 0 && ldw >= g->n && ldo >= snp_end - snp_begin, "bad weighted-sum-of-squares shape");
         if (n_weights == 0 || snp_end == snp_begin) return;
+        // int32 limb sums over N samples of squared counts (<= 4) times balanced digits (|d| <= 128)
+        require(g->n <= 4000000, "weighted sums of squares support at most 4,000,000 samples");
         hipStream_t st = S(stream);
-        const int64_t n_pad = round_up(ceil_div(g->n, 4), 16) * 4;
-        for (int e0 = 0; e0 < n_weights; e0 += 16) {
-            const int nw = std::min(16, n_weights - e0);
-            const int ec = nw <= 8 ? 8 : 16;
-            double* wsm = static_cast<double*>(g->scratch("wss_w", (size_t)n_pad * 16 * sizeof(double)));
-            hipLaunchKernelGGL(k_weights_sample_major, dim3((unsigned)ceil_div(n_pad * ec, 256)), dim3(256), 0, st,
-                               w_dev + (int64_t)e0 * ldw, ldw, nw, g->n, n_pad, ec, wsm);
-            GPE_HIP(hipGetLastError());
-            launch_weighted_sumsq(g->snp_work, g->snp_stride, snp_begin, snp_end, g->n, g->mean, g->inv_sd, wsm, ec,
-                                  nw, out_dev + (int64_t)e0 * ldo, ldo, st);
+        // x_jn^2 = inv_sd_j^2 (g_jn - mu_j)^2 on observed calls (0 where missing, mean mode), and g^2 = g + 2[g=2],
+        // so sum_n w_n x_jn^2 = inv_sd_j^2 (mu_j^2 (W - ME) - 2 mu_j GE + G2E) with exact int64 plane sums of the
+        // quantized weights: GE (counts), ME (missing flags), G2E (squared counts), W (all samples)
+        const LimbCfg lc = limb_cfg(6);
+        const int64_t L = snp_end - snp_begin;
+        for (int e0 = 0; e0 < n_weights; e0 += 1024) {
+            const int nw = std::min(1024, n_weights - e0);
+            const int v_pad = (int)round_up(nw, 32);
+            const int groups = v_pad / 32;
+            const int64_t coef_ctiles = (int64_t)groups * lc.nl * lc.ctc;
+            const int64_t kcount = ceil_div(g->n, kKChunk);
+            uint64_t* colmax = static_cast<uint64_t*>(g->scratch("colmax", v_pad * sizeof(uint64_t)));
+            int32_t* shift = static_cast<int32_t*>(g->scratch("shift", v_pad * sizeof(int32_t)));
+            double* col_scale = static_cast<double*>(g->scratch("col_scale", v_pad * sizeof(double)));
+            int64_t* col_sum = static_cast<int64_t*>(g->scratch("col_sum", v_pad * sizeof(int64_t)));
+            launch_fill_u64(colmax, v_pad, 0, st);
+            launch_fill_u64(reinterpret_cast<uint64_t*>(col_sum), v_pad, 0, st);
+            const double* w = w_dev + (int64_t)e0 * ldw;
+            launch_colmax_f64(w, ldw, g->n, nw, nw, v_pad, colmax, st);
+            const int prec = std::min(8 * lc.nl - 2, 62 - bitlen((uint64_t)(2 * g->n)));
+            launch_shift_from_colmax(colmax, v_pad, prec, shift, col_scale, st);
+            int8_t* coef = static_cast<int8_t*>(g->scratch("coef", kcount * coef_ctiles * 4096));
+            if (v_pad != nw) launch_fill_u8(reinterpret_cast<uint8_t*>(coef), kcount * coef_ctiles * 4096, 0, st);
+            launch_quant_p2(w, ldw, g->n, 1, nw, v_pad, shift, lc.nl, lc.ctc, coef_ctiles, coef, col_sum, st);
+            int64_t* ge = static_cast<int64_t*>(g->scratch("wss_ge", (size_t)L * v_pad * sizeof(int64_t)));
+            int64_t* me = static_cast<int64_t*>(g->scratch("wss_me", (size_t)L * v_pad * sizeof(int64_t)));
+            int64_t* g2 = static_cast<int64_t*>(g->scratch("wss_g2", (size_t)L * v_pad * sizeof(int64_t)));
+            ContractArgs a{};
+            a.packed = g->snp_work;
+            a.stride = g->snp_stride;
+            a.r0 = snp_begin;
+            a.r1 = snp_end;
+            a.coef = coef;
+            a.coef_ctiles = coef_ctiles;
+            a.kc0 = 0;
+            a.kc1 = kcount;
+            a.kc_per_split = kcount;
+            a.ctiles_used = v_pad / 32;
+            a.ldo = v_pad;
+            a.cols_out = v_pad;
+            a.out_row0 = snp_begin;
+            const bool miss = g->impute_mode == GPE_IMPUTE_MEAN && g->n_missing > 0;
+            a.out_i64 = ge;
+            a.out2_i64 = me;
+            launch_contract(lc.nl, lc.ctc, miss, miss ? EPI_I64_2P : EPI_I64_ROWMAJOR, a, groups, 1, st);
+            a.out_i64 = g2;
+            a.out2_i64 = nullptr;
+            launch_contract(lc.nl, lc.ctc, false, EPI_I64_SQ, a, groups, 1, st);
+            launch_wss_finalize(ge, miss ? me : nullptr, g2, v_pad, snp_begin, snp_end, nw, g->mean, g->inv_sd,
+                                col_scale, col_sum, out_dev + (int64_t)e0 * ldo, ldo, st);
         }
     });
 }

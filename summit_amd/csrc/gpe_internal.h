@@ -48,6 +48,8 @@ enum EpiMode : int {
     EPI_I64_ROWMAJOR = 2,// combine limbs, store int64 to out_i64[r * ldo + c]
     EPI_P2 = 3,          // phase-2 epilogue: standardized Work, per-tile sums of Work^2, Work^4
     EPI_WORK = 4,        // phase-2 epilogue writing standardized Work to out_f64[r * ldo + c]
+    EPI_I64_2P = 5,      // MISS: int64 row-major of the genotype plane to out_i64, missing plane to out2_i64
+    EPI_I64_SQ = 6,      // squared-genotype plane (codes 00 -> 0, 01 -> 0, 10 -> 1, 11 -> 4), int64 row-major
 };
 
 struct ContractArgs {
@@ -75,6 +77,7 @@ struct ContractArgs {
     int has_miss;
     int32_t cols_out;        // I32/I64 epilogues: columns >= cols_out are not written
     int64_t out_row0;        // I32/I64 epilogues: output row index = r - out_row0
+    int64_t* out2_i64;       // EPI_I64_2P: missing-plane output (same layout as out_i64)
 };
 
 // Banded Gram (windowed LD) arguments
@@ -172,6 +175,11 @@ void launch_philox_rademacher(const uint64_t* keys, const int64_t* starts, const
 void launch_weighted_sumsq(const uint8_t* snp, int64_t stride, int64_t r0, int64_t r1, int64_t n, const double* mean,
                            const double* inv_sd, const double* W_sample_major, int ec, int n_w, double* out,
                            int64_t ldo, hipStream_t st);
+// out[e * ldo + (j - r0)] = inv_sd_j^2 scale_e (mu_j^2 (wsum_e - me) - 2 mu_j ge + g2) for the
+// three int64 plane sums (row-major [j - r0][e], leading dimension ld)
+void launch_wss_finalize(const int64_t* ge, const int64_t* me, const int64_t* g2, int64_t ld, int64_t r0, int64_t r1,
+                         int n_w, const double* mean, const double* inv_sd, const double* col_scale,
+                         const int64_t* col_sum, double* out, int64_t ldo, hipStream_t st);
 void launch_window_scale(const int32_t* nobs, const int64_t* sum, const int64_t* sumsq, int64_t m, int64_t n,
                          int completed, double* scale, hipStream_t st);
 void launch_resvar_inv(const int32_t* nobs, const int64_t* sum, const int64_t* sumsq, const double* mean,

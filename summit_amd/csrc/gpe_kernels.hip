@@ -51,6 +51,8 @@ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
 //   missing flag   01 -> 1, otherwise 0.
 constexpr uint32_t kGenoTable = 0x02010000u;
 constexpr uint32_t kMissTable = 0x00000100u;
+// squared genotype count 00 -> 0, 01 (missing) -> 0, 10 -> 1, 11 -> 4
+constexpr uint32_t kSqTable = 0x04010000u;
 
 __device__ __forceinline__ void code_bytes(uint32_t w, uint32_t s[4]) {
     s[0] = w & 0x03030303u;
@@ -69,11 +71,12 @@ __device__ __forceinline__ void unpack16(uint32_t w, v4i& g, v4i& m) {
             (int32_t)__builtin_amdgcn_perm(0u, kMissTable, s[2]), (int32_t)__builtin_amdgcn_perm(0u, kMissTable, s[3])};
 }
 
+template <uint32_t TABLE = kGenoTable>
 __device__ __forceinline__ void unpack16_g(uint32_t w, v4i& g) {
     uint32_t s[4];
     code_bytes(w, s);
-    g = v4i{(int32_t)__builtin_amdgcn_perm(0u, kGenoTable, s[0]), (int32_t)__builtin_amdgcn_perm(0u, kGenoTable, s[1]),
-            (int32_t)__builtin_amdgcn_perm(0u, kGenoTable, s[2]), (int32_t)__builtin_amdgcn_perm(0u, kGenoTable, s[3])};
+    g = v4i{(int32_t)__builtin_amdgcn_perm(0u, TABLE, s[0]), (int32_t)__builtin_amdgcn_perm(0u, TABLE, s[1]),
+            (int32_t)__builtin_amdgcn_perm(0u, TABLE, s[2]), (int32_t)__builtin_amdgcn_perm(0u, TABLE, s[3])};
 }
 
 // Offset of coefficient byte (c-tile tg, value index kk in [0,128), lane row lr in [0,32))
@@ -146,6 +149,7 @@ __global__ __launch_bounds__(256, (NL * CTC <= 2) ? 4 : 2) void k_contract(Contr
     constexpr int RPW = MISS ? 32 : 64;
     constexpr int RPB = 4 * RPW;
     constexpr int G = MISS ? 1 : 2;  // packed-row loads per lane per chunk
+    constexpr uint32_t TBL = EPI == EPI_I64_SQ ? kSqTable : kGenoTable;
     __shared__ __attribute__((aligned(16))) int8_t lds[3][CT * 4096];
 
     const int tid = threadIdx.x;
@@ -201,8 +205,8 @@ __global__ __launch_bounds__(256, (NL * CTC <= 2) ? 4 : 2) void k_contract(Contr
     if (MISS) {
         unpack16((uint32_t)g0a[0], op0, op1);
     } else {
-        unpack16_g((uint32_t)g0a[0], op0);
-        unpack16_g((uint32_t)g0b[0], op1);
+        unpack16_g<TBL>((uint32_t)g0a[0], op0);
+        unpack16_g<TBL>((uint32_t)g0b[0], op1);
     }
     int buf = 0;  // LDS buffer of chunk kc
     for (int64_t kc = kc_begin; kc < kc_end; ++kc) {
@@ -232,8 +236,8 @@ __global__ __launch_bounds__(256, (NL * CTC <= 2) ? 4 : 2) void k_contract(Contr
                 unpack16(wn0, nx0, nx1);
             } else {
                 const uint32_t wn1 = s < 3 ? (uint32_t)g0b[s + 1] : (uint32_t)g1b[0];
-                unpack16_g(wn0, nx0);
-                unpack16_g(wn1, nx1);
+                unpack16_g<TBL>(wn0, nx0);
+                unpack16_g<TBL>(wn1, nx1);
             }
 #pragma unroll
             for (int t = 0; t < CT; ++t) {
@@ -272,7 +276,29 @@ __global__ __launch_bounds__(256, (NL * CTC <= 2) ? 4 : 2) void k_contract(Contr
                 }
             }
         }
-    } else if constexpr (EPI == EPI_I64_ATOMIC || EPI == EPI_I64_ROWMAJOR) {
+    } else if constexpr (EPI == EPI_I64_2P) {
+        static_assert(MISS, "two-plane output needs the missing plane");
+#pragma unroll
+        for (int ctc = 0; ctc < CTC; ++ctc) {
+            const int ct = cg * CTC + ctc;
+            if (ct >= a.ctiles_used) continue;
+            const int64_t r = rbase + lr;
+            if (r >= a.r1) continue;
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                int64_t ge = 0, me = 0;
+#pragma unroll
+                for (int l = 0; l < NL; ++l) {
+                    ge = ge * 256 + (int64_t)acc[l * CTC + ctc][0][e];
+                    me = me * 256 + (int64_t)acc[l * CTC + ctc][1][e];
+                }
+                const int c = ct * 32 + 8 * (e >> 2) + 4 * half + (e & 3);
+                if (c >= a.cols_out) continue;
+                a.out_i64[(r - a.out_row0) * a.ldo + c] = ge;
+                a.out2_i64[(r - a.out_row0) * a.ldo + c] = me;
+            }
+        }
+    } else if constexpr (EPI == EPI_I64_ATOMIC || EPI == EPI_I64_ROWMAJOR || EPI == EPI_I64_SQ) {
 #pragma unroll
         for (int ctc = 0; ctc < CTC; ++ctc) {
             const int ct = cg * CTC + ctc;
@@ -385,6 +411,10 @@ void launch_contract(int nl, int ctc, bool miss, EpiMode mode, const ContractArg
     // weighted phase 1 (sample-major rows, int64 partial sums)
     GPE_CASE(5, 1, false, EPI_I64_ATOMIC)
     GPE_CASE(6, 1, false, EPI_I64_ATOMIC)
+    // plane sums of weighted sums of squares (genotype + missing, squared genotype)
+    GPE_CASE(6, 1, true, EPI_I64_2P)
+    GPE_CASE(6, 1, false, EPI_I64_ROWMAJOR)
+    GPE_CASE(6, 1, false, EPI_I64_SQ)
     // weighted phase 2
     GPE_CASE(5, 1, false, EPI_P2)
     GPE_CASE(5, 1, true, EPI_P2)
@@ -1687,6 +1717,34 @@ __global__ __launch_bounds__(256) void k_weighted_sumsq(const uint8_t* __restric
     const double s2 = inv_sd[j] * inv_sd[j];
     for (int e = 0; e < n_w; ++e) out[(int64_t)e * ldo + (j - r0)] = acc[e] * s2;
 }
+__global__ void k_wss_finalize(const int64_t* __restrict__ ge, const int64_t* __restrict__ me,
+                               const int64_t* __restrict__ g2, int64_t ld, int64_t r0, int64_t r1, int n_w,
+                               const double* __restrict__ mean, const double* __restrict__ inv_sd,
+                               const double* __restrict__ col_scale, const int64_t* __restrict__ col_sum,
+                               double* __restrict__ out, int64_t ldo) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t L = r1 - r0;
+    if (idx >= L * n_w) return;
+    const int e = (int)(idx / L);
+    const int64_t jr = idx - (int64_t)e * L;
+    const int64_t j = r0 + jr;
+    const double mu = mean[j], isd = inv_sd[j];
+    const int64_t o = jr * ld + e;
+    const double obs = (double)(col_sum[e] - (me ? me[o] : 0));
+    const double v = mu * mu * obs - 2.0 * mu * (double)ge[o] + (double)g2[o];
+    out[(int64_t)e * ldo + jr] = isd * isd * col_scale[e] * v;
+}
+
+void launch_wss_finalize(const int64_t* ge, const int64_t* me, const int64_t* g2, int64_t ld, int64_t r0, int64_t r1,
+                         int n_w, const double* mean, const double* inv_sd, const double* col_scale,
+                         const int64_t* col_sum, double* out, int64_t ldo, hipStream_t st) {
+    const int64_t total = (r1 - r0) * (int64_t)n_w;
+    if (total <= 0) return;
+    hipLaunchKernelGGL(k_wss_finalize, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, st, ge, me, g2, ld, r0, r1,
+                       n_w, mean, inv_sd, col_scale, col_sum, out, ldo);
+    GPE_HIP(hipGetLastError());
+}
+
 void launch_weighted_sumsq(const uint8_t* snp, int64_t stride, int64_t r0, int64_t r1, int64_t n, const double* mean,
                            const double* inv_sd, const double* W_sample_major, int ec, int n_w, double* out,
                            int64_t ldo, hipStream_t st) {

@@ -72,17 +72,19 @@ def test_gxe_multi_environment_matches_oracle(oracle, per_env_cov):
     g.close()
 
 
-@pytest.mark.parametrize("n_w", [1, 3, 17])
-def test_weighted_sumsq_matches_dense(oracle, n_w):
-    """sum_n w_n x_jn^2 for several weight vectors in one call (16 per pass) against the
-    oracle's standardized matrix; a SNP range that starts mid-matrix."""
+@pytest.mark.parametrize("n_w,impute", [(1, "mean"), (3, "mean"), (17, "mean"), (40, "mean"), (5, "hwe")])
+def test_weighted_sumsq_matches_dense(oracle, n_w, impute):
+    """sum_n w_n x_jn^2 for several weight vectors in one call (plane sums on int8
+    MFMA, two column groups at 40 weights) against the oracle's standardized matrix;
+    a SNP range that starts mid-matrix; signed weights; HWE-completed codes."""
     from summit_amd import engine, gxe
     m, n = 300, 1037
     codes = random_codes(m, n, seed=41, miss_rate=0.03, monomorphic=1)
-    g = engine.Genotype.from_bed_bytes(oracle.pack_codes(codes), n, m).prepare(impute="mean")
+    g = engine.Genotype.from_bed_bytes(oracle.pack_codes(codes), n, m).prepare(impute=impute, impute_seed=5)
     rng = np.random.default_rng(n_w)
     W = rng.standard_normal((n_w, n)) ** 2
-    X = oracle.read_block_standardized(oracle.pack_codes(codes), n, 0, m, None, 1, "mean", 0)
+    W[-1] -= 0.5  # a weight vector with both signs
+    X = oracle.read_block_standardized(oracle.pack_codes(codes), n, 0, m, None, 1, impute, 5)
     got = gxe.weighted_sumsq(g, torch.as_tensor(W, device=engine.default_device()), 20, 290).cpu().numpy()
     ref = W @ (X[:, 20:290] ** 2)
     np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-9)
