@@ -225,6 +225,17 @@ int gpe_phase1_xz_z8(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const doub
                      const double* inv_dev, const int8_t* z_dev, int64_t ldz, int32_t v_count, double* xz_dev,
                      int64_t ldxz, const uint8_t* bin_mode_dev, int32_t limbs, void* stream);
 
+/* Small-rank FP64 projection of n_cols columns of length n, in place:
+ *   x_q <- x_q - A (B x_q),  then, when center != 0, x_q <- x_q - mean(x_q),
+ * x column q at x_dev[q * ldx + i]; A n x p column-major (a_dev[i + k * lda]);
+ * B p x n row-major (b_dev[k * ldb + i]).  With A = C, B = R it is the
+ * covariate projection (I - C R) of src/native/gwldcore.cpp:1119-1132 and
+ * :3134-3207; with A = Q, B = Q^T and centring the GxE projection off the
+ * intercept and an orthonormal covariate basis (src/native/gxeldcore.cpp:7578-7675).
+ * Fixed-order reductions; a per-device workspace is kept for the process. */
+int gpe_project_f64(double* x_dev, int64_t ldx, int64_t n, int32_t n_cols, const double* a_dev, int64_t lda,
+                    const double* b_dev, int64_t ldb, int32_t p, int32_t center, void* stream);
+
 /* out[e * ldo + (j - snp_begin)] = sum_n w[e * ldw + n] x_jn^2 for each of
  * n_weights weight vectors (standardized working codes): the ||e . x_j||^2
  * feature moments of the GxE interaction columns, every environment in one

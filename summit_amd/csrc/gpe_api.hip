@@ -1197,6 +1197,40 @@ int gpe_phase1_xz_z8(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const doub
     });
 }
 
+int gpe_project_f64(double* x_dev, int64_t ldx, int64_t n, int32_t n_cols, const double* a_dev, int64_t lda,
+                    const double* b_dev, int64_t ldb, int32_t p, int32_t center, void* stream) {
+    return guarded([&] {
+        require(n >= 0 && n_cols >= 0 && p >= 0, "projection sizes must be non-negative");
+        require(n_cols <= 65535, "at most 65535 columns per projection call");
+        if (n == 0 || n_cols == 0 || (p == 0 && !center)) return;
+        require(x_dev != nullptr && ldx >= n, "projected matrix needs ldx >= n");
+        if (p > 0) require(a_dev != nullptr && b_dev != nullptr && lda >= n && ldb >= n, "projection factor shapes");
+        // per-device workspace, kept for the process (grown on demand)
+        static std::mutex mu;
+        static std::map<int, Buf> ws;
+        int dev = 0;
+        GPE_HIP(hipGetDevice(&dev));
+        const size_t need = ((size_t)std::max(p, 1) * n_cols + 2 * (size_t)n_cols + (size_t)std::max(p, 1)) *
+                            sizeof(double);
+        std::lock_guard<std::mutex> lock(mu);
+        Buf& b = ws[dev];
+        if (b.bytes < need) {
+            hipStream_t st0 = S(stream);
+            GPE_HIP(hipStreamSynchronize(st0));
+            if (b.p) GPE_HIP(hipFree(b.p));
+            b.p = nullptr;
+            b.bytes = 0;
+            GPE_HIP(hipMalloc(&b.p, need));
+            b.bytes = need;
+        }
+        double* T = static_cast<double*>(b.p);
+        double* Ssum = T + (size_t)std::max(p, 1) * n_cols;
+        double* mean = Ssum + n_cols;
+        double* a = mean + n_cols;
+        launch_project(x_dev, ldx, n, n_cols, a_dev, lda, b_dev, ldb, p, center != 0, T, Ssum, a, mean, S(stream));
+    });
+}
+
 int gpe_weighted_sumsq(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const double* w_dev, int32_t n_weights,
                        int64_t ldw, double* out_dev, int64_t ldo, void* stream) {
     return guarded([&] {

@@ -186,6 +186,11 @@ void launch_resvar_inv(const int32_t* nobs, const int64_t* sum, const int64_t* s
                        const double* inv_sd, int64_t r0, int64_t r1, int64_t n, int completed, double eps,
                        double* inv_out, hipStream_t st);
 
+// x_q <- x_q - A (B x_q) [- mean] for q_count FP64 columns; scratch T (p x q_count),
+// S, mean (q_count) and a (p)
+void launch_project(double* x, int64_t ldx, int64_t n, int q_count, const double* A, int64_t lda, const double* B,
+                    int64_t ldb, int p, bool center, double* T, double* S, double* a, double* mean, hipStream_t st);
+
 // Records the calling thread's failure message (returned by gpe_last_error);
 // entry points of other translation units report through it.
 void set_last_error(const char* msg);

@@ -1854,4 +1854,112 @@ void launch_synth_markov(uint8_t* snp, int64_t stride, int64_t m, int64_t n, uin
     GPE_HIP(hipGetLastError());
 }
 
+
+// ---------------------------------------------------------------------------
+// Small-rank FP64 projection of columns: x_q <- x_q - A (B x_q), optionally
+// followed by centring (x_q <- x_q - mean(x_q)).  A is n x p column-major
+// (A[i + k lda]), B is p x n row-major (B[k ldb + i]), x column q at
+// x[q ldx + i].  The covariate projection I - C R of phase 1/2 inputs and
+// the GxE projection I - Q Q^T plus intercept.  All reductions run in a
+// fixed order.
+// ---------------------------------------------------------------------------
+constexpr int kProjRows = 32;  // rows of B per coefficient pass
+
+// T[k][q] = sum_i B[k0 + k][i] x[q][i] (k < kb), S[q] = sum_i x[q][i] (when S != null)
+__global__ __launch_bounds__(256) void k_proj_coef(const double* __restrict__ x, int64_t ldx, int64_t n,
+                                                   const double* __restrict__ B, int64_t ldb, int k0, int kb,
+                                                   double* __restrict__ T, int64_t ldt, double* __restrict__ S) {
+    __shared__ double red[kProjRows + 1][4];
+    const int q = blockIdx.x;
+    const int tid = threadIdx.x;
+    double acc[kProjRows + 1];
+#pragma unroll
+    for (int k = 0; k <= kProjRows; ++k) acc[k] = 0.0;
+    const double* xq = x + (int64_t)q * ldx;
+    for (int64_t i = tid; i < n; i += 256) {
+        const double v = xq[i];
+        acc[kProjRows] += v;
+#pragma unroll
+        for (int k = 0; k < kProjRows; ++k)
+            if (k < kb) acc[k] = fma(B[(int64_t)(k0 + k) * ldb + i], v, acc[k]);
+    }
+    const int wave = tid >> 6, lane = tid & 63;
+#pragma unroll
+    for (int k = 0; k <= kProjRows; ++k) {
+        const double w = wave_sum_f64(acc[k]);
+        if (lane == 0) red[k][wave] = w;
+    }
+    __syncthreads();
+    if (tid <= kProjRows) {
+        const double v = (red[tid][0] + red[tid][1]) + (red[tid][2] + red[tid][3]);
+        if (tid < kb) T[(int64_t)(k0 + tid) * ldt + q] = v;
+        if (tid == kProjRows && S) S[q] = v;
+    }
+}
+
+// a[k] = sum_i A[i + k lda]
+__global__ __launch_bounds__(256) void k_proj_colsum(const double* __restrict__ A, int64_t lda, int64_t n,
+                                                     double* __restrict__ a) {
+    __shared__ double red[256];
+    const int k = blockIdx.x, tid = threadIdx.x;
+    double s = 0.0;
+    for (int64_t i = tid; i < n; i += 256) s += A[i + (int64_t)k * lda];
+    red[tid] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) red[tid] += red[tid + w];
+        __syncthreads();
+    }
+    if (tid == 0) a[k] = red[0];
+}
+
+// mean[q] = (S[q] - sum_k a[k] T[k][q]) / n: the mean of the projected column
+__global__ void k_proj_mean(const double* __restrict__ S, const double* __restrict__ a, const double* __restrict__ T,
+                            int64_t ldt, int p, int q_count, int64_t n, double* __restrict__ mean) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= q_count) return;
+    double v = S[q];
+    for (int k = 0; k < p; ++k) v -= a[k] * T[(int64_t)k * ldt + q];
+    mean[q] = v / (double)n;
+}
+
+// x[q][i] -= sum_k A[i][k] T[k][q] (+ mean[q])
+__global__ __launch_bounds__(256) void k_proj_apply(double* __restrict__ x, int64_t ldx, int64_t n,
+                                                    const double* __restrict__ A, int64_t lda,
+                                                    const double* __restrict__ T, int64_t ldt, int p,
+                                                    const double* __restrict__ mean) {
+    const int q = blockIdx.y;
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    double v = x[(int64_t)q * ldx + i];
+    for (int k = 0; k < p; ++k) v -= A[i + (int64_t)k * lda] * T[(int64_t)k * ldt + q];
+    if (mean) v -= mean[q];
+    x[(int64_t)q * ldx + i] = v;
+}
+
+void launch_project(double* x, int64_t ldx, int64_t n, int q_count, const double* A, int64_t lda, const double* B,
+                    int64_t ldb, int p, bool center, double* T, double* S, double* a, double* mean, hipStream_t st) {
+    if (n <= 0 || q_count <= 0) return;
+    for (int k0 = 0; k0 < p || (k0 == 0 && center); k0 += kProjRows) {
+        const int kb = std::max(0, std::min(kProjRows, p - k0));
+        hipLaunchKernelGGL(k_proj_coef, dim3((unsigned)q_count), dim3(256), 0, st, x, ldx, n, B, ldb, k0, kb, T,
+                           (int64_t)q_count, (k0 == 0 && center) ? S : nullptr);
+        GPE_HIP(hipGetLastError());
+        if (p == 0) break;
+    }
+    if (center) {
+        if (p > 0) {
+            hipLaunchKernelGGL(k_proj_colsum, dim3((unsigned)p), dim3(256), 0, st, A, lda, n, a);
+            GPE_HIP(hipGetLastError());
+        }
+        hipLaunchKernelGGL(k_proj_mean, dim3((unsigned)ceil_div(q_count, 256)), dim3(256), 0, st, S, a, T,
+                           (int64_t)q_count, p, q_count, n, mean);
+        GPE_HIP(hipGetLastError());
+    }
+    if (p == 0 && !center) return;
+    hipLaunchKernelGGL(k_proj_apply, dim3((unsigned)ceil_div(n, 256), (unsigned)q_count), dim3(256), 0, st, x, ldx, n,
+                       A, lda, T, (int64_t)q_count, p, center ? mean : nullptr);
+    GPE_HIP(hipGetLastError());
+}
+
 }  // namespace gpe

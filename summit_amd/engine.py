@@ -271,6 +271,35 @@ def xtv(g: Genotype, v: torch.Tensor, snp_begin: int = 0, snp_end: int | None = 
     return out
 
 
+def project(x_qn: torch.Tensor, A: torch.Tensor | None, B: torch.Tensor | None, center: bool = False,
+            stream=None) -> torch.Tensor:
+    """In place: x_q <- x_q - A (B x_q) [then minus its mean] for the rows q of
+    x_qn (Q, N) FP64 (= column-major (N, Q)); A (N, p), B (p, N) FP64 device
+    tensors or None (centring only).  Returns x_qn."""
+    if x_qn.dtype != torch.float64 or x_qn.dim() != 2 or x_qn.stride(1) != 1:
+        raise RuntimeError("projected matrix must be (Q, N) FP64 with unit stride along N")
+    Q, N = x_qn.shape
+    p = 0
+    a_ptr = b_ptr = 0
+    lda = ldb = N
+    if A is not None:
+        A_cm = A.to(torch.float64).t().contiguous()  # (p, N) rows = columns of A
+        B_rm = B.to(torch.float64).contiguous()
+        if A_cm.shape[1] != N or B_rm.shape != A_cm.shape:
+            raise RuntimeError("projection factors must be (N, p) and (p, N)")
+        p = int(A_cm.shape[0])
+        a_ptr, b_ptr = _ptr(A_cm), _ptr(B_rm)
+    if p == 0 and not center:
+        return x_qn
+    call("gpe_project_f64", _ptr(x_qn), x_qn.stride(0), N, Q, a_ptr, lda, b_ptr, ldb, p, int(bool(center)),
+         _stream(stream))
+    if A is not None and stream is not None:
+        # the factor copies were made on torch's current stream; keep them until `stream` is done
+        A_cm.record_stream(stream)
+        B_rm.record_stream(stream)
+    return x_qn
+
+
 def apply_grm(g: Genotype, panel: torch.Tensor, inv_all: torch.Tensor, nsnps: int | None = None,
               step_size: int | None = None, limbs: int = 5, stream=None) -> torch.Tensor:
     """panel_out = (1/M) X diag(inv^2) X^T panel_in over the first nsnps SNPs

@@ -160,19 +160,25 @@ def _write_back_colmajor(dst, dev):
     dst[...] = dev.cpu().numpy().T.astype(dst.dtype, copy=False)
 
 
+def _fresh(x: torch.Tensor) -> torch.Tensor:
+    """A new contiguous FP64 copy of a device matrix."""
+    out = torch.empty(x.shape, dtype=torch.float64, device=x.device)
+    out.copy_(x)
+    return out
+
+
 def _project_dev(x_qn: torch.Tensor, C, R) -> torch.Tensor:
-    """x <- x - C (R x) for x column-major (N, Q) held as (Q, N): rocBLAS FP64 via torch."""
-    Cd = _to_dev(C)
-    Rd = _to_dev(R)
-    return x_qn - (Cd @ (Rd @ x_qn.T)).T
+    """(I - C R) x for x column-major (N, Q) held as (Q, N), into a new tensor
+    (gpe_project_f64: FP64 coefficients C^T-side reduction, then the update)."""
+    return engine.project(_fresh(x_qn), _to_dev(C), _to_dev(R))
 
 
 def _project_t_dev(x_qn: torch.Tensor, C, R) -> torch.Tensor:
-    """x <- x - R^T (C^T x): the transpose projection that carries phase 2's
-    projected genotypes onto X.Z (G_p^T x = G^T (I - CR)^T x)."""
+    """x - R^T (C^T x): the transpose projection that carries phase 2's
+    projected genotypes onto X.Z (G_p^T x = G^T (I - CR)^T x), into a new tensor."""
     Cd = _to_dev(C)
     Rd = _to_dev(R)
-    return x_qn - (Rd.T @ (Cd.T @ x_qn.T)).T
+    return engine.project(_fresh(x_qn), Rd.T, Cd.T)
 
 
 # --------------------------------------------------------------------------- plumbing exports

@@ -129,10 +129,14 @@ def weighted_sumsq(g: engine.Genotype, w: torch.Tensor, snp_begin: int = 0, snp_
 
 
 def _project(M_qn: torch.Tensor, C: torch.Tensor | None) -> torch.Tensor:
-    """P applied to columns of a column-major (N, Q) matrix held as (Q, N)."""
+    """P applied to columns of a column-major (N, Q) matrix held as (Q, N), into
+    a new tensor: off the orthonormal basis C (x - C (C^T x)), then centred
+    (gpe_project_f64)."""
+    out = torch.empty(M_qn.shape, dtype=torch.float64, device=M_qn.device)
+    out.copy_(M_qn)
     if C is not None and C.shape[1] > 0:
-        M_qn = M_qn - (M_qn @ C) @ C.T
-    return M_qn - M_qn.mean(dim=1, keepdim=True)
+        return engine.project(out, C, C.T, center=True)
+    return engine.project(out, None, None, center=True)
 
 
 def features(g: engine.Genotype, env: torch.Tensor, C: torch.Tensor | None, eps_var: float = 1e-10,

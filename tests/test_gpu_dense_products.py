@@ -50,3 +50,24 @@ def test_apply_grm_panel_matches_dense(oracle):
     Xs = X[:, :450] * inv[None, :450]
     ref = Xs @ (Xs.T @ pin) / 450.0
     assert np.max(np.abs(pout - ref)) <= 1e-11 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("p,center", [(0, True), (3, False), (3, True), (40, True)])
+def test_projection_kernel_matches_numpy(p, center):
+    """x <- x - A (B x) [- mean] in FP64 on the GPU (gpe_project_f64), including
+    more factor rows than one coefficient pass (40 > 32) and centring only."""
+    from summit_amd import engine
+    rng = np.random.default_rng(p + 10 * center)
+    Q, N = 7, 1000
+    x = rng.standard_normal((Q, N))
+    A = rng.standard_normal((N, p))
+    B = rng.standard_normal((p, N)) / N
+    ref = x.T - A @ (B @ x.T) if p else x.T.copy()
+    if center:
+        ref = ref - ref.mean(axis=0, keepdims=True)
+    xd = torch.from_numpy(x).cuda()
+    if p:
+        engine.project(xd, torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda(), center=center)
+    else:
+        engine.project(xd, None, None, center=center)
+    np.testing.assert_allclose(xd.cpu().numpy(), ref.T, rtol=1e-12, atol=1e-12)
