@@ -471,9 +471,12 @@ def main() -> int:
     from summit_amd import autotune
     tuned = None
     if not args.no_tune:
-        tuned = autotune.apply_cached(N, M, 1, B, args.limbs)
-        if tuned is None:
-            tuned = autotune.tune_phase1(geno, B, 1, args.limbs, seed=seed)
+        tuned = {"phase1": autotune.apply_cached(N, M, 1, B, args.limbs),
+                 "phase2": autotune.apply_cached_phase2(M, N, 1, B, args.limbs)}
+        if tuned["phase1"] is None:
+            tuned["phase1"] = autotune.tune_phase1(geno, B, 1, args.limbs, seed=seed)
+        if tuned["phase2"] is None:
+            tuned["phase2"] = autotune.tune_phase2(geno, B, 1, args.limbs, seed=seed)
         log(f"autotune: {tuned}")
 
     annot = torch.ones((M, 1), dtype=torch.float64, device=dev)

@@ -320,10 +320,19 @@ int gpe_profile_reset(void);
  *   GPE_SCHED_P1_SPLITS:        K splits of the phase-1 contraction (1..64)
  *   GPE_SCHED_P1_FINALIZE_KIB:  SNP-range size of the phase-1 missing-call
  *                               finalize, in KiB of records (64..65536)
+ *   GPE_SCHED_P1_CPB / P2_CPB:  128-value chunks staged per LDS slot and
+ *                               barrier in the five-limb contractions (1 or 2;
+ *                               2 when no entry)
  * Phase-1 keys: rows = samples, length = SNPs of the call, columns = bins x
  * probes rounded up to 32 per bin.
  * ------------------------------------------------------------------------ */
-enum { GPE_SCHED_P1_SPLITS = 0, GPE_SCHED_P1_FINALIZE_KIB = 1, GPE_SCHED_KNOBS = 2 };
+enum {
+    GPE_SCHED_P1_SPLITS = 0,
+    GPE_SCHED_P1_FINALIZE_KIB = 1,
+    GPE_SCHED_P1_CPB = 2,  /* chunks per barrier of the five-limb phase-1 contraction (1 or 2) */
+    GPE_SCHED_P2_CPB = 3,  /* the same for phase 2; key rows = SNPs, length = samples */
+    GPE_SCHED_KNOBS = 4
+};
 int gpe_schedule_set(int32_t knob, int64_t rows, int64_t length, int32_t columns, int32_t limbs, int32_t value);
 /* The value in force for a key (0 = the heuristic decides). */
 int gpe_schedule_get(int32_t knob, int64_t rows, int64_t length, int32_t columns, int32_t limbs, int32_t* value);

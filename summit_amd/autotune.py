@@ -28,7 +28,10 @@ from ._native import c_i32, call
 
 SCHED_P1_SPLITS = 0
 SCHED_P1_FINALIZE_KIB = 1
-_KNOBS = {"p1_splits": SCHED_P1_SPLITS, "p1_finalize_kib": SCHED_P1_FINALIZE_KIB}
+SCHED_P1_CPB = 2
+SCHED_P2_CPB = 3
+_KNOBS = {"p1_splits": SCHED_P1_SPLITS, "p1_finalize_kib": SCHED_P1_FINALIZE_KIB, "p1_cpb": SCHED_P1_CPB,
+          "p2_cpb": SCHED_P2_CPB}
 
 
 def cache_path() -> str:
@@ -148,12 +151,73 @@ def tune_phase1(g: engine.Genotype, v_count: int, n_bins: int = 1, limbs: int = 
         set_value("p1_finalize_kib", key, best_range)
     else:
         best_range = None
+    # chunks per LDS slot and barrier of the phase-1 contraction
+    cpb_ms = {}
+    for cpb in (1, 2):
+        set_value("p1_cpb", key, cpb)
+        cpb_ms[cpb] = round(_time_phase1(g, L, annot, inv, v_count, seed, xz, limbs, reps, "p1_contract"), 4)
+    best_cpb = min(cpb_ms, key=cpb_ms.get)
+    set_value("p1_cpb", key, best_cpb)
     torch.cuda.synchronize()
-    choice = {"p1_splits": best_split}
+    choice = {"p1_splits": best_split, "p1_cpb": best_cpb}
     if best_range is not None:
         choice["p1_finalize_kib"] = best_range
     entry = {"choice": choice, "heuristic_ms": heuristic, "p1_contract_ms": split_ms, "p1_finalize_ms": range_ms,
+             "p1_cpb_ms": cpb_ms,
              "tune_seconds": round(time.perf_counter() - t0, 3)}
     if store:
         _store_cache(path or cache_path(), _entry_key(key), entry)
+    return entry
+
+
+def p2_key(n_snps: int, n: int, n_bins: int, v_count: int, limbs: int) -> tuple:
+    """Phase-2 shape key: (rows = SNPs, length = samples, columns, limbs)."""
+    return (int(n_snps), int(n), int(n_bins) * ((int(v_count) + 31) // 32 * 32), int(limbs))
+
+
+def apply_cached_phase2(n_snps: int, n: int, n_bins: int, v_count: int, limbs: int = 5,
+                        path: str | None = None) -> dict | None:
+    key = p2_key(n_snps, n, n_bins, v_count, limbs)
+    entry = _load_cache(path or cache_path()).get("p2|" + _entry_key(key))
+    if not entry:
+        return None
+    for knob, value in entry.get("choice", {}).items():
+        if knob in _KNOBS:
+            set_value(knob, key, value)
+    return entry
+
+
+def tune_phase2(g: engine.Genotype, v_count: int, n_bins: int = 1, limbs: int = 5, seed: int = 1, reps: int = 3,
+                path: str | None = None, store: bool = True) -> dict:
+    """Time the phase-2 contraction with one and two chunks per barrier over
+    every SNP of ``g`` against ``n_bins`` x ``v_count`` columns; keep the faster."""
+    if g.prepared_with is None:
+        raise RuntimeError("genotype handle is not prepared")
+    dev = engine.default_device()
+    N, M = g.n, g.m
+    key = p2_key(M, N, n_bins, v_count, limbs)
+    inv = engine.resvar_inv(g, 0, M)
+    gen = torch.Generator(device="cpu").manual_seed(int(seed))
+    xz = torch.randn((n_bins * v_count, N), generator=gen, dtype=torch.float64).to(dev)
+    meansq = torch.zeros((M, n_bins), dtype=torch.float64, device=dev)
+    t0 = time.perf_counter()
+    set_value("p2_cpb", key, 0)
+    engine.phase2_accum(g, 0, M, inv, v_count, xz, meansq, N, limbs=limbs)  # warm-up
+    ms = {}
+    for cpb in (1, 2):
+        set_value("p2_cpb", key, cpb)
+        times = []
+        for _ in range(reps):
+            engine.profile_enable(True)
+            engine.profile_reset()
+            engine.phase2_accum(g, 0, M, inv, v_count, xz, meansq, N, limbs=limbs)
+            times.append(engine.profile_read("p2_contract")[0])
+            engine.profile_enable(False)
+        times.sort()
+        ms[cpb] = round(times[len(times) // 2], 4)
+    best = min(ms, key=ms.get)
+    set_value("p2_cpb", key, best)
+    entry = {"choice": {"p2_cpb": best}, "p2_cpb_ms": ms, "tune_seconds": round(time.perf_counter() - t0, 3)}
+    if store:
+        _store_cache(path or cache_path(), "p2|" + _entry_key(key), entry)
     return entry

@@ -371,6 +371,8 @@ int gpe_schedule_set(int32_t knob, int64_t rows, int64_t length, int32_t columns
         if (knob == GPE_SCHED_P1_SPLITS) require(value <= 64, "phase-1 splits must be in [1, 64]");
         if (knob == GPE_SCHED_P1_FINALIZE_KIB)
             require(value <= 0 || (value >= 64 && value <= 65536), "finalize range must be in [64, 65536] KiB");
+        if (knob == GPE_SCHED_P1_CPB || knob == GPE_SCHED_P2_CPB)
+            require(value <= 2, "chunks per barrier must be 1 or 2");
         std::lock_guard<std::mutex> lock(sched_mu());
         const SchedKey k{knob, rows, length, columns, limbs};
         if (value <= 0)
@@ -860,8 +862,10 @@ void run_phase1(gpe_geno* g, int64_t s, int64_t e, const double* w, int n_bins, 
     splits = (int)std::min<int64_t>(std::max<int64_t>(splits, ceil_div(kcount, max_chunks)), kcount);
     a.kc_per_split = ceil_div(kcount, splits);
     splits = (int)ceil_div(kcount, a.kc_per_split);
+    // two chunks per barrier unless the schedule says one (measured ~3% faster at the headline shape)
+    const int cpb = sched_lookup(GPE_SCHED_P1_CPB, g->n, L, ncols_pad, lc.nl) == 1 ? 1 : 2;
     ProfScope ps(GPE_PROF_P1_CONTRACT, st);
-    launch_contract(lc.nl, lc.ctc, false, EPI_I64_ATOMIC, a, groups, splits, st);
+    launch_contract(lc.nl, lc.ctc, false, EPI_I64_ATOMIC, a, groups, splits, st, cpb);
     ps.stop();
     const bool miss = g->impute_mode == GPE_IMPUTE_MEAN && g->n_missing > 0;
     if (miss && z_i8) {
@@ -920,8 +924,9 @@ void run_phase2(gpe_geno* g, int64_t s, int64_t e, const double* x, int64_t ldx,
     a.col_sum = col_sum;
     const bool miss = g->impute_mode == GPE_IMPUTE_MEAN && g->n_missing > 0;
     a.has_miss = miss ? 1 : 0;
+    const int cpb = sched_lookup(GPE_SCHED_P2_CPB, e - s, g->n, ncols_pad, lc.nl) == 1 ? 1 : 2;
     ProfScope ps(GPE_PROF_P2_CONTRACT, st);
-    launch_contract(lc.nl, lc.ctc, miss, mode, a, groups, 1, st);
+    launch_contract(lc.nl, lc.ctc, miss, mode, a, groups, 1, st, cpb);
     ps.stop();
     if (ctiles_used_out) *ctiles_used_out = ncols / 32;
     if (v_pad_out) *v_pad_out = v_pad;

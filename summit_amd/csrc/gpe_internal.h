@@ -104,8 +104,10 @@ struct GramArgs {
 };
 
 // Launchers (gpe_kernels.hip)
+// cpb: chunks per barrier (1, or 2 for the five-limb phase-1 / phase-2 variants; other
+// variants ignore it)
 void launch_contract(int nl, int ctc, bool miss, EpiMode mode, const ContractArgs& a, int n_col_groups,
-                     int n_splits, hipStream_t st);
+                     int n_splits, hipStream_t st, int cpb = 1);
 // workgroups of one contraction variant resident on the whole device at once
 int contract_slots(int nl, int ctc, bool miss, EpiMode mode);
 void launch_clear_tail(uint8_t* snp, int64_t m, int64_t stride, int64_t n, hipStream_t st);

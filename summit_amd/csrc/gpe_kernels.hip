@@ -143,14 +143,49 @@ __device__ __forceinline__ void coef_chunk_to_lds(const int8_t* src_chunk, int8_
             (__attribute__((address_space(3))) void*)(dst + (t * 256 + wave * 64) * 16), 16, 0, 0);
 }
 
-template <int NL, int CTC, bool MISS, int EPI>
+// The four MFMA steps of one 128-value chunk whose coefficients sit at L:
+// operands op0/op1 of step 0 are already unpacked; each step unpacks the next
+// step's operands first (from ga/gb, after step 3 from the words na/nb of the
+// following chunk) so the unpack interleaves with this step's MFMAs.
+template <int CT, bool MISS, uint32_t TBL>
+__device__ __forceinline__ void chunk_steps(const int8_t* L, int lane, const v4i& ga, const v4i& gb, uint32_t na,
+                                            uint32_t nb, v4i& op0, v4i& op1, v16i (&acc)[CT][2]) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const uint32_t wn0 = s < 3 ? (uint32_t)ga[s + 1] : na;
+        v4i nx0, nx1;
+        if (MISS) {
+            unpack16(wn0, nx0, nx1);
+        } else {
+            const uint32_t wn1 = s < 3 ? (uint32_t)gb[s + 1] : nb;
+            unpack16_g<TBL>(wn0, nx0);
+            unpack16_g<TBL>(wn1, nx1);
+        }
+#pragma unroll
+        for (int t = 0; t < CT; ++t) {
+            const v4i av = *reinterpret_cast<const v4i*>(L + ((t * 4 + s) * 64 + lane) * 16);
+            acc[t][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, op0, acc[t][0], 0, 0, 0);
+            acc[t][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, op1, acc[t][1], 0, 0, 0);
+        }
+        op0 = nx0;
+        op1 = nx1;
+    }
+}
+
+// CPB = chunks per barrier: 1 -> a three-slot LDS ring of one chunk per slot,
+// loads issued two chunks ahead, one barrier per chunk; 2 -> a two-slot ring of
+// two chunks per slot, loads issued one slot (two chunks) ahead, one barrier
+// per two chunks (needs CT <= 5 to keep two workgroups per CU in 160 KiB of LDS).
+template <int NL, int CTC, bool MISS, int EPI, int CPB = 1>
 __global__ __launch_bounds__(256, (NL * CTC <= 2) ? 4 : 2) void k_contract(ContractArgs a) {
     constexpr int CT = NL * CTC;
     constexpr int RPW = MISS ? 32 : 64;
     constexpr int RPB = 4 * RPW;
     constexpr int G = MISS ? 1 : 2;  // packed-row loads per lane per chunk
     constexpr uint32_t TBL = EPI == EPI_I64_SQ ? kSqTable : kGenoTable;
-    __shared__ __attribute__((aligned(16))) int8_t lds[3][CT * 4096];
+    static_assert(CPB == 1 || CPB == 2, "one or two chunks per barrier");
+    constexpr int SLOTS = CPB == 1 ? 3 : 2;
+    __shared__ __attribute__((aligned(16))) int8_t lds[SLOTS][CPB * CT * 4096];
 
     const int tid = threadIdx.x;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -187,72 +222,126 @@ __global__ __launch_bounds__(256, (NL * CTC <= 2) ? 4 : 2) void k_contract(Contr
     const int8_t* coef_grp = a.coef + (int64_t)cg * CT * 4096;
     const int64_t coef_chunk_bytes = a.coef_ctiles * 4096;
 
-    // packed words of chunks kc (g0), kc+1 (g1); plain unrolled code, no lambdas,
-    // so these stay in VGPRs.  Prefetches past the end re-read the last chunk
-    // (in bounds, never used).
-    v4i g0a = v4i{}, g0b = v4i{}, g1a = v4i{}, g1b = v4i{};
-    if (kc_begin < kc_end) {
-        const int64_t k1 = kc_begin + 1 < kc_end ? kc_begin + 1 : kc_end - 1;
-        coef_chunk_to_lds<CT>(coef_grp + (kc_begin - a.kc0) * coef_chunk_bytes, lds[0], tid, wave);
-        g0a = *reinterpret_cast<const v4i*>(prow0 + kc_begin * 32);
-        if (!MISS) g0b = *reinterpret_cast<const v4i*>(prow1 + kc_begin * 32);
-        coef_chunk_to_lds<CT>(coef_grp + (k1 - a.kc0) * coef_chunk_bytes, lds[1], tid, wave);
-        g1a = *reinterpret_cast<const v4i*>(prow0 + k1 * 32);
-        if (!MISS) g1b = *reinterpret_cast<const v4i*>(prow1 + k1 * 32);
-    }
-    // operands of step 0 of the first chunk; later ones are unpacked one step ahead
-    v4i op0, op1;
-    if (MISS) {
-        unpack16((uint32_t)g0a[0], op0, op1);
-    } else {
-        unpack16_g<TBL>((uint32_t)g0a[0], op0);
-        unpack16_g<TBL>((uint32_t)g0b[0], op1);
-    }
-    int buf = 0;  // LDS buffer of chunk kc
-    for (int64_t kc = kc_begin; kc < kc_end; ++kc) {
-        // chunk kc's coefficients (and packed words) have landed once only the
-        // CT + G loads issued for chunk kc+1 may still be in flight; the barrier
-        // then publishes every wave's part of the chunk
-        __builtin_amdgcn_s_waitcnt(waitcnt_vm(CT + G));
-        asm volatile("" ::: "memory");
-        __builtin_amdgcn_s_barrier();
-        asm volatile("" ::: "memory");
-        // chunk kc+2 into the buffer read two chunks ago (k2 passes through an empty
-        // asm so these loads cannot be hoisted above the wait and barrier)
-        int64_t k2 = kc + 2 < kc_end ? kc + 2 : kc_end - 1;
-        asm volatile("" : "+s"(k2));
-        const int buf2 = buf == 0 ? 2 : buf - 1;
-        coef_chunk_to_lds<CT>(coef_grp + (k2 - a.kc0) * coef_chunk_bytes, lds[buf2], tid, wave);
-        const v4i g2a = *reinterpret_cast<const v4i*>(prow0 + k2 * 32);
-        const v4i g2b = MISS ? v4i{} : *reinterpret_cast<const v4i*>(prow1 + k2 * 32);
-        const int8_t* L = lds[buf];
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            // unpack the operands of the next step (step 0 of the next chunk after
-            // step 3) before this step's MFMAs, so the two are independent and interleave
-            const uint32_t wn0 = s < 3 ? (uint32_t)g0a[s + 1] : (uint32_t)g1a[0];
-            v4i nx0, nx1;
-            if (MISS) {
-                unpack16(wn0, nx0, nx1);
-            } else {
-                const uint32_t wn1 = s < 3 ? (uint32_t)g0b[s + 1] : (uint32_t)g1b[0];
-                unpack16_g<TBL>(wn0, nx0);
-                unpack16_g<TBL>(wn1, nx1);
-            }
-#pragma unroll
-            for (int t = 0; t < CT; ++t) {
-                const v4i av = *reinterpret_cast<const v4i*>(L + ((t * 4 + s) * 64 + lane) * 16);
-                acc[t][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, op0, acc[t][0], 0, 0, 0);
-                acc[t][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, op1, acc[t][1], 0, 0, 0);
-            }
-            op0 = nx0;
-            op1 = nx1;
+    if constexpr (CPB == 1) {
+        // packed words of chunks kc (g0), kc+1 (g1); plain unrolled code, no lambdas,
+        // so these stay in VGPRs.  Prefetches past the end re-read the last chunk
+        // (in bounds, never used).
+        v4i g0a = v4i{}, g0b = v4i{}, g1a = v4i{}, g1b = v4i{};
+        if (kc_begin < kc_end) {
+            const int64_t k1 = kc_begin + 1 < kc_end ? kc_begin + 1 : kc_end - 1;
+            coef_chunk_to_lds<CT>(coef_grp + (kc_begin - a.kc0) * coef_chunk_bytes, lds[0], tid, wave);
+            g0a = *reinterpret_cast<const v4i*>(prow0 + kc_begin * 32);
+            if (!MISS) g0b = *reinterpret_cast<const v4i*>(prow1 + kc_begin * 32);
+            coef_chunk_to_lds<CT>(coef_grp + (k1 - a.kc0) * coef_chunk_bytes, lds[1], tid, wave);
+            g1a = *reinterpret_cast<const v4i*>(prow0 + k1 * 32);
+            if (!MISS) g1b = *reinterpret_cast<const v4i*>(prow1 + k1 * 32);
         }
-        g0a = g1a;
-        g0b = g1b;
-        g1a = g2a;
-        g1b = g2b;
-        buf = buf == 2 ? 0 : buf + 1;
+        // operands of step 0 of the first chunk; later ones are unpacked one step ahead
+        v4i op0, op1;
+        if (MISS) {
+            unpack16((uint32_t)g0a[0], op0, op1);
+        } else {
+            unpack16_g<TBL>((uint32_t)g0a[0], op0);
+            unpack16_g<TBL>((uint32_t)g0b[0], op1);
+        }
+        int buf = 0;  // LDS buffer of chunk kc
+        for (int64_t kc = kc_begin; kc < kc_end; ++kc) {
+            // chunk kc's coefficients (and packed words) have landed once only the
+            // CT + G loads issued for chunk kc+1 may still be in flight; the barrier
+            // then publishes every wave's part of the chunk
+            __builtin_amdgcn_s_waitcnt(waitcnt_vm(CT + G));
+            asm volatile("" ::: "memory");
+            __builtin_amdgcn_s_barrier();
+            asm volatile("" ::: "memory");
+            // chunk kc+2 into the buffer read two chunks ago (k2 passes through an empty
+            // asm so these loads cannot be hoisted above the wait and barrier)
+            int64_t k2 = kc + 2 < kc_end ? kc + 2 : kc_end - 1;
+            asm volatile("" : "+s"(k2));
+            const int buf2 = buf == 0 ? 2 : buf - 1;
+            coef_chunk_to_lds<CT>(coef_grp + (k2 - a.kc0) * coef_chunk_bytes, lds[buf2], tid, wave);
+            const v4i g2a = *reinterpret_cast<const v4i*>(prow0 + k2 * 32);
+            const v4i g2b = MISS ? v4i{} : *reinterpret_cast<const v4i*>(prow1 + k2 * 32);
+            const int8_t* L = lds[buf];
+    #pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                // unpack the operands of the next step (step 0 of the next chunk after
+                // step 3) before this step's MFMAs, so the two are independent and interleave
+                const uint32_t wn0 = s < 3 ? (uint32_t)g0a[s + 1] : (uint32_t)g1a[0];
+                v4i nx0, nx1;
+                if (MISS) {
+                    unpack16(wn0, nx0, nx1);
+                } else {
+                    const uint32_t wn1 = s < 3 ? (uint32_t)g0b[s + 1] : (uint32_t)g1b[0];
+                    unpack16_g<TBL>(wn0, nx0);
+                    unpack16_g<TBL>(wn1, nx1);
+                }
+    #pragma unroll
+                for (int t = 0; t < CT; ++t) {
+                    const v4i av = *reinterpret_cast<const v4i*>(L + ((t * 4 + s) * 64 + lane) * 16);
+                    acc[t][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, op0, acc[t][0], 0, 0, 0);
+                    acc[t][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, op1, acc[t][1], 0, 0, 0);
+                }
+                op0 = nx0;
+                op1 = nx1;
+            }
+            g0a = g1a;
+            g0b = g1b;
+            g1a = g2a;
+            g1b = g2b;
+            buf = buf == 2 ? 0 : buf + 1;
+        }
+    } else {
+        // super chunk = chunks kc, kc+1 in one slot; words of the current pair in
+        // g0/g1, of the next pair in g2/g3 (loaded one slot ahead)
+        constexpr int64_t kSlot = (int64_t)CT * 4096;
+        v4i g0a = v4i{}, g0b = v4i{}, g1a = v4i{}, g1b = v4i{};
+        if (kc_begin < kc_end) {
+            const int64_t c1 = kc_begin + 1 < kc_end ? kc_begin + 1 : kc_end - 1;
+            coef_chunk_to_lds<CT>(coef_grp + (kc_begin - a.kc0) * coef_chunk_bytes, lds[0], tid, wave);
+            coef_chunk_to_lds<CT>(coef_grp + (c1 - a.kc0) * coef_chunk_bytes, lds[0] + kSlot, tid, wave);
+            g0a = *reinterpret_cast<const v4i*>(prow0 + kc_begin * 32);
+            if (!MISS) g0b = *reinterpret_cast<const v4i*>(prow1 + kc_begin * 32);
+            g1a = *reinterpret_cast<const v4i*>(prow0 + c1 * 32);
+            if (!MISS) g1b = *reinterpret_cast<const v4i*>(prow1 + c1 * 32);
+        }
+        v4i op0, op1;
+        if (MISS) {
+            unpack16((uint32_t)g0a[0], op0, op1);
+        } else {
+            unpack16_g<TBL>((uint32_t)g0a[0], op0);
+            unpack16_g<TBL>((uint32_t)g0b[0], op1);
+        }
+        int buf = 0;
+        for (int64_t kc = kc_begin; kc < kc_end; kc += 2) {
+            // everything issued for this pair (coefficients into LDS by every wave,
+            // this lane's packed words) has landed; the barrier publishes the slot
+            // and frees the other one, read during the previous pair
+            __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
+            asm volatile("" ::: "memory");
+            __builtin_amdgcn_s_barrier();
+            asm volatile("" ::: "memory");
+            int64_t n0 = kc + 2 < kc_end ? kc + 2 : kc_end - 1;
+            int64_t n1 = kc + 3 < kc_end ? kc + 3 : kc_end - 1;
+            asm volatile("" : "+s"(n0));
+            asm volatile("" : "+s"(n1));
+            int8_t* nxt = lds[buf ^ 1];
+            coef_chunk_to_lds<CT>(coef_grp + (n0 - a.kc0) * coef_chunk_bytes, nxt, tid, wave);
+            coef_chunk_to_lds<CT>(coef_grp + (n1 - a.kc0) * coef_chunk_bytes, nxt + kSlot, tid, wave);
+            const v4i g2a = *reinterpret_cast<const v4i*>(prow0 + n0 * 32);
+            const v4i g2b = MISS ? v4i{} : *reinterpret_cast<const v4i*>(prow1 + n0 * 32);
+            const v4i g3a = *reinterpret_cast<const v4i*>(prow0 + n1 * 32);
+            const v4i g3b = MISS ? v4i{} : *reinterpret_cast<const v4i*>(prow1 + n1 * 32);
+            const int8_t* L = lds[buf];
+            chunk_steps<CT, MISS, TBL>(L, lane, g0a, g0b, (uint32_t)g1a[0], (uint32_t)g1b[0], op0, op1, acc);
+            if (kc + 1 < kc_end)
+                chunk_steps<CT, MISS, TBL>(L + kSlot, lane, g1a, g1b, (uint32_t)g2a[0], (uint32_t)g2b[0], op0, op1,
+                                           acc);
+            g0a = g2a;
+            g0b = g2b;
+            g1a = g3a;
+            g1b = g3b;
+            buf ^= 1;
+        }
     }
     // no LDS-load may still be writing when the workgroup ends
     __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
@@ -388,18 +477,33 @@ __global__ __launch_bounds__(256, (NL * CTC <= 2) ? 4 : 2) void k_contract(Contr
     }
 }
 
-template <int NL, int CTC, bool MISS, int EPI>
+template <int NL, int CTC, bool MISS, int EPI, int CPB = 1>
 static void launch_one(const ContractArgs& a, int n_col_groups, int n_splits, hipStream_t st) {
     constexpr int RPB = 4 * (MISS ? 32 : 64);
     const int64_t rows = a.r1 - a.r0;
     if (rows <= 0 || a.kc1 <= a.kc0) return;
     dim3 grid((unsigned)ceil_div(rows, RPB), (unsigned)n_col_groups, (unsigned)n_splits);
-    hipLaunchKernelGGL((k_contract<NL, CTC, MISS, EPI>), grid, dim3(256), 0, st, a);
+    hipLaunchKernelGGL((k_contract<NL, CTC, MISS, EPI, CPB>), grid, dim3(256), 0, st, a);
     GPE_HIP(hipGetLastError());
 }
 
 void launch_contract(int nl, int ctc, bool miss, EpiMode mode, const ContractArgs& a, int n_col_groups,
-                     int n_splits, hipStream_t st) {
+                     int n_splits, hipStream_t st, int cpb) {
+    // two chunks per barrier: the phase-1 and phase-2 variants of the five-limb path
+    if (cpb == 2) {
+        if (nl == 5 && ctc == 1 && !miss && mode == EPI_I64_ATOMIC) {
+            launch_one<5, 1, false, EPI_I64_ATOMIC, 2>(a, n_col_groups, n_splits, st);
+            return;
+        }
+        if (nl == 5 && ctc == 1 && miss && mode == EPI_P2) {
+            launch_one<5, 1, true, EPI_P2, 2>(a, n_col_groups, n_splits, st);
+            return;
+        }
+        if (nl == 5 && ctc == 1 && !miss && mode == EPI_P2) {
+            launch_one<5, 1, false, EPI_P2, 2>(a, n_col_groups, n_splits, st);
+            return;
+        }
+    }
 #define GPE_CASE(NL_, CTC_, MISS_, EPI_)                                                          \
     if (nl == NL_ && ctc == CTC_ && miss == MISS_ && mode == EPI_) {                              \
         launch_one<NL_, CTC_, MISS_, EPI_>(a, n_col_groups, n_splits, st);                        \

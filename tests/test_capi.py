@@ -68,6 +68,10 @@ def test_schedule_table_set_get_clear_and_validation():
         autotune.set_value("p1_splits", key, 65)
     with pytest.raises(_native.GpeError, match="finalize range"):
         autotune.set_value("p1_finalize_kib", key, 8)
+    with pytest.raises(_native.GpeError, match="chunks per barrier"):
+        autotune.set_value("p2_cpb", key, 3)
+    autotune.set_value("p1_cpb", key, 2)
+    assert autotune.get_value("p1_cpb", key) == 2
     with pytest.raises(_native.GpeError, match="unknown schedule knob"):
         _native.call("gpe_schedule_set", 7, 1, 1, 1, 1, 1)
     autotune.clear()

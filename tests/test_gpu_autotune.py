@@ -46,3 +46,39 @@ def test_tune_phase1_caches_and_keeps_results():
     assert autotune.get_value("p1_splits", key) == entry["choice"]["p1_splits"]
     autotune.clear()
     g.close()
+
+
+@pytest.mark.parametrize("n,impute", [(1000, "mean"), (1100, "mean"), (1100, "hwe")])
+def test_two_chunks_per_barrier_is_bit_identical(n, impute):
+    """The two-chunks-per-barrier contraction (a schedule knob) gives bit-identical
+    phase-1 and phase-2 results: even and odd chunk counts (N = 1000 -> 8 chunks,
+    1100 -> 9), K splits with odd chunk counts per split, missing plane or not."""
+    from summit_amd import autotune, engine
+    dev = engine.default_device()
+    M, V = 3000, 40
+    print(f"cpb parity N={n} M={M} {impute}", flush=True)
+    g = engine.Genotype.synthetic(n, M, seed=21, maf_lo=0.05, maf_hi=0.5, miss_rate=0.01, block_len=100,
+                                  copy_prob=0.5)
+    g.prepare(ddof=1, impute=impute, impute_seed=4)
+    annot = torch.ones((M, 1), dtype=torch.float64, device=dev)
+    inv = engine.resvar_inv(g, 0, M)
+    k1 = autotune.shape_key(n, M, 1, V, 5)
+    k2 = (M, n, k1[2], 5)
+    autotune.clear()
+    outs = {}
+    for cpb in (1, 2):
+        for splits in (1, 3):
+            autotune.set_value("p1_cpb", k1, cpb)
+            autotune.set_value("p2_cpb", k2, cpb)
+            autotune.set_value("p1_splits", k1, splits)
+            xz = torch.zeros((V, n), dtype=torch.float64, device=dev)
+            engine.phase1_xz(g, 0, M, 1000, annot, inv, 0, V, 99, xz)
+            meansq = torch.zeros((M, 1), dtype=torch.float64, device=dev)
+            engine.phase2_accum(g, 0, M, inv, V, xz, meansq, n)
+            outs[(cpb, splits)] = (xz.cpu().numpy(), meansq.cpu().numpy())
+    autotune.clear()
+    g.close()
+    ref = outs[(1, 1)]
+    for key, (xz, ms) in outs.items():
+        assert np.array_equal(xz, ref[0]), key
+        assert np.array_equal(ms, ref[1]), key
