@@ -111,7 +111,8 @@ def wall_to_one_percent(engine, ldscore, torch, dev) -> dict:
     missing, seeds 101 / 1101).  The exact target L_j = sum_i r_ij^2 - M/(N-1)
     (r with the residual scales of the estimator) comes from a dense FP64
     X X^T on the GPU.  Probe counts double from 16,384 until the error bar is
-    met; the reported time is genotype preparation plus the run that met it."""
+    met; the reported time is genotype preparation plus the run that met it.
+    V-tiles of 256 probes, eight thousand probes per traversal pair."""
     N, M, root = 5000, 10000, 1101
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -140,7 +141,9 @@ def wall_to_one_percent(engine, ldscore, torch, dev) -> dict:
     for B in (16384, 32768, 65536, 131072, 262144):
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        res = ldscore.genomewide_ldscore(g, nvecs=B, seed=root, vtiles=ldscore.bounded_vtile_sizes(B, 8192))
+        # V-tiles of 256 probes (the reference's own granularity rule), served 8,192 at a time
+        res = ldscore.genomewide_ldscore(g, nvecs=B, seed=root, vtiles=ldscore.bounded_vtile_sizes(B, 256),
+                                         traversal_width=8192)
         t_run = time.perf_counter() - t1
         err = float(np.mean(np.abs(res.ldscore[:, 0] - exact) / np.maximum(np.abs(exact), 1e-12)))
         tried.append({"probes": B, "seconds": round(t_run, 4), "mean_rel_err": round(err, 5)})
@@ -382,7 +385,7 @@ def learned_cv(engine, ldscore, torch, dev, plain_max: int = 98304, plain_tile: 
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         xz.zero_()
-        engine.phase1_xz(gb, 0, M, 1000, annot1, inv, v0, plain_tile, 1506, xz)
+        engine.phase1_xz(gb, 0, M, 1000, annot1, inv, v0, plain_tile, 1506, xz, probe_tile=256)
         engine.phase2_accum(gb, 0, M, inv, plain_tile, xz, meansq, N)
         torch.cuda.synchronize()
         elapsed += time.perf_counter() - t0
@@ -418,7 +421,7 @@ def main() -> int:
     ap.add_argument("--no-exact", action="store_true")
     ap.add_argument("--no-cv", action="store_true", help="skip the learned control-variate measurement")
     ap.add_argument("--no-tune", action="store_true", help="keep the built-in schedule heuristics")
-    ap.add_argument("--only", choices=["learned_cv", "partitioned", "gxe", "windowed"],
+    ap.add_argument("--only", choices=["learned_cv", "partitioned", "gxe", "windowed", "wall_to_1pct"],
                     help="run only one extra measurement and print it")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the 1%%-error wall-time run and the host->HBM ingest measurement")
@@ -456,6 +459,9 @@ def main() -> int:
         return 0
     if args.only == "windowed":
         print(json.dumps({"windowed": windowed_rate(engine, torch)}), flush=True)
+        return 0
+    if args.only == "wall_to_1pct":
+        print(json.dumps({"wall_to_1pct": wall_to_one_percent(engine, ldscore, torch, dev)}), flush=True)
         return 0
     N, M, B = args.n, args.m, args.b
     seed = 1303

@@ -154,6 +154,19 @@ int gpe_phase1_xz(gpe_geno* g, int64_t snp_begin, int64_t snp_end, int64_t step_
                   int32_t v_count, int32_t rand_dist, uint64_t root_seed, double* xz_dev, int64_t ldxz,
                   const uint8_t* bin_mode_dev, int32_t limbs, void* stream);
 
+/* gpe_phase1_xz over several of the reference's V-tiles at once: probe
+ * columns [t*probe_tile, (t+1)*probe_tile) carry the streams of V-tile
+ * v_start + t*probe_tile (make_seed(root, block start, v_start + t*probe_tile)),
+ * exactly as separate calls with v_count = probe_tile would, in one traversal.
+ * The reference's probe values depend on its V-tile split
+ * (src/summit/ldscore/gw_ldscore.py:687-697); this lets a wide traversal keep a
+ * narrow split, and gives the GPU one probe stream per (block, tile) to run in
+ * parallel. */
+int gpe_phase1_xz_tiled(gpe_geno* g, int64_t snp_begin, int64_t snp_end, int64_t step_size, const double* annot_dev,
+                        int32_t n_bins, const double* inv_dev, int32_t v_start, int32_t v_count, int32_t probe_tile,
+                        int32_t rand_dist, uint64_t root_seed, double* xz_dev, int64_t ldxz,
+                        const uint8_t* bin_mode_dev, int32_t limbs, void* stream);
+
 /* Phase 2, X^T(X.Z): for SNPs j in [snp_begin, snp_end) and bins k,
  *   meansq[j][k] += sum_v (x_j . Xz[:, k*tile_nvecs + v])^2 * inv_j^2/(N_denom-1)^2
  * with the non-finite / extreme-update guards of src/native/gwldcore.cpp:295-417

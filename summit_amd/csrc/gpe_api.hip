@@ -1053,7 +1053,17 @@ int gpe_phase1_xz(gpe_geno* g, int64_t snp_begin, int64_t snp_end, int64_t step_
                   int32_t n_bins, const double* inv_dev, int32_t v_start, int32_t v_count, int32_t rand_dist,
                   uint64_t root_seed, double* xz_dev, int64_t ldxz, const uint8_t* bin_mode_dev, int32_t limbs,
                   void* stream) {
+    return gpe_phase1_xz_tiled(g, snp_begin, snp_end, step_size, annot_dev, n_bins, inv_dev, v_start, v_count, v_count,
+                               rand_dist, root_seed, xz_dev, ldxz, bin_mode_dev, limbs, stream);
+}
+
+int gpe_phase1_xz_tiled(gpe_geno* g, int64_t snp_begin, int64_t snp_end, int64_t step_size, const double* annot_dev,
+                        int32_t n_bins, const double* inv_dev, int32_t v_start, int32_t v_count, int32_t probe_tile,
+                        int32_t rand_dist, uint64_t root_seed, double* xz_dev, int64_t ldxz,
+                        const uint8_t* bin_mode_dev, int32_t limbs, void* stream) {
     return guarded([&] {
+        require(probe_tile > 0, "probe_tile must be positive");
+        const int sub = std::min(probe_tile, v_count);
         require_prepared(g);
         require(snp_begin >= 0 && snp_end <= g->m && snp_begin <= snp_end, "blk_end exceeds #SNPs in BIM");
         require(step_size > 0, "step_size must be positive");
@@ -1078,15 +1088,18 @@ int gpe_phase1_xz(gpe_geno* g, int64_t snp_begin, int64_t snp_end, int64_t step_
             z_i8 = static_cast<int8_t*>(g->scratch("p1_z8", g->m_alloc * ldz));
             int64_t* starts_dev = static_cast<int64_t*>(g->scratch("p1_starts", nb * sizeof(int64_t)));
             GPE_HIP(hipMemcpyAsync(starts_dev, starts.data(), nb * sizeof(int64_t), hipMemcpyHostToDevice, st));
-            launch_mt_rademacher(root_seed, starts_dev, nb, snp_end, v_start, v_count, z_i8, ldz, st);
+            launch_mt_rademacher(root_seed, starts_dev, nb, snp_end, v_start, v_count, z_i8, ldz, st, sub);
         } else if (is_real_probe_dist(rand_dist)) {
             z_f64 = static_cast<double*>(g->scratch("p1_z64", g->m_alloc * ldz * sizeof(double)));
-            std::vector<double> zh((size_t)(L * v_count));
-            require(gpe_probes(rand_dist, root_seed, starts.data(), nb, snp_end, v_start, v_count, zh.data()) == 0,
-                    g_err);
             std::vector<double> zp((size_t)(L * ldz), 0.0);
-            for (int64_t r = 0; r < L; ++r)
-                for (int c = 0; c < v_count; ++c) zp[(size_t)(r * ldz + c)] = zh[(size_t)(r * v_count + c)];
+            for (int v0 = 0; v0 < v_count; v0 += sub) {
+                const int vc = std::min(sub, v_count - v0);
+                std::vector<double> zh((size_t)(L * vc));
+                require(gpe_probes(rand_dist, root_seed, starts.data(), nb, snp_end, v_start + v0, vc, zh.data()) == 0,
+                        g_err);
+                for (int64_t r = 0; r < L; ++r)
+                    for (int c = 0; c < vc; ++c) zp[(size_t)(r * ldz + v0 + c)] = zh[(size_t)(r * vc + c)];
+            }
             GPE_HIP(hipMemcpyAsync(z_f64 + snp_begin * ldz, zp.data(), zp.size() * sizeof(double),
                                    hipMemcpyHostToDevice, st));
             GPE_HIP(hipStreamSynchronize(st));

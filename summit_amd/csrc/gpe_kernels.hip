@@ -903,13 +903,19 @@ __device__ __forceinline__ uint64_t mt_twist_one(uint64_t a, uint64_t b, uint64_
     return c ^ (x >> 1) ^ ((x & 1ULL) ? kMtA : 0ULL);
 }
 
+// One wave per (SNP block, probe tile): the stream of the reference's V-tile
+// [v_start + t*sub, + sub) of that block, written to probe columns t*sub.. .
 __global__ __launch_bounds__(64) void k_mt_rademacher(uint64_t root, const int64_t* blk_starts, int n_blocks,
-                                                     int64_t blk_end_all, int v_start, int v_count, int8_t* z,
-                                                     int64_t ldz) {
+                                                     int64_t blk_end_all, int v_start, int v_count_all, int sub,
+                                                     int8_t* z_all, int64_t ldz) {
     __shared__ uint64_t mt[kMtN];
     const int b = blockIdx.x;
     const int lane = threadIdx.x;
     if (b >= n_blocks) return;
+    const int v0 = (int)blockIdx.y * sub;
+    if (v0 >= v_count_all) return;
+    const int v_count = v_count_all - v0 < sub ? v_count_all - v0 : sub;
+    int8_t* z = z_all + v0;
     const int64_t s = blk_starts[b];
     const int64_t e = (b + 1 < n_blocks) ? blk_starts[b + 1] : blk_end_all;
     const int64_t L = e - s;
@@ -917,7 +923,7 @@ __global__ __launch_bounds__(64) void k_mt_rademacher(uint64_t root, const int64
         uint64_t seed = 0x1234abcdULL;
         seed ^= mix64(root);
         seed ^= mix64((uint64_t)(int64_t)(int32_t)s + 0x9e37ULL);
-        seed ^= mix64((uint64_t)(int64_t)v_start + 0x85ebULL);
+        seed ^= mix64((uint64_t)(int64_t)(v_start + v0) + 0x85ebULL);
         mt[0] = seed;
         for (int i = 1; i < kMtN; ++i) mt[i] = 6364136223846793005ULL * (mt[i - 1] ^ (mt[i - 1] >> 62)) + (uint64_t)i;
     }
@@ -969,10 +975,12 @@ __global__ __launch_bounds__(64) void k_mt_rademacher(uint64_t root, const int64
     }
 }
 void launch_mt_rademacher(uint64_t root, const int64_t* blk_starts, int n_blocks, int64_t blk_end_all,
-                          int v_start, int v_count, int8_t* z, int64_t ldz, hipStream_t st) {
+                          int v_start, int v_count, int8_t* z, int64_t ldz, hipStream_t st, int sub) {
     if (n_blocks <= 0 || v_count <= 0) return;
-    hipLaunchKernelGGL(k_mt_rademacher, dim3((unsigned)n_blocks), dim3(64), 0, st, root, blk_starts, n_blocks,
-                       blk_end_all, v_start, v_count, z, ldz);
+    if (sub <= 0 || sub > v_count) sub = v_count;
+    const int n_sub = (int)ceil_div(v_count, sub);
+    hipLaunchKernelGGL(k_mt_rademacher, dim3((unsigned)n_blocks, (unsigned)n_sub), dim3(64), 0, st, root, blk_starts,
+                       n_blocks, blk_end_all, v_start, v_count, sub, z, ldz);
     GPE_HIP(hipGetLastError());
 }
 

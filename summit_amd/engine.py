@@ -194,8 +194,10 @@ def resvar_inv(g: Genotype, snp_begin: int = 0, snp_end: int | None = None, eps:
 def phase1_xz(g: Genotype, snp_begin: int, snp_end: int, step_size: int, annot: torch.Tensor,
               inv: torch.Tensor, v_start: int, v_count: int, seed: int, xz: torch.Tensor,
               rand_dist: str = "rademacher", bin_mode: torch.Tensor | None = None, limbs: int = 5,
-              stream=None, probe_fp32: bool = False) -> torch.Tensor:
-    """Xz[k*v_count+v, :] (+)= sum_i x_i inv_i sqrt(a_ik) z_iv over [snp_begin, snp_end)."""
+              stream=None, probe_fp32: bool = False, probe_tile: int | None = None) -> torch.Tensor:
+    """Xz[k*v_count+v, :] (+)= sum_i x_i inv_i sqrt(a_ik) z_iv over [snp_begin, snp_end).
+    probe_tile: probe columns come in V-tiles of this many, each with its own
+    per-block streams (as separate calls of that width would draw)."""
     L = snp_end - snp_begin
     annot = annot.to(torch.float64).contiguous()
     inv = inv.to(torch.float64).contiguous()
@@ -206,9 +208,10 @@ def phase1_xz(g: Genotype, snp_begin: int, snp_end: int, step_size: int, annot: 
     B = annot.shape[1]
     if xz.dtype != torch.float64 or xz.dim() != 2 or xz.shape[0] != B * v_count or xz.shape[1] != g.n:
         raise RuntimeError("Xz2d_chunk shape must be (N, B*v_count)")
-    call("gpe_phase1_xz", g.handle, snp_begin, snp_end, step_size, _ptr(annot), B, _ptr(inv), v_start, v_count,
-         rand_code(rand_dist, probe_fp32), int(seed) & 0xFFFFFFFFFFFFFFFF, _ptr(xz), xz.stride(0), _ptr(bin_mode),
-         limbs, _stream(stream))
+    tile = v_count if probe_tile is None else int(probe_tile)
+    call("gpe_phase1_xz_tiled", g.handle, snp_begin, snp_end, step_size, _ptr(annot), B, _ptr(inv), v_start, v_count,
+         tile, rand_code(rand_dist, probe_fp32), int(seed) & 0xFFFFFFFFFFFFFFFF, _ptr(xz), xz.stride(0),
+         _ptr(bin_mode), limbs, _stream(stream))
     return xz
 
 

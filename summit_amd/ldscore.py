@@ -45,12 +45,31 @@ class LDScoreResult:
     mc_var: np.ndarray | None = None  # per-SNP MC variance of ldscore (mc=True)
 
 
+def traversal_plan(vtiles: list[int], width: int | None) -> list[tuple[int, int, int]]:
+    """(v_start, v_count, probe_tile) per traversal: consecutive V-tiles of one
+    size merge into traversals of up to ``width`` probe columns (None: one
+    traversal per V-tile)."""
+    plan, v0, i = [], 0, 0
+    while i < len(vtiles):
+        P, k = int(vtiles[i]), 1
+        if width:
+            while i + k < len(vtiles) and int(vtiles[i + k]) == P and (k + 1) * P <= width:
+                k += 1
+        plan.append((v0, k * P, P))
+        v0 += k * P
+        i += k
+    return plan
+
+
 def genomewide_ldscore(geno: engine.Genotype, nvecs: int, seed: int, annot: np.ndarray | None = None,
                        step_size: int = 1000, ddof: int = 1, vtiles: list[int] | None = None,
                        rand_dist: str = "rademacher", eps: float = 1e-10, limbs: int = 5, mc: bool = False,
-                       stream=None) -> LDScoreResult:
+                       stream=None, traversal_width: int | None = None) -> LDScoreResult:
     """The prepared ``geno`` must have been prepared with the same ddof (and the
-    impute mode / seed of the run)."""
+    impute mode / seed of the run).  ``vtiles`` are the reference's V-tiles
+    (they fix the probe streams); ``traversal_width`` lets one pair of
+    traversals serve several equal V-tiles (same estimate, fewer, wider
+    traversals, and one probe stream per block and tile in parallel)."""
     if geno.prepared_with is None:
         raise RuntimeError("genotype handle is not prepared")
     M, N = geno.m, geno.n
@@ -70,17 +89,15 @@ def genomewide_ldscore(geno: engine.Genotype, nvecs: int, seed: int, annot: np.n
     mc_agg = torch.zeros(K, dtype=torch.float64, device=dev) if mc else None
     mc_ps = torch.zeros((M, K), dtype=torch.float64, device=dev) if mc else None
     n_denom = N - ddof + 1
-    v_start = 0
     guards = [0, 0]
-    for Vt in vtiles:
+    for v_start, Vt, tile in traversal_plan(vtiles, traversal_width):
         xz = torch.zeros((K * Vt, N), dtype=torch.float64, device=dev)
         engine.phase1_xz(geno, 0, M, step_size, annot_d, inv, v_start, Vt, seed, xz, rand_dist=rand_dist,
-                         limbs=limbs, stream=stream)
+                         limbs=limbs, stream=stream, probe_tile=tile)
         gc = engine.phase2_accum(geno, 0, M, inv, Vt, xz, meansq, n_denom, mc_aggregate_sumsq=mc_agg,
                                  mc_per_snp_sumsq=mc_ps, limbs=limbs, stream=stream, want_guards=True)
         guards[0] += gc[0]
         guards[1] += gc[1]
-        v_start += Vt
     torch.cuda.synchronize()
     meansq_h = meansq.cpu().numpy()
     nsnps_bin = annot.sum(axis=0, dtype=np.float64)

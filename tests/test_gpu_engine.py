@@ -233,3 +233,31 @@ def test_ldscore_matches_oracle(engine, oracle, mode, limbs):
     tol = 1e-10 * np.maximum(np.abs(ref.ldscore), 1.0)
     assert np.all(np.abs(res.ldscore - ref.ldscore) <= tol)
     g.close()
+
+
+@pytest.mark.parametrize("rand_dist", ["rademacher", "gaussian"])
+def test_wide_traversals_keep_the_vtile_streams(engine, oracle, rand_dist):
+    """Several equal V-tiles served by one traversal pair (probe columns in tiles,
+    each tile with its own per-block streams) give the reference's estimate for
+    that V-tile split: tiles [8, 8, 8, 4] in traversals of 24 + 4 columns; a
+    SNP block count (1500 / 400 -> 4 blocks, the last short) with several
+    streams per launch; bins."""
+    _say(f"wide traversals {rand_dist}")
+    from summit_amd import ldscore
+    m, n, step = 1500, 600, 400
+    codes = random_codes(m, n, seed=33, miss_rate=0.01)
+    seed = 77
+    annot = np.ones((m, 2))
+    annot[:, 1] = np.random.default_rng(4).random(m) < 0.4
+    g = _geno(engine, oracle, codes).prepare(impute="mean")
+    vt = [8, 8, 8, 4]
+    assert ldscore.traversal_plan(vt, 24) == [(0, 24, 8), (24, 4, 4)]
+    res = ldscore.genomewide_ldscore(g, 28, seed, annot=annot, step_size=step, vtiles=vt, traversal_width=24,
+                                     rand_dist=rand_dist)
+    ref = oracle.gw_ldscore(oracle.pack_codes(codes), n, m, annot=annot, nvecs=28, step_size=step, seed=seed,
+                            impute="mean", vtiles=vt, rand_dist=rand_dist)
+    tol = 1e-10 * np.maximum(np.abs(ref.ldscore), 1.0)
+    assert np.all(np.abs(res.ldscore - ref.ldscore) <= tol)
+    one = ldscore.genomewide_ldscore(g, 28, seed, annot=annot, step_size=step, vtiles=vt, rand_dist=rand_dist)
+    np.testing.assert_allclose(res.meansq_accum, one.meansq_accum, rtol=1e-12)
+    g.close()
