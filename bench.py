@@ -570,6 +570,23 @@ def main() -> int:
         del u, w, z
 
     if not args.no_extras:
+        # the same step with the reference's float32 Gaussian probes (host std::normal_distribution<float>
+        # per SNP block, real coefficients on the same int8-limb contraction)
+        def gstep(i: int) -> None:
+            xz.zero_()
+            engine.phase1_xz(geno, 0, M, 1000, annot, inv, i * B, B, seed, xz, rand_dist="gaussian", probe_fp32=True,
+                             limbs=args.limbs)
+            engine.phase2_accum(geno, 0, M, inv, B, xz, meansq, N, limbs=args.limbs)
+        gstep(0)
+        torch.cuda.synchronize()
+        tg = time.perf_counter()
+        for i in range(3):
+            gstep(1 + i)
+        torch.cuda.synchronize()
+        gms = (time.perf_counter() - tg) * 1000.0 / 3
+        extra["gaussian_f32"] = {"ms_per_step": round(gms, 3), "gop_s": round(pairs / (gms / 1000.0) / 1e9, 1),
+                                 "workload": f"N={N} M={M} B={B} float32 Gaussian probes, 3 steps"}
+        log(f"gaussian f32: {extra['gaussian_f32']}")
         extra["ingest"] = ingest_rate(engine, torch, geno, min(M, 100_000))
         log(f"ingest: {extra['ingest']}")
 

@@ -780,7 +780,9 @@ int gpe_probes(int32_t rand_dist, uint64_t root_seed, const int64_t* blk_starts_
             dfree(z);
             for (size_t i = 0; i < zh.size(); ++i) z_host[i] = (double)zh[i];
         } else if (is_real_probe_dist(rand_dist)) {
-            for (int b = 0; b < n_blocks; ++b) {
+            // every block's stream is independent of the others and writes its own
+            // rows, so blocks are spread over host threads
+            auto fill = [&](int b) {
                 const int64_t s = blk_starts_host[b];
                 const int64_t e = b + 1 < n_blocks ? blk_starts_host[b + 1] : blk_end;
                 if (rand_dist >= GPE_RAND_GAUSSIAN_F32)
@@ -789,6 +791,18 @@ int gpe_probes(int32_t rand_dist, uint64_t root_seed, const int64_t* blk_starts_
                 else
                     real_probe_fill<double>(root_seed, s, v_start, e - s, v_count, rand_dist == GPE_RAND_SPHERICAL,
                                             z_host + (s - s0) * v_count, v_count);
+            };
+            const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+            const int nt = std::min({hw, 16, n_blocks});
+            if (nt <= 1 || (int64_t)L * v_count < (1 << 16)) {
+                for (int b = 0; b < n_blocks; ++b) fill(b);
+            } else {
+                std::vector<std::thread> pool;
+                for (int t = 0; t < nt; ++t)
+                    pool.emplace_back([&, t] {
+                        for (int b = t; b < n_blocks; b += nt) fill(b);
+                    });
+                for (auto& th : pool) th.join();
             }
         } else {
             throw Error("rand_dist must be rademacher, gaussian or spherical");
