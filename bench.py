@@ -421,6 +421,7 @@ def main() -> int:
     ap.add_argument("--no-exact", action="store_true")
     ap.add_argument("--no-cv", action="store_true", help="skip the learned control-variate measurement")
     ap.add_argument("--no-tune", action="store_true", help="keep the built-in schedule heuristics")
+    ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph replay timing of the same steps")
     ap.add_argument("--only", choices=["learned_cv", "partitioned", "gxe", "windowed", "wall_to_1pct"],
                     help="run only one extra measurement and print it")
     ap.add_argument("--no-extras", action="store_true",
@@ -515,6 +516,26 @@ def main() -> int:
     value = pairs / (ms_per_step / 1000.0) / 1e9
     log(f"{args.steps} steps: {ms_per_step:.2f} ms/step, p1 contract {p1_ms / max(p1_n, 1):.2f} ms, "
         f"p2 contract {p2_ms / max(p2_n, 1):.2f} ms")
+
+    # the same steps replayed from one HIP graph (each replay = the next probe tile)
+    graph = None
+    if not args.no_graph:
+        gms = torch.zeros((M, 1), dtype=torch.float64, device=dev)
+        gt = ldscore.GraphedTiles(geno, B, seed, annot, inv, gms, limbs=args.limbs,
+                                  v_start=(args.warmup + args.steps) * B)
+        gt.eager()
+        gt.capture()
+        gt.replay()
+        torch.cuda.synchronize()
+        tg0 = time.perf_counter()
+        for _ in range(args.steps):
+            gt.replay()
+        torch.cuda.synchronize()
+        g_ms = (time.perf_counter() - tg0) * 1000.0 / args.steps
+        graph = {"ms_per_step": round(g_ms, 3), "gop_s": round(2.0 * N * M * B / (g_ms / 1000.0) / 1e9, 1),
+                 "steps": args.steps}
+        log(f"graph replays: {graph}")
+        del gt, gms
 
     # dominant kernel against its roofline: algorithmic int ops (2 per genotype-probe MAC) per launch
     if p2_ms >= p1_ms:
@@ -628,6 +649,7 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "autotune": tuned,
+        "graph_replay": graph,
         **extra,
     }
     print(json.dumps(line), flush=True)

@@ -167,6 +167,21 @@ int gpe_phase1_xz_tiled(gpe_geno* g, int64_t snp_begin, int64_t snp_end, int64_t
                         int32_t rand_dist, uint64_t root_seed, double* xz_dev, int64_t ldxz,
                         const uint8_t* bin_mode_dev, int32_t limbs, void* stream);
 
+/* gpe_phase1_xz_tiled with Rademacher probes whose first probe id (the
+ * v_start of the reference's stream seed) is read on the device from
+ * *v_start_dev when the kernels run.  With gpe_add_i32 advancing it, a probe
+ * tile (phase 1, phase 2, advance) can be captured once in a HIP graph and
+ * replayed tile after tile.  Under capture the workspaces and the device copy
+ * of the SNP-block starts must already exist: run the same call once
+ * outside capture first (otherwise the call fails with that message). */
+int gpe_phase1_xz_indirect(gpe_geno* g, int64_t snp_begin, int64_t snp_end, int64_t step_size,
+                           const double* annot_dev, int32_t n_bins, const double* inv_dev,
+                           const int32_t* v_start_dev, int32_t v_count, int32_t probe_tile, uint64_t root_seed,
+                           double* xz_dev, int64_t ldxz, const uint8_t* bin_mode_dev, int32_t limbs, void* stream);
+
+/* *value_dev += increment on the device (stream-ordered; graph-capturable). */
+int gpe_add_i32(int32_t* value_dev, int32_t increment, void* stream);
+
 /* Phase 2, X^T(X.Z): for SNPs j in [snp_begin, snp_end) and bins k,
  *   meansq[j][k] += sum_v (x_j . Xz[:, k*tile_nvecs + v])^2 * inv_j^2/(N_denom-1)^2
  * with the non-finite / extreme-update guards of src/native/gwldcore.cpp:295-417

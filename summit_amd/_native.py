@@ -39,6 +39,9 @@ SIGNATURES = {
                                      c_vp, c_i64, c_vp, c_i32, c_vp]),
     "gpe_phase1_xz_tiled": (ctypes.c_int, [c_vp, c_i64, c_i64, c_i64, c_vp, c_i32, c_vp, c_i32, c_i32, c_i32, c_i32,
                                            c_u64, c_vp, c_i64, c_vp, c_i32, c_vp]),
+    "gpe_phase1_xz_indirect": (ctypes.c_int, [c_vp, c_i64, c_i64, c_i64, c_vp, c_i32, c_vp, c_vp, c_i32, c_i32,
+                                              c_u64, c_vp, c_i64, c_vp, c_i32, c_vp]),
+    "gpe_add_i32": (ctypes.c_int, [c_vp, c_i32, c_vp]),
     "gpe_phase2_accum": (ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_i32, c_vp, c_i64, c_i32, c_i32, c_vp, c_i64,
                                         c_vp, c_vp, c_dbl, c_i32, c_i32, c_vp, c_vp]),
     "gpe_probes": (ctypes.c_int, [c_i32, c_u64, c_vp, c_i32, c_i64, c_i32, c_i32, c_vp]),

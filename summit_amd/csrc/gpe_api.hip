@@ -53,10 +53,18 @@ struct gpe_geno {
     int64_t* miss_ptr = nullptr;
     int32_t* miss_idx = nullptr;
     std::map<std::string, Buf> ws;
+    // set while an entry point enqueues onto a stream under HIP graph capture:
+    // workspaces may then not grow (no allocation may enter a graph)
+    bool capturing = false;
+    // device copy of the phase-1 SNP-block starts for (begin, end, step)
+    int64_t starts_key[3] = {-1, -1, -1};
 
     void* scratch(const std::string& name, size_t bytes) {
         Buf& b = ws[name];
         if (b.bytes < bytes) {
+            if (capturing)
+                throw Error("workspace '" + name +
+                            "' is not sized for this call: run the same call once outside graph capture first");
             if (b.p) GPE_HIP(hipFree(b.p));
             b.p = nullptr;
             b.bytes = 0;
@@ -87,6 +95,22 @@ int guarded(F&& f) {
 }
 
 inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Marks the handle as capturing while an entry point enqueues onto a stream
+// under HIP graph capture.
+struct CaptureScope {
+    gpe_geno* g;
+    bool active = false;
+    CaptureScope(gpe_geno* g_, hipStream_t st) : g(g_) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        GPE_HIP(hipStreamIsCapturing(st, &cs));
+        active = cs == hipStreamCaptureStatusActive;
+        if (g) g->capturing = active;
+    }
+    ~CaptureScope() {
+        if (g) g->capturing = false;
+    }
+};
 
 }  // namespace
 
@@ -1059,6 +1083,72 @@ void windowed_ld_impl(gpe_geno* g, int n_chr, const int64_t* bounds, const int64
     GPE_HIP(hipStreamSynchronize(st));
 }
 
+// Phase 1 entry: weights, probe streams (device MT19937-64 for Rademacher, host
+// std::normal_distribution for real probes), then the traversal.  With
+// v_start_dev the tile's first probe id is read on the device (Rademacher
+// only), so a captured graph can advance it between replays.
+void phase1_entry(gpe_geno* g, int64_t snp_begin, int64_t snp_end, int64_t step_size, const double* annot_dev,
+                  int32_t n_bins, const double* inv_dev, int32_t v_start, const int32_t* v_start_dev,
+                  int32_t v_count, int32_t probe_tile, int32_t rand_dist, uint64_t root_seed, double* xz_dev,
+                  int64_t ldxz, const uint8_t* bin_mode_dev, int32_t limbs, hipStream_t st) {
+    require(probe_tile > 0, "probe_tile must be positive");
+    require_prepared(g);
+    require(snp_begin >= 0 && snp_end <= g->m && snp_begin <= snp_end, "blk_end exceeds #SNPs in BIM");
+    require(step_size > 0, "step_size must be positive");
+    require(n_bins > 0 && v_count > 0, "annotation count and v_count must be positive");
+    require(ldxz >= g->n, "Xz2d_chunk shape must be (N, B*v_count)");
+    if (snp_end == snp_begin) return;
+    (void)limb_cfg(limbs);
+    CaptureScope cap(g, st);
+    require(!cap.active || !prof().on, "kernel timing must be off during graph capture");
+    require(!cap.active || rand_dist == GPE_RAND_RADEMACHER,
+            "only Rademacher probes (generated on the device) can be captured in a graph");
+    const int sub = std::min(probe_tile, v_count);
+    const int64_t L = snp_end - snp_begin;
+    const int v_pad = (int)round_up(v_count, 32);
+    double* w = static_cast<double*>(g->scratch("p1_w", g->m_alloc * n_bins * sizeof(double)));
+    hipLaunchKernelGGL(k_make_w, dim3((unsigned)ceil_div(L, 256)), dim3(256), 0, st, annot_dev, n_bins, inv_dev,
+                       g->inv_sd, g->nobs, g->sum, g->sumsq, snp_begin, snp_end, w);
+    GPE_HIP(hipGetLastError());
+    std::vector<int64_t> starts;
+    for (int64_t s = snp_begin; s < snp_end; s += step_size) starts.push_back(s);
+    const int nb = (int)starts.size();
+    int8_t* z_i8 = nullptr;
+    double* z_f64 = nullptr;
+    const int64_t ldz = v_pad;
+    if (rand_dist == GPE_RAND_RADEMACHER) {
+        z_i8 = static_cast<int8_t*>(g->scratch("p1_z8", g->m_alloc * ldz));
+        int64_t* starts_dev = static_cast<int64_t*>(g->scratch("p1_starts", nb * sizeof(int64_t)));
+        // the block starts stay on the device between calls with the same (begin, end, step)
+        if (!(g->starts_key[0] == snp_begin && g->starts_key[1] == snp_end && g->starts_key[2] == step_size)) {
+            require(!cap.active, "SNP-block starts not on the device: run the same call once outside graph capture first");
+            GPE_HIP(hipMemcpyAsync(starts_dev, starts.data(), nb * sizeof(int64_t), hipMemcpyHostToDevice, st));
+            GPE_HIP(hipStreamSynchronize(st));
+            g->starts_key[0] = snp_begin;
+            g->starts_key[1] = snp_end;
+            g->starts_key[2] = step_size;
+        }
+        launch_mt_rademacher(root_seed, starts_dev, nb, snp_end, v_start, v_count, z_i8, ldz, st, sub, v_start_dev);
+    } else if (is_real_probe_dist(rand_dist)) {
+        z_f64 = static_cast<double*>(g->scratch("p1_z64", g->m_alloc * ldz * sizeof(double)));
+        std::vector<double> zp((size_t)(L * ldz), 0.0);
+        for (int v0 = 0; v0 < v_count; v0 += sub) {
+            const int vc = std::min(sub, v_count - v0);
+            std::vector<double> zh((size_t)(L * vc));
+            require(gpe_probes(rand_dist, root_seed, starts.data(), nb, snp_end, v_start + v0, vc, zh.data()) == 0,
+                    g_err);
+            for (int64_t r = 0; r < L; ++r)
+                for (int c = 0; c < vc; ++c) zp[(size_t)(r * ldz + v0 + c)] = zh[(size_t)(r * vc + c)];
+        }
+        GPE_HIP(hipMemcpyAsync(z_f64 + snp_begin * ldz, zp.data(), zp.size() * sizeof(double),
+                               hipMemcpyHostToDevice, st));
+        GPE_HIP(hipStreamSynchronize(st));
+    } else {
+        throw Error("rand_dist must be 'rademacher', 'gaussian' or 'spherical'");
+    }
+    run_phase1(g, snp_begin, snp_end, w, n_bins, z_i8, z_f64, ldz, v_count, xz_dev, ldxz, bin_mode_dev, limbs, st);
+}
+
 }  // namespace
 
 extern "C" {
@@ -1076,52 +1166,26 @@ int gpe_phase1_xz_tiled(gpe_geno* g, int64_t snp_begin, int64_t snp_end, int64_t
                         int32_t rand_dist, uint64_t root_seed, double* xz_dev, int64_t ldxz,
                         const uint8_t* bin_mode_dev, int32_t limbs, void* stream) {
     return guarded([&] {
-        require(probe_tile > 0, "probe_tile must be positive");
-        const int sub = std::min(probe_tile, v_count);
-        require_prepared(g);
-        require(snp_begin >= 0 && snp_end <= g->m && snp_begin <= snp_end, "blk_end exceeds #SNPs in BIM");
-        require(step_size > 0, "step_size must be positive");
-        require(n_bins > 0 && v_count > 0, "annotation count and v_count must be positive");
-        require(ldxz >= g->n, "Xz2d_chunk shape must be (N, B*v_count)");
-        if (snp_end == snp_begin) return;
-        (void)limb_cfg(limbs);
-        hipStream_t st = S(stream);
-        const int64_t L = snp_end - snp_begin;
-        const int v_pad = (int)round_up(v_count, 32);
-        double* w = static_cast<double*>(g->scratch("p1_w", g->m_alloc * n_bins * sizeof(double)));
-        hipLaunchKernelGGL(k_make_w, dim3((unsigned)ceil_div(L, 256)), dim3(256), 0, st, annot_dev, n_bins, inv_dev,
-                           g->inv_sd, g->nobs, g->sum, g->sumsq, snp_begin, snp_end, w);
-        GPE_HIP(hipGetLastError());
-        std::vector<int64_t> starts;
-        for (int64_t s = snp_begin; s < snp_end; s += step_size) starts.push_back(s);
-        const int nb = (int)starts.size();
-        int8_t* z_i8 = nullptr;
-        double* z_f64 = nullptr;
-        const int64_t ldz = v_pad;
-        if (rand_dist == GPE_RAND_RADEMACHER) {
-            z_i8 = static_cast<int8_t*>(g->scratch("p1_z8", g->m_alloc * ldz));
-            int64_t* starts_dev = static_cast<int64_t*>(g->scratch("p1_starts", nb * sizeof(int64_t)));
-            GPE_HIP(hipMemcpyAsync(starts_dev, starts.data(), nb * sizeof(int64_t), hipMemcpyHostToDevice, st));
-            launch_mt_rademacher(root_seed, starts_dev, nb, snp_end, v_start, v_count, z_i8, ldz, st, sub);
-        } else if (is_real_probe_dist(rand_dist)) {
-            z_f64 = static_cast<double*>(g->scratch("p1_z64", g->m_alloc * ldz * sizeof(double)));
-            std::vector<double> zp((size_t)(L * ldz), 0.0);
-            for (int v0 = 0; v0 < v_count; v0 += sub) {
-                const int vc = std::min(sub, v_count - v0);
-                std::vector<double> zh((size_t)(L * vc));
-                require(gpe_probes(rand_dist, root_seed, starts.data(), nb, snp_end, v_start + v0, vc, zh.data()) == 0,
-                        g_err);
-                for (int64_t r = 0; r < L; ++r)
-                    for (int c = 0; c < vc; ++c) zp[(size_t)(r * ldz + v0 + c)] = zh[(size_t)(r * vc + c)];
-            }
-            GPE_HIP(hipMemcpyAsync(z_f64 + snp_begin * ldz, zp.data(), zp.size() * sizeof(double),
-                                   hipMemcpyHostToDevice, st));
-            GPE_HIP(hipStreamSynchronize(st));
-        } else {
-            throw Error("rand_dist must be 'rademacher', 'gaussian' or 'spherical'");
-        }
-        run_phase1(g, snp_begin, snp_end, w, n_bins, z_i8, z_f64, ldz, v_count, xz_dev, ldxz, bin_mode_dev, limbs,
-                   st);
+        phase1_entry(g, snp_begin, snp_end, step_size, annot_dev, n_bins, inv_dev, v_start, nullptr, v_count,
+                     probe_tile, rand_dist, root_seed, xz_dev, ldxz, bin_mode_dev, limbs, S(stream));
+    });
+}
+
+int gpe_phase1_xz_indirect(gpe_geno* g, int64_t snp_begin, int64_t snp_end, int64_t step_size,
+                           const double* annot_dev, int32_t n_bins, const double* inv_dev,
+                           const int32_t* v_start_dev, int32_t v_count, int32_t probe_tile, uint64_t root_seed,
+                           double* xz_dev, int64_t ldxz, const uint8_t* bin_mode_dev, int32_t limbs, void* stream) {
+    return guarded([&] {
+        require(v_start_dev != nullptr, "v_start_dev is null");
+        phase1_entry(g, snp_begin, snp_end, step_size, annot_dev, n_bins, inv_dev, 0, v_start_dev, v_count,
+                     probe_tile, GPE_RAND_RADEMACHER, root_seed, xz_dev, ldxz, bin_mode_dev, limbs, S(stream));
+    });
+}
+
+int gpe_add_i32(int32_t* value_dev, int32_t increment, void* stream) {
+    return guarded([&] {
+        require(value_dev != nullptr, "value_dev is null");
+        launch_add_i32(value_dev, increment, S(stream));
     });
 }
 
@@ -1361,6 +1425,9 @@ int gpe_phase2_accum(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const doub
         require(ldxz >= g->n, "Xz2d shape mismatch in phase2_accum_XtXz_bed");
         require(ld_meansq >= n_bins, "meansq_accum shape mismatch in phase2_accum_XtXz_bed");
         hipStream_t st = S(stream);
+        CaptureScope cap(g, st);
+        require(!cap.active || (guard_counts_host == nullptr && !prof().on),
+                "under graph capture the guard counts cannot be returned and kernel timing must be off");
         unsigned long long* gc = static_cast<unsigned long long*>(g->scratch("guards", 2 * sizeof(unsigned long long)));
         launch_fill_u64(reinterpret_cast<uint64_t*>(gc), 2, 0, st);
         if (snp_end > snp_begin) {

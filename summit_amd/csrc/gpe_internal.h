@@ -128,8 +128,11 @@ void launch_miss_fill(const uint8_t* smp, int64_t stride, int64_t n, const int64
 void launch_fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t st);
 void launch_fill_u64(uint64_t* p, int64_t n, uint64_t v, hipStream_t st);
 // sub > 0: probe columns in tiles of sub, each tile the reference's stream for V-tile v_start + t*sub
+// v_start_dev (may be null): read the first probe id on the device instead of v_start
 void launch_mt_rademacher(uint64_t root, const int64_t* blk_starts, int n_blocks, int64_t blk_end_all,
-                          int v_start, int v_count, int8_t* z, int64_t ldz, hipStream_t st, int sub = 0);
+                          int v_start, int v_count, int8_t* z, int64_t ldz, hipStream_t st, int sub = 0,
+                          const int32_t* v_start_dev = nullptr);
+void launch_add_i32(int32_t* v, int32_t inc, hipStream_t st);
 void launch_coef_p1_colmax(const double* w, int64_t ldw, const int8_t* z_i8, const double* z_f64, int64_t ldz,
                            int64_t k0, int64_t k1, int n_bins, int v_count, int v_pad, uint64_t* colmax_bits,
                            hipStream_t st);

@@ -906,12 +906,13 @@ __device__ __forceinline__ uint64_t mt_twist_one(uint64_t a, uint64_t b, uint64_
 // One wave per (SNP block, probe tile): the stream of the reference's V-tile
 // [v_start + t*sub, + sub) of that block, written to probe columns t*sub.. .
 __global__ __launch_bounds__(64) void k_mt_rademacher(uint64_t root, const int64_t* blk_starts, int n_blocks,
-                                                     int64_t blk_end_all, int v_start, int v_count_all, int sub,
-                                                     int8_t* z_all, int64_t ldz) {
+                                                     int64_t blk_end_all, int v_start_arg, const int32_t* v_start_dev,
+                                                     int v_count_all, int sub, int8_t* z_all, int64_t ldz) {
     __shared__ uint64_t mt[kMtN];
     const int b = blockIdx.x;
     const int lane = threadIdx.x;
     if (b >= n_blocks) return;
+    const int v_start = v_start_dev ? v_start_dev[0] : v_start_arg;
     const int v0 = (int)blockIdx.y * sub;
     if (v0 >= v_count_all) return;
     const int v_count = v_count_all - v0 < sub ? v_count_all - v0 : sub;
@@ -975,12 +976,22 @@ __global__ __launch_bounds__(64) void k_mt_rademacher(uint64_t root, const int64
     }
 }
 void launch_mt_rademacher(uint64_t root, const int64_t* blk_starts, int n_blocks, int64_t blk_end_all,
-                          int v_start, int v_count, int8_t* z, int64_t ldz, hipStream_t st, int sub) {
+                          int v_start, int v_count, int8_t* z, int64_t ldz, hipStream_t st, int sub,
+                          const int32_t* v_start_dev) {
     if (n_blocks <= 0 || v_count <= 0) return;
     if (sub <= 0 || sub > v_count) sub = v_count;
     const int n_sub = (int)ceil_div(v_count, sub);
     hipLaunchKernelGGL(k_mt_rademacher, dim3((unsigned)n_blocks, (unsigned)n_sub), dim3(64), 0, st, root, blk_starts,
-                       n_blocks, blk_end_all, v_start, v_count, sub, z, ldz);
+                       n_blocks, blk_end_all, v_start, v_start_dev, v_count, sub, z, ldz);
+    GPE_HIP(hipGetLastError());
+}
+
+__global__ void k_add_i32(int32_t* v, int32_t inc) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) v[0] += inc;
+}
+
+void launch_add_i32(int32_t* v, int32_t inc, hipStream_t st) {
+    hipLaunchKernelGGL(k_add_i32, dim3(1), dim3(64), 0, st, v, inc);
     GPE_HIP(hipGetLastError());
 }
 

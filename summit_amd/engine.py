@@ -194,7 +194,8 @@ def resvar_inv(g: Genotype, snp_begin: int = 0, snp_end: int | None = None, eps:
 def phase1_xz(g: Genotype, snp_begin: int, snp_end: int, step_size: int, annot: torch.Tensor,
               inv: torch.Tensor, v_start: int, v_count: int, seed: int, xz: torch.Tensor,
               rand_dist: str = "rademacher", bin_mode: torch.Tensor | None = None, limbs: int = 5,
-              stream=None, probe_fp32: bool = False, probe_tile: int | None = None) -> torch.Tensor:
+              stream=None, probe_fp32: bool = False, probe_tile: int | None = None,
+              v_start_dev: torch.Tensor | None = None) -> torch.Tensor:
     """Xz[k*v_count+v, :] (+)= sum_i x_i inv_i sqrt(a_ik) z_iv over [snp_begin, snp_end).
     probe_tile: probe columns come in V-tiles of this many, each with its own
     per-block streams (as separate calls of that width would draw)."""
@@ -209,10 +210,25 @@ def phase1_xz(g: Genotype, snp_begin: int, snp_end: int, step_size: int, annot: 
     if xz.dtype != torch.float64 or xz.dim() != 2 or xz.shape[0] != B * v_count or xz.shape[1] != g.n:
         raise RuntimeError("Xz2d_chunk shape must be (N, B*v_count)")
     tile = v_count if probe_tile is None else int(probe_tile)
+    if v_start_dev is not None:
+        # first probe id read on the device (graph-capturable; Rademacher only)
+        if rand_dist != "rademacher" or v_start_dev.dtype != torch.int32 or v_start_dev.numel() != 1:
+            raise RuntimeError("v_start_dev needs Rademacher probes and a one-element int32 device tensor")
+        call("gpe_phase1_xz_indirect", g.handle, snp_begin, snp_end, step_size, _ptr(annot), B, _ptr(inv),
+             _ptr(v_start_dev), v_count, tile, int(seed) & 0xFFFFFFFFFFFFFFFF, _ptr(xz), xz.stride(0),
+             _ptr(bin_mode), limbs, _stream(stream))
+        return xz
     call("gpe_phase1_xz_tiled", g.handle, snp_begin, snp_end, step_size, _ptr(annot), B, _ptr(inv), v_start, v_count,
          tile, rand_code(rand_dist, probe_fp32), int(seed) & 0xFFFFFFFFFFFFFFFF, _ptr(xz), xz.stride(0),
          _ptr(bin_mode), limbs, _stream(stream))
     return xz
+
+
+def add_i32(value: torch.Tensor, increment: int, stream=None) -> None:
+    """value[0] += increment on the device, in stream order (graph-capturable)."""
+    if value.dtype != torch.int32 or value.numel() < 1:
+        raise RuntimeError("value must be an int32 device tensor")
+    call("gpe_add_i32", _ptr(value), int(increment), _stream(stream))
 
 
 def phase2_accum(g: Genotype, snp_begin: int, snp_end: int, inv_left: torch.Tensor, tile_nvecs: int,

@@ -107,6 +107,56 @@ def genomewide_ldscore(geno: engine.Genotype, nvecs: int, seed: int, annot: np.n
     return LDScoreResult(ld, meansq_h, inv.cpu().numpy(), int(nvecs), nsnps_bin, mc_ps_h, tuple(guards), mc_var)
 
 
+class GraphedTiles:
+    """Probe tiles of the genome-wide estimator replayed from one HIP graph.
+
+    One tile = clear X.Z, phase 1 over every SNP block (Rademacher streams of
+    V-tile ``v_start`` read on the device), phase 2 accumulating into
+    ``meansq``, and ``v_start += tile``: the launches of a tile are captured
+    once (after one eager tile sizes every workspace) and each ``replay()``
+    runs the next tile.  Results equal eager tiles bit for bit."""
+
+    def __init__(self, geno: engine.Genotype, tile: int, seed: int, annot: torch.Tensor, inv: torch.Tensor,
+                 meansq: torch.Tensor, step_size: int = 1000, limbs: int = 5, probe_tile: int | None = None,
+                 v_start: int = 0):
+        dev = engine.default_device()
+        self.g, self.tile, self.seed = geno, int(tile), int(seed)
+        self.annot, self.inv, self.meansq = annot, inv, meansq
+        self.step_size, self.limbs = int(step_size), int(limbs)
+        self.probe_tile = probe_tile
+        K = annot.shape[1]
+        self.xz = torch.zeros((K * self.tile, geno.n), dtype=torch.float64, device=dev)
+        self.v_start = torch.tensor([int(v_start)], dtype=torch.int32, device=dev)
+        self.n_denom = geno.n - 1 + 1  # ddof 1
+        self.graph = None
+
+    def _tile(self, stream=None) -> None:
+        self.xz.zero_()
+        engine.phase1_xz(self.g, 0, self.g.m, self.step_size, self.annot, self.inv, 0, self.tile, self.seed,
+                         self.xz, limbs=self.limbs, probe_tile=self.probe_tile, v_start_dev=self.v_start,
+                         stream=stream)
+        engine.phase2_accum(self.g, 0, self.g.m, self.inv, self.tile, self.xz, self.meansq, self.n_denom,
+                            limbs=self.limbs, stream=stream)
+        engine.add_i32(self.v_start, self.tile, stream=stream)
+
+    def eager(self) -> None:
+        """One tile without the graph (also sizes the workspaces)."""
+        self._tile()
+
+    def capture(self) -> None:
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph, stream=side):
+            self._tile(stream=side)
+        torch.cuda.current_stream().wait_stream(side)
+
+    def replay(self) -> None:
+        if self.graph is None:
+            raise RuntimeError("capture() first (after one eager tile)")
+        self.graph.replay()
+
+
 def write_gw_ldscore_table(outpath, snpdf, scores_df, *, snp_var=None, ld_rel=None) -> str:
     """``<outpath>.gw.ldscore.gz``: the SNP columns (CHR SNP BP), V when given, one
     column per annotation and LD_REL_<column> when given; tab-separated, six
