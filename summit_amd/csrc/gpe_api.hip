@@ -58,6 +58,20 @@ struct gpe_geno {
     bool capturing = false;
     // device copy of the phase-1 SNP-block starts for (begin, end, step)
     int64_t starts_key[3] = {-1, -1, -1};
+    // pinned host staging of host-generated (real-valued) probes, reused
+    double* host_stage = nullptr;
+    size_t host_stage_bytes = 0;
+
+    double* stage(size_t bytes) {
+        if (host_stage_bytes < bytes) {
+            if (host_stage) GPE_HIP(hipHostFree(host_stage));
+            host_stage = nullptr;
+            host_stage_bytes = 0;
+            GPE_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_stage), bytes, hipHostMallocDefault));
+            host_stage_bytes = bytes;
+        }
+        return host_stage;
+    }
 
     void* scratch(const std::string& name, size_t bytes) {
         Buf& b = ws[name];
@@ -254,6 +268,12 @@ struct ProfScope {
     }
 };
 
+// Real-valued probes of blocks [starts[b], next start or blk_end) for V-tile
+// v_start, written to dst[(row - starts[0]) * ldz + c] for c < v_count; every
+// block's stream is independent, so blocks are spread over host threads.
+void real_probes_into(int rand_dist, uint64_t root, const int64_t* starts, int n_blocks, int64_t blk_end, int v_start,
+                      int v_count, double* dst, int64_t ldz);
+
 bool is_real_probe_dist(int d) {
     return d == GPE_RAND_GAUSSIAN || d == GPE_RAND_SPHERICAL || d == GPE_RAND_GAUSSIAN_F32 ||
            d == GPE_RAND_SPHERICAL_F32;
@@ -357,6 +377,33 @@ int pick_splits(int64_t grid_xy, int64_t kcount, int64_t max_chunks_per_split, i
     }
     while (ceil_div(kcount, best) > max_chunks_per_split) ++best;
     return (int)best;
+}
+
+void real_probes_into(int rand_dist, uint64_t root, const int64_t* starts, int n_blocks, int64_t blk_end, int v_start,
+                      int v_count, double* dst, int64_t ldz) {
+    const int64_t s0 = starts[0];
+    auto fill = [&](int b) {
+        const int64_t s = starts[b];
+        const int64_t e = b + 1 < n_blocks ? starts[b + 1] : blk_end;
+        if (rand_dist >= GPE_RAND_GAUSSIAN_F32)
+            real_probe_fill<float>(root, s, v_start, e - s, v_count, rand_dist == GPE_RAND_SPHERICAL_F32,
+                                   dst + (s - s0) * ldz, ldz);
+        else
+            real_probe_fill<double>(root, s, v_start, e - s, v_count, rand_dist == GPE_RAND_SPHERICAL,
+                                    dst + (s - s0) * ldz, ldz);
+    };
+    const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+    const int nt = std::min({hw, 16, n_blocks});
+    if (nt <= 1 || (blk_end - s0) * (int64_t)v_count < (1 << 16)) {
+        for (int b = 0; b < n_blocks; ++b) fill(b);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; ++t)
+        pool.emplace_back([&, t] {
+            for (int b = t; b < n_blocks; b += nt) fill(b);
+        });
+    for (auto& th : pool) th.join();
 }
 
 }  // namespace
@@ -487,6 +534,7 @@ int gpe_geno_free(gpe_geno* g) {
         dfree(g->mean);
         dfree(g->inv_sd);
         for (auto& kv : g->ws) dfree(kv.second.p);
+        if (g->host_stage) (void)hipHostFree(g->host_stage);
         delete g;
     });
 }
@@ -804,30 +852,8 @@ int gpe_probes(int32_t rand_dist, uint64_t root_seed, const int64_t* blk_starts_
             dfree(z);
             for (size_t i = 0; i < zh.size(); ++i) z_host[i] = (double)zh[i];
         } else if (is_real_probe_dist(rand_dist)) {
-            // every block's stream is independent of the others and writes its own
-            // rows, so blocks are spread over host threads
-            auto fill = [&](int b) {
-                const int64_t s = blk_starts_host[b];
-                const int64_t e = b + 1 < n_blocks ? blk_starts_host[b + 1] : blk_end;
-                if (rand_dist >= GPE_RAND_GAUSSIAN_F32)
-                    real_probe_fill<float>(root_seed, s, v_start, e - s, v_count, rand_dist == GPE_RAND_SPHERICAL_F32,
-                                           z_host + (s - s0) * v_count, v_count);
-                else
-                    real_probe_fill<double>(root_seed, s, v_start, e - s, v_count, rand_dist == GPE_RAND_SPHERICAL,
-                                            z_host + (s - s0) * v_count, v_count);
-            };
-            const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
-            const int nt = std::min({hw, 16, n_blocks});
-            if (nt <= 1 || (int64_t)L * v_count < (1 << 16)) {
-                for (int b = 0; b < n_blocks; ++b) fill(b);
-            } else {
-                std::vector<std::thread> pool;
-                for (int t = 0; t < nt; ++t)
-                    pool.emplace_back([&, t] {
-                        for (int b = t; b < n_blocks; b += nt) fill(b);
-                    });
-                for (auto& th : pool) th.join();
-            }
+            real_probes_into(rand_dist, root_seed, blk_starts_host, n_blocks, blk_end, v_start, v_count, z_host,
+                             v_count);
         } else {
             throw Error("rand_dist must be rademacher, gaussian or spherical");
         }
@@ -914,6 +940,33 @@ void run_phase1(gpe_geno* g, int64_t s, int64_t e, const double* w, int n_bins, 
         launch_p1_finalize_z8(y, ldy, g->n, n_bins, v_count, v_pad, col_scale, cst, g->miss_ptr, g->miss_idx, s, e,
                               rec, out, ldo, bin_mode, (int64_t)kib << 10, st);
         pf.stop();
+        return;
+    }
+    if (miss && z_f64) {
+        // real-valued probes: the missing-call term sum_{i missing at n} mu_i c_iv as a second
+        // int8-MFMA traversal, the missing-flag plane against the quantized mu_i w_ik z_iv (the
+        // per-call records of int8 probes would carry 64 FP64 values per call here)
+        double* w2 = static_cast<double*>(g->scratch("p1_w2", g->m_alloc * n_bins * sizeof(double)));
+        launch_scale_rows(w, g->mean, s, e, n_bins, w2, st);
+        uint64_t* colmax2 = static_cast<uint64_t*>(g->scratch("colmax2", ncols_pad * sizeof(uint64_t)));
+        int32_t* shift2 = static_cast<int32_t*>(g->scratch("shift2", ncols_pad * sizeof(int32_t)));
+        double* cs2 = static_cast<double*>(g->scratch("col_scale2", ncols_pad * sizeof(double)));
+        double* cst2 = static_cast<double*>(g->scratch("cst2", ncols_pad * sizeof(double)));
+        double* cst_part2 =
+            static_cast<double*>(g->scratch("cst_part2", (size_t)ncols_pad * kCstParts * sizeof(double)));
+        launch_fill_u64(colmax2, ncols_pad, 0, st);
+        launch_coef_p1_colmax(w2, n_bins, nullptr, z_f64, ldz, s, e, n_bins, v_count, v_pad, colmax2, st);
+        launch_shift_from_colmax(colmax2, ncols_pad, prec, shift2, cs2, st);
+        // the main contraction has consumed the coefficient buffer (stream order); reuse it
+        if (ncols_pad != ncols) launch_fill_u8(reinterpret_cast<uint8_t*>(coef), kcount * coef_ctiles * 4096, 0, st);
+        launch_coef_p1_quant(w2, n_bins, nullptr, z_f64, ldz, s, e, n_bins, v_count, v_pad, shift2, lc.nl, lc.ctc,
+                             coef_ctiles, kc0, coef, g->mean, cst2, cst_part2, st);
+        int64_t* y2 = static_cast<int64_t*>(g->scratch("p1_y2", (int64_t)ncols_pad * ldy * sizeof(int64_t)));
+        launch_fill_u64(reinterpret_cast<uint64_t*>(y2), (int64_t)ncols_pad * ldy, 0, st);
+        ContractArgs a2 = a;
+        a2.out_i64 = y2;
+        launch_contract(lc.nl, lc.ctc, false, EPI_I64_ATOMIC_Q, a2, groups, splits, st, cpb);
+        launch_p1_combine2(y, y2, ldy, g->n, n_bins, v_count, v_pad, col_scale, cst, cs2, out, ldo, bin_mode, st);
         return;
     }
     launch_p1_finalize(y, ldy, g->n, n_bins, v_count, v_pad, col_scale, cst, miss ? g->miss_ptr : nullptr,
@@ -1131,18 +1184,18 @@ void phase1_entry(gpe_geno* g, int64_t snp_begin, int64_t snp_end, int64_t step_
         launch_mt_rademacher(root_seed, starts_dev, nb, snp_end, v_start, v_count, z_i8, ldz, st, sub, v_start_dev);
     } else if (is_real_probe_dist(rand_dist)) {
         z_f64 = static_cast<double*>(g->scratch("p1_z64", g->m_alloc * ldz * sizeof(double)));
-        std::vector<double> zp((size_t)(L * ldz), 0.0);
+        // the previous call's copy out of the staging buffer must be done before it is rewritten
+        GPE_HIP(hipStreamSynchronize(st));
+        double* zp = g->stage((size_t)(L * ldz) * sizeof(double));
         for (int v0 = 0; v0 < v_count; v0 += sub) {
             const int vc = std::min(sub, v_count - v0);
-            std::vector<double> zh((size_t)(L * vc));
-            require(gpe_probes(rand_dist, root_seed, starts.data(), nb, snp_end, v_start + v0, vc, zh.data()) == 0,
-                    g_err);
-            for (int64_t r = 0; r < L; ++r)
-                for (int c = 0; c < vc; ++c) zp[(size_t)(r * ldz + v0 + c)] = zh[(size_t)(r * vc + c)];
+            real_probes_into(rand_dist, root_seed, starts.data(), nb, snp_end, v_start + v0, vc, zp + v0, ldz);
         }
-        GPE_HIP(hipMemcpyAsync(z_f64 + snp_begin * ldz, zp.data(), zp.size() * sizeof(double),
+        if (v_pad > v_count)
+            for (int64_t r = 0; r < L; ++r)
+                std::fill(zp + r * ldz + v_count, zp + (r + 1) * ldz, 0.0);
+        GPE_HIP(hipMemcpyAsync(z_f64 + snp_begin * ldz, zp, (size_t)(L * ldz) * sizeof(double),
                                hipMemcpyHostToDevice, st));
-        GPE_HIP(hipStreamSynchronize(st));
     } else {
         throw Error("rand_dist must be 'rademacher', 'gaussian' or 'spherical'");
     }

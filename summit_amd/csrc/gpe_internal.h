@@ -50,6 +50,7 @@ enum EpiMode : int {
     EPI_WORK = 4,        // phase-2 epilogue writing standardized Work to out_f64[r * ldo + c]
     EPI_I64_2P = 5,      // MISS: int64 row-major of the genotype plane to out_i64, missing plane to out2_i64
     EPI_I64_SQ = 6,      // squared-genotype plane (codes 00 -> 0, 01 -> 0, 10 -> 1, 11 -> 4), int64 row-major
+    EPI_I64_ATOMIC_Q = 7,// missing-flag plane (01 -> 1), otherwise as EPI_I64_ATOMIC
 };
 
 struct ContractArgs {
@@ -183,6 +184,13 @@ void launch_weighted_sumsq(const uint8_t* snp, int64_t stride, int64_t r0, int64
                            int64_t ldo, hipStream_t st);
 // out[e * ldo + (j - r0)] = inv_sd_j^2 scale_e (mu_j^2 (wsum_e - me) - 2 mu_j ge + g2) for the
 // three int64 plane sums (row-major [j - r0][e], leading dimension ld)
+// xz[(bin*v_count + v) * ldxz + n] (=|+=) y[c][n] cs[c] - cst[c] + y2[c][n] cs2[c] (c = bin*v_pad + v)
+void launch_p1_combine2(const int64_t* y, const int64_t* y2, int64_t ldy, int64_t n, int n_bins, int v_count,
+                        int v_pad, const double* cs, const double* cst, const double* cs2, double* xz, int64_t ldxz,
+                        const uint8_t* col_mode, hipStream_t st);
+// out[i * n_bins + k] = mean[i] * w[i * n_bins + k] for i in [s, e)
+void launch_scale_rows(const double* w, const double* mean, int64_t s, int64_t e, int n_bins, double* out,
+                       hipStream_t st);
 void launch_wss_finalize(const int64_t* ge, const int64_t* me, const int64_t* g2, int64_t ld, int64_t r0, int64_t r1,
                          int n_w, const double* mean, const double* inv_sd, const double* col_scale,
                          const int64_t* col_sum, double* out, int64_t ldo, hipStream_t st);

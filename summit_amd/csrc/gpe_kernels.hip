@@ -182,7 +182,7 @@ __global__ __launch_bounds__(256, (NL * CTC <= 2) ? 4 : 2) void k_contract(Contr
     constexpr int RPW = MISS ? 32 : 64;
     constexpr int RPB = 4 * RPW;
     constexpr int G = MISS ? 1 : 2;  // packed-row loads per lane per chunk
-    constexpr uint32_t TBL = EPI == EPI_I64_SQ ? kSqTable : kGenoTable;
+    constexpr uint32_t TBL = EPI == EPI_I64_SQ ? kSqTable : (EPI == EPI_I64_ATOMIC_Q ? kMissTable : kGenoTable);
     static_assert(CPB == 1 || CPB == 2, "one or two chunks per barrier");
     constexpr int SLOTS = CPB == 1 ? 3 : 2;
     __shared__ __attribute__((aligned(16))) int8_t lds[SLOTS][CPB * CT * 4096];
@@ -387,7 +387,8 @@ __global__ __launch_bounds__(256, (NL * CTC <= 2) ? 4 : 2) void k_contract(Contr
                 a.out2_i64[(r - a.out_row0) * a.ldo + c] = me;
             }
         }
-    } else if constexpr (EPI == EPI_I64_ATOMIC || EPI == EPI_I64_ROWMAJOR || EPI == EPI_I64_SQ) {
+    } else if constexpr (EPI == EPI_I64_ATOMIC || EPI == EPI_I64_ROWMAJOR || EPI == EPI_I64_SQ ||
+                         EPI == EPI_I64_ATOMIC_Q) {
 #pragma unroll
         for (int ctc = 0; ctc < CTC; ++ctc) {
             const int ct = cg * CTC + ctc;
@@ -404,7 +405,7 @@ __global__ __launch_bounds__(256, (NL * CTC <= 2) ? 4 : 2) void k_contract(Contr
                     for (int l = 0; l < NL; ++l) v = v * 256 + (int64_t)acc[l * CTC + ctc][bt][e];
                     const int c = ct * 32 + 8 * (e >> 2) + 4 * half + (e & 3);
                     if (c >= a.cols_out) continue;
-                    if (EPI == EPI_I64_ATOMIC)
+                    if (EPI == EPI_I64_ATOMIC || EPI == EPI_I64_ATOMIC_Q)
                         atomicAdd(reinterpret_cast<unsigned long long*>(
                                       &a.out_i64[(int64_t)c * a.ldo + (r - a.out_row0)]),
                                   (unsigned long long)v);
@@ -503,6 +504,10 @@ void launch_contract(int nl, int ctc, bool miss, EpiMode mode, const ContractArg
             launch_one<5, 1, false, EPI_P2, 2>(a, n_col_groups, n_splits, st);
             return;
         }
+        if (nl == 5 && ctc == 1 && !miss && mode == EPI_I64_ATOMIC_Q) {
+            launch_one<5, 1, false, EPI_I64_ATOMIC_Q, 2>(a, n_col_groups, n_splits, st);
+            return;
+        }
     }
 #define GPE_CASE(NL_, CTC_, MISS_, EPI_)                                                          \
     if (nl == NL_ && ctc == CTC_ && miss == MISS_ && mode == EPI_) {                              \
@@ -515,6 +520,9 @@ void launch_contract(int nl, int ctc, bool miss, EpiMode mode, const ContractArg
     // weighted phase 1 (sample-major rows, int64 partial sums)
     GPE_CASE(5, 1, false, EPI_I64_ATOMIC)
     GPE_CASE(6, 1, false, EPI_I64_ATOMIC)
+    // missing-flag plane of phase 1 with real-valued probes
+    GPE_CASE(5, 1, false, EPI_I64_ATOMIC_Q)
+    GPE_CASE(6, 1, false, EPI_I64_ATOMIC_Q)
     // plane sums of weighted sums of squares (genotype + missing, squared genotype)
     GPE_CASE(6, 1, true, EPI_I64_2P)
     GPE_CASE(6, 1, false, EPI_I64_ROWMAJOR)
@@ -1210,6 +1218,46 @@ __global__ void k_p1_finalize(const int64_t* y, int64_t ldy, int64_t n, int n_bi
         }
     }
 }
+__global__ void k_p1_combine2(const int64_t* __restrict__ y, const int64_t* __restrict__ y2, int64_t ldy, int64_t n,
+                              int n_bins, int v_count, int v_pad, const double* __restrict__ cs,
+                              const double* __restrict__ cst, const double* __restrict__ cs2, double* __restrict__ xz,
+                              int64_t ldxz, const uint8_t* __restrict__ col_mode) {
+    const int64_t smp = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    const int k = c / v_pad, v = c % v_pad;
+    if (smp >= n || v >= v_count) return;
+    const double val = (double)y[(int64_t)c * ldy + smp] * cs[c] - cst[c] + (double)y2[(int64_t)c * ldy + smp] * cs2[c];
+    const int mode = col_mode ? col_mode[k] : 2;
+    double* dst = xz + (int64_t)(k * v_count + v) * ldxz + smp;
+    if (mode == 1) *dst = val;
+    else *dst += val;
+}
+
+void launch_p1_combine2(const int64_t* y, const int64_t* y2, int64_t ldy, int64_t n, int n_bins, int v_count,
+                        int v_pad, const double* cs, const double* cst, const double* cs2, double* xz, int64_t ldxz,
+                        const uint8_t* col_mode, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_p1_combine2, dim3((unsigned)ceil_div(n, 256), (unsigned)(n_bins * v_pad)), dim3(256), 0, st,
+                       y, y2, ldy, n, n_bins, v_count, v_pad, cs, cst, cs2, xz, ldxz, col_mode);
+    GPE_HIP(hipGetLastError());
+}
+
+__global__ void k_scale_rows(const double* __restrict__ w, const double* __restrict__ mean, int64_t s, int64_t e,
+                             int n_bins, double* __restrict__ out) {
+    const int64_t idx = s * n_bins + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= e * n_bins) return;
+    out[idx] = mean[idx / n_bins] * w[idx];
+}
+
+void launch_scale_rows(const double* w, const double* mean, int64_t s, int64_t e, int n_bins, double* out,
+                       hipStream_t st) {
+    const int64_t total = (e - s) * (int64_t)n_bins;
+    if (total <= 0) return;
+    hipLaunchKernelGGL(k_scale_rows, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, st, w, mean, s, e, n_bins,
+                       out);
+    GPE_HIP(hipGetLastError());
+}
+
 void launch_p1_finalize(const int64_t* y, int64_t ldy, int64_t n, int n_bins, int v_count, int v_pad,
                         const double* col_scale, const double* cst, const int64_t* miss_ptr,
                         const int32_t* miss_idx, int64_t k0, int64_t k1, const double* w, int64_t ldw,
