@@ -281,10 +281,45 @@ bool is_real_probe_dist(int d) {
 
 // The reference's real-valued probe fill for one SNP block
 // (src/native/gwldcore.cpp:1679-1716): std::mt19937_64 seeded by
-// make_seed(root, block start, v_start), one std::normal_distribution<T> for the
+// make_seed(root, block start, v_start), one std::normal_distribution<T> (PolarNormal) for the
 // whole block, column by column; spherical probes rescale each column to norm
 // sqrt(L) with the sum of squares in long double.  T is the precision of the
 // caller's arrays (float32 callers draw float variates).  Writes z[r * ldz + c].
+// std::normal_distribution<T> over std::mt19937_64 as libstdc++ defines it
+// (Marsaglia's polar method; each uniform is generate_canonical<T, digits>, one
+// 64-bit draw u giving T(u) / 2^64, clamped below 1; the second variate of an
+// accepted pair is kept for the next call), written out so that the constant
+// log2 of the engine range, which libstdc++ evaluates in long double at every
+// uniform unless the compiler folds it, is not recomputed per draw.
+template <class T>
+struct PolarNormal {
+    bool saved_available = false;
+    T saved = T(0);
+    static T canonical(std::mt19937_64& rng) {
+        const T ret = T(rng() - std::mt19937_64::min()) / T(18446744073709551616.0L);
+        return ret >= T(1) ? std::nextafter(T(1), T(0)) : ret;
+    }
+    T operator()(std::mt19937_64& rng) {
+        T ret;
+        if (saved_available) {
+            saved_available = false;
+            ret = saved;
+        } else {
+            T x, y, r2;
+            do {
+                x = T(2.0) * canonical(rng) - 1.0;
+                y = T(2.0) * canonical(rng) - 1.0;
+                r2 = x * x + y * y;
+            } while (r2 > 1.0 || r2 == 0.0);
+            const T mult = std::sqrt(-2 * std::log(r2) / r2);
+            saved = x * mult;
+            saved_available = true;
+            ret = y * mult;
+        }
+        return ret * T(1) + T(0);
+    }
+};
+
 template <class T>
 void real_probe_fill(uint64_t root, int64_t blk_start, int v_start, int64_t L, int V, bool spherical, double* z,
                      int64_t ldz) {
@@ -299,7 +334,7 @@ void real_probe_fill(uint64_t root, int64_t blk_start, int v_start, int64_t L, i
     seed ^= mix((uint64_t)(int64_t)(int32_t)blk_start + 0x9e37ULL);
     seed ^= mix((uint64_t)(int64_t)v_start + 0x85ebULL);
     std::mt19937_64 rng(seed);
-    std::normal_distribution<T> gN(0, (T)1);
+    PolarNormal<T> gN;
     std::vector<T> col((size_t)L);
     for (int c = 0; c < V; ++c) {
         long double ss = 0.0L;
@@ -1187,15 +1222,29 @@ void phase1_entry(gpe_geno* g, int64_t snp_begin, int64_t snp_end, int64_t step_
         // the previous call's copy out of the staging buffer must be done before it is rewritten
         GPE_HIP(hipStreamSynchronize(st));
         double* zp = g->stage((size_t)(L * ldz) * sizeof(double));
-        for (int v0 = 0; v0 < v_count; v0 += sub) {
-            const int vc = std::min(sub, v_count - v0);
-            real_probes_into(rand_dist, root_seed, starts.data(), nb, snp_end, v_start + v0, vc, zp + v0, ldz);
+        // the host draws overlap the device: the SNP blocks are cut into parts, and each
+        // part's copy and traversal are queued while the host draws the next part (later
+        // parts add to what the first one wrote)
+        const int parts = nb >= 16 ? 4 : 1;
+        int b0 = 0;
+        for (int p = 0; p < parts; ++p) {
+            const int b1 = (int)((int64_t)nb * (p + 1) / parts);
+            const int64_t ps = starts[b0], pe = b1 < nb ? starts[b1] : snp_end;
+            double* zpp = zp + (ps - snp_begin) * ldz;
+            for (int v0 = 0; v0 < v_count; v0 += sub) {
+                const int vc = std::min(sub, v_count - v0);
+                real_probes_into(rand_dist, root_seed, starts.data() + b0, b1 - b0, pe, v_start + v0, vc, zpp + v0,
+                                 ldz);
+            }
+            if (v_pad > v_count)
+                for (int64_t r = 0; r < pe - ps; ++r) std::fill(zpp + r * ldz + v_count, zpp + (r + 1) * ldz, 0.0);
+            GPE_HIP(hipMemcpyAsync(z_f64 + ps * ldz, zpp, (size_t)((pe - ps) * ldz) * sizeof(double),
+                                   hipMemcpyHostToDevice, st));
+            run_phase1(g, ps, pe, w, n_bins, nullptr, z_f64, ldz, v_count, xz_dev, ldxz,
+                       p == 0 ? bin_mode_dev : nullptr, limbs, st);
+            b0 = b1;
         }
-        if (v_pad > v_count)
-            for (int64_t r = 0; r < L; ++r)
-                std::fill(zp + r * ldz + v_count, zp + (r + 1) * ldz, 0.0);
-        GPE_HIP(hipMemcpyAsync(z_f64 + snp_begin * ldz, zp, (size_t)(L * ldz) * sizeof(double),
-                               hipMemcpyHostToDevice, st));
+        return;
     } else {
         throw Error("rand_dist must be 'rademacher', 'gaussian' or 'spherical'");
     }

@@ -114,6 +114,37 @@ def test_probe_streams(engine, oracle, dist, fp32):
     np.testing.assert_array_equal(z, ref)
 
 
+@pytest.mark.parametrize("fp32", [False, True])
+def test_phase1_gaussian_probes_many_blocks_match_oracle(engine, oracle, fp32):
+    """Phase 1 with Gaussian probes over enough SNP blocks that the host draws
+    are cut into parts (each part's traversal queued while the next is drawn):
+    the first part writes the set-mode bins, later parts add."""
+    _say(f"phase1 gaussian many blocks fp32={fp32}")
+    m, n, K, V, step = 2100, 300, 2, 7, 100
+    codes = random_codes(m, n, seed=17, miss_rate=0.02)
+    rng = np.random.default_rng(8)
+    annot = np.ones((m, K))
+    annot[:, 1] = rng.random(m) < 0.4
+    inv = rng.uniform(0.8, 1.2, size=m)
+    g = _geno(engine, oracle, codes).prepare(impute="mean")
+    prior = rng.standard_normal((K * V, n))
+    xz = torch.from_numpy(prior.copy()).cuda()
+    mode = torch.tensor([1, 2], dtype=torch.uint8, device="cuda")  # bin 0 set, bin 1 add
+    engine.phase1_xz(g, 0, m, step, torch.from_numpy(annot).cuda(), torch.from_numpy(inv).cuda(), 5, V, 77, xz,
+                     rand_dist="gaussian", bin_mode=mode, probe_fp32=fp32)
+    bed = oracle.pack_codes(codes)
+    ref = np.zeros((n, K * V))
+    for s in range(0, m, step):
+        e = min(m, s + step)
+        X = oracle.read_block_standardized(bed, n, s, e, None, 1, "mean", 0)
+        ref += oracle.phase1_chunk(X, annot[s:e], inv[s:e], oracle.probe_block(77, s, 5, e - s, V, "gaussian",
+                                                                                fp32=fp32))
+    ref[:, V:] += prior[V:].T
+    err = np.abs(xz.cpu().numpy().T - ref) / np.abs(ref).max(axis=0, keepdims=True)
+    assert err.max() < 3e-11, err.max()
+    g.close()
+
+
 def test_phase1_spherical_probes_match_oracle(engine, oracle):
     """Phase 1 with spherical probes (FP64 probe operand, per-block column rescale)."""
     _say("phase1 spherical")
