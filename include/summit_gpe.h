@@ -331,7 +331,8 @@ enum {
     GPE_PROF_GTU = 3,         /* exact G^T U */
     GPE_PROF_GRAM = 4,        /* banded Gram of windowed LD */
     GPE_PROF_P1_FINALIZE = 5, /* phase-1 missing-call finalize (all SNP-range launches of one traversal) */
-    GPE_PROF_KINDS = 6
+    GPE_PROF_P2_MISS = 6,     /* phase-2 missing-call sums gathered over the per-SNP missing lists */
+    GPE_PROF_KINDS = 7
 };
 int gpe_profile_enable(int32_t on);
 /* Accumulated milliseconds and launch count of one kernel kind (synchronizes
@@ -351,6 +352,12 @@ int gpe_profile_reset(void);
  *   GPE_SCHED_P1_CPB / P2_CPB:  128-value chunks staged per LDS slot and
  *                               barrier in the five-limb contractions (1 or 2;
  *                               2 when no entry)
+ *   GPE_SCHED_P2_MISS:          phase-2 missing calls (mean imputation): 1 a
+ *                               missing-flag MFMA plane, 2 exact sums gathered
+ *                               over per-SNP missing lists (no entry: 2 up to
+ *                               0.55% missing calls, else 1); results are identical
+ *   GPE_SCHED_P2_MISS_PARTS:    sample ranges of the gathered sums (1, or a
+ *                               multiple of 8 up to 64; 8 when no entry)
  * Phase-1 keys: rows = samples, length = SNPs of the call, columns = bins x
  * probes rounded up to 32 per bin.
  * ------------------------------------------------------------------------ */
@@ -359,7 +366,9 @@ enum {
     GPE_SCHED_P1_FINALIZE_KIB = 1,
     GPE_SCHED_P1_CPB = 2,  /* chunks per barrier of the five-limb phase-1 contraction (1 or 2) */
     GPE_SCHED_P2_CPB = 3,  /* the same for phase 2; key rows = SNPs, length = samples */
-    GPE_SCHED_KNOBS = 4
+    GPE_SCHED_P2_MISS = 4, /* phase-2 missing-call method (phase-2 key) */
+    GPE_SCHED_P2_MISS_PARTS = 5, /* sample ranges of the gathered missing-call sums (phase-2 key) */
+    GPE_SCHED_KNOBS = 6
 };
 int gpe_schedule_set(int32_t knob, int64_t rows, int64_t length, int32_t columns, int32_t limbs, int32_t value);
 /* The value in force for a key (0 = the heuristic decides). */

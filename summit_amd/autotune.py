@@ -30,8 +30,10 @@ SCHED_P1_SPLITS = 0
 SCHED_P1_FINALIZE_KIB = 1
 SCHED_P1_CPB = 2
 SCHED_P2_CPB = 3
+SCHED_P2_MISS = 4
+SCHED_P2_MISS_PARTS = 5
 _KNOBS = {"p1_splits": SCHED_P1_SPLITS, "p1_finalize_kib": SCHED_P1_FINALIZE_KIB, "p1_cpb": SCHED_P1_CPB,
-          "p2_cpb": SCHED_P2_CPB}
+          "p2_cpb": SCHED_P2_CPB, "p2_miss": SCHED_P2_MISS, "p2_miss_parts": SCHED_P2_MISS_PARTS}
 
 
 def cache_path() -> str:
@@ -190,7 +192,9 @@ def apply_cached_phase2(n_snps: int, n: int, n_bins: int, v_count: int, limbs: i
 def tune_phase2(g: engine.Genotype, v_count: int, n_bins: int = 1, limbs: int = 5, seed: int = 1, reps: int = 3,
                 path: str | None = None, store: bool = True) -> dict:
     """Time the phase-2 contraction with one and two chunks per barrier over
-    every SNP of ``g`` against ``n_bins`` x ``v_count`` columns; keep the faster."""
+    every SNP of ``g`` against ``n_bins`` x ``v_count`` columns, and, with
+    missing calls, both missing-call methods (missing-flag MFMA plane, or sums
+    gathered over per-SNP lists: identical results); keep the fastest."""
     if g.prepared_with is None:
         raise RuntimeError("genotype handle is not prepared")
     dev = engine.default_device()
@@ -202,22 +206,32 @@ def tune_phase2(g: engine.Genotype, v_count: int, n_bins: int = 1, limbs: int = 
     meansq = torch.zeros((M, n_bins), dtype=torch.float64, device=dev)
     t0 = time.perf_counter()
     set_value("p2_cpb", key, 0)
-    engine.phase2_accum(g, 0, M, inv, v_count, xz, meansq, N, limbs=limbs)  # warm-up
+    set_value("p2_miss", key, 0)
+    methods = (1, 2) if g.n_missing() > 0 else (0,)
     ms = {}
-    for cpb in (1, 2):
-        set_value("p2_cpb", key, cpb)
-        times = []
-        for _ in range(reps):
-            engine.profile_enable(True)
-            engine.profile_reset()
-            engine.phase2_accum(g, 0, M, inv, v_count, xz, meansq, N, limbs=limbs)
-            times.append(engine.profile_read("p2_contract")[0])
-            engine.profile_enable(False)
-        times.sort()
-        ms[cpb] = round(times[len(times) // 2], 4)
-    best = min(ms, key=ms.get)
-    set_value("p2_cpb", key, best)
-    entry = {"choice": {"p2_cpb": best}, "p2_cpb_ms": ms, "tune_seconds": round(time.perf_counter() - t0, 3)}
+    for method in methods:
+        set_value("p2_miss", key, method)
+        engine.phase2_accum(g, 0, M, inv, v_count, xz, meansq, N, limbs=limbs)  # warm-up, workspaces
+        for cpb in (1, 2):
+            set_value("p2_cpb", key, cpb)
+            times = []
+            for _ in range(reps):
+                engine.profile_enable(True)
+                engine.profile_reset()
+                engine.phase2_accum(g, 0, M, inv, v_count, xz, meansq, N, limbs=limbs)
+                times.append(engine.profile_read("p2_contract")[0] + engine.profile_read("p2_miss")[0])
+                engine.profile_enable(False)
+            times.sort()
+            ms[(method, cpb)] = round(times[len(times) // 2], 4)
+    best_method, best_cpb = min(ms, key=ms.get)
+    set_value("p2_cpb", key, best_cpb)
+    set_value("p2_miss", key, best_method)
+    choice = {"p2_cpb": best_cpb}
+    if best_method:
+        choice["p2_miss"] = best_method
+    entry = {"choice": choice, "p2_cpb_ms": {cpb: t for (mth, cpb), t in ms.items() if mth == best_method},
+             "p2_ms": {f"miss{mth}_cpb{cpb}": t for (mth, cpb), t in ms.items()},
+             "tune_seconds": round(time.perf_counter() - t0, 3)}
     if store:
         _store_cache(path or cache_path(), "p2|" + _entry_key(key), entry)
     return entry

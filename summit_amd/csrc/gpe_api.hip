@@ -50,8 +50,12 @@ struct gpe_geno {
     double* mean = nullptr;
     double* inv_sd = nullptr;
     int64_t n_missing = 0;
-    int64_t* miss_ptr = nullptr;
+    int64_t* miss_ptr = nullptr;   // per sample: sorted SNP indices of its missing calls
     int32_t* miss_idx = nullptr;
+    int64_t* smiss_ptr = nullptr;  // per SNP: sorted sample indices of its missing calls
+    int32_t* smiss_idx = nullptr;
+    int64_t* smiss_rptr = nullptr; // per SNP: list offsets of rptr_parts sample ranges
+    int rptr_parts = 0;
     std::map<std::string, Buf> ws;
     // set while an entry point enqueues onto a stream under HIP graph capture:
     // workspaces may then not grow (no allocation may enter a graph)
@@ -198,24 +202,20 @@ __global__ void k_absmax_i32(const int32_t* u, int64_t ldu, int64_t n, int n_col
     atomicMax(out, mx);
 }
 
-__global__ void k_scan_small(const int32_t* counts, int64_t n, int64_t* ptr) {
-    // single-thread exclusive scan; n is the sample count (done once per prepare)
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    int64_t acc = 0;
-    for (int64_t i = 0; i < n; ++i) {
-        ptr[i] = acc;
-        acc += counts[i];
-    }
-    ptr[n] = acc;
-}
-
 void free_prepared(gpe_geno* g) {
     if (g->snp_work && g->snp_work != g->snp_raw) dfree(g->snp_work);
     g->snp_work = g->snp_raw;
     dfree(g->miss_ptr);
     dfree(g->miss_idx);
+    dfree(g->smiss_ptr);
+    dfree(g->smiss_idx);
+    dfree(g->smiss_rptr);
+    g->smiss_rptr = nullptr;
+    g->rptr_parts = 0;
     g->miss_ptr = nullptr;
     g->miss_idx = nullptr;
+    g->smiss_ptr = nullptr;
+    g->smiss_idx = nullptr;
     g->n_missing = 0;
     g->prepared = false;
 }
@@ -479,6 +479,10 @@ int gpe_schedule_set(int32_t knob, int64_t rows, int64_t length, int32_t columns
             require(value <= 0 || (value >= 64 && value <= 65536), "finalize range must be in [64, 65536] KiB");
         if (knob == GPE_SCHED_P1_CPB || knob == GPE_SCHED_P2_CPB)
             require(value <= 2, "chunks per barrier must be 1 or 2");
+        if (knob == GPE_SCHED_P2_MISS) require(value <= 2, "phase-2 missing-call method must be 1 or 2");
+        if (knob == GPE_SCHED_P2_MISS_PARTS)
+            require(value <= 0 || value == 1 || (value % 8 == 0 && value <= 64),
+                    "missing-call sample ranges must be 1 or a multiple of 8 up to 64");
         std::lock_guard<std::mutex> lock(sched_mu());
         const SchedKey k{knob, rows, length, columns, limbs};
         if (value <= 0)
@@ -695,8 +699,7 @@ int gpe_geno_prepare(gpe_geno* g, int32_t ddof, int32_t impute_mode, uint64_t im
         int32_t* counts = static_cast<int32_t*>(g->scratch("miss_counts", g->n * sizeof(int32_t)));
         launch_miss_count(g->smp_work, g->smp_stride, g->n, counts, st);
         g->miss_ptr = dalloc<int64_t>(g->n + 1);
-        hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(1), 0, st, counts, g->n, g->miss_ptr);
-        GPE_HIP(hipGetLastError());
+        launch_scan_counts(counts, g->n, g->miss_ptr, st);
         int64_t total = 0;
         GPE_HIP(hipMemcpyAsync(&total, g->miss_ptr + g->n, sizeof(int64_t), hipMemcpyDeviceToHost, st));
         GPE_HIP(hipStreamSynchronize(st));
@@ -704,6 +707,17 @@ int gpe_geno_prepare(gpe_geno* g, int32_t ddof, int32_t impute_mode, uint64_t im
         if (total > 0) {
             g->miss_idx = dalloc<int32_t>(total);
             launch_miss_fill(g->smp_work, g->smp_stride, g->n, g->miss_ptr, g->miss_idx, st);
+            // the same calls listed per SNP (phase-2 missing-call sums)
+            int32_t* scounts = static_cast<int32_t*>(g->scratch("smiss_counts", g->m * sizeof(int32_t)));
+            launch_miss_count(g->snp_work, g->snp_stride, g->m, scounts, st);
+            g->smiss_ptr = dalloc<int64_t>(g->m + 1);
+            launch_scan_counts(scounts, g->m, g->smiss_ptr, st);
+            int64_t total_snp = 0;
+            GPE_HIP(hipMemcpyAsync(&total_snp, g->smiss_ptr + g->m, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+            GPE_HIP(hipStreamSynchronize(st));
+            require(total_snp == total, "missing-call lists by SNP and by sample disagree");
+            g->smiss_idx = dalloc<int32_t>(total);
+            launch_miss_fill(g->snp_work, g->snp_stride, g->m, g->smiss_ptr, g->smiss_idx, st);
         }
         GPE_HIP(hipStreamSynchronize(st));
         g->prepared = true;
@@ -1009,6 +1023,14 @@ void run_phase1(gpe_geno* g, int64_t s, int64_t e, const double* w, int n_bins, 
                        bin_mode, st);
 }
 
+// Missing-call density (missing calls / (N M)) up to which phase 2 gathers the
+// missing-call sums instead of running a missing-flag MFMA plane: the gathers
+// cost about 7 ms per 0.5% at N = 100,000, M = 500,000, 64 columns, the plane
+// about 8 ms at any density (break-even near 0.55%)
+constexpr double kSparseMissDensity = 0.0055;
+// sample ranges of the gathered missing-call sums (one per XCD)
+constexpr int kMissParts = 8;
+
 // Phase-2 type product on the SNP-major layout against x (column-major (N, n_bins*v_count)):
 // quantize x into limbs and run the contraction with the given epilogue.
 void run_phase2(gpe_geno* g, int64_t s, int64_t e, const double* x, int64_t ldx, int n_bins, int v_count, int limbs,
@@ -1051,6 +1073,58 @@ void run_phase2(gpe_geno* g, int64_t s, int64_t e, const double* x, int64_t ldx,
     const bool miss = g->impute_mode == GPE_IMPUTE_MEAN && g->n_missing > 0;
     a.has_miss = miss ? 1 : 0;
     const int cpb = sched_lookup(GPE_SCHED_P2_CPB, e - s, g->n, ncols_pad, lc.nl) == 1 ? 1 : 2;
+    // missing calls: a second (missing-flag) MFMA plane, or exact sums gathered over
+    // the per-SNP missing lists when calls are sparse (the same int64 values)
+    const int p2m = sched_lookup(GPE_SCHED_P2_MISS, e - s, g->n, ncols_pad, lc.nl);
+    const double density = (double)g->n_missing / ((double)g->n * (double)g->m);
+    const bool gather = miss && mode == EPI_P2 && (p2m == 2 || (p2m != 1 && density <= kSparseMissDensity));
+    if (gather) {
+        const int nch = (int)ceil_div(ncols_pad, 64);
+        int parts = sched_lookup(GPE_SCHED_P2_MISS_PARTS, e - s, g->n, ncols_pad, lc.nl);
+        if (parts <= 0) parts = kMissParts;
+        // per-SNP list offsets of the sample ranges, kept on the handle for the range count in use
+        if (g->rptr_parts != parts) {
+            require(!g->capturing, "missing-list ranges not on the device: run the same call once outside graph "
+                                   "capture first");
+            dfree(g->smiss_rptr);
+            g->smiss_rptr = nullptr;
+            g->rptr_parts = 0;
+            g->smiss_rptr = dalloc<int64_t>(g->m * (parts + 1));
+            launch_range_ptrs(g->smiss_ptr, g->smiss_idx, g->m, parts, ceil_div(g->n, parts), g->smiss_rptr, st);
+            g->rptr_parts = parts;
+        }
+        int8_t* xrow = static_cast<int8_t*>(g->scratch("p2_xrow", (size_t)(g->n + 1) * nch * lc.nl * 64));
+        launch_xrow_limbs(x, ldx, g->n, n_bins, v_count, v_pad, shift, lc.nl, nch, xrow, st);
+        const int64_t ld_ms = (int64_t)nch * 64;
+        // SNP slices keep the per-range sums near 4 GiB
+        const int64_t slice =
+            std::max<int64_t>(256, round_up(((int64_t)1 << 32) / ((int64_t)parts * ld_ms * 8), 256));
+        const int64_t rows = std::min<int64_t>(slice, e - s);
+        int64_t* msum =
+            static_cast<int64_t*>(g->scratch("p2_msum", (size_t)parts * rows * ld_ms * sizeof(int64_t)));
+        a.msum = msum;
+        a.ld_ms = ld_ms;
+        a.ms_parts = parts;
+        a.ms_part_stride = rows * ld_ms;
+        for (int64_t rs = s; rs < e; rs += slice) {
+            const int64_t re = std::min(e, rs + slice);
+            ProfScope pm(GPE_PROF_P2_MISS, st);
+            launch_miss_sums(g->smiss_rptr, parts, g->smiss_idx, rs, re, xrow, g->n, lc.nl, nch, msum, rows * ld_ms,
+                             ld_ms, st);
+            pm.stop();
+            a.r0 = rs;
+            a.r1 = re;
+            ProfScope ps(GPE_PROF_P2_CONTRACT, st);
+            launch_contract(lc.nl, lc.ctc, false, EPI_P2X, a, groups, 1, st, cpb);
+            ps.stop();
+        }
+        a.r0 = s;
+        a.r1 = e;
+        a.msum = nullptr;
+        if (ctiles_used_out) *ctiles_used_out = ncols / 32;
+        if (v_pad_out) *v_pad_out = v_pad;
+        return;
+    }
     ProfScope ps(GPE_PROF_P2_CONTRACT, st);
     launch_contract(lc.nl, lc.ctc, miss, mode, a, groups, 1, st, cpb);
     ps.stop();

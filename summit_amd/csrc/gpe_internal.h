@@ -51,6 +51,7 @@ enum EpiMode : int {
     EPI_I64_2P = 5,      // MISS: int64 row-major of the genotype plane to out_i64, missing plane to out2_i64
     EPI_I64_SQ = 6,      // squared-genotype plane (codes 00 -> 0, 01 -> 0, 10 -> 1, 11 -> 4), int64 row-major
     EPI_I64_ATOMIC_Q = 7,// missing-flag plane (01 -> 1), otherwise as EPI_I64_ATOMIC
+    EPI_P2X = 8,         // EPI_P2 with the missing-call sums read from msum instead of a missing plane
 };
 
 struct ContractArgs {
@@ -79,6 +80,10 @@ struct ContractArgs {
     int32_t cols_out;        // I32/I64 epilogues: columns >= cols_out are not written
     int64_t out_row0;        // I32/I64 epilogues: output row index = r - out_row0
     int64_t* out2_i64;       // EPI_I64_2P: missing-plane output (same layout as out_i64)
+    const int64_t* msum;     // EPI_P2X: exact missing-call sums over sample ranges,
+    int64_t ld_ms;           //   msum[p * ms_part_stride + (r - r0) * ld_ms + c] for p < ms_parts
+    int32_t ms_parts;
+    int64_t ms_part_stride;
 };
 
 // Banded Gram (windowed LD) arguments
@@ -124,6 +129,22 @@ void launch_hwe_impute(uint8_t* snp, int64_t stride, int64_t m, int64_t n, const
 void launch_transpose(const uint8_t* snp, int64_t snp_stride, int64_t m_pad, uint8_t* smp, int64_t smp_stride,
                       int64_t n_pad, hipStream_t st);
 void launch_miss_count(const uint8_t* smp, int64_t stride, int64_t n, int32_t* counts, hipStream_t st);
+// ptr[i] = sum_{t < i} counts[t] for i <= n (one workgroup)
+void launch_scan_counts(const int32_t* counts, int64_t n, int64_t* ptr, hipStream_t st);
+// Row-major limbs of the phase-2 operand for gathers by sample: xrow[((s * nch + h) * nl + l) * 64 + cc] =
+// digit l of q = round(x * 2^shift) of column c = 64 h + cc of sample s (0 past the real columns), the
+// same digits launch_quant_p2 swizzles for the contraction; rows s = 0..n, row n all zeros
+void launch_xrow_limbs(const double* x, int64_t ldx, int64_t n, int n_bins, int v_count, int v_pad,
+                       const int32_t* shift, int nl, int nch, int8_t* xrow, hipStream_t st);
+// rptr[j * (parts + 1) + p] = first entry of SNP j's missing list (ptr/idx, sorted
+// by sample) at a sample >= p * range, p = 0..parts
+void launch_range_ptrs(const int64_t* ptr, const int32_t* idx, int64_t m, int parts, int64_t range, int64_t* rptr,
+                       hipStream_t st);
+// out[p * part_stride + (j - r0) * ld_ms + c] = sum over the missing calls (n, j) with n in sample
+// range p of q[n][c], exact, for SNPs j in [r0, r1), from the range pointers of launch_range_ptrs
+// and the limb rows of launch_xrow_limbs
+void launch_miss_sums(const int64_t* rptr, int parts, const int32_t* idx, int64_t r0, int64_t r1, const int8_t* xrow,
+                      int64_t n, int nl, int nch, int64_t* out, int64_t part_stride, int64_t ld_ms, hipStream_t st);
 void launch_miss_fill(const uint8_t* smp, int64_t stride, int64_t n, const int64_t* ptr, int32_t* idx,
                       hipStream_t st);
 void launch_fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t st);

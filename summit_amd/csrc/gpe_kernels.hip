@@ -439,7 +439,7 @@ __global__ __launch_bounds__(256, (NL * CTC <= 2) ? 4 : 2) void k_contract(Contr
                 }
             }
         }
-    } else {  // EPI_P2
+    } else {  // EPI_P2, EPI_P2X
 #pragma unroll
         for (int ctc = 0; ctc < CTC; ++ctc) {
             const int ct = cg * CTC + ctc;
@@ -460,6 +460,10 @@ __global__ __launch_bounds__(256, (NL * CTC <= 2) ? 4 : 2) void k_contract(Contr
                             if (MISS) me = me * 256 + (int64_t)acc[l * CTC + ctc][1][e];
                         }
                         const int c = ct * 32 + 8 * (e >> 2) + 4 * half + (e & 3);
+                        if constexpr (EPI == EPI_P2X) {
+                            const int64_t* ms = a.msum + (r - a.r0) * a.ld_ms + c;
+                            for (int pp = 0; pp < a.ms_parts; ++pp) me += ms[pp * a.ms_part_stride];
+                        }
                         const double obs_sum = (double)(a.col_sum[c] - me);
                         const double w = isd * a.col_scale[c] * ((double)ge - mu * obs_sum);
                         const double w2 = w * w;
@@ -504,6 +508,10 @@ void launch_contract(int nl, int ctc, bool miss, EpiMode mode, const ContractArg
             launch_one<5, 1, false, EPI_P2, 2>(a, n_col_groups, n_splits, st);
             return;
         }
+        if (nl == 5 && ctc == 1 && !miss && mode == EPI_P2X) {
+            launch_one<5, 1, false, EPI_P2X, 2>(a, n_col_groups, n_splits, st);
+            return;
+        }
         if (nl == 5 && ctc == 1 && !miss && mode == EPI_I64_ATOMIC_Q) {
             launch_one<5, 1, false, EPI_I64_ATOMIC_Q, 2>(a, n_col_groups, n_splits, st);
             return;
@@ -532,6 +540,8 @@ void launch_contract(int nl, int ctc, bool miss, EpiMode mode, const ContractArg
     GPE_CASE(5, 1, true, EPI_P2)
     GPE_CASE(6, 1, false, EPI_P2)
     GPE_CASE(6, 1, true, EPI_P2)
+    GPE_CASE(5, 1, false, EPI_P2X)
+    GPE_CASE(6, 1, false, EPI_P2X)
     // dense standardized X^T V
     GPE_CASE(5, 1, false, EPI_WORK)
     GPE_CASE(5, 1, true, EPI_WORK)
@@ -856,7 +866,8 @@ void launch_miss_count(const uint8_t* smp, int64_t stride, int64_t n, int32_t* c
     GPE_HIP(hipGetLastError());
 }
 
-// Sorted SNP indices of the missing calls of each sample row.
+// Sorted indices of the missing calls of each packed row: SNP indices for
+// sample-major rows, sample indices for SNP-major rows.
 __global__ void k_miss_fill(const uint8_t* smp, int64_t stride, int64_t n, const int64_t* ptr, int32_t* idx) {
     const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
@@ -891,6 +902,37 @@ void launch_miss_fill(const uint8_t* smp, int64_t stride, int64_t n, const int64
                       hipStream_t st) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_miss_fill, dim3((unsigned)ceil_div(n, 4)), dim3(256), 0, st, smp, stride, n, ptr, idx);
+    GPE_HIP(hipGetLastError());
+}
+
+// Exclusive scan of n counts in one workgroup: each thread sums a contiguous
+// run, the 1024 run totals are scanned in LDS, then each run is written out.
+__global__ __launch_bounds__(1024) void k_scan_counts(const int32_t* __restrict__ counts, int64_t n,
+                                                      int64_t* __restrict__ ptr) {
+    __shared__ int64_t part[1024];
+    const int t = threadIdx.x;
+    const int64_t per = (n + 1023) / 1024;
+    const int64_t a = t * per < n ? t * per : n;
+    const int64_t b = a + per < n ? a + per : n;
+    int64_t s = 0;
+    for (int64_t i = a; i < b; ++i) s += counts[i];
+    part[t] = s;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const int64_t v = t >= o ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int64_t acc = part[t] - s;
+    for (int64_t i = a; i < b; ++i) {
+        ptr[i] = acc;
+        acc += counts[i];
+    }
+    if (t == 1023) ptr[n] = part[1023];
+}
+void launch_scan_counts(const int32_t* counts, int64_t n, int64_t* ptr, hipStream_t st) {
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, counts, n, ptr);
     GPE_HIP(hipGetLastError());
 }
 
@@ -1544,6 +1586,206 @@ void launch_quant_p2(const double* x, int64_t ldx, int64_t n, int n_bins, int v_
     else if (nl == 6)
         hipLaunchKernelGGL(k_quant_p2<6>, grid, dim3(256), 0, st, x, ldx, n, v_count, v_pad, shift, ctc, coef_ctiles,
                            kcount, coef, col_sum);
+    else
+        throw Error("phase-2 operand limbs must be 5 or 6");
+    GPE_HIP(hipGetLastError());
+}
+
+// Limb rows of the phase-2 operand for gathers by sample.  One workgroup per
+// 64-sample x 64-column tile: q and its digits as k_quant_p2 forms them, staged
+// in LDS and written out so that each sample's nl x 64 bytes are contiguous.
+// Rows run to n inclusive: row n is all zeros (the pad entry of the gathers).
+template <int NL>
+__global__ __launch_bounds__(256) void k_xrow_limbs(const double* __restrict__ x, int64_t ldx, int64_t n, int ncols,
+                                                    int v_count, int v_pad, const int32_t* __restrict__ shift,
+                                                    int nch, int8_t* __restrict__ xrow) {
+    constexpr int DW = NL * 16;  // dwords per (sample, chunk)
+    __shared__ __attribute__((aligned(16))) int8_t tile[64][NL * 64];
+    const int tid = threadIdx.x;
+    const int64_t n0 = (int64_t)blockIdx.x * 64;
+    const int h = blockIdx.y;
+    for (int i = tid; i < 64 * 64; i += 256) {
+        const int cc = i >> 6, sl = i & 63;
+        const int c = h * 64 + cc;
+        const int64_t smp = n0 + sl;
+        int64_t q = 0;
+        if (c < ncols && smp < n) {
+            const int k = c / v_pad, v = c % v_pad;
+            if (v < v_count) q = __double2ll_rn(ldexp(x[(int64_t)(k * v_count + v) * ldx + smp], shift[c]));
+        }
+        int8_t d[NL];
+        balanced_digits<NL>(q, d);
+#pragma unroll
+        for (int l = 0; l < NL; ++l) tile[sl][l * 64 + cc] = d[l];
+    }
+    __syncthreads();
+    for (int i = tid; i < 64 * DW; i += 256) {
+        const int sl = i / DW, dw = i % DW;
+        const int64_t smp = n0 + sl;
+        if (smp <= n)
+            reinterpret_cast<uint32_t*>(xrow)[(smp * nch + h) * DW + dw] =
+                reinterpret_cast<const uint32_t*>(tile[sl])[dw];
+    }
+}
+void launch_xrow_limbs(const double* x, int64_t ldx, int64_t n, int n_bins, int v_count, int v_pad,
+                       const int32_t* shift, int nl, int nch, int8_t* xrow, hipStream_t st) {
+    if (n <= 0) return;
+    const dim3 grid((unsigned)ceil_div(n + 1, 64), (unsigned)nch);
+    if (nl == 5)
+        hipLaunchKernelGGL(k_xrow_limbs<5>, grid, dim3(256), 0, st, x, ldx, n, n_bins * v_pad, v_count, v_pad, shift,
+                           nch, xrow);
+    else if (nl == 6)
+        hipLaunchKernelGGL(k_xrow_limbs<6>, grid, dim3(256), 0, st, x, ldx, n, n_bins * v_pad, v_count, v_pad, shift,
+                           nch, xrow);
+    else
+        throw Error("phase-2 operand limbs must be 5 or 6");
+    GPE_HIP(hipGetLastError());
+}
+
+// rptr[j * (P + 1) + p] = first entry of SNP j's (sorted) missing list whose
+// sample is >= p R, for p = 0..P: the list's part in sample range p.
+__global__ void k_range_ptrs(const int64_t* __restrict__ ptr, const int32_t* __restrict__ idx, int64_t m, int parts,
+                             int64_t range, int64_t* __restrict__ rptr) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m * (parts + 1)) return;
+    const int64_t j = t / (parts + 1);
+    const int64_t bound = (t % (parts + 1)) * range;
+    int64_t lo = ptr[j], hi = ptr[j + 1];
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (idx[mid] < bound) lo = mid + 1;
+        else hi = mid;
+    }
+    rptr[t] = lo;
+}
+void launch_range_ptrs(const int64_t* ptr, const int32_t* idx, int64_t m, int parts, int64_t range, int64_t* rptr,
+                       hipStream_t st) {
+    const int64_t total = m * (parts + 1);
+    if (total <= 0) return;
+    hipLaunchKernelGGL(k_range_ptrs, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, st, ptr, idx, m, parts, range,
+                       rptr);
+    GPE_HIP(hipGetLastError());
+}
+
+// Exact missing-call sums of the phase-2 operand over one sample range, one
+// wave per (range p, column chunk h, SNP j):
+//   part_p[j - r0][64 h + cc] = sum over samples n in range p missing at j of q[n][64 h + cc].
+// A sample's limb row is D = nl x 16 dwords (k_xrow_limbs).  Entries are taken
+// U = 16 at a time: each entry's first 64 dwords are one full-wave load (lane
+// = dword), and its last D - 64 dwords share a full-wave load with the next
+// 64 / (D - 64) - 1 entries (lane groups of D - 64).  Each lane sums the four
+// signed bytes of its dwords in int32 (|sum| <= 128 n < 2^31); the tail lane
+// groups are folded with shuffles and the limbs combined to int64 at the end.
+// A batch's entries come 64 at a time (lane t loads entry t's sample index,
+// entries past the list read the all-zero row n), so every group is full and
+// its 16 + 16 (D - 64) / 64 loads are in flight together.  With P a multiple of
+// the XCD count, workgroup b runs on XCD b % 8 and the ranges are dealt so that
+// each XCD gathers only from its own ranges, one after another: a range's rows
+// stay in that XCD's L2 (at N = 100,000, P = 8: 90% L2 hits).  The gathers are
+// then bound by the misses a CU keeps in flight to L2, about 7 ms for 2.5e8
+// calls of 320 bytes, against about 8 ms for the missing-flag MFMA plane.
+template <int NL>
+__global__ __launch_bounds__(256) void k_miss_sums(const int64_t* __restrict__ rptr, int parts,
+                                                   const int32_t* __restrict__ idx, int64_t r0, int64_t r1,
+                                                   const int8_t* __restrict__ xrow, int n, int nch,
+                                                   int64_t* __restrict__ out, int64_t part_stride, int64_t ld_ms) {
+    constexpr int D = NL * 16;
+    constexpr int X = D - 64;        // tail dwords per entry (16 or 32)
+    constexpr int E = 64 / X;        // entries per tail load
+    constexpr int U = 16;            // entries per group
+    static_assert(X > 0 && 64 % X == 0 && U % E == 0, "limb rows of 5 or 6 limbs");
+    __shared__ int32_t red[4][NL * 64];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int64_t rows = r1 - r0;
+    const int64_t per_range = rows * nch;
+    int p;
+    int64_t item;
+    if (parts % kXcds == 0) {
+        const int64_t br = (per_range + 3) / 4;  // workgroups per range
+        const int64_t i = blockIdx.x / kXcds;
+        const int q = (int)(i / br);
+        p = (int)(blockIdx.x % kXcds) + kXcds * q;
+        item = (i % br) * 4 + wave;
+        if (q >= parts / kXcds || item >= per_range) return;
+    } else {
+        const int64_t w = (int64_t)blockIdx.x * 4 + wave;
+        p = (int)(w / per_range);
+        item = w % per_range;
+        if (p >= parts) return;
+    }
+    const int h = (int)(item / rows);
+    const int64_t j = r0 + item % rows;
+    const int64_t b0 = rptr[j * (parts + 1) + p], b1 = rptr[j * (parts + 1) + p + 1];
+    // dword offsets (the table holds fewer than 2^32 dwords)
+    const uint32_t rstride = (uint32_t)(nch * D);
+    const uint32_t* tab = reinterpret_cast<const uint32_t*>(xrow) + (uint32_t)h * D;
+    const uint32_t tail_off = 64u + (uint32_t)(lane % X);
+    const int tail_grp = lane / X;
+    int32_t s0[4] = {0, 0, 0, 0}, s1[4] = {0, 0, 0, 0};
+    auto add4 = [](int32_t* acc, uint32_t v) {
+        acc[0] += (int32_t)(int8_t)(v & 0xffu);
+        acc[1] += (int32_t)(int8_t)((v >> 8) & 0xffu);
+        acc[2] += (int32_t)(int8_t)((v >> 16) & 0xffu);
+        acc[3] += (int32_t)(int8_t)(v >> 24);
+    };
+    for (int64_t p0 = b0; p0 < b1; p0 += 64) {
+        const int cnt = (int)((b1 - p0) < 64 ? (b1 - p0) : 64);
+        const int my = lane < cnt ? idx[p0 + lane] : n;
+        for (int t = 0; t < cnt; t += U) {
+            uint32_t v0[U], v1[U / E];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t smp = (uint32_t)__builtin_amdgcn_readlane(my, t + u);
+                v0[u] = tab[smp * rstride + (uint32_t)lane];
+            }
+#pragma unroll
+            for (int w = 0; w < U / E; ++w) {
+                const uint32_t smp = (uint32_t)__shfl(my, t + w * E + tail_grp, 64);
+                v1[w] = tab[smp * rstride + tail_off];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) add4(s0, v0[u]);
+#pragma unroll
+            for (int w = 0; w < U / E; ++w) add4(s1, v1[w]);
+        }
+    }
+    // fold the tail lane groups: lanes < X then hold dword 64 + lane
+#pragma unroll
+    for (int o = X; o < 64; o <<= 1) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) s1[k] += __shfl_down(s1[k], o, 64);
+    }
+    // dword d holds limb d / 16 of columns 4 (d % 16) .. +3, i.e. red[4 d + k]
+    int32_t* rd = red[wave];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) rd[4 * lane + k] = s0[k];
+    if (lane < X) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) rd[4 * (64 + lane) + k] = s1[k];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    int64_t me = 0;
+#pragma unroll
+    for (int l = 0; l < NL; ++l) me = me * 256 + (int64_t)rd[l * 64 + lane];
+    out[(int64_t)p * part_stride + (j - r0) * ld_ms + h * 64 + lane] = me;
+}
+void launch_miss_sums(const int64_t* rptr, int parts, const int32_t* idx, int64_t r0, int64_t r1, const int8_t* xrow,
+                      int64_t n, int nl, int nch, int64_t* out, int64_t part_stride, int64_t ld_ms, hipStream_t st) {
+    const int64_t per_range = (r1 - r0) * nch;
+    if (per_range <= 0) return;
+    if ((n + 1) * nch * nl * 16 >= ((int64_t)1 << 32)) throw Error("missing-call gather table exceeds 2^32 dwords");
+    const int64_t blocks = parts % kXcds == 0 ? kXcds * (int64_t)(parts / kXcds) * ceil_div(per_range, 4)
+                                              : ceil_div(per_range * parts, 4);
+    const dim3 grid((unsigned)blocks);
+    if (nl == 5)
+        hipLaunchKernelGGL(k_miss_sums<5>, grid, dim3(256), 0, st, rptr, parts, idx, r0, r1, xrow, (int)n, nch, out,
+                           part_stride, ld_ms);
+    else if (nl == 6)
+        hipLaunchKernelGGL(k_miss_sums<6>, grid, dim3(256), 0, st, rptr, parts, idx, r0, r1, xrow, (int)n, nch, out,
+                           part_stride, ld_ms);
     else
         throw Error("phase-2 operand limbs must be 5 or 6");
     GPE_HIP(hipGetLastError());

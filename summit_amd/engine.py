@@ -348,7 +348,7 @@ def probes(root_seed: int, blk_starts, blk_end: int, v_start: int, v_count: int,
     return out
 
 
-PROF_KINDS = {"p1_contract": 0, "p2_contract": 1, "gz": 2, "gtu": 3, "gram": 4, "p1_finalize": 5}
+PROF_KINDS = {"p1_contract": 0, "p2_contract": 1, "gz": 2, "gtu": 3, "gram": 4, "p1_finalize": 5, "p2_miss": 6}
 
 
 def profile_enable(on: bool = True) -> None:
