@@ -510,6 +510,8 @@ def main() -> int:
     t1 = time.perf_counter()
     p1_ms, p1_n = engine.profile_read("p1_contract")
     p2_ms, p2_n = engine.profile_read("p2_contract")
+    fin_ms, fin_n = engine.profile_read("p1_finalize")
+    pm_ms, pm_n = engine.profile_read("p2_miss")
     engine.profile_enable(False)
     ms_per_step = (t1 - t0) * 1000.0 / args.steps
     pairs = 2.0 * N * M * B
@@ -554,15 +556,18 @@ def main() -> int:
         except Exception:
             traffic = None
     # int8 MFMA multiply-adds the kernel issues per algorithmic genotype-probe pair: one per limb of
-    # the real-valued operand, doubled in phase 2 when the missing-call plane is contracted too
-    miss_plane = args.impute == "mean" and geno.n_missing() > 0
+    # the real-valued operand, doubled in phase 2 when the missing-call plane is contracted too (with
+    # sparse missing calls their sums are gathered over per-SNP lists instead: p2_miss_sums)
+    miss_plane = args.impute == "mean" and geno.n_missing() > 0 and pm_n == 0
     per_pair = args.limbs * (2 if (kname.startswith("k_contract phase 2") and miss_plane) else 1)
     roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / INT8_PEAK_TOPS, 4), "traffic": traffic, "kernel": kname,
                 "int8_macs_per_pair": per_pair, "mfma_pipe_frac": round(achieved * per_pair / INT8_PEAK_TOPS, 4),
                 "avg_ms": round(avg_ms, 3),
                 "phase_ms": {"p1_contract": round(p1_ms / max(p1_n, 1), 3),
-                             "p2_contract": round(p2_ms / max(p2_n, 1), 3)}}
+                             "p1_missing_finalize": round(fin_ms / args.steps, 3),
+                             "p2_contract": round(p2_ms / max(p2_n, 1), 3),
+                             "p2_miss_sums": round(pm_ms / args.steps, 3)}}
 
     extra = {}
     if not args.no_exact:
