@@ -244,6 +244,52 @@ def gxe_rate(engine, torch) -> dict:
             "pairs": pairs, "gpairs_per_s": round(pairs / t / 1e9, 1)}
 
 
+def cpu_mailman_c1(engine, torch) -> dict:
+    """The reference's CPU path for at most 10 probes (Mailman lookup tables,
+    OpenMP over segments; the oracle's restatement, pinned to the dense integer
+    products) on N=5,000, M=10,000 i.i.d. genotypes with 1% missing calls and
+    B=10 Rademacher probes: G.Z then G^T (G.Z), exact int64, on all host cores,
+    beside the GPU's exact integer products on the same genotypes."""
+    import numpy as np
+
+    from oracle import oracle as o
+
+    n, m, b = 5000, 10_000, 10
+    g = engine.Genotype.synthetic(n, m, seed=101, maf_lo=0.05, maf_hi=0.5, miss_rate=0.01, block_len=100,
+                                  copy_prob=0.0)
+    g.prepare(ddof=1, impute="mean")
+    packed = np.ascontiguousarray(g.download_rows("snp_raw", 0, m)[:, :(n + 3) // 4])
+    cores = len(os.sched_getaffinity(0))
+    rng = np.random.default_rng(1101)
+    z = (rng.integers(0, 2, size=(m, b)) * 2 - 1).astype(np.int8)
+    u = o.mailman_gz(packed, n, z, threads=cores)  # warm-up
+    reps, t0 = 0, time.perf_counter()
+    while reps < 3 or time.perf_counter() - t0 < 2.0:
+        u = o.mailman_gz(packed, n, z, threads=cores)
+        w = o.mailman_gtu(packed, n, u, threads=cores)
+        reps += 1
+    cpu_s = (time.perf_counter() - t0) / reps
+    pairs = 2.0 * n * m * b
+    zd = torch.from_numpy(z).to(engine.default_device())
+    ud = engine.gz_i32(g, zd)
+    wd = engine.gtu_i64(g, ud)
+    torch.cuda.synchronize()
+    same = bool(np.array_equal(ud.cpu().numpy().T.astype(np.int64), u)) and bool(np.array_equal(wd.cpu().numpy(), w))
+    t1 = time.perf_counter()
+    for _ in range(20):
+        ud = engine.gz_i32(g, zd)
+        wd = engine.gtu_i64(g, ud)
+    torch.cuda.synchronize()
+    gpu_s = (time.perf_counter() - t1) / 20
+    g.close()
+    return {"workload": f"N={n} M={m} B={b} Rademacher, i.i.d. genotypes, 1% missing (as 0): G.Z and G^T(G.Z)",
+            "cpu_mailman": {"gpairs_per_s": round(pairs / cpu_s / 1e9, 2), "seconds": round(cpu_s, 4),
+                            "cores": cores, "reps": reps, "kind": "port",
+                            "segment_snps": o.mailman_segment_size(n)},
+            "gpu_exact_int": {"gpairs_per_s": round(pairs / gpu_s / 1e9, 1), "seconds": round(gpu_s, 6)},
+            "results_equal": same}
+
+
 def ingest_rate(engine, torch, geno, n_snps: int) -> dict:
     """Host -> HBM rate of the BED path: the first n_snps packed SNP rows of the
     benchmark genotypes, copied to host memory, then uploaded (pinned,
@@ -422,7 +468,8 @@ def main() -> int:
     ap.add_argument("--no-cv", action="store_true", help="skip the learned control-variate measurement")
     ap.add_argument("--no-tune", action="store_true", help="keep the built-in schedule heuristics")
     ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph replay timing of the same steps")
-    ap.add_argument("--only", choices=["learned_cv", "partitioned", "gxe", "windowed", "wall_to_1pct"],
+    ap.add_argument("--only", choices=["learned_cv", "partitioned", "gxe", "windowed", "wall_to_1pct",
+                                       "cpu_mailman_c1"],
                     help="run only one extra measurement and print it")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the 1%%-error wall-time run and the host->HBM ingest measurement")
@@ -460,6 +507,9 @@ def main() -> int:
         return 0
     if args.only == "windowed":
         print(json.dumps({"windowed": windowed_rate(engine, torch)}), flush=True)
+        return 0
+    if args.only == "cpu_mailman_c1":
+        print(json.dumps({"cpu_mailman_c1": cpu_mailman_c1(engine, torch)}), flush=True)
         return 0
     if args.only == "wall_to_1pct":
         print(json.dumps({"wall_to_1pct": wall_to_one_percent(engine, ldscore, torch, dev)}), flush=True)
@@ -615,6 +665,9 @@ def main() -> int:
         log(f"gaussian f32: {extra['gaussian_f32']}")
         extra["ingest"] = ingest_rate(engine, torch, geno, min(M, 100_000))
         log(f"ingest: {extra['ingest']}")
+        if not args.no_cpu:
+            extra["cpu_mailman_c1"] = cpu_mailman_c1(engine, torch)
+            log(f"cpu mailman (C1 shape): {extra['cpu_mailman_c1']}")
 
     cpu = None
     if not args.no_cpu:

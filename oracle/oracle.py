@@ -54,6 +54,12 @@ def _lib():
         lib.or_mt64_nth.argtypes = [ctypes.c_uint64, ctypes.c_int64]
         lib.or_make_seed.restype = ctypes.c_uint64
         lib.or_make_seed.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int]
+        lib.or_mailman_segment_size.restype = ctypes.c_int
+        lib.or_mailman_segment_size.argtypes = [ctypes.c_int64]
+        for fn in (lib.or_mailman_gz, lib.or_mailman_gtu):
+            fn.restype = None
+            fn.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                           ctypes.c_void_p, ctypes.c_int]
         for fn in (lib.or_rademacher_block, lib.or_gaussian_block):
             fn.restype = None
             fn.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -576,6 +582,35 @@ def gxe_scores(bed, n_total: int, m_total: int, env, C_int, annot=None, nvecs: i
 # ----------------------------------------------------------------------------
 def masked_geno(codes) -> np.ndarray:
     return np.array([0, 0, 1, 2], dtype=np.int64)[codes]
+
+
+def mailman_segment_size(n: int) -> int:
+    """src/native/common/genotype.cpp:186-200 (no environment override)."""
+    return int(_lib().or_mailman_segment_size(int(n)))
+
+
+def mailman_gz(packed: np.ndarray, n: int, Z, threads: int = 0) -> np.ndarray:
+    """U (n, b) int64 = G.Z through the Mailman post-multiply
+    (src/native/common/mailman.hpp:213-270) over packed SNP-major BED rows
+    (M, stride) with missing calls as 0; Z (M, b) int8."""
+    p = np.ascontiguousarray(packed, dtype=np.uint8)
+    z = np.ascontiguousarray(Z, dtype=np.int8)
+    m, b = z.shape
+    out = np.empty((n, b), dtype=np.int64)
+    _lib().or_mailman_gz(p.ctypes.data, p.shape[1], n, m, z.ctypes.data, b, out.ctypes.data, int(threads))
+    return out
+
+
+def mailman_gtu(packed: np.ndarray, n: int, U, threads: int = 0) -> np.ndarray:
+    """W (M, b) int64 = G^T U through the Mailman pre-multiply
+    (src/native/common/mailman.hpp:17-78); U (n, b) int64."""
+    p = np.ascontiguousarray(packed, dtype=np.uint8)
+    u = np.ascontiguousarray(U, dtype=np.int64)
+    b = u.shape[1]
+    m = p.shape[0]
+    out = np.empty((m, b), dtype=np.int64)
+    _lib().or_mailman_gtu(p.ctypes.data, p.shape[1], n, m, u.ctypes.data, b, out.ctypes.data, int(threads))
+    return out
 
 
 def gz_int(g_masked, Z) -> np.ndarray:
