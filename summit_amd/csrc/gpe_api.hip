@@ -274,6 +274,21 @@ struct ProfScope {
 void real_probes_into(int rand_dist, uint64_t root, const int64_t* starts, int n_blocks, int64_t blk_end, int v_start,
                       int v_count, double* dst, int64_t ldz);
 
+// Packed rows [src, src + rows * stride) to host rows of host_stride bytes (the
+// first `used` bytes of each): whole rows in 64 MiB pieces through the pinned
+// staging buffer, then compacted on the host (plain contiguous copies only).
+void download_rows_staged(gpe_geno* g, uint8_t* host, int64_t host_stride, const uint8_t* src, int64_t stride,
+                          int64_t used, int64_t rows) {
+    GPE_HIP(hipDeviceSynchronize());
+    const int64_t piece = std::max<int64_t>(1, ((int64_t)64 << 20) / stride);
+    uint8_t* stage = reinterpret_cast<uint8_t*>(g->stage((size_t)(std::min(rows, piece) * stride)));
+    for (int64_t r0 = 0; r0 < rows; r0 += piece) {
+        const int64_t nr = std::min(piece, rows - r0);
+        GPE_HIP(hipMemcpy(stage, src + r0 * stride, (size_t)(nr * stride), hipMemcpyDeviceToHost));
+        for (int64_t r = 0; r < nr; ++r) std::memcpy(host + (r0 + r) * host_stride, stage + r * stride, (size_t)used);
+    }
+}
+
 bool is_real_probe_dist(int d) {
     return d == GPE_RAND_GAUSSIAN || d == GPE_RAND_SPHERICAL || d == GPE_RAND_GAUSSIAN_F32 ||
            d == GPE_RAND_SPHERICAL_F32;
@@ -749,9 +764,7 @@ int gpe_geno_download_rows(gpe_geno* g, int32_t layout, int64_t row_begin, int64
         require(row_begin >= 0 && row_count >= 0 && row_begin + row_count <= rows, "row range out of bounds");
         require(host_stride >= used, "host_stride too small");
         if (row_count == 0) return;
-        GPE_HIP(hipDeviceSynchronize());
-        GPE_HIP(hipMemcpy2D(host, host_stride, src + row_begin * stride, stride, used, row_count,
-                            hipMemcpyDeviceToHost));
+        download_rows_staged(g, host, host_stride, src + row_begin * stride, stride, used, row_count);
     });
 }
 
@@ -775,8 +788,7 @@ int gpe_geno_download(gpe_geno* g, int32_t layout, uint8_t* host, int64_t host_s
             throw Error("unknown layout");
         }
         require(host_stride >= used, "host_stride too small");
-        GPE_HIP(hipDeviceSynchronize());
-        GPE_HIP(hipMemcpy2D(host, host_stride, src, stride, used, rows, hipMemcpyDeviceToHost));
+        download_rows_staged(g, host, host_stride, src, stride, used, rows);
     });
 }
 
