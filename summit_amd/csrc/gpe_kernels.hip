@@ -980,6 +980,15 @@ __global__ __launch_bounds__(64) void k_mt_rademacher(uint64_t root, const int64
     }
     __syncthreads();
     const int64_t total = L * (int64_t)v_count;
+    // output t = v L + r of this lane's slots i = lane + 64 q, advanced by kMtN per
+    // twist without a division
+    int32_t ov[5], orow[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        const int i = lane + 64 * q;
+        ov[q] = (int32_t)(i / L);
+        orow[q] = (int32_t)(i % L);
+    }
     for (int64_t base = 0; base < total; base += kMtN) {
         uint64_t nv[3];
         // phase A: i in [0, 156) reads old values only
@@ -1010,17 +1019,22 @@ __global__ __launch_bounds__(64) void k_mt_rademacher(uint64_t root, const int64
         __syncthreads();
         if (lane == 0) mt[kMtN - 1] = mt_twist_one(mt[kMtN - 1], mt[0], mt[kMtM - 1]);
         __syncthreads();
-        for (int i = lane; i < kMtN; i += 64) {
-            const int64_t t = base + i;
-            if (t >= total) break;
-            uint64_t y = mt[i];
-            y ^= (y >> 29) & 0x5555555555555555ULL;
-            y ^= (y << 17) & 0x71D67FFFEDA60000ULL;
-            y ^= (y << 37) & 0xFFF7EEE000000000ULL;
-            y ^= (y >> 43);
-            const int64_t v = t / L;
-            const int64_t r = t - v * L;
-            z[(s + r) * ldz + v] = (y & 1ULL) ? (int8_t)1 : (int8_t)-1;
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+            const int i = lane + 64 * q;
+            if (i < kMtN && ov[q] < v_count) {
+                uint64_t y = mt[i];
+                y ^= (y >> 29) & 0x5555555555555555ULL;
+                y ^= (y << 17) & 0x71D67FFFEDA60000ULL;
+                y ^= (y << 37) & 0xFFF7EEE000000000ULL;
+                y ^= (y >> 43);
+                z[(s + orow[q]) * ldz + ov[q]] = (y & 1ULL) ? (int8_t)1 : (int8_t)-1;
+            }
+            orow[q] += kMtN;
+            while (orow[q] >= L) {
+                orow[q] -= (int32_t)L;
+                ++ov[q];
+            }
         }
         __syncthreads();
     }
@@ -1070,6 +1084,26 @@ __global__ void k_coef_p1_colmax(const double* w, int64_t ldw, const int8_t* z_i
     if ((threadIdx.x & 63) == 0 && mx > 0.0)
         atomicMax(reinterpret_cast<unsigned long long*>(colmax_bits + c), (unsigned long long)__double_as_longlong(mx));
 }
+// int8 probes: every column of bin k gets max_i |w_ik| (see k_coef_p1_colmax); it
+// is reduced once per bin into column k * v_pad, then copied to the bin's columns
+__global__ void k_coef_p1_binmax(const double* w, int64_t ldw, int64_t k0, int64_t k1, int v_pad,
+                                 uint64_t* colmax_bits) {
+    const int k = blockIdx.y;
+    double mx = 0.0;
+    for (int64_t i = k0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < k1; i += (int64_t)gridDim.x * blockDim.x)
+        mx = fmax(mx, fabs(w[i * ldw + k]));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+    if ((threadIdx.x & 63) == 0 && mx > 0.0)
+        atomicMax(reinterpret_cast<unsigned long long*>(colmax_bits + (int64_t)k * v_pad),
+                  (unsigned long long)__double_as_longlong(mx));
+}
+__global__ void k_colmax_bin_copy(int n_bins, int v_count, int v_pad, uint64_t* colmax_bits) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n_bins * v_pad) return;
+    const int k = c / v_pad, v = c % v_pad;
+    if (v >= 1 && v < v_count) colmax_bits[c] = colmax_bits[(int64_t)k * v_pad];
+}
 void launch_coef_p1_colmax(const double* w, int64_t ldw, const int8_t* z_i8, const double* z_f64, int64_t ldz,
                            int64_t k0, int64_t k1, int n_bins, int v_count, int v_pad, uint64_t* colmax_bits,
                            hipStream_t st) {
@@ -1077,6 +1111,15 @@ void launch_coef_p1_colmax(const double* w, int64_t ldw, const int8_t* z_i8, con
     if (L <= 0) return;
     int64_t gx = ceil_div(L, 256);
     if (gx > 64) gx = 64;
+    if (z_i8) {
+        hipLaunchKernelGGL(k_coef_p1_binmax, dim3((unsigned)gx, (unsigned)n_bins), dim3(256), 0, st, w, ldw, k0, k1,
+                           v_pad, colmax_bits);
+        GPE_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_colmax_bin_copy, dim3((unsigned)ceil_div(n_bins * v_pad, 256)), dim3(256), 0, st, n_bins,
+                           v_count, v_pad, colmax_bits);
+        GPE_HIP(hipGetLastError());
+        return;
+    }
     hipLaunchKernelGGL(k_coef_p1_colmax, dim3((unsigned)gx, (unsigned)(n_bins * v_pad)), dim3(256), 0, st, w, ldw,
                        z_i8, z_f64, ldz, k0, k1, v_count, v_pad, colmax_bits);
     GPE_HIP(hipGetLastError());
@@ -1325,42 +1368,56 @@ void launch_p1_finalize(const int64_t* y, int64_t ldy, int64_t n, int n_bins, in
 // mean_i * deq(coef) is z_iv a_k bit for bit.
 __host__ __device__ constexpr int64_t miss_rec_bytes(int n_bins) { return ((16 + 8 * (int64_t)n_bins + 15) / 16) * 16; }
 
+// One thread per (SNP, 64-probe word): the word's 64 probe bytes in four
+// 16-byte loads (the probe rows are 32-byte aligned), sign bits of the first
+// v_count, then the bins' a_k.
 __global__ void k_p1_miss_rec(const double* w, int64_t ldw, const int8_t* z, int64_t ldz, int64_t k0, int64_t k1,
                               int n_bins, int v_count, int v_pad, const int32_t* shift, const double* col_scale,
                               const double* mean, uint8_t* rec) {
-    const int64_t i = k0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= k1) return;
     const int vw = (v_pad + 63) / 64;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (k1 - k0) * vw) return;
+    const int64_t i = k0 + t / vw;
+    const int wd = (int)(t % vw);
     const int64_t rb = miss_rec_bytes(n_bins);
-    for (int wd = 0; wd < vw; ++wd) {
-        uint64_t pos = 0, neg = 0;
-        for (int b = 0; b < 64; ++b) {
-            const int v = wd * 64 + b;
-            if (v >= v_count) break;
-            const int8_t zz = z[i * ldz + v];
-            if (zz > 0) pos |= 1ULL << b;
-            else if (zz < 0) neg |= 1ULL << b;
+    uint64_t pos = 0, neg = 0;
+    const int nb = v_count - wd * 64 < 64 ? v_count - wd * 64 : 64;  // probe columns of this word
+    const int8_t* zr = z + i * ldz + wd * 64;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (q * 16 >= nb) break;
+        const v4u d = *reinterpret_cast<const v4u*>(zr + 16 * q);
+#pragma unroll
+        for (int b = 0; b < 16; ++b) {
+            const int bit = q * 16 + b;
+            const int8_t zz = (int8_t)((d[b >> 2] >> (8 * (b & 3))) & 0xffu);
+            if (bit < nb) {
+                if (zz > 0) pos |= 1ULL << bit;
+                else if (zz < 0) neg |= 1ULL << bit;
+            }
         }
-        uint8_t* r = rec + ((i - k0) * vw + wd) * rb;
-        uint32_t* hdr = reinterpret_cast<uint32_t*>(r);
-        hdr[0] = (uint32_t)pos;
-        hdr[1] = (uint32_t)neg;
-        hdr[2] = (uint32_t)(pos >> 32);
-        hdr[3] = (uint32_t)(neg >> 32);
-        double* a = reinterpret_cast<double*>(r + 16);
-        for (int k = 0; k < n_bins; ++k) {
-            const int c = k * v_pad;
-            const double q = (double)__double2ll_rn(ldexp(w[i * ldw + k], shift[c]));
-            a[k] = mean[i] * (q * col_scale[c]);
-        }
-        for (int64_t b = 16 + 8 * (int64_t)n_bins; b < rb; ++b) r[b] = 0;
     }
+    uint8_t* r = rec + ((i - k0) * vw + wd) * rb;
+    uint32_t* hdr = reinterpret_cast<uint32_t*>(r);
+    hdr[0] = (uint32_t)pos;
+    hdr[1] = (uint32_t)neg;
+    hdr[2] = (uint32_t)(pos >> 32);
+    hdr[3] = (uint32_t)(neg >> 32);
+    double* a = reinterpret_cast<double*>(r + 16);
+    for (int k = 0; k < n_bins; ++k) {
+        const int c = k * v_pad;
+        const double q = (double)__double2ll_rn(ldexp(w[i * ldw + k], shift[c]));
+        a[k] = mean[i] * (q * col_scale[c]);
+    }
+    for (int64_t b = 16 + 8 * (int64_t)n_bins; b < rb; ++b) r[b] = 0;
 }
 void launch_p1_miss_rec(const double* w, int64_t ldw, const int8_t* z, int64_t ldz, int64_t k0, int64_t k1,
                         int n_bins, int v_count, int v_pad, const int32_t* shift, const double* col_scale,
                         const double* mean, void* rec, hipStream_t st) {
     if (k1 <= k0) return;
-    hipLaunchKernelGGL(k_p1_miss_rec, dim3((unsigned)ceil_div(k1 - k0, 256)), dim3(256), 0, st, w, ldw, z, ldz, k0,
+    if (ldz % 16 != 0) throw Error("probe rows must be 16-byte aligned");
+    const int64_t items = (k1 - k0) * ceil_div(v_pad, 64);
+    hipLaunchKernelGGL(k_p1_miss_rec, dim3((unsigned)ceil_div(items, 256)), dim3(256), 0, st, w, ldw, z, ldz, k0,
                        k1, n_bins, v_count, v_pad, shift, col_scale, mean, static_cast<uint8_t*>(rec));
     GPE_HIP(hipGetLastError());
 }
