@@ -330,7 +330,8 @@ enum {
     GPE_PROF_GZ = 2,          /* exact G.Z */
     GPE_PROF_GTU = 3,         /* exact G^T U */
     GPE_PROF_GRAM = 4,        /* banded Gram of windowed LD */
-    GPE_PROF_P1_FINALIZE = 5, /* phase-1 missing-call finalize (all SNP-range launches of one traversal) */
+    GPE_PROF_P1_FINALIZE = 5, /* phase-1 missing-call term: the records' finalize (all SNP-range launches of
+                                 one traversal) or the missing-flag plane (quantization, traversal, combine) */
     GPE_PROF_P2_MISS = 6,     /* phase-2 missing-call sums gathered over the per-SNP missing lists */
     GPE_PROF_KINDS = 7
 };
@@ -358,6 +359,10 @@ int gpe_profile_reset(void);
  *                               0.55% missing calls, else 1); results are identical
  *   GPE_SCHED_P2_MISS_PARTS:    sample ranges of the gathered sums (1, or a
  *                               multiple of 8 up to 64; 8 when no entry)
+ *   GPE_SCHED_P1_MISS:          phase-1 missing calls with int8 probes: 1
+ *                               per-call records, 2 a missing-flag MFMA plane
+ *                               (no entry: the plane above 2% missing calls, or
+ *                               above 0.4% with 8+ words of 64 probe columns)
  * Phase-1 keys: rows = samples, length = SNPs of the call, columns = bins x
  * probes rounded up to 32 per bin.
  * ------------------------------------------------------------------------ */
@@ -368,7 +373,8 @@ enum {
     GPE_SCHED_P2_CPB = 3,  /* the same for phase 2; key rows = SNPs, length = samples */
     GPE_SCHED_P2_MISS = 4, /* phase-2 missing-call method (phase-2 key) */
     GPE_SCHED_P2_MISS_PARTS = 5, /* sample ranges of the gathered missing-call sums (phase-2 key) */
-    GPE_SCHED_KNOBS = 6
+    GPE_SCHED_P1_MISS = 6, /* phase-1 missing-call method for int8 probes (phase-1 key) */
+    GPE_SCHED_KNOBS = 7
 };
 int gpe_schedule_set(int32_t knob, int64_t rows, int64_t length, int32_t columns, int32_t limbs, int32_t value);
 /* The value in force for a key (0 = the heuristic decides). */

@@ -32,8 +32,10 @@ SCHED_P1_CPB = 2
 SCHED_P2_CPB = 3
 SCHED_P2_MISS = 4
 SCHED_P2_MISS_PARTS = 5
+SCHED_P1_MISS = 6
 _KNOBS = {"p1_splits": SCHED_P1_SPLITS, "p1_finalize_kib": SCHED_P1_FINALIZE_KIB, "p1_cpb": SCHED_P1_CPB,
-          "p2_cpb": SCHED_P2_CPB, "p2_miss": SCHED_P2_MISS, "p2_miss_parts": SCHED_P2_MISS_PARTS}
+          "p2_cpb": SCHED_P2_CPB, "p2_miss": SCHED_P2_MISS, "p2_miss_parts": SCHED_P2_MISS_PARTS,
+          "p1_miss": SCHED_P1_MISS}
 
 
 def cache_path() -> str:
@@ -102,13 +104,15 @@ def apply_cached(n: int, length: int, n_bins: int, v_count: int, limbs: int = 5,
 
 
 def _time_phase1(g, L, annot, inv, v_count, seed, xz, limbs, reps, kind) -> float:
-    """Median milliseconds of one kernel kind over reps phase-1 traversals."""
+    """Median milliseconds of one kernel kind (or the sum of a tuple of kinds)
+    over reps phase-1 traversals."""
+    kinds = kind if isinstance(kind, tuple) else (kind,)
     times = []
     for r in range(reps):
         engine.profile_enable(True)
         engine.profile_reset()
         engine.phase1_xz(g, 0, L, 1000, annot, inv, r * v_count, v_count, seed, xz, limbs=limbs)
-        ms, n = engine.profile_read(kind)
+        ms = sum(engine.profile_read(k)[0] for k in kinds)
         engine.profile_enable(False)
         times.append(ms)
     times.sort()
@@ -153,6 +157,18 @@ def tune_phase1(g: engine.Genotype, v_count: int, n_bins: int = 1, limbs: int = 
         set_value("p1_finalize_kib", key, best_range)
     else:
         best_range = None
+    # missing calls: per-call records (best range) against the missing-flag plane, each
+    # timed as contraction + missing-call term
+    method_ms = {}
+    best_method = None
+    if g.n_missing() > 0:
+        for method in (1, 2):
+            set_value("p1_miss", key, method)
+            engine.phase1_xz(g, 0, L, 1000, annot, inv, 0, v_count, seed, xz, limbs=limbs)  # workspaces
+            method_ms[method] = round(
+                _time_phase1(g, L, annot, inv, v_count, seed, xz, limbs, reps, ("p1_contract", "p1_finalize")), 4)
+        best_method = min(method_ms, key=method_ms.get)
+        set_value("p1_miss", key, best_method)
     # chunks per LDS slot and barrier of the phase-1 contraction
     cpb_ms = {}
     for cpb in (1, 2):
@@ -164,8 +180,10 @@ def tune_phase1(g: engine.Genotype, v_count: int, n_bins: int = 1, limbs: int = 
     choice = {"p1_splits": best_split, "p1_cpb": best_cpb}
     if best_range is not None:
         choice["p1_finalize_kib"] = best_range
+    if best_method is not None:
+        choice["p1_miss"] = best_method
     entry = {"choice": choice, "heuristic_ms": heuristic, "p1_contract_ms": split_ms, "p1_finalize_ms": range_ms,
-             "p1_cpb_ms": cpb_ms,
+             "p1_miss_ms": method_ms, "p1_cpb_ms": cpb_ms,
              "tune_seconds": round(time.perf_counter() - t0, 3)}
     if store:
         _store_cache(path or cache_path(), _entry_key(key), entry)

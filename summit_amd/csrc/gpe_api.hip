@@ -494,7 +494,8 @@ int gpe_schedule_set(int32_t knob, int64_t rows, int64_t length, int32_t columns
             require(value <= 0 || (value >= 64 && value <= 65536), "finalize range must be in [64, 65536] KiB");
         if (knob == GPE_SCHED_P1_CPB || knob == GPE_SCHED_P2_CPB)
             require(value <= 2, "chunks per barrier must be 1 or 2");
-        if (knob == GPE_SCHED_P2_MISS) require(value <= 2, "phase-2 missing-call method must be 1 or 2");
+        if (knob == GPE_SCHED_P2_MISS || knob == GPE_SCHED_P1_MISS)
+            require(value <= 2, "missing-call method must be 1 or 2");
         if (knob == GPE_SCHED_P2_MISS_PARTS)
             require(value <= 0 || value == 1 || (value % 8 == 0 && value <= 64),
                     "missing-call sample ranges must be 1 or a multiple of 8 up to 64");
@@ -993,7 +994,17 @@ void run_phase1(gpe_geno* g, int64_t s, int64_t e, const double* w, int n_bins, 
     launch_contract(lc.nl, lc.ctc, false, EPI_I64_ATOMIC, a, groups, splits, st, cpb);
     ps.stop();
     const bool miss = g->impute_mode == GPE_IMPUTE_MEAN && g->n_missing > 0;
-    if (miss && z_i8) {
+    // missing calls with int8 probes: per-call records summed by k_p1_finalize_z8, or the
+    // missing-flag plane as a second traversal (always for real-valued probes).  The records
+    // cost grows with missing calls x probe words, the plane with N x SNPs x probe words:
+    // measured, records win at N=100,000, M=500,000, 64 probes, 0.5% missing (3.3 ms vs
+    // 11.7 ms), the plane at N=5,000, M=10,000, 8,192 probes, 1% missing (1.6 ms vs 3.0 ms)
+    const int p1m = sched_lookup(GPE_SCHED_P1_MISS, g->n, L, ncols_pad, lc.nl);
+    const double density = (double)g->n_missing / ((double)g->n * (double)g->m);
+    const int words = ncols / 64 > 0 ? ncols / 64 : 1;
+    const bool plane = miss && (z_f64 != nullptr || p1m == 2 ||
+                                (p1m != 1 && (density > 0.02 || (words >= 8 && density > 0.004))));
+    if (miss && z_i8 && !plane) {
         void* rec = g->scratch("p1_mrec", p1_miss_rec_bytes(L, n_bins, v_pad));
         launch_p1_miss_rec(w, n_bins, z_i8, ldz, s, e, n_bins, v_count, v_pad, shift, col_scale, g->mean, rec, st);
         const int32_t kib = sched_lookup(GPE_SCHED_P1_FINALIZE_KIB, g->n, L, ncols_pad, lc.nl);
@@ -1003,31 +1014,31 @@ void run_phase1(gpe_geno* g, int64_t s, int64_t e, const double* w, int n_bins, 
         pf.stop();
         return;
     }
-    if (miss && z_f64) {
-        // real-valued probes: the missing-call term sum_{i missing at n} mu_i c_iv as a second
-        // int8-MFMA traversal, the missing-flag plane against the quantized mu_i w_ik z_iv (the
-        // per-call records of int8 probes would carry 64 FP64 values per call here)
+    if (plane) {
+        // the missing-call term sum_{i missing at n} mu_i c_iv as a second int8-MFMA traversal:
+        // the missing-flag plane against the quantized mu_i w_ik z_iv (for real-valued probes a
+        // per-call record would carry 64 FP64 values per call)
+        ProfScope pf(GPE_PROF_P1_FINALIZE, st);
         double* w2 = static_cast<double*>(g->scratch("p1_w2", g->m_alloc * n_bins * sizeof(double)));
         launch_scale_rows(w, g->mean, s, e, n_bins, w2, st);
         uint64_t* colmax2 = static_cast<uint64_t*>(g->scratch("colmax2", ncols_pad * sizeof(uint64_t)));
         int32_t* shift2 = static_cast<int32_t*>(g->scratch("shift2", ncols_pad * sizeof(int32_t)));
         double* cs2 = static_cast<double*>(g->scratch("col_scale2", ncols_pad * sizeof(double)));
-        double* cst2 = static_cast<double*>(g->scratch("cst2", ncols_pad * sizeof(double)));
-        double* cst_part2 =
-            static_cast<double*>(g->scratch("cst_part2", (size_t)ncols_pad * kCstParts * sizeof(double)));
         launch_fill_u64(colmax2, ncols_pad, 0, st);
-        launch_coef_p1_colmax(w2, n_bins, nullptr, z_f64, ldz, s, e, n_bins, v_count, v_pad, colmax2, st);
+        launch_coef_p1_colmax(w2, n_bins, z_i8, z_f64, ldz, s, e, n_bins, v_count, v_pad, colmax2, st);
         launch_shift_from_colmax(colmax2, ncols_pad, prec, shift2, cs2, st);
         // the main contraction has consumed the coefficient buffer (stream order); reuse it
         if (ncols_pad != ncols) launch_fill_u8(reinterpret_cast<uint8_t*>(coef), kcount * coef_ctiles * 4096, 0, st);
-        launch_coef_p1_quant(w2, n_bins, nullptr, z_f64, ldz, s, e, n_bins, v_count, v_pad, shift2, lc.nl, lc.ctc,
-                             coef_ctiles, kc0, coef, g->mean, cst2, cst_part2, st);
+        // (no Cst: the flag plane has no mean-centring term)
+        launch_coef_p1_quant(w2, n_bins, z_i8, z_f64, ldz, s, e, n_bins, v_count, v_pad, shift2, lc.nl, lc.ctc,
+                             coef_ctiles, kc0, coef, g->mean, nullptr, nullptr, st);
         int64_t* y2 = static_cast<int64_t*>(g->scratch("p1_y2", (int64_t)ncols_pad * ldy * sizeof(int64_t)));
         launch_fill_u64(reinterpret_cast<uint64_t*>(y2), (int64_t)ncols_pad * ldy, 0, st);
         ContractArgs a2 = a;
         a2.out_i64 = y2;
         launch_contract(lc.nl, lc.ctc, false, EPI_I64_ATOMIC_Q, a2, groups, splits, st, cpb);
         launch_p1_combine2(y, y2, ldy, g->n, n_bins, v_count, v_pad, col_scale, cst, cs2, out, ldo, bin_mode, st);
+        pf.stop();
         return;
     }
     launch_p1_finalize(y, ldy, g->n, n_bins, v_count, v_pad, col_scale, cst, miss ? g->miss_ptr : nullptr,

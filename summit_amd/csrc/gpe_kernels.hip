@@ -1148,28 +1148,46 @@ void launch_shift_from_colmax(const uint64_t* colmax_bits, int n, int precision_
     GPE_HIP(hipGetLastError());
 }
 
+// Swizzled coefficient limbs of phase 1: thread (group g of 16 consecutive SNPs
+// of the chunk range, column c) quantizes q = round(w_ik z_iv 2^shift_c) for the
+// 16 SNPs (0 outside [k0, k1) or past v_count), places each limb's 16 digits at
+// the transposed byte positions of one 16-byte operand unit and stores the unit
+// whole; consecutive threads are consecutive columns (coalesced probe reads and
+// unit stores).
 template <int NL>
-__global__ void k_coef_p1_quant(const double* w, int64_t ldw, const int8_t* z_i8, const double* z_f64, int64_t ldz,
-                                int64_t k0, int64_t k1, int v_count, int v_pad, const int32_t* shift, int ctc,
-                                int64_t coef_ctiles, int64_t kc0, int64_t kcount, int8_t* coef) {
-    // x: SNP offset inside the chunk range, y: column
-    const int64_t kk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int c = blockIdx.y;
-    if (kk >= kcount * kKChunk) return;
-    const int64_t i = kc0 * kKChunk + kk;
+__global__ __launch_bounds__(256) void k_coef_p1_quant16(const double* w, int64_t ldw, const int8_t* z_i8,
+                                                         const double* z_f64, int64_t ldz, int64_t k0, int64_t k1,
+                                                         int v_count, int v_pad, int ncols, const int32_t* shift,
+                                                         int ctc, int64_t coef_ctiles, int64_t kc0, int8_t* coef) {
+    const int64_t grp = blockIdx.x;
+    const int c = (int)blockIdx.y * 256 + (int)threadIdx.x;
+    if (c >= ncols) return;
     const int k = c / v_pad, v = c % v_pad;
-    int64_t q = 0;
-    if (v < v_count && i >= k0 && i < k1) q = __double2ll_rn(ldexp(p1_coef(w, ldw, z_i8, z_f64, ldz, i, k, v), shift[c]));
-    int8_t d[NL];
-    balanced_digits<NL>(q, d);
+    const int sh = shift[c];
+    uint32_t unit[NL][4];
+#pragma unroll
+    for (int l = 0; l < NL; ++l) unit[l][0] = unit[l][1] = unit[l][2] = unit[l][3] = 0u;
+    const int64_t i0 = kc0 * kKChunk + grp * 16;
+#pragma unroll 4
+    for (int j = 0; j < 16; ++j) {
+        const int64_t i = i0 + j;
+        int64_t q = 0;
+        if (v < v_count && i >= k0 && i < k1) q = __double2ll_rn(ldexp(p1_coef(w, ldw, z_i8, z_f64, ldz, i, k, v), sh));
+        int8_t d[NL];
+        balanced_digits<NL>(q, d);
+        const int b = ((j & 3) << 2) | (j >> 2);  // the unpacked operand's byte order (swz_offset)
+#pragma unroll
+        for (int l = 0; l < NL; ++l) unit[l][b >> 2] |= (uint32_t)(uint8_t)d[l] << (8 * (b & 3));
+    }
+    const int64_t kl = grp / (kKChunk / 16);
+    const int kk = (int)(grp % (kKChunk / 16)) * 16;
     const int ct = c >> 5, lr = c & 31;
     const int cgp = ct / ctc, ctcl = ct % ctc;
-    const int64_t kl = kk / kKChunk;
-    const int kin = (int)(kk % kKChunk);
 #pragma unroll
     for (int l = 0; l < NL; ++l) {
         const int64_t tg = (int64_t)cgp * (NL * ctc) + l * ctc + ctcl;
-        coef[swz_offset(kl, coef_ctiles, tg, kin, lr)] = d[l];
+        *reinterpret_cast<v4u*>(coef + swz_offset(kl, coef_ctiles, tg, kk, lr)) =
+            v4u{unit[l][0], unit[l][1], unit[l][2], unit[l][3]};
     }
 }
 
@@ -1216,13 +1234,15 @@ void launch_coef_p1_quant(const double* w, int64_t ldw, const int8_t* z_i8, cons
     const int64_t kcount = kc1 - kc0;
     const int ncols = n_bins * v_pad;
     if (kcount > 0) {
-        dim3 grid((unsigned)ceil_div(kcount * kKChunk, 256), (unsigned)ncols);
+        // one thread per (16 consecutive SNPs, column): the 16 values of one 16-byte
+        // operand unit per limb, written with one vector store each
+        const dim3 grid((unsigned)(kcount * (kKChunk / 16)), (unsigned)ceil_div(ncols, 256));
         if (nl == 5)
-            hipLaunchKernelGGL(k_coef_p1_quant<5>, grid, dim3(256), 0, st, w, ldw, z_i8, z_f64, ldz, k0, k1, v_count,
-                               v_pad, shift, ctc, coef_ctiles, kc0, kcount, coef);
+            hipLaunchKernelGGL(k_coef_p1_quant16<5>, grid, dim3(256), 0, st, w, ldw, z_i8, z_f64, ldz, k0, k1,
+                               v_count, v_pad, ncols, shift, ctc, coef_ctiles, kc0, coef);
         else if (nl == 6)
-            hipLaunchKernelGGL(k_coef_p1_quant<6>, grid, dim3(256), 0, st, w, ldw, z_i8, z_f64, ldz, k0, k1, v_count,
-                               v_pad, shift, ctc, coef_ctiles, kc0, kcount, coef);
+            hipLaunchKernelGGL(k_coef_p1_quant16<6>, grid, dim3(256), 0, st, w, ldw, z_i8, z_f64, ldz, k0, k1,
+                               v_count, v_pad, ncols, shift, ctc, coef_ctiles, kc0, coef);
         else
             throw Error("phase-1 coefficient limbs must be 5 or 6");
         GPE_HIP(hipGetLastError());
@@ -1607,42 +1627,63 @@ void launch_colmax_f64(const double* x, int64_t ldx, int64_t n, int n_cols, int 
 }
 
 // Phase-2 operand: limbs of round(Xz * 2^shift) over samples, plus exact column sums.
+// Swizzled limbs of the phase-2 operand x (row c' = bin * v_count + v of length n,
+// leading dimension ldx): thread (group g of 16 consecutive samples, column
+// c = bin * v_pad + v) quantizes q = round(x 2^shift_c) for its 16 samples (0
+// past n or v_count; one 128-byte row segment per thread), adds them to the
+// column's exact sum, and stores each limb's 16 digits as one 16-byte operand
+// unit in the transposed byte order of the unpacked genotype operand.
 template <int NL>
-__global__ void k_quant_p2(const double* x, int64_t ldx, int64_t n, int v_count, int v_pad, const int32_t* shift,
-                           int ctc, int64_t coef_ctiles, int64_t kcount, int8_t* coef, int64_t* col_sum) {
-    const int64_t kk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int c = blockIdx.y;
+__global__ __launch_bounds__(256) void k_quant_p2_16(const double* x, int64_t ldx, int64_t n, int v_count, int v_pad,
+                                                     int ncols, const int32_t* shift, int ctc, int64_t coef_ctiles,
+                                                     int8_t* coef, int64_t* col_sum) {
+    const int64_t grp = blockIdx.x;
+    const int c = (int)blockIdx.y * 256 + (int)threadIdx.x;
+    if (c >= ncols) return;
     const int k = c / v_pad, v = c % v_pad;
-    int64_t q = 0;
-    if (kk < kcount * kKChunk && v < v_count && kk < n)
-        q = __double2ll_rn(ldexp(x[(int64_t)(k * v_count + v) * ldx + kk], shift[c]));
-    const int64_t qs = wave_sum_i64(q);
-    if ((threadIdx.x & 63) == 0 && qs != 0)
-        atomicAdd(reinterpret_cast<unsigned long long*>(col_sum + c), (unsigned long long)qs);
-    if (kk >= kcount * kKChunk) return;
-    int8_t d[NL];
-    balanced_digits<NL>(q, d);
+    const int sh = shift[c];
+    const double* xr = x + (int64_t)(k * v_count + v) * ldx;
+    uint32_t unit[NL][4];
+#pragma unroll
+    for (int l = 0; l < NL; ++l) unit[l][0] = unit[l][1] = unit[l][2] = unit[l][3] = 0u;
+    int64_t qs = 0;
+    const int64_t s0 = grp * 16;
+#pragma unroll 4
+    for (int j = 0; j < 16; ++j) {
+        const int64_t smp = s0 + j;
+        int64_t q = 0;
+        if (v < v_count && smp < n) q = __double2ll_rn(ldexp(xr[smp], sh));
+        qs += q;
+        int8_t d[NL];
+        balanced_digits<NL>(q, d);
+        const int b = ((j & 3) << 2) | (j >> 2);  // the unpacked operand's byte order (swz_offset)
+#pragma unroll
+        for (int l = 0; l < NL; ++l) unit[l][b >> 2] |= (uint32_t)(uint8_t)d[l] << (8 * (b & 3));
+    }
+    if (qs != 0) atomicAdd(reinterpret_cast<unsigned long long*>(col_sum + c), (unsigned long long)qs);
+    const int64_t kl = grp / (kKChunk / 16);
+    const int kk = (int)(grp % (kKChunk / 16)) * 16;
     const int ct = c >> 5, lr = c & 31;
     const int cgp = ct / ctc, ctcl = ct % ctc;
-    const int64_t kl = kk / kKChunk;
-    const int kin = (int)(kk % kKChunk);
 #pragma unroll
     for (int l = 0; l < NL; ++l) {
         const int64_t tg = (int64_t)cgp * (NL * ctc) + l * ctc + ctcl;
-        coef[swz_offset(kl, coef_ctiles, tg, kin, lr)] = d[l];
+        *reinterpret_cast<v4u*>(coef + swz_offset(kl, coef_ctiles, tg, kk, lr)) =
+            v4u{unit[l][0], unit[l][1], unit[l][2], unit[l][3]};
     }
 }
 void launch_quant_p2(const double* x, int64_t ldx, int64_t n, int n_bins, int v_count, int v_pad,
                      const int32_t* shift, int nl, int ctc, int64_t coef_ctiles, int8_t* coef, int64_t* col_sum,
                      hipStream_t st) {
     const int64_t kcount = ceil_div(n, kKChunk);
-    dim3 grid((unsigned)ceil_div(kcount * kKChunk, 256), (unsigned)(n_bins * v_pad));
+    const int ncols = n_bins * v_pad;
+    const dim3 grid((unsigned)(kcount * (kKChunk / 16)), (unsigned)ceil_div(ncols, 256));
     if (nl == 5)
-        hipLaunchKernelGGL(k_quant_p2<5>, grid, dim3(256), 0, st, x, ldx, n, v_count, v_pad, shift, ctc, coef_ctiles,
-                           kcount, coef, col_sum);
+        hipLaunchKernelGGL(k_quant_p2_16<5>, grid, dim3(256), 0, st, x, ldx, n, v_count, v_pad, ncols, shift, ctc,
+                           coef_ctiles, coef, col_sum);
     else if (nl == 6)
-        hipLaunchKernelGGL(k_quant_p2<6>, grid, dim3(256), 0, st, x, ldx, n, v_count, v_pad, shift, ctc, coef_ctiles,
-                           kcount, coef, col_sum);
+        hipLaunchKernelGGL(k_quant_p2_16<6>, grid, dim3(256), 0, st, x, ldx, n, v_count, v_pad, ncols, shift, ctc,
+                           coef_ctiles, coef, col_sum);
     else
         throw Error("phase-2 operand limbs must be 5 or 6");
     GPE_HIP(hipGetLastError());

@@ -184,9 +184,13 @@ def _phase1_ref(oracle, codes, annot, inv, seed, v_start, v, step, mode="mean", 
 
 
 @pytest.mark.parametrize("mode", ["mean", "hwe"])
-def test_phase1_matches_oracle(engine, oracle, mode, K=3, V=7):
-    _say(f"phase1 {mode} bins={K} probes={V}")
+def test_phase1_matches_oracle(engine, oracle, mode, K=3, V=7, method=0):
+    """method: missing-call term from per-call records (1), the missing-flag
+    plane (2), or the built-in rule (0)."""
+    from summit_amd import autotune
+    _say(f"phase1 {mode} bins={K} probes={V} method={method}")
     m, n, step = 700, 333, 100
+    autotune.set_value("p1_miss", autotune.shape_key(n, m, K, V, 5), method)
     codes = random_codes(m, n, seed=11, miss_rate=0.03, monomorphic=2)
     rng = np.random.default_rng(3)
     annot = (rng.random((m, K)) < 0.6).astype(np.float64) * rng.uniform(0.5, 2.0, size=(m, K))
@@ -201,19 +205,28 @@ def test_phase1_matches_oracle(engine, oracle, mode, K=3, V=7):
     ref = _phase1_ref(oracle, codes, annot, inv, 777, 5, V, step, mode, iseed)
     scale = np.abs(ref).max(axis=0, keepdims=True)
     err = np.abs(got - ref) / scale
+    autotune.set_value("p1_miss", autotune.shape_key(n, m, K, V, 5), 0)
     assert err.max() < 1e-11, err.max()
     g.close()
+
+
+@pytest.mark.parametrize("method", [1, 2])
+def test_phase1_missing_methods_match_oracle(engine, oracle, method):
+    """Per-call records and the missing-flag plane, each against the oracle,
+    with several bins and 600 probes (ten 64-column words, padded columns)."""
+    test_phase1_matches_oracle(engine, oracle, "mean", 3, 7, method)
+    test_phase1_matches_oracle(engine, oracle, "mean", 2, 600, method)
 
 
 @pytest.mark.parametrize("K", [7, 17])
 def test_phase1_many_bins_match_oracle(engine, oracle, K):
     """Bin counts the missing-call finalize splits into groups (7 = 4 + 2 + 1, 17 = 16 + 1)."""
-    test_phase1_matches_oracle(engine, oracle, "mean", K)
+    test_phase1_matches_oracle(engine, oracle, "mean", K, method=1)
 
 
 def test_phase1_wide_probe_tile_matches_oracle(engine, oracle):
     """100 probes: two 64-column sign words per missing-call record, padded columns."""
-    test_phase1_matches_oracle(engine, oracle, "mean", 2, 100)
+    test_phase1_matches_oracle(engine, oracle, "mean", 2, 100, method=1)
 
 
 @pytest.mark.parametrize("mode,limbs,rtol", [("mean", 5, 5e-11), ("hwe", 5, 5e-11), ("mean", 6, 1e-12)])
