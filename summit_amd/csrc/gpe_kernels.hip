@@ -1628,31 +1628,42 @@ void launch_colmax_f64(const double* x, int64_t ldx, int64_t n, int n_cols, int 
 
 // Phase-2 operand: limbs of round(Xz * 2^shift) over samples, plus exact column sums.
 // Swizzled limbs of the phase-2 operand x (row c' = bin * v_count + v of length n,
-// leading dimension ldx): thread (group g of 16 consecutive samples, column
-// c = bin * v_pad + v) quantizes q = round(x 2^shift_c) for its 16 samples (0
-// past n or v_count; one 128-byte row segment per thread), adds them to the
-// column's exact sum, and stores each limb's 16 digits as one 16-byte operand
-// unit in the transposed byte order of the unpacked genotype operand.
+// leading dimension ldx).  One workgroup per 64-column x 64-sample tile: the
+// quantized values q = round(x 2^shift_c) (0 past n or v_count) are read with
+// threads over samples (coalesced) into LDS; then thread (column, group of 16
+// samples) adds its 16 values to the column's exact sum and stores each limb's
+// 16 digits as one 16-byte operand unit in the transposed byte order of the
+// unpacked genotype operand (threads over columns: consecutive units).
 template <int NL>
-__global__ __launch_bounds__(256) void k_quant_p2_16(const double* x, int64_t ldx, int64_t n, int v_count, int v_pad,
-                                                     int ncols, const int32_t* shift, int ctc, int64_t coef_ctiles,
-                                                     int8_t* coef, int64_t* col_sum) {
-    const int64_t grp = blockIdx.x;
-    const int c = (int)blockIdx.y * 256 + (int)threadIdx.x;
+__global__ __launch_bounds__(256) void k_quant_p2_t(const double* x, int64_t ldx, int64_t n, int v_count, int v_pad,
+                                                    int ncols, const int32_t* shift, int ctc, int64_t coef_ctiles,
+                                                    int8_t* coef, int64_t* col_sum) {
+    __shared__ int64_t qt[64][65];
+    const int tid = threadIdx.x;
+    const int64_t n0 = (int64_t)blockIdx.x * 64;
+    const int c0 = (int)blockIdx.y * 64;
+    for (int idx = tid; idx < 64 * 64; idx += 256) {
+        const int cl = idx >> 6, sl = idx & 63;
+        const int c = c0 + cl;
+        const int64_t smp = n0 + sl;
+        int64_t q = 0;
+        if (c < ncols && smp < n) {
+            const int k = c / v_pad, v = c % v_pad;
+            if (v < v_count) q = __double2ll_rn(ldexp(x[(int64_t)(k * v_count + v) * ldx + smp], shift[c]));
+        }
+        qt[cl][sl] = q;
+    }
+    __syncthreads();
+    const int cl = tid & 63, gq = tid >> 6;
+    const int c = c0 + cl;
     if (c >= ncols) return;
-    const int k = c / v_pad, v = c % v_pad;
-    const int sh = shift[c];
-    const double* xr = x + (int64_t)(k * v_count + v) * ldx;
     uint32_t unit[NL][4];
 #pragma unroll
     for (int l = 0; l < NL; ++l) unit[l][0] = unit[l][1] = unit[l][2] = unit[l][3] = 0u;
     int64_t qs = 0;
-    const int64_t s0 = grp * 16;
 #pragma unroll 4
     for (int j = 0; j < 16; ++j) {
-        const int64_t smp = s0 + j;
-        int64_t q = 0;
-        if (v < v_count && smp < n) q = __double2ll_rn(ldexp(xr[smp], sh));
+        const int64_t q = qt[cl][gq * 16 + j];
         qs += q;
         int8_t d[NL];
         balanced_digits<NL>(q, d);
@@ -1661,8 +1672,9 @@ __global__ __launch_bounds__(256) void k_quant_p2_16(const double* x, int64_t ld
         for (int l = 0; l < NL; ++l) unit[l][b >> 2] |= (uint32_t)(uint8_t)d[l] << (8 * (b & 3));
     }
     if (qs != 0) atomicAdd(reinterpret_cast<unsigned long long*>(col_sum + c), (unsigned long long)qs);
-    const int64_t kl = grp / (kKChunk / 16);
-    const int kk = (int)(grp % (kKChunk / 16)) * 16;
+    const int64_t s16 = n0 + gq * 16;  // first sample of the unit
+    const int64_t kl = s16 / kKChunk;
+    const int kk = (int)(s16 % kKChunk);
     const int ct = c >> 5, lr = c & 31;
     const int cgp = ct / ctc, ctcl = ct % ctc;
 #pragma unroll
@@ -1677,12 +1689,12 @@ void launch_quant_p2(const double* x, int64_t ldx, int64_t n, int n_bins, int v_
                      hipStream_t st) {
     const int64_t kcount = ceil_div(n, kKChunk);
     const int ncols = n_bins * v_pad;
-    const dim3 grid((unsigned)(kcount * (kKChunk / 16)), (unsigned)ceil_div(ncols, 256));
+    const dim3 grid((unsigned)(kcount * (kKChunk / 64)), (unsigned)ceil_div(ncols, 64));
     if (nl == 5)
-        hipLaunchKernelGGL(k_quant_p2_16<5>, grid, dim3(256), 0, st, x, ldx, n, v_count, v_pad, ncols, shift, ctc,
+        hipLaunchKernelGGL(k_quant_p2_t<5>, grid, dim3(256), 0, st, x, ldx, n, v_count, v_pad, ncols, shift, ctc,
                            coef_ctiles, coef, col_sum);
     else if (nl == 6)
-        hipLaunchKernelGGL(k_quant_p2_16<6>, grid, dim3(256), 0, st, x, ldx, n, v_count, v_pad, ncols, shift, ctc,
+        hipLaunchKernelGGL(k_quant_p2_t<6>, grid, dim3(256), 0, st, x, ldx, n, v_count, v_pad, ncols, shift, ctc,
                            coef_ctiles, coef, col_sum);
     else
         throw Error("phase-2 operand limbs must be 5 or 6");

@@ -114,6 +114,18 @@ def test_probe_streams(engine, oracle, dist, fp32):
     np.testing.assert_array_equal(z, ref)
 
 
+def test_rademacher_long_block_streams(engine, oracle):
+    """Long and short SNP blocks in one launch (13,000 / 500 / 500 SNPs x 40
+    probes): each stream's output index is advanced without divisions."""
+    _say("probes rademacher long blocks")
+    starts = [0, 13000, 13500]
+    end = 14000
+    z = engine.probes(1101, starts, end, v_start=64, v_count=40, rand_dist="rademacher")
+    ref = np.concatenate([oracle.probe_block(1101, s, 64, e - s, 40, "rademacher")
+                          for s, e in zip(starts, starts[1:] + [end])])
+    np.testing.assert_array_equal(z, ref)
+
+
 @pytest.mark.parametrize("fp32", [False, True])
 def test_phase1_gaussian_probes_many_blocks_match_oracle(engine, oracle, fp32):
     """Phase 1 with Gaussian probes over enough SNP blocks that the host draws
