@@ -361,8 +361,10 @@ int gpe_profile_reset(void);
  *                               multiple of 8 up to 64; 8 when no entry)
  *   GPE_SCHED_P1_MISS:          phase-1 missing calls with int8 probes: 1
  *                               per-call records, 2 a missing-flag MFMA plane
- *                               (no entry: the plane above 2% missing calls, or
- *                               above 0.4% with 8+ words of 64 probe columns)
+ *                               (no entry: a cost model of missing-call density
+ *                               d and the call's SNP count L: the plane when
+ *                               L (0.248 - 12.2 d) ps < 2.52 ns, e.g. for
+ *                               L = 10,000 at 1%, records for L = 500,000 at 0.5%)
  * Phase-1 keys: rows = samples, length = SNPs of the call, columns = bins x
  * probes rounded up to 32 per bin.
  * ------------------------------------------------------------------------ */

@@ -995,15 +995,16 @@ void run_phase1(gpe_geno* g, int64_t s, int64_t e, const double* w, int n_bins, 
     ps.stop();
     const bool miss = g->impute_mode == GPE_IMPUTE_MEAN && g->n_missing > 0;
     // missing calls with int8 probes: per-call records summed by k_p1_finalize_z8, or the
-    // missing-flag plane as a second traversal (always for real-valued probes).  The records
-    // cost grows with missing calls x probe words, the plane with N x SNPs x probe words:
-    // measured, records win at N=100,000, M=500,000, 64 probes, 0.5% missing (3.3 ms vs
-    // 11.7 ms), the plane at N=5,000, M=10,000, 8,192 probes, 1% missing (1.6 ms vs 3.0 ms)
+    // missing-flag plane as a second traversal (always for real-valued probes).  Per sample
+    // and 64-probe word the records cost about 3.48 ns + 12.2 ps per missing call, the plane
+    // (quantization, traversal, combine) about 0.96 ns + 0.248 ps per SNP -- fitted to
+    // N=100,000, M=500,000, 64 probes, 0.5% missing (records 3.4 ms, plane 12.5 ms) and
+    // N=5,000, M=10,000, 8,192 probes, 1% missing (records 3.0 ms, plane 2.2 ms) -- so the
+    // plane wins when L (0.248 ps - 12.2 ps d) < 2.52 ns for L SNPs at missing density d
     const int p1m = sched_lookup(GPE_SCHED_P1_MISS, g->n, L, ncols_pad, lc.nl);
     const double density = (double)g->n_missing / ((double)g->n * (double)g->m);
-    const int words = ncols / 64 > 0 ? ncols / 64 : 1;
-    const bool plane = miss && (z_f64 != nullptr || p1m == 2 ||
-                                (p1m != 1 && (density > 0.02 || (words >= 8 && density > 0.004))));
+    const bool plane_cheaper = (double)L * (0.248e-12 - 12.2e-12 * density) < 2.52e-9;
+    const bool plane = miss && (z_f64 != nullptr || p1m == 2 || (p1m != 1 && plane_cheaper));
     if (miss && z_i8 && !plane) {
         void* rec = g->scratch("p1_mrec", p1_miss_rec_bytes(L, n_bins, v_pad));
         launch_p1_miss_rec(w, n_bins, z_i8, ldz, s, e, n_bins, v_count, v_pad, shift, col_scale, g->mean, rec, st);

@@ -31,7 +31,15 @@ def test_tune_phase1_caches_and_keeps_results():
                                  path=path)
     assert entry["choice"]["p1_splits"] in (1, 2, 7)
     assert entry["choice"]["p1_finalize_kib"] in (64, 4096)
+    assert entry["choice"]["p1_miss"] in (1, 2)
     assert autotune.get_value("p1_splits", key) == entry["choice"]["p1_splits"]
+    # splits and finalize ranges keep the results of one missing-call method to 1e-13
+    # (the two methods quantize the missing-call term separately: ~1e-12 apart)
+    autotune.set_value("p1_miss", key, 1)
+    autotune.set_value("p1_splits", key, 0)
+    autotune.set_value("p1_finalize_kib", key, 0)
+    ref.zero_()
+    engine.phase1_xz(g, 0, M, 1000, annot, inv, 0, V, 77, ref)
     for s in (1, 2, 7):
         for kib in (64, 4096):
             autotune.set_value("p1_splits", key, s)
