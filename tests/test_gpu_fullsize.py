@@ -6,7 +6,9 @@ no oracle finishes there.
   SNP sums s_i of the statistics; the SNP-sum of W = G^T U equals
   sum_n U1_n U_bn with U1 = G.1; G.(Z1 - Z2) = G.Z1 - G.Z2;
 * the FP64-semantics estimator: 5- and 6-limb LD scores agree (the fixed-point
-  error of the 5-limb path, which is what separates it from FP64).
+  error of the 5-limb path, which is what separates it from FP64);
+* phase 2's two missing-call methods (missing-flag MFMA plane, sums gathered
+  over per-SNP lists in XCD-affine sample ranges) give bit-identical scores.
 """
 from __future__ import annotations
 
@@ -61,3 +63,21 @@ def test_five_and_six_limbs_agree_at_full_size(geno):
     print(f"max |L5 - L6| / max(|L6|, 1) = {rel:.3e}", flush=True)
     assert rel < 1e-10
     assert np.all(np.isfinite(r5.ldscore))
+
+
+def test_phase2_missing_methods_identical_at_full_size(geno):
+    from summit_amd import autotune, engine
+    dev = engine.default_device()
+    gen = torch.Generator(device="cpu").manual_seed(13)
+    xz = torch.randn((B, N), generator=gen, dtype=torch.float64).to(dev)
+    inv = engine.resvar_inv(geno, 0, M)
+    key = autotune.p2_key(M, N, 1, B, 5)
+    out = {}
+    for method in (1, 2):
+        autotune.set_value("p2_miss", key, method)
+        ms = torch.zeros((M, 1), dtype=torch.float64, device=dev)
+        engine.phase2_accum(geno, 0, M, inv, B, xz, ms, N)
+        out[method] = ms.cpu().numpy()
+    autotune.set_value("p2_miss", key, 0)
+    assert np.array_equal(out[1], out[2])
+    assert np.all(np.isfinite(out[1])) and np.abs(out[1]).max() > 0
