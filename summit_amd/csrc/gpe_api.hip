@@ -1128,13 +1128,13 @@ void run_phase2(gpe_geno* g, int64_t s, int64_t e, const double* x, int64_t ldx,
             static_cast<int64_t*>(g->scratch("p2_msum", (size_t)parts * rows * ld_ms * sizeof(int64_t)));
         a.msum = msum;
         a.ld_ms = ld_ms;
-        a.ms_parts = parts;
-        a.ms_part_stride = rows * ld_ms;
         for (int64_t rs = s; rs < e; rs += slice) {
             const int64_t re = std::min(e, rs + slice);
             ProfScope pm(GPE_PROF_P2_MISS, st);
             launch_miss_sums(g->smiss_rptr, parts, g->smiss_idx, rs, re, xrow, g->n, lc.nl, nch, msum, rows * ld_ms,
                              ld_ms, st);
+            // the sample ranges' sums added into the first (the epilogue reads one value)
+            launch_sum_parts(msum, parts, rows * ld_ms, (re - rs) * ld_ms, st);
             pm.stop();
             a.r0 = rs;
             a.r1 = re;

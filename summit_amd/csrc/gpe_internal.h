@@ -80,10 +80,8 @@ struct ContractArgs {
     int32_t cols_out;        // I32/I64 epilogues: columns >= cols_out are not written
     int64_t out_row0;        // I32/I64 epilogues: output row index = r - out_row0
     int64_t* out2_i64;       // EPI_I64_2P: missing-plane output (same layout as out_i64)
-    const int64_t* msum;     // EPI_P2X: exact missing-call sums over sample ranges,
-    int64_t ld_ms;           //   msum[p * ms_part_stride + (r - r0) * ld_ms + c] for p < ms_parts
-    int32_t ms_parts;
-    int64_t ms_part_stride;
+    const int64_t* msum;     // EPI_P2X: exact missing-call sums, msum[(r - r0) * ld_ms + c]
+    int64_t ld_ms;
 };
 
 // Banded Gram (windowed LD) arguments
@@ -145,6 +143,8 @@ void launch_range_ptrs(const int64_t* ptr, const int32_t* idx, int64_t m, int pa
 // and the limb rows of launch_xrow_limbs
 void launch_miss_sums(const int64_t* rptr, int parts, const int32_t* idx, int64_t r0, int64_t r1, const int8_t* xrow,
                       int64_t n, int nl, int nch, int64_t* out, int64_t part_stride, int64_t ld_ms, hipStream_t st);
+// parts[i] = sum_p parts[p * stride + i] for i < n (the sample ranges' sums into part 0)
+void launch_sum_parts(int64_t* parts, int n_parts, int64_t stride, int64_t n, hipStream_t st);
 void launch_miss_fill(const uint8_t* smp, int64_t stride, int64_t n, const int64_t* ptr, int32_t* idx,
                       hipStream_t st);
 void launch_fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t st);

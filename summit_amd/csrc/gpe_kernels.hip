@@ -460,10 +460,7 @@ __global__ __launch_bounds__(256, (NL * CTC <= 2) ? 4 : 2) void k_contract(Contr
                             if (MISS) me = me * 256 + (int64_t)acc[l * CTC + ctc][1][e];
                         }
                         const int c = ct * 32 + 8 * (e >> 2) + 4 * half + (e & 3);
-                        if constexpr (EPI == EPI_P2X) {
-                            const int64_t* ms = a.msum + (r - a.r0) * a.ld_ms + c;
-                            for (int pp = 0; pp < a.ms_parts; ++pp) me += ms[pp * a.ms_part_stride];
-                        }
+                        if constexpr (EPI == EPI_P2X) me = a.msum[(r - a.r0) * a.ld_ms + c];
                         const double obs_sum = (double)(a.col_sum[c] - me);
                         const double w = isd * a.col_scale[c] * ((double)ge - mu * obs_sum);
                         const double w2 = w * w;
@@ -1882,6 +1879,20 @@ __global__ __launch_bounds__(256) void k_miss_sums(const int64_t* __restrict__ r
     for (int l = 0; l < NL; ++l) me = me * 256 + (int64_t)rd[l * 64 + lane];
     out[(int64_t)p * part_stride + (j - r0) * ld_ms + h * 64 + lane] = me;
 }
+// out[i] = sum_p parts[p * stride + i] for i < n, into parts[i] (part 0), in order
+__global__ void k_sum_parts(int64_t* parts, int n_parts, int64_t stride, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int64_t acc = parts[i];
+    for (int p = 1; p < n_parts; ++p) acc += parts[(int64_t)p * stride + i];
+    parts[i] = acc;
+}
+void launch_sum_parts(int64_t* parts, int n_parts, int64_t stride, int64_t n, hipStream_t st) {
+    if (n <= 0 || n_parts <= 1) return;
+    hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, parts, n_parts, stride, n);
+    GPE_HIP(hipGetLastError());
+}
+
 void launch_miss_sums(const int64_t* rptr, int parts, const int32_t* idx, int64_t r0, int64_t r1, const int8_t* xrow,
                       int64_t n, int nl, int nch, int64_t* out, int64_t part_stride, int64_t ld_ms, hipStream_t st) {
     const int64_t per_range = (r1 - r0) * nch;
