@@ -1699,7 +1699,7 @@ void launch_quant_p2(const double* x, int64_t ldx, int64_t n, int n_bins, int v_
 }
 
 // Limb rows of the phase-2 operand for gathers by sample.  One workgroup per
-// 64-sample x 64-column tile: q and its digits as k_quant_p2 forms them, staged
+// 64-sample x 64-column tile: q and its digits as k_quant_p2_t forms them, staged
 // in LDS and written out so that each sample's nl x 64 bytes are contiguous.
 // Rows run to n inclusive: row n is all zeros (the pad entry of the gathers).
 template <int NL>
