@@ -1375,10 +1375,9 @@ void launch_p1_finalize(const int64_t* y, int64_t ldy, int64_t n, int n_bins, in
 // Missing-call records for phase 1 with int8 probes (entries -1, 0, +1): for
 // SNP i of [k0, k1) and probe word wd (columns 64wd .. 64wd+63) one record of
 // rec_bytes = round_up(16 + 8 n_bins, 16) bytes at byte ((i - k0) vw + wd) rec_bytes:
-//   uint32 { pos bits 0-31, neg bits 0-31, pos bits 32-63, neg bits 32-63 },
-//   then double a_k for every bin k,
-// pos / neg: bit b set when z[i][64wd + b] is > 0 / < 0 (lane l of a wave reads
-// the pos/neg pair of its half, 8 bytes at offset 8 (l / 32));
+//   uint32 code words t = 0..3, column 64wd + 16t + b in bits 2b, 2b+1 as a
+//   signed 2-bit field (+1 -> 01, -1 -> 11, 0 -> 00: one signed bit-field
+//   extract gives z), then double a_k for every bin k,
 // a_k = mean_i * (round(w_ik 2^shift_k) * 2^-shift_k).
 // Rounding to nearest even commutes with negation, so the quantized
 // coefficient of column (k, v) is z_iv round(w_ik 2^shift_k) and
@@ -1386,7 +1385,7 @@ void launch_p1_finalize(const int64_t* y, int64_t ldy, int64_t n, int n_bins, in
 __host__ __device__ constexpr int64_t miss_rec_bytes(int n_bins) { return ((16 + 8 * (int64_t)n_bins + 15) / 16) * 16; }
 
 // One thread per (SNP, 64-probe word): the word's 64 probe bytes in four
-// 16-byte loads (the probe rows are 32-byte aligned), sign bits of the first
+// 16-byte loads (the probe rows are 32-byte aligned), code words of the first
 // v_count, then the bins' a_k.
 __global__ void k_p1_miss_rec(const double* w, int64_t ldw, const int8_t* z, int64_t ldz, int64_t k0, int64_t k1,
                               int n_bins, int v_count, int v_pad, const int32_t* shift, const double* col_scale,
@@ -1397,7 +1396,7 @@ __global__ void k_p1_miss_rec(const double* w, int64_t ldw, const int8_t* z, int
     const int64_t i = k0 + t / vw;
     const int wd = (int)(t % vw);
     const int64_t rb = miss_rec_bytes(n_bins);
-    uint64_t pos = 0, neg = 0;
+    uint32_t code[4] = {0u, 0u, 0u, 0u};
     const int nb = v_count - wd * 64 < 64 ? v_count - wd * 64 : 64;  // probe columns of this word
     const int8_t* zr = z + i * ldz + wd * 64;
 #pragma unroll
@@ -1406,20 +1405,15 @@ __global__ void k_p1_miss_rec(const double* w, int64_t ldw, const int8_t* z, int
         const v4u d = *reinterpret_cast<const v4u*>(zr + 16 * q);
 #pragma unroll
         for (int b = 0; b < 16; ++b) {
-            const int bit = q * 16 + b;
             const int8_t zz = (int8_t)((d[b >> 2] >> (8 * (b & 3))) & 0xffu);
-            if (bit < nb) {
-                if (zz > 0) pos |= 1ULL << bit;
-                else if (zz < 0) neg |= 1ULL << bit;
+            if (q * 16 + b < nb) {
+                if (zz > 0) code[q] |= 1u << (2 * b);
+                else if (zz < 0) code[q] |= 3u << (2 * b);
             }
         }
     }
     uint8_t* r = rec + ((i - k0) * vw + wd) * rb;
-    uint32_t* hdr = reinterpret_cast<uint32_t*>(r);
-    hdr[0] = (uint32_t)pos;
-    hdr[1] = (uint32_t)neg;
-    hdr[2] = (uint32_t)(pos >> 32);
-    hdr[3] = (uint32_t)(neg >> 32);
+    *reinterpret_cast<v4u*>(r) = v4u{code[0], code[1], code[2], code[3]};
     double* a = reinterpret_cast<double*>(r + 16);
     for (int k = 0; k < n_bins; ++k) {
         const int c = k * v_pad;
@@ -1448,8 +1442,8 @@ int64_t p1_miss_rec_bytes(int64_t n_snps, int n_bins, int v_pad) {
 // over 64 probe columns per pass.  The sample's entries are handled 64 at a
 // time: lane j loads entry j's record (the next group's loads are in flight
 // while the current group is summed) and stages it in the wave's LDS slot;
-// every lane then reads each record back (its own half of the sign words once,
-// the KB shared a_k) and adds z * a_k with one FMA per bin (z in {-1, 0, +1}, exact).
+// every lane then reads each record back (its column's code word once, the KB
+// shared a_k) and adds z * a_k with one FMA per bin (z in {-1, 0, +1}, exact).
 constexpr int kMissGroup = 64;
 
 template <int KB>
@@ -1485,8 +1479,8 @@ __global__ __launch_bounds__(256) void k_p1_finalize_z8(
     const int64_t me = lo;
     const int vw = (v_pad + 63) / 64;
     const int64_t rbytes = miss_rec_bytes(n_bins);
-    const int half = lane >> 5;
-    const uint32_t lbit = (uint32_t)(lane & 31);
+    const int cword = lane >> 4;                         // code word of this lane's column
+    const uint32_t cshift = 30u - 2u * (uint32_t)(lane & 15);  // its field moved to bits 30-31
     uint8_t* st = stage[wave];
     for (int wd = 0; wd < vw; ++wd) {
         const int v = wd * 64 + lane;
@@ -1530,8 +1524,8 @@ __global__ __launch_bounds__(256) void k_p1_finalize_z8(
             const int cnt = (me - p0) < kMissGroup ? (int)(me - p0) : kMissGroup;
             for (int jx = 0; jx < cnt; ++jx) {
                 const uint8_t* se = st + jx * kStage;
-                const uint2 pn = *reinterpret_cast<const uint2*>(se + 8 * half);
-                const double zs = (double)((int)((pn.x >> lbit) & 1u) - (int)((pn.y >> lbit) & 1u));
+                const uint32_t cw = *reinterpret_cast<const uint32_t*>(se + 4 * cword);
+                const double zs = (double)((int32_t)(cw << cshift) >> 30);
                 const double* av = reinterpret_cast<const double*>(se + 16);
 #pragma unroll
                 for (int k = 0; k < KB; ++k) acc[k] = fma(av[k], zs, acc[k]);
@@ -1556,6 +1550,105 @@ __global__ __launch_bounds__(256) void k_p1_finalize_z8(
     }
 }
 
+// One bin of k_p1_finalize_z8 without staging: one wave per sample, lane j
+// takes the sample's entries j, j + 64, ... of [ra, rb) and keeps all 64
+// columns' sums of z * a in registers, so each entry's record (code words and
+// the bin's a) reaches one lane instead of being broadcast to 64 through LDS.
+// Per (entry, column) one signed bit-field extract, one conversion and one
+// FMA.  The wave's 64 x 64 partial sums are then combined by a halving
+// exchange (xor 32, 16, ..., 1; at each step a lane keeps the half of its
+// values whose column bit equals its lane bit and adds the partner's copy of
+// that half), which leaves column c's total in lane c.
+__global__ __launch_bounds__(256) void k_p1_finalize_z8_lanes(
+    const int64_t* __restrict__ y, int64_t ldy, int64_t n, int n_bins, int kb0, int v_count, int v_pad,
+    const double* __restrict__ col_scale, const double* __restrict__ cst, const int64_t* __restrict__ miss_ptr,
+    const int32_t* __restrict__ miss_idx, int64_t k0, int64_t ra, int64_t rb, const uint8_t* __restrict__ rec,
+    double* __restrict__ xz, int64_t ldxz, const uint8_t* __restrict__ col_mode) {
+    const bool first = ra == k0;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t smp = (int64_t)blockIdx.x * 4 + wave;
+    const int lane = threadIdx.x & 63;
+    if (smp >= n) return;
+    const int64_t b0 = miss_ptr[smp], b1 = miss_ptr[smp + 1];
+    int64_t lo = b0, hi = b1;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (miss_idx[mid] < ra) lo = mid + 1;
+        else hi = mid;
+    }
+    const int64_t mb = lo;
+    hi = b1;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (miss_idx[mid] < rb) lo = mid + 1;
+        else hi = mid;
+    }
+    const int64_t me = lo;
+    const int vw = (v_pad + 63) / 64;
+    const int64_t rbytes = miss_rec_bytes(n_bins);
+    const int64_t a_off = 16 + 8 * (int64_t)kb0;
+    for (int wd = 0; wd < vw; ++wd) {
+        const uint8_t* rbase = rec + (int64_t)wd * rbytes;
+        double acc[64];
+#pragma unroll
+        for (int c = 0; c < 64; ++c) acc[c] = 0.0;
+        // a lane without an entry holds zero code words and a = 0 (adds nothing)
+        v4u h = v4u{};
+        double av = 0.0;
+        if (mb + lane < me) {
+            const uint8_t* e = rbase + ((int64_t)miss_idx[mb + lane] - k0) * vw * rbytes;
+            h = *reinterpret_cast<const v4u*>(e);
+            av = *reinterpret_cast<const double*>(e + a_off);
+        }
+        int32_t idx1 = mb + 64 + lane < me ? miss_idx[mb + 64 + lane] : (int32_t)k0;
+        // the first entry and the next index have landed before the loop, so the
+        // loop header sees no pending load from outside and each iteration waits
+        // only where it moves its prefetched entry into place (after the sums)
+        __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
+        for (int64_t p0 = mb; p0 < me; p0 += 64) {
+            // the lane's next entry (its index loaded one iteration ago) and the
+            // index after it are loaded while this entry is summed
+            const int64_t pn = p0 + 64 + lane;
+            v4u hn = v4u{};
+            double an = 0.0;
+            if (pn < me) {
+                const uint8_t* e = rbase + ((int64_t)idx1 - k0) * vw * rbytes;
+                hn = *reinterpret_cast<const v4u*>(e);
+                an = *reinterpret_cast<const double*>(e + a_off);
+            }
+            const int32_t idx2 = pn + 64 < me ? miss_idx[pn + 64] : (int32_t)k0;
+#pragma unroll
+            for (int c = 0; c < 64; ++c) {
+                const int32_t zc = (int32_t)(h[c >> 4] << (30 - 2 * (c & 15))) >> 30;
+                acc[c] = fma(av, (double)zc, acc[c]);
+            }
+            h = hn;
+            av = an;
+            idx1 = idx2;
+        }
+#pragma unroll
+        for (int sft = 32; sft >= 1; sft >>= 1) {
+            const bool up = (lane & sft) != 0;
+#pragma unroll
+            for (int i = 0; i < sft; ++i) {
+                const double keep = up ? acc[i + sft] : acc[i];
+                const double send = up ? acc[i] : acc[i + sft];
+                acc[i] = keep + __shfl_xor(send, sft, 64);
+            }
+        }
+        const int v = wd * 64 + lane;
+        if (v < v_count) {
+            const int c = kb0 * v_pad + v;
+            double val = acc[0];
+            if (first) val += (double)y[(int64_t)c * ldy + smp] * col_scale[c] - cst[c];
+            const int mode = (col_mode && first) ? col_mode[kb0] : 2;
+            double* dst = xz + (int64_t)(kb0 * v_count + v) * ldxz + smp;
+            if (mode == 1) *dst = val;
+            else *dst += val;
+        }
+    }
+}
+
 void launch_p1_finalize_z8(const int64_t* y, int64_t ldy, int64_t n, int n_bins, int v_count, int v_pad,
                            const double* col_scale, const double* cst, const int64_t* miss_ptr,
                            const int32_t* miss_idx, int64_t k0, int64_t k1, const void* rec, double* xz,
@@ -1565,14 +1658,16 @@ void launch_p1_finalize_z8(const int64_t* y, int64_t ldy, int64_t n, int n_bins,
     const int64_t rec_stride = (int64_t)vw * miss_rec_bytes(n_bins);
     // SNP ranges of range_bytes of records, one launch per range (and bin
     // group), in order.  Small ranges keep a range L2-resident while all
-    // samples walk it; measured at N=100,000, M=500,000, B=64 fewer, larger
-    // launches win (2 MiB 3.9 ms, 16 MiB and up 3.3 ms), hence the 64 MiB default
-    if (range_bytes <= 0) range_bytes = (int64_t)64 << 20;
+    // samples walk it.  Measured at N=100,000, M=500,000, B=64: the one-bin
+    // kernel (no LDS staging) is fastest at 8 MiB (2.45 ms; 3.1 ms at 16 MiB
+    // and up, 2.9 ms at 4 MiB), while the LDS-staged kernel of several bins
+    // favoured fewer, larger launches (2 MiB 3.9 ms, 16 MiB and up 3.3 ms)
+    if (range_bytes <= 0) range_bytes = n_bins == 1 ? (int64_t)8 << 20 : (int64_t)64 << 20;
     const int64_t range = std::max<int64_t>(4096, range_bytes / rec_stride);
     const uint8_t* r = static_cast<const uint8_t*>(rec);
     for (int64_t ra = k0; ra < k1 || ra == k0; ra += range) {
         const int64_t rb = std::min(k1, ra + range);
-        // bins in groups of 16, 8, 4, 2, 1
+        // bins in groups of 16, 8, 4, 2 (staged) and 1 (k_p1_finalize_z8_lanes)
         for (int kb0 = 0; kb0 < n_bins;) {
             const int left = n_bins - kb0;
             const dim3 grid((unsigned)ceil_div(n, 4));
@@ -1589,7 +1684,9 @@ void launch_p1_finalize_z8(const int64_t* y, int64_t ldy, int64_t n, int n_bins,
             } else if (left >= 2) {
                 GPE_FZ8(2);
             } else {
-                GPE_FZ8(1);
+                hipLaunchKernelGGL(k_p1_finalize_z8_lanes, grid, dim3(256), 0, st, y, ldy, n, n_bins, kb0, v_count,
+                                   v_pad, col_scale, cst, miss_ptr, miss_idx, k0, ra, rb, r, xz, ldxz, col_mode);
+                kb0 += 1;
             }
 #undef GPE_FZ8
             GPE_HIP(hipGetLastError());
