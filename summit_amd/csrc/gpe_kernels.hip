@@ -1372,6 +1372,28 @@ void launch_p1_finalize(const int64_t* y, int64_t ldy, int64_t n, int n_bins, in
     GPE_HIP(hipGetLastError());
 }
 
+// First position p in [lo, hi) of the sorted idx with idx[p] >= t (hi if none),
+// searched by the whole wave: each step the 64 lanes probe 64 evenly spaced
+// positions and a ballot of idx < t (a prefix of the lanes, idx being sorted)
+// narrows the interval about 65-fold, so a sample's list of thousands of
+// entries takes two or three dependent loads instead of a dozen.  Invariant:
+// the answer lies in [lo, hi]; positions below lo hold idx < t and position
+// hi, when below the original end, holds idx >= t.  lo, hi, t wave-uniform.
+__device__ __forceinline__ int64_t wave_lower_bound(const int32_t* __restrict__ idx, int64_t lo, int64_t hi,
+                                                    int64_t t, int lane) {
+    while (hi - lo > 64) {
+        const int64_t len = hi - lo;
+        const int64_t pos = lo + (len * (lane + 1)) / 65;  // in [lo, hi), increasing with the lane
+        const int cnt = __popcll(__ballot(idx[pos] < t));
+        const int64_t nlo = cnt > 0 ? lo + (len * cnt) / 65 + 1 : lo;
+        const int64_t nhi = cnt < 64 ? lo + (len * (cnt + 1)) / 65 : hi;
+        lo = nlo;
+        hi = nhi;
+    }
+    const bool less = lo + lane < hi && idx[lo + lane] < t;
+    return lo + __popcll(__ballot(less));
+}
+
 // Missing-call records for phase 1 with int8 probes (entries -1, 0, +1): for
 // SNP i of [k0, k1) and probe word wd (columns 64wd .. 64wd+63) one record of
 // rec_bytes = round_up(16 + 8 n_bins, 16) bytes at byte ((i - k0) vw + wd) rec_bytes:
@@ -1463,20 +1485,8 @@ __global__ __launch_bounds__(256) void k_p1_finalize_z8(
     const int lane = threadIdx.x & 63;
     if (smp >= n) return;
     const int64_t b0 = miss_ptr[smp], b1 = miss_ptr[smp + 1];
-    int64_t lo = b0, hi = b1;
-    while (lo < hi) {
-        const int64_t mid = (lo + hi) >> 1;
-        if (miss_idx[mid] < ra) lo = mid + 1;
-        else hi = mid;
-    }
-    const int64_t mb = lo;
-    hi = b1;
-    while (lo < hi) {
-        const int64_t mid = (lo + hi) >> 1;
-        if (miss_idx[mid] < rb) lo = mid + 1;
-        else hi = mid;
-    }
-    const int64_t me = lo;
+    const int64_t mb = wave_lower_bound(miss_idx, b0, b1, ra, lane);
+    const int64_t me = wave_lower_bound(miss_idx, mb, b1, rb, lane);
     const int vw = (v_pad + 63) / 64;
     const int64_t rbytes = miss_rec_bytes(n_bins);
     const int cword = lane >> 4;                         // code word of this lane's column
@@ -1570,20 +1580,8 @@ __global__ __launch_bounds__(256) void k_p1_finalize_z8_lanes(
     const int lane = threadIdx.x & 63;
     if (smp >= n) return;
     const int64_t b0 = miss_ptr[smp], b1 = miss_ptr[smp + 1];
-    int64_t lo = b0, hi = b1;
-    while (lo < hi) {
-        const int64_t mid = (lo + hi) >> 1;
-        if (miss_idx[mid] < ra) lo = mid + 1;
-        else hi = mid;
-    }
-    const int64_t mb = lo;
-    hi = b1;
-    while (lo < hi) {
-        const int64_t mid = (lo + hi) >> 1;
-        if (miss_idx[mid] < rb) lo = mid + 1;
-        else hi = mid;
-    }
-    const int64_t me = lo;
+    const int64_t mb = wave_lower_bound(miss_idx, b0, b1, ra, lane);
+    const int64_t me = wave_lower_bound(miss_idx, mb, b1, rb, lane);
     const int vw = (v_pad + 63) / 64;
     const int64_t rbytes = miss_rec_bytes(n_bins);
     const int64_t a_off = 16 + 8 * (int64_t)kb0;
