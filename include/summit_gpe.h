@@ -349,7 +349,8 @@ int gpe_profile_reset(void);
  * which times the candidates on the device.  value <= 0 removes the entry.
  *   GPE_SCHED_P1_SPLITS:        K splits of the phase-1 contraction (1..64)
  *   GPE_SCHED_P1_FINALIZE_KIB:  SNP-range size of the phase-1 missing-call
- *                               finalize, in KiB of records (64..65536)
+ *                               finalize, in KiB of records (64..65536; no
+ *                               entry: 8 MiB for one bin, 64 MiB for several)
  *   GPE_SCHED_P1_CPB / P2_CPB:  128-value chunks staged per LDS slot and
  *                               barrier in the five-limb contractions (1 or 2;
  *                               2 when no entry)
