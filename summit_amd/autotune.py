@@ -3,9 +3,9 @@
 The phase-1 contraction splits its K range (SNPs) over ``splits`` workgroup
 layers whose int64 partial sums meet atomically, and the phase-1 missing-call
 finalize walks the missing-call records in SNP ranges sized to stay resident
-in each XCD's L2.  The library picks both by heuristics (the smallest split
-count whose last wave of workgroups fills >= 90% of the device's resident
-slots; 8 MiB ranges for one bin, 64 MiB for several).  ``tune_phase1`` times the candidates on the device
+in each XCD's L2.  The library picks both by heuristics (splits until the
+grid holds 16 waves of resident workgroups with >= 128 chunks per split;
+8 MiB ranges for one bin, 64 MiB for several).  ``tune_phase1`` times the candidates on the device
 with HIP events around the kernels themselves (``gpe_profile_*``), keeps the
 fastest in the library's schedule table for that shape key, and records the
 choice in a JSON cache so later processes on the same device and shape start
