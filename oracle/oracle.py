@@ -27,6 +27,13 @@ Pins (each function is checked by something other than a retyping of itself):
   Carlo standard errors, variance ``2[(sum r^2)^2 - sum r^4]/B``).
 * Integer products (``gz_int``, ``gtu_int``): brute-force int64 loops on tiny
   inputs and the identity ``sum_b U[:,b]^2 == (Z^T G G^T Z)`` trace checks.
+* GxE features (``gxe_project_center`` and the ``inv_x``/``inv_w`` of
+  ``gxe_scores``): the reference test's dense feature formula
+  (``tests/test_gxe_native_core.py:95-123``: ``p = I - q q^T`` with the
+  intercept in ``q``, rank ``N - q.shape[1]``), the invariants ``C^T x = 0``,
+  ``1^T x = 0``, ``||x_j||^2 / (N - p - 1) = 1``, and planted mistakes (no
+  centring, environment applied after the projection, a dropped degree of
+  freedom) that the formula rejects.
 """
 from __future__ import annotations
 
