@@ -190,6 +190,9 @@ int gpe_add_i32(int32_t* value_dev, int32_t increment, void* stream);
  * meansq: row-major (n_snps_total, n_bins) with ld_meansq >= n_bins, indexed by
  * absolute SNP j.  inv_left_dev: per SNP of the range.  guard_counts_host
  * (may be NULL) receives {nonfinite, extreme} counts (the call synchronizes).
+ * A NaN or infinity anywhere in a column of Xz makes that column's product with
+ * every SNP non-finite, as in the reference's dense GEMM, so the updates of its
+ * bin are counted as non-finite and dropped.
  * Replaces phase2_accum_XtXz_bed (src/native/gwldcore.cpp:2329-2649, exported
  * at :3759-3770) and, with HWE-prepared handles, the Mailman variants
  * (:2651-3119). */
@@ -198,6 +201,22 @@ int gpe_phase2_accum(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const doub
                      int32_t n_denom, double* meansq_dev, int64_t ld_meansq, double* mc_aggregate_sumsq_dev,
                      double* mc_per_snp_sumsq_dev, double extreme_warn, int32_t drop_extreme, int32_t limbs,
                      unsigned long long* guard_counts_host, void* stream);
+
+/* gpe_phase2_accum with the centring taken from the caller's column sums:
+ * Work_jc = inv_sd_j (g_j . Xz_c - mu_j sum_xz_dev[c]) (minus the missing-call
+ * terms of a mean-imputed handle) instead of the exact sums of Xz, for
+ * c < n_bins * tile_nvecs (device FP64).  Replaces
+ * phase2_accum_XtXz_bed_mailman_rowmajor (src/native/gwldcore.cpp:2919-3119,
+ * exported at :3800-3809), whose per-column centring uses the caller's sum_Xz
+ * (:2965-2967, :874-880); its callers pass the column sums of the projected
+ * X.Z (src/summit/ldscore/gw_ldscore.py:2010-2016).  A non-finite sum makes its
+ * column's updates non-finite (counted and dropped like any other). */
+int gpe_phase2_accum_colsums(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const double* inv_left_dev,
+                             int32_t tile_nvecs, const double* xz_dev, int64_t ldxz, int32_t n_bins,
+                             const double* sum_xz_dev, int32_t n_denom, double* meansq_dev, int64_t ld_meansq,
+                             double* mc_aggregate_sumsq_dev, double* mc_per_snp_sumsq_dev, double extreme_warn,
+                             int32_t drop_extreme, int32_t limbs, unsigned long long* guard_counts_host,
+                             void* stream);
 
 /* Dense products with the standardized genotype matrix X (N x M, the handle's
  * working codes, x_in = (g_in - mu_i) inv_sd_i, 0 where missing in mean mode):
@@ -263,6 +282,13 @@ int gpe_phase1_xz_z8(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const doub
  * Fixed-order reductions; a per-device workspace is kept for the process. */
 int gpe_project_f64(double* x_dev, int64_t ldx, int64_t n, int32_t n_cols, const double* a_dev, int64_t lda,
                     const double* b_dev, int64_t ldb, int32_t p, int32_t center, void* stream);
+
+/* out_dev[q] = sum_{i < n} x_dev[q * ldx + i] for q < n_cols (FP64, fixed-order
+ * reduction): the column sums of a row-major (N, Q) X.Z held column by column.
+ * Replaces compute_col_sums_rowmajor (src/native/gwldcore.cpp:3121-3132,
+ * exported at :3720-3721) and the sums of project_rowmajor_inplace_and_col_sums
+ * (:3134-3171) after gpe_project_f64. */
+int gpe_col_sums_f64(const double* x_dev, int64_t ldx, int64_t n, int32_t n_cols, double* out_dev, void* stream);
 
 /* out[e * ldo + (j - snp_begin)] = sum_n w[e * ldw + n] x_jn^2 for each of
  * n_weights weight vectors (standardized working codes): the ||e . x_j||^2

@@ -404,6 +404,54 @@ def phase2_accum(X, inv_left, tile_nvecs: int, Xz, meansq_accum, N_denom: int, b
 # ----------------------------------------------------------------------------
 # Driver: src/summit/ldscore/gw_ldscore.py:1753-2223 (BED, no covariates)
 # ----------------------------------------------------------------------------
+def phase2_accum_mailman(G, mean, inv_std, inv_left, tile_nvecs: int, Y, sum_y, meansq_accum, N_denom: int,
+                         blk_start: int, mc_per_snp_sumsq=None, drop_extreme: bool = True):
+    """Mailman phase 2 with the caller's column sums (src/native/gwldcore.cpp:2919-3119,
+    per-SNP update :774-908): for HWE-completed counts G (N, L) and Y (N, Q),
+    w = G^T Y (the exact products the 3^s tables give), and per SNP j and bin k
+    upd = a_j sum_c w_jc^2 - 2 mu_j a_j sum_c w_jc s_c + mu_j^2 a_j sum_c s_c^2,
+    a_j = inv_j^2 inv_std_j^2 / (N_denom - 1)^2, s = sum_y (the caller's sums),
+    with the non-finite / extreme guards on the group length; the MC sum is
+    a_j^2 sum_c (w_jc - mu_j s_c)^4."""
+    G = np.asarray(G, dtype=np.float64)
+    Y = np.asarray(Y, dtype=np.float64)
+    s = np.asarray(sum_y, dtype=np.float64)
+    L = G.shape[1]
+    Q = Y.shape[1]
+    if tile_nvecs <= 0 or Q % tile_nvecs:
+        raise RuntimeError("Xz2d col count must be a multiple of tile_nvecs")
+    if s.shape != (Q,):
+        raise RuntimeError("sum_Xz shape mismatch in phase2_accum_XtXz_bed_mailman_rm")
+    denom = float(N_denom) - 1.0
+    if denom <= 0.0:
+        denom = 1.0
+    inv = np.asarray(inv_left, dtype=np.float64)
+    alpha = np.where(np.isfinite(inv) & (inv > 0), inv * inv / (denom * denom), 0.0) * \
+        np.asarray(inv_std, dtype=np.float64) ** 2
+    mu = np.asarray(mean, dtype=np.float64)
+    counts = {"nonfinite": 0, "extreme": 0}
+    w = G.T @ Y  # (L, Q)
+    B = Q // tile_nvecs
+    for k in range(B):
+        sl = slice(k * tile_nvecs, (k + 1) * tile_nvecs)
+        wk, sk = w[:, sl], s[sl]
+        upd = alpha * np.sum(wk * wk, axis=1) - 2.0 * mu * alpha * (wk @ sk) + mu * mu * alpha * float(sk @ sk)
+        live = np.isfinite(alpha) & (alpha > 0)  # other SNPs are skipped before the guards
+        nonfin = live & ~np.isfinite(upd)
+        extreme = live & np.isfinite(upd) & (upd > EXTREME_LD_WARN * float(tile_nvecs))
+        counts["nonfinite"] += int(nonfin.sum())
+        counts["extreme"] += int(extreme.sum())
+        keep = live & ~nonfin & ~(extreme & drop_extreme)
+        upd = np.where(upd < 0.0, 0.0, upd)
+        rows = blk_start + np.arange(L)
+        meansq_accum[rows[keep], k] += upd[keep]
+        if mc_per_snp_sumsq is not None:
+            cen = wk - mu[:, None] * sk[None, :]
+            c2 = cen * cen
+            mc_per_snp_sumsq[rows[keep], k] += (alpha * alpha * np.sum(c2 * c2, axis=1))[keep]
+    return counts
+
+
 def bounded_vtile_sizes(nvecs: int, vmax: int, granularity: int = 64):
     """src/summit/ldscore/gw_ldscore.py:687-697"""
     width = min(nvecs, vmax)

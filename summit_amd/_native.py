@@ -44,11 +44,14 @@ SIGNATURES = {
     "gpe_add_i32": (ctypes.c_int, [c_vp, c_i32, c_vp]),
     "gpe_phase2_accum": (ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_i32, c_vp, c_i64, c_i32, c_i32, c_vp, c_i64,
                                         c_vp, c_vp, c_dbl, c_i32, c_i32, c_vp, c_vp]),
+    "gpe_phase2_accum_colsums": (ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_i32, c_vp, c_i64, c_i32, c_vp, c_i32,
+                                                c_vp, c_i64, c_vp, c_vp, c_dbl, c_i32, c_i32, c_vp, c_vp]),
     "gpe_probes": (ctypes.c_int, [c_i32, c_u64, c_vp, c_i32, c_i64, c_i32, c_i32, c_vp]),
     "gpe_philox_rademacher": (ctypes.c_int, [c_vp, c_vp, c_i32, c_i64, c_i32, c_vp, c_i64, c_vp]),
     "gpe_phase1_xz_z8": (ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_i32, c_vp, c_vp, c_i64, c_i32, c_vp, c_i64, c_vp,
                                         c_i32, c_vp]),
     "gpe_project_f64": (ctypes.c_int, [c_vp, c_i64, c_i64, c_i32, c_vp, c_i64, c_vp, c_i64, c_i32, c_i32, c_vp]),
+    "gpe_col_sums_f64": (ctypes.c_int, [c_vp, c_i64, c_i64, c_i32, c_vp, c_vp]),
     "gpe_weighted_sumsq": (ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_i32, c_i64, c_vp, c_i64, c_vp]),
     "gpe_windowed_ld": (ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_dbl, c_i64, c_vp, c_i32, c_vp, c_vp]),
     "gpe_windowed_ld_multi": (ctypes.c_int, [c_vp, c_i32, c_vp, c_vp, c_dbl, c_i64, c_vp, c_i32, c_vp, c_vp]),

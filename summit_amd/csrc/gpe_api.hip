@@ -1072,8 +1072,11 @@ constexpr int kMissParts = 8;
 
 // Phase-2 type product on the SNP-major layout against x (column-major (N, n_bins*v_count)):
 // quantize x into limbs and run the contraction with the given epilogue.
+// sum_override (may be null): per column of x, sums that replace the exact column
+// sums of the quantized operand in the centring term.
 void run_phase2(gpe_geno* g, int64_t s, int64_t e, const double* x, int64_t ldx, int n_bins, int v_count, int limbs,
-                EpiMode mode, ContractArgs& a, hipStream_t st, int* ctiles_used_out, int* v_pad_out) {
+                EpiMode mode, ContractArgs& a, hipStream_t st, int* ctiles_used_out, int* v_pad_out,
+                const double* sum_override = nullptr) {
     const LimbCfg lc = limb_cfg(limbs);
     const int v_pad = (int)round_up(v_count, 32);
     require((int64_t)n_bins * v_pad <= 65535,
@@ -1095,6 +1098,7 @@ void run_phase2(gpe_geno* g, int64_t s, int64_t e, const double* x, int64_t ldx,
     int8_t* coef = static_cast<int8_t*>(g->scratch("coef", kcount * coef_ctiles * 4096));
     if (ncols_pad != ncols) launch_fill_u8(reinterpret_cast<uint8_t*>(coef), kcount * coef_ctiles * 4096, 0, st);
     launch_quant_p2(x, ldx, g->n, n_bins, v_count, v_pad, shift, lc.nl, lc.ctc, coef_ctiles, coef, col_sum, st);
+    if (sum_override) launch_col_sum_override(sum_override, n_bins, v_count, v_pad, shift, col_scale, col_sum, st);
     a.packed = g->snp_work;
     a.stride = g->snp_stride;
     a.r0 = s;
@@ -1459,28 +1463,43 @@ int gpe_philox_rademacher(const uint64_t* keys_host, const int64_t* blk_starts_h
             max_len = std::max(max_len, lens[(size_t)b]);
         }
         const size_t nkeys = (size_t)n_blocks * (size_t)v_count * 2;
-        uint64_t* keys = nullptr;
-        int64_t* starts = nullptr;
-        int64_t* lens_d = nullptr;
-        GPE_HIP(hipMalloc(&keys, nkeys * sizeof(uint64_t)));
-        GPE_HIP(hipMalloc(&starts, n_blocks * sizeof(int64_t)));
-        GPE_HIP(hipMalloc(&lens_d, n_blocks * sizeof(int64_t)));
-        try {
-            GPE_HIP(hipMemcpyAsync(keys, keys_host, nkeys * sizeof(uint64_t), hipMemcpyHostToDevice, st));
-            GPE_HIP(hipMemcpyAsync(starts, blk_starts_host, n_blocks * sizeof(int64_t), hipMemcpyHostToDevice, st));
-            GPE_HIP(hipMemcpyAsync(lens_d, lens.data(), n_blocks * sizeof(int64_t), hipMemcpyHostToDevice, st));
-            launch_philox_rademacher(keys, starts, lens_d, n_blocks, v_count, max_len, blk_starts_host[0], z_dev, ldz,
-                                     st);
-            GPE_HIP(hipStreamSynchronize(st));
-        } catch (...) {
-            dfree(keys);
-            dfree(starts);
-            dfree(lens_d);
-            throw;
+        const size_t bytes = nkeys * sizeof(uint64_t) + 2 * (size_t)n_blocks * sizeof(int64_t);
+        // per-device key / block tables (device + pinned host staging), kept for the
+        // process and reused once the previous call's launch has read them
+        struct Tables {
+            uint8_t* dev = nullptr;
+            uint8_t* host = nullptr;
+            size_t bytes = 0;
+            hipEvent_t done = nullptr;
+        };
+        static std::mutex mu;
+        static std::map<int, Tables> tables;
+        int device = 0;
+        GPE_HIP(hipGetDevice(&device));
+        std::lock_guard<std::mutex> lock(mu);
+        Tables& t = tables[device];
+        if (!t.done) GPE_HIP(hipEventCreateWithFlags(&t.done, hipEventDisableTiming));
+        GPE_HIP(hipEventSynchronize(t.done));
+        if (t.bytes < bytes) {
+            if (t.dev) GPE_HIP(hipFree(t.dev));
+            if (t.host) GPE_HIP(hipHostFree(t.host));
+            t.dev = t.host = nullptr;
+            t.bytes = 0;
+            const size_t cap = std::max<size_t>(bytes, 1 << 16);
+            GPE_HIP(hipMalloc(reinterpret_cast<void**>(&t.dev), cap));
+            GPE_HIP(hipHostMalloc(reinterpret_cast<void**>(&t.host), cap, hipHostMallocDefault));
+            t.bytes = cap;
         }
-        dfree(keys);
-        dfree(starts);
-        dfree(lens_d);
+        std::memcpy(t.host, keys_host, nkeys * sizeof(uint64_t));
+        std::memcpy(t.host + nkeys * sizeof(uint64_t), blk_starts_host, n_blocks * sizeof(int64_t));
+        std::memcpy(t.host + nkeys * sizeof(uint64_t) + n_blocks * sizeof(int64_t), lens.data(),
+                    n_blocks * sizeof(int64_t));
+        GPE_HIP(hipMemcpyAsync(t.dev, t.host, bytes, hipMemcpyHostToDevice, st));
+        const uint64_t* keys = reinterpret_cast<const uint64_t*>(t.dev);
+        const int64_t* starts = reinterpret_cast<const int64_t*>(t.dev + nkeys * sizeof(uint64_t));
+        const int64_t* lens_d = starts + n_blocks;
+        launch_philox_rademacher(keys, starts, lens_d, n_blocks, v_count, max_len, blk_starts_host[0], z_dev, ldz, st);
+        GPE_HIP(hipEventRecord(t.done, st));
     });
 }
 
@@ -1539,6 +1558,16 @@ int gpe_project_f64(double* x_dev, int64_t ldx, int64_t n, int32_t n_cols, const
         double* mean = Ssum + n_cols;
         double* a = mean + n_cols;
         launch_project(x_dev, ldx, n, n_cols, a_dev, lda, b_dev, ldb, p, center != 0, T, Ssum, a, mean, S(stream));
+    });
+}
+
+int gpe_col_sums_f64(const double* x_dev, int64_t ldx, int64_t n, int32_t n_cols, double* out_dev, void* stream) {
+    return guarded([&] {
+        require(n >= 0 && n_cols >= 0, "column-sum sizes must be non-negative");
+        require(n_cols <= 2147483647 && (n_cols == 0 || (x_dev != nullptr && out_dev != nullptr && ldx >= n)),
+                "column sums need ldx >= n");
+        if (n_cols == 0) return;
+        launch_col_sums(x_dev, ldx, n, n_cols, out_dev, S(stream));
     });
 }
 
@@ -1628,12 +1657,16 @@ int gpe_windowed_ld_multi(gpe_geno* g, int32_t n_chr, const int64_t* chr_bounds_
     });
 }
 
-int gpe_phase2_accum(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const double* inv_left_dev,
-                     int32_t tile_nvecs, const double* xz_dev, int64_t ldxz, int32_t n_bins, int32_t n_denom,
-                     double* meansq_dev, int64_t ld_meansq, double* mc_aggregate_sumsq_dev,
-                     double* mc_per_snp_sumsq_dev, double extreme_warn, int32_t drop_extreme, int32_t limbs,
-                     unsigned long long* guard_counts_host, void* stream) {
-    return guarded([&] {
+}  // extern "C"
+
+namespace {
+
+void phase2_entry(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const double* inv_left_dev, int32_t tile_nvecs,
+                  const double* xz_dev, int64_t ldxz, int32_t n_bins, int32_t n_denom, double* meansq_dev,
+                  int64_t ld_meansq, double* mc_aggregate_sumsq_dev, double* mc_per_snp_sumsq_dev,
+                  double extreme_warn, int32_t drop_extreme, int32_t limbs, const double* sum_xz_dev,
+                  unsigned long long* guard_counts_host, void* stream) {
+    {
         require_prepared(g);
         require(snp_begin >= 0 && snp_end <= g->m && snp_begin <= snp_end, "blk_end exceeds #SNPs in BIM");
         require(tile_nvecs > 0 && n_bins > 0, "Xz2d col count must be a multiple of tile_nvecs");
@@ -1655,7 +1688,7 @@ int gpe_phase2_accum(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const doub
             a.s4 = mc ? static_cast<double*>(g->scratch("p2_s4", g->m_alloc * ctiles_used * sizeof(double))) : nullptr;
             a.ld_s = ctiles_used;
             run_phase2(g, snp_begin, snp_end, xz_dev, ldxz, n_bins, tile_nvecs, limbs, EPI_P2, a, st, nullptr,
-                       nullptr);
+                       nullptr, sum_xz_dev);
             double denom = (double)n_denom - 1.0;
             if (denom <= 0.0) denom = 1.0;
             const double inv_denom2 = 1.0 / (denom * denom);
@@ -1667,6 +1700,36 @@ int gpe_phase2_accum(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const doub
             GPE_HIP(hipMemcpyAsync(guard_counts_host, gc, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
             GPE_HIP(hipStreamSynchronize(st));
         }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int gpe_phase2_accum(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const double* inv_left_dev,
+                     int32_t tile_nvecs, const double* xz_dev, int64_t ldxz, int32_t n_bins, int32_t n_denom,
+                     double* meansq_dev, int64_t ld_meansq, double* mc_aggregate_sumsq_dev,
+                     double* mc_per_snp_sumsq_dev, double extreme_warn, int32_t drop_extreme, int32_t limbs,
+                     unsigned long long* guard_counts_host, void* stream) {
+    return guarded([&] {
+        phase2_entry(g, snp_begin, snp_end, inv_left_dev, tile_nvecs, xz_dev, ldxz, n_bins, n_denom, meansq_dev,
+                     ld_meansq, mc_aggregate_sumsq_dev, mc_per_snp_sumsq_dev, extreme_warn, drop_extreme, limbs,
+                     nullptr, guard_counts_host, stream);
+    });
+}
+
+int gpe_phase2_accum_colsums(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const double* inv_left_dev,
+                             int32_t tile_nvecs, const double* xz_dev, int64_t ldxz, int32_t n_bins,
+                             const double* sum_xz_dev, int32_t n_denom, double* meansq_dev, int64_t ld_meansq,
+                             double* mc_aggregate_sumsq_dev, double* mc_per_snp_sumsq_dev, double extreme_warn,
+                             int32_t drop_extreme, int32_t limbs, unsigned long long* guard_counts_host,
+                             void* stream) {
+    return guarded([&] {
+        require(sum_xz_dev != nullptr, "sum_Xz shape mismatch in phase2_accum_XtXz_bed_mailman_rm");
+        phase2_entry(g, snp_begin, snp_end, inv_left_dev, tile_nvecs, xz_dev, ldxz, n_bins, n_denom, meansq_dev,
+                     ld_meansq, mc_aggregate_sumsq_dev, mc_per_snp_sumsq_dev, extreme_warn, drop_extreme, limbs,
+                     sum_xz_dev, guard_counts_host, stream);
     });
 }
 

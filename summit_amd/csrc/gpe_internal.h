@@ -183,6 +183,8 @@ void launch_colmax_f64(const double* x, int64_t ldx, int64_t n, int n_cols, int 
 void launch_quant_p2(const double* x, int64_t ldx, int64_t n, int n_bins, int v_count, int v_pad,
                      const int32_t* shift, int nl, int ctc, int64_t coef_ctiles, int8_t* coef,
                      int64_t* col_sum, hipStream_t st);
+void launch_col_sum_override(const double* sum, int n_bins, int v_count, int v_pad, const int32_t* shift,
+                             double* col_scale, int64_t* col_sum, hipStream_t st);
 void launch_shift_from_colmax(const uint64_t* colmax_bits, int n, int precision_bits, int32_t* shift,
                               double* col_scale, hipStream_t st);
 void launch_p2_finalize(const double* s2, const double* s4, int64_t ld_s, int64_t r0, int64_t r1, int n_bins,
@@ -223,6 +225,8 @@ void launch_resvar_inv(const int32_t* nobs, const int64_t* sum, const int64_t* s
 
 // x_q <- x_q - A (B x_q) [- mean] for q_count FP64 columns; scratch T (p x q_count),
 // S, mean (q_count) and a (p)
+// S[q] = sum_{i < n} x[q ldx + i] (fixed-order reduction, one workgroup per column)
+void launch_col_sums(const double* x, int64_t ldx, int64_t n, int q_count, double* S, hipStream_t st);
 void launch_project(double* x, int64_t ldx, int64_t n, int q_count, const double* A, int64_t lda, const double* B,
                     int64_t ldb, int p, bool center, double* T, double* S, double* a, double* mean, hipStream_t st);
 

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdint>
 #include <map>
@@ -100,6 +101,13 @@ __device__ __forceinline__ void balanced_digits(int64_t q, int8_t* d) {
         d[l] = (int8_t)r;
         q = (q - r) / 256;
     }
+}
+
+// round(v 2^sh) as an integer; 0 for a non-finite or out-of-range value (the
+// column's scale is NaN then, so the value never reaches an output).
+__device__ __forceinline__ int64_t quant_rn(double v, int sh) {
+    const double t = ldexp(v, sh);
+    return fabs(t) < 4.0e18 ? __double2ll_rn(t) : 0;
 }
 
 __device__ __forceinline__ double wave_sum_f64(double v) {
@@ -1135,7 +1143,10 @@ __global__ void k_shift_from_colmax(const uint64_t* colmax_bits, int n, int prec
         sh = precision_bits - e;
     }
     shift[c] = sh;
-    col_scale[c] = ldexp(1.0, -sh);
+    // a column holding a non-finite value (marked NaN by k_colmax_f64) gets a NaN
+    // scale: every product with it is NaN, as a NaN in a dense FP64 GEMM operand
+    // makes every dot product with that column NaN
+    col_scale[c] = isnan(mx) ? __builtin_nan("") : ldexp(1.0, -sh);
 }
 void launch_shift_from_colmax(const uint64_t* colmax_bits, int n, int precision_bits, int32_t* shift,
                               double* col_scale, hipStream_t st) {
@@ -1701,12 +1712,21 @@ __global__ void k_colmax_f64(const double* x, int64_t ldx, int64_t n, int v_coun
     if (v >= v_count) return;
     const double* col = x + (int64_t)(k * v_count + v) * ldx;
     double mx = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        mx = fmax(mx, fabs(col[i]));
+    int bad = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double a = fabs(col[i]);
+        if (a <= DBL_MAX)
+            mx = fmax(mx, a);
+        else
+            bad = 1;  // NaN or infinity
+    }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
-    if ((threadIdx.x & 63) == 0 && mx > 0.0)
-        atomicMax(reinterpret_cast<unsigned long long*>(colmax_bits + c), (unsigned long long)__double_as_longlong(mx));
+    bad = __any(bad);
+    // a positive quiet NaN's bits exceed every finite value's: the column stays marked
+    if ((threadIdx.x & 63) == 0 && (bad || mx > 0.0))
+        atomicMax(reinterpret_cast<unsigned long long*>(colmax_bits + c),
+                  bad ? 0x7FF8000000000000ULL : (unsigned long long)__double_as_longlong(mx));
 }
 void launch_colmax_f64(const double* x, int64_t ldx, int64_t n, int n_cols, int v_count, int v_pad,
                        uint64_t* colmax_bits, hipStream_t st) {
@@ -1741,7 +1761,7 @@ __global__ __launch_bounds__(256) void k_quant_p2_t(const double* x, int64_t ldx
         int64_t q = 0;
         if (c < ncols && smp < n) {
             const int k = c / v_pad, v = c % v_pad;
-            if (v < v_count) q = __double2ll_rn(ldexp(x[(int64_t)(k * v_count + v) * ldx + smp], shift[c]));
+            if (v < v_count) q = quant_rn(x[(int64_t)(k * v_count + v) * ldx + smp], shift[c]);
         }
         qt[cl][sl] = q;
     }
@@ -1793,6 +1813,35 @@ void launch_quant_p2(const double* x, int64_t ldx, int64_t n, int n_bins, int v_
     GPE_HIP(hipGetLastError());
 }
 
+// Column sums supplied by the caller in place of the exact sums of the quantized
+// operand: col_sum[c] = round(sum[k * v_count + v] 2^shift_c) for the padded
+// column c = k * v_pad + v (the centring term of the reference's Mailman phase 2,
+// src/native/gwldcore.cpp:2965-2967, :874-880).  A non-finite sum makes the
+// column's scale NaN, as the reference's product with it is NaN.
+__global__ void k_col_sum_override(const double* sum, int v_count, int v_pad, int ncols, const int32_t* shift,
+                                   double* col_scale, int64_t* col_sum) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncols) return;
+    const int k = c / v_pad, v = c % v_pad;
+    if (v >= v_count) return;
+    const double s = sum[k * v_count + v];
+    const double t = ldexp(s, shift[c]);
+    if (fabs(t) < 4.0e18) {
+        col_sum[c] = __double2ll_rn(t);
+    } else {
+        col_sum[c] = 0;
+        col_scale[c] = __builtin_nan("");
+    }
+}
+void launch_col_sum_override(const double* sum, int n_bins, int v_count, int v_pad, const int32_t* shift,
+                             double* col_scale, int64_t* col_sum, hipStream_t st) {
+    const int ncols = n_bins * v_pad;
+    if (ncols <= 0) return;
+    hipLaunchKernelGGL(k_col_sum_override, dim3((unsigned)ceil_div(ncols, 256)), dim3(256), 0, st, sum, v_count,
+                       v_pad, ncols, shift, col_scale, col_sum);
+    GPE_HIP(hipGetLastError());
+}
+
 // Limb rows of the phase-2 operand for gathers by sample.  One workgroup per
 // 64-sample x 64-column tile: q and its digits as k_quant_p2_t forms them, staged
 // in LDS and written out so that each sample's nl x 64 bytes are contiguous.
@@ -1813,7 +1862,7 @@ __global__ __launch_bounds__(256) void k_xrow_limbs(const double* __restrict__ x
         int64_t q = 0;
         if (c < ncols && smp < n) {
             const int k = c / v_pad, v = c % v_pad;
-            if (v < v_count) q = __double2ll_rn(ldexp(x[(int64_t)(k * v_count + v) * ldx + smp], shift[c]));
+            if (v < v_count) q = quant_rn(x[(int64_t)(k * v_count + v) * ldx + smp], shift[c]);
         }
         int8_t d[NL];
         balanced_digits<NL>(q, d);
@@ -2564,6 +2613,13 @@ __global__ __launch_bounds__(256) void k_proj_apply(double* __restrict__ x, int6
     for (int k = 0; k < p; ++k) v -= A[i + (int64_t)k * lda] * T[(int64_t)k * ldt + q];
     if (mean) v -= mean[q];
     x[(int64_t)q * ldx + i] = v;
+}
+
+void launch_col_sums(const double* x, int64_t ldx, int64_t n, int q_count, double* S, hipStream_t st) {
+    if (q_count <= 0) return;
+    hipLaunchKernelGGL(k_proj_coef, dim3((unsigned)q_count), dim3(256), 0, st, x, ldx, n, nullptr, (int64_t)0, 0, 0,
+                       nullptr, (int64_t)q_count, S);
+    GPE_HIP(hipGetLastError());
 }
 
 void launch_project(double* x, int64_t ldx, int64_t n, int q_count, const double* A, int64_t lda, const double* B,

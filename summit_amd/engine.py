@@ -234,10 +234,12 @@ def add_i32(value: torch.Tensor, increment: int, stream=None) -> None:
 def phase2_accum(g: Genotype, snp_begin: int, snp_end: int, inv_left: torch.Tensor, tile_nvecs: int,
                  xz: torch.Tensor, meansq: torch.Tensor, n_denom: int, mc_aggregate_sumsq=None,
                  mc_per_snp_sumsq=None, extreme_warn: float = 1.0e10, drop_extreme: bool = True,
-                 limbs: int = 5, meansq_row0: int = 0, stream=None, want_guards: bool = False):
+                 limbs: int = 5, meansq_row0: int = 0, stream=None, want_guards: bool = False, sum_xz=None):
     """meansq[j, k] += sum_v (x_j . Xz[k*V+v])^2 inv_j^2/(N_denom-1)^2 for j in [snp_begin, snp_end).
 
-    ``meansq`` may hold only rows [meansq_row0, ...); it is addressed by absolute SNP index."""
+    ``meansq`` may hold only rows [meansq_row0, ...); it is addressed by absolute SNP index.
+    ``sum_xz`` (device FP64, one value per column of Xz) replaces the column sums of
+    Xz in the centring x_j . Xz_c = inv_sd_j (g_j . Xz_c - mu_j sum_xz[c])."""
     if xz.dtype != torch.float64 or xz.dim() != 2 or xz.shape[1] != g.n:
         raise RuntimeError("Xz2d shape mismatch in phase2_accum_XtXz_bed")
     Q = xz.shape[0]
@@ -253,9 +255,18 @@ def phase2_accum(g: Genotype, snp_begin: int, snp_end: int, inv_left: torch.Tens
     base = _ptr(meansq) - shift
     mcp = 0 if mc_per_snp_sumsq is None else _ptr(mc_per_snp_sumsq) - shift
     counts = (ctypes.c_ulonglong * 2)()
-    call("gpe_phase2_accum", g.handle, snp_begin, snp_end, _ptr(inv_left), tile_nvecs, _ptr(xz), xz.stride(0), B,
-         int(n_denom), base, B, _ptr(mc_aggregate_sumsq), mcp, float(extreme_warn), int(bool(drop_extreme)), limbs,
-         ctypes.cast(counts, c_vp) if want_guards else None, _stream(stream))
+    guards = ctypes.cast(counts, c_vp) if want_guards else None
+    if sum_xz is None:
+        call("gpe_phase2_accum", g.handle, snp_begin, snp_end, _ptr(inv_left), tile_nvecs, _ptr(xz), xz.stride(0),
+             B, int(n_denom), base, B, _ptr(mc_aggregate_sumsq), mcp, float(extreme_warn), int(bool(drop_extreme)),
+             limbs, guards, _stream(stream))
+    else:
+        sum_xz = sum_xz.to(torch.float64).contiguous()
+        if sum_xz.numel() != Q:
+            raise RuntimeError("sum_Xz shape mismatch in phase2_accum_XtXz_bed_mailman_rm")
+        call("gpe_phase2_accum_colsums", g.handle, snp_begin, snp_end, _ptr(inv_left), tile_nvecs, _ptr(xz),
+             xz.stride(0), B, _ptr(sum_xz), int(n_denom), base, B, _ptr(mc_aggregate_sumsq), mcp,
+             float(extreme_warn), int(bool(drop_extreme)), limbs, guards, _stream(stream))
     if want_guards:
         return int(counts[0]), int(counts[1])
     return None
@@ -317,6 +328,16 @@ def project(x_qn: torch.Tensor, A: torch.Tensor | None, B: torch.Tensor | None, 
         A_cm.record_stream(stream)
         B_rm.record_stream(stream)
     return x_qn
+
+
+def col_sums(x_qn: torch.Tensor, stream=None) -> torch.Tensor:
+    """sum over N of each row q of x_qn (Q, N) FP64 (= column sums of column-major (N, Q))."""
+    if x_qn.dtype != torch.float64 or x_qn.dim() != 2 or x_qn.stride(1) != 1:
+        raise RuntimeError("column sums need a (Q, N) FP64 matrix with unit stride along N")
+    Q, N = x_qn.shape
+    out = torch.empty(Q, dtype=torch.float64, device=x_qn.device)
+    call("gpe_col_sums_f64", _ptr(x_qn), x_qn.stride(0), N, Q, _ptr(out), _stream(stream))
+    return out
 
 
 def apply_grm(g: Genotype, panel: torch.Tensor, inv_all: torch.Tensor, nsnps: int | None = None,

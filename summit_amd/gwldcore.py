@@ -13,8 +13,10 @@ libsummit_gpe on the GPU.  Differences a caller can see:
 * Arrays may be NumPy arrays (host, copied to and from the GPU per call) or
   torch tensors already on the GPU (no copies: ``Xz2d`` as a contiguous (Q, N)
   tensor for the column-major layout, ``meansq_accum`` as (M, B)).
-* Covariate projection (``C``/``R``) is applied on the GPU with rocBLAS FP64
-  GEMMs through torch; ``resvar_gram`` is honoured the same way.
+* Covariate projection (``C``/``R``), column sums and ``resvar_gram`` run on
+  the GPU in the library's own FP64 kernels (gpe_project_f64, gpe_col_sums_f64,
+  the phase-2 contraction); the Cholesky factor of the small p x p
+  ``resvar_gram`` is taken on the host in FP64.
 """
 from __future__ import annotations
 
@@ -377,27 +379,30 @@ def residual_inv_block(g: engine.Genotype, s: int, e: int, eps: float, C=None, R
     df = float(N - p - 1)
     ones = torch.ones(e - s, dtype=torch.float64, device=_dev())
     # plain ||x_j||^2 from the integer moments: inv0 = 1/sqrt(max(ss/(N-1), eps)) -> recover ss
-    inv0 = engine.resvar_inv(g, s, e, 0.0)
-    ss = (float(N - 1)) / (inv0 * inv0)
-    ss = torch.where(torch.isfinite(ss), ss, torch.zeros_like(ss))
+    inv0 = engine.resvar_inv(g, s, e, 0.0).cpu().numpy()
+    with np.errstate(divide="ignore", invalid="ignore"):
+        ss = float(N - 1) / (inv0 * inv0)
+    ss = np.where(np.isfinite(ss), ss, 0.0)
 
-    def sumsq_rows(Mrows: torch.Tensor) -> torch.Tensor:
+    def sumsq_rows(Mrows: torch.Tensor) -> np.ndarray:
         acc = torch.zeros((g.m, 1), dtype=torch.float64, device=_dev())
         engine.phase2_accum(g, s, e, ones, Mrows.shape[0], Mrows.contiguous(), acc, n_denom=2,
                             extreme_warn=0.0, limbs=max(LIMBS, 6))
-        return acc[s:e, 0]
+        return acc[s:e, 0].cpu().numpy()
 
     h = sumsq_rows(Rd)
     if gram is not None:
-        Gd = _to_dev(gram)
-        Lc = torch.linalg.cholesky(Gd)
-        q = sumsq_rows(Lc.T @ Rd)
+        # q = (R x)^T G (R x) = ||L^T R x||^2 with G = L L^T (p x p, host FP64)
+        Gh = np.asarray(gram.cpu().numpy() if isinstance(gram, torch.Tensor) else gram, dtype=np.float64)
+        Rh = np.asarray(R.cpu().numpy() if isinstance(R, torch.Tensor) else R, dtype=np.float64)
+        Lc = np.linalg.cholesky(Gh)
+        q = sumsq_rows(_to_dev(Lc.T @ Rh))
     else:
         q = h
-    ssp = ss + q - 2.0 * h
-    var = ssp / df
-    safe = torch.where(torch.isnan(var), var, torch.clamp(var, min=eps))
-    return 1.0 / torch.sqrt(safe)
+    var = (ss + q - 2.0 * h) / df
+    with np.errstate(invalid="ignore"):
+        safe = np.where(np.isnan(var), var, np.maximum(var, eps))
+    return _to_dev(1.0 / np.sqrt(safe))
 
 
 def precompute_residual_variances_bed(bed_prefix, fam_path, nsnps, step_size, row_sel=None, ddof=1, eps=1e-10,
@@ -422,7 +427,7 @@ def precompute_residual_variances_bed(bed_prefix, fam_path, nsnps, step_size, ro
 
 # --------------------------------------------------------------------------- phase 2
 def _phase2(bed_prefix, fam_path, blk_start, blk_end, row_sel, ddof, inv_left, tile_nvecs, Xz2d, meansq_accum, C, R,
-            N_denom, impute_mode, impute_seed, mc_aggregate_sumsq, mc_per_snp_sumsq, rowmajor):
+            N_denom, impute_mode, impute_seed, mc_aggregate_sumsq, mc_per_snp_sumsq, rowmajor, sum_Xz=None):
     g, n_total, m_total = _geno(bed_prefix, fam_path, row_sel, ddof, impute_mode, impute_seed)
     blk_start, blk_end = int(blk_start), int(blk_end)
     if blk_end > m_total:
@@ -454,6 +459,12 @@ def _phase2(bed_prefix, fam_path, blk_start, blk_end, row_sel, ddof, inv_left, t
         raise RuntimeError("mc_aggregate_sumsq length must equal the annotation count")
     if mc_per_snp_sumsq is not None and tuple(mc_per_snp_sumsq.shape) != tuple(meansq_accum.shape):
         raise RuntimeError("mc_per_snp_sumsq shape must match meansq_accum")
+    sum_d = None
+    if sum_Xz is not None:
+        sum_np = sum_Xz.cpu().numpy() if isinstance(sum_Xz, torch.Tensor) else np.asarray(sum_Xz)
+        if sum_np.ndim != 1 or sum_np.shape[0] != Q:
+            raise RuntimeError("sum_Xz shape mismatch in phase2_accum_XtXz_bed_mailman_rm")
+        sum_d = _to_dev(sum_np)
     if C is not None and R is not None:
         xz = _project_t_dev(xz, C, R)
     inv_d = _to_dev(inv_left)
@@ -472,7 +483,7 @@ def _phase2(bed_prefix, fam_path, blk_start, blk_end, row_sel, ddof, inv_left, t
     nonfin, extreme = engine.phase2_accum(
         g, blk_start, blk_end, inv_d, int(tile_nvecs), xz, ms, int(N_denom), mc_aggregate_sumsq=mca,
         mc_per_snp_sumsq=mcp, extreme_warn=EXTREME_LD_WARN, drop_extreme=DROP_EXTREME, limbs=LIMBS,
-        meansq_row0=row0, want_guards=True)
+        meansq_row0=row0, want_guards=True, sum_xz=sum_d)
     if not dev_out:
         upd = ms.cpu().numpy()
         meansq_accum[blk_start:blk_end] += upd.astype(meansq_accum.dtype, copy=False)
@@ -507,29 +518,44 @@ def phase2_accum_XtXz_bed_mailman(bed_prefix, fam_path, blk_start, blk_end, row_
 def phase2_accum_XtXz_bed_mailman_rowmajor(bed_prefix, fam_path, blk_start, blk_end, row_sel=None, ddof=1, *,
                                            inv_left, tile_nvecs, Xz2d, meansq_accum, sum_Xz, N_denom=0,
                                            impute_seed=None, mc_aggregate_sumsq=None, mc_per_snp_sumsq=None):
-    """Row-major HWE phase 2 (src/native/gwldcore.cpp:2919-3119); sum_Xz is recomputed exactly on the GPU."""
+    """Row-major HWE phase 2 (src/native/gwldcore.cpp:2919-3119): each column c is centred
+    with the caller's sum_Xz[c] (the column sums of the projected X.Z the reference's
+    driver passes, src/summit/ldscore/gw_ldscore.py:2010-2016)."""
+    if sum_Xz is None:
+        raise RuntimeError("sum_Xz shape mismatch in phase2_accum_XtXz_bed_mailman_rm")
     _phase2(bed_prefix, fam_path, blk_start, blk_end, row_sel, ddof, inv_left, tile_nvecs, Xz2d, meansq_accum,
-            None, None, N_denom, "hwe", impute_seed, mc_aggregate_sumsq, mc_per_snp_sumsq, rowmajor=True)
+            None, None, N_denom, "hwe", impute_seed, mc_aggregate_sumsq, mc_per_snp_sumsq, rowmajor=True,
+            sum_Xz=sum_Xz)
 
 
 # --------------------------------------------------------------------------- column helpers
+def _rowmajor_qn_dev(Xz2d):
+    """C-contiguous (N, Q) array or tensor -> contiguous (Q, N) FP64 device tensor."""
+    if isinstance(Xz2d, torch.Tensor):
+        if Xz2d.dim() != 2:
+            raise RuntimeError("Xz2d must be a 2D array")
+        return Xz2d.to(device=_dev(), dtype=torch.float64).T.contiguous()
+    a = np.asarray(Xz2d)
+    if a.ndim != 2:
+        raise RuntimeError("Xz2d must be a 2D array")
+    return torch.from_numpy(np.ascontiguousarray(a.T, dtype=np.float64)).to(_dev())
+
+
 def compute_col_sums_rowmajor(Xz2d):
     """Column sums of a C-contiguous (N, Q) array (src/native/gwldcore.cpp:3121-3132)."""
-    x = Xz2d if isinstance(Xz2d, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(Xz2d, dtype=np.float64))
-    return x.to(_dev(), torch.float64).sum(dim=0).cpu().numpy()
+    return engine.col_sums(_rowmajor_qn_dev(Xz2d)).cpu().numpy()
 
 
 def project_rowmajor_inplace_and_col_sums(Xz2d, C=None, R=None):
     """Xz <- (I - C R) Xz for C-contiguous (N, Q), then column sums (src/native/gwldcore.cpp:3134-3171)."""
-    x = torch.from_numpy(np.ascontiguousarray(Xz2d, dtype=np.float64)).to(_dev()) \
-        if not isinstance(Xz2d, torch.Tensor) else Xz2d.to(_dev(), torch.float64)
+    x = _rowmajor_qn_dev(Xz2d)
     if C is not None and R is not None:
-        x = _project_dev(x.T, C, R).T
+        x = _project_dev(x, C, R)
         if isinstance(Xz2d, torch.Tensor):
-            Xz2d.copy_(x.to(Xz2d.dtype))
+            Xz2d.copy_(x.T.to(Xz2d.dtype))
         else:
-            Xz2d[...] = x.cpu().numpy().astype(Xz2d.dtype, copy=False)
-    return x.sum(dim=0).cpu().numpy()
+            Xz2d[...] = x.cpu().numpy().T.astype(Xz2d.dtype, copy=False)
+    return engine.col_sums(x).cpu().numpy()
 
 
 def project_colmajor_inplace(Xz2d, C=None, R=None) -> None:

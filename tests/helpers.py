@@ -18,3 +18,17 @@ def random_codes(m: int, n: int, seed: int, miss_rate: float = 0.02, maf_lo: flo
     for j in range(min(all_missing, m)):
         codes[m - 1 - j, :] = 1
     return codes
+
+
+def assert_ld_close(ld, ref, null, rtol: float = 1e-10):
+    """LD scores L = meansq/V - null_k against the oracle: |dL| <= rtol (|L| + null_k)
+    per element, i.e. rtol relative to the accumulated statistic meansq/V = L + null_k
+    that the estimator computes (the null subtraction M_k/N_denom can cancel it to
+    near zero, where a bare relative bar on L would measure the cancellation)."""
+    ld = np.asarray(ld, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    scale = np.abs(ref) + np.abs(np.asarray(null, dtype=np.float64))[None, :]
+    err = np.abs(ld - ref)
+    bad = err > rtol * scale
+    assert not np.any(bad), (f"{int(bad.sum())} LD scores beyond {rtol:g} relative; worst "
+                             f"{float(np.max(err / np.maximum(scale, 1e-300))):.3g}")
