@@ -283,6 +283,18 @@ int gpe_phase1_xz_z8(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const doub
 int gpe_project_f64(double* x_dev, int64_t ldx, int64_t n, int32_t n_cols, const double* a_dev, int64_t lda,
                     const double* b_dev, int64_t ldb, int32_t p, int32_t center, void* stream);
 
+/* gpe_project_f64 without any allocation: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments, kernels
+ * and results as gpe_project_f64; its partial results live in one fixed 4 MiB
+ * device array per GPU, held until the process ends, so a call needs
+ * (max(p,1) + 2) * n_cols + max(p,1) <= 524288 (larger sizes return GPE_EINVAL).
+ * Every call on one GPU shares that array: calls that overlap on one GPU (other
+ * streams or threads) race and must be serialized by the caller. */
+int gpe_project_f64_noalloc(double* x_dev, int64_t ldx, int64_t n, int32_t n_cols, const double* a_dev,
+                            int64_t lda, const double* b_dev, int64_t ldb, int32_t p, int32_t center,
+                            void* stream);
+
 /* out_dev[q] = sum_{i < n} x_dev[q * ldx + i] for q < n_cols (FP64, fixed-order
  * reduction): the column sums of a row-major (N, Q) X.Z held column by column.
  * Replaces compute_col_sums_rowmajor (src/native/gwldcore.cpp:3121-3132,

@@ -51,6 +51,8 @@ SIGNATURES = {
     "gpe_phase1_xz_z8": (ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_i32, c_vp, c_vp, c_i64, c_i32, c_vp, c_i64, c_vp,
                                         c_i32, c_vp]),
     "gpe_project_f64": (ctypes.c_int, [c_vp, c_i64, c_i64, c_i32, c_vp, c_i64, c_vp, c_i64, c_i32, c_i32, c_vp]),
+    "gpe_project_f64_noalloc": (ctypes.c_int, [c_vp, c_i64, c_i64, c_i32, c_vp, c_i64, c_vp, c_i64, c_i32, c_i32,
+                                               c_vp]),
     "gpe_col_sums_f64": (ctypes.c_int, [c_vp, c_i64, c_i64, c_i32, c_vp, c_vp]),
     "gpe_weighted_sumsq": (ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_i32, c_i64, c_vp, c_i64, c_vp]),
     "gpe_windowed_ld": (ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_dbl, c_i64, c_vp, c_i32, c_vp, c_vp]),

@@ -1561,6 +1561,60 @@ int gpe_project_f64(double* x_dev, int64_t ldx, int64_t n, int32_t n_cols, const
     });
 }
 
+}  // extern "C"
+
+namespace {
+
+// Scratch of gpe_project_f64_noalloc: one fixed array per device, kept for the
+// process.  Host code must never pass the symbol itself to a kernel (on the host
+// it names a stand-in, not device memory): the device address is looked up once
+// per device with hipGetSymbolAddress and only that pointer is used.
+constexpr size_t kNoAllocScratchDoubles = (size_t)1 << 19;  // 4 MiB
+__device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+double* noalloc_scratch_ptr() {
+    static std::mutex mu;
+    static std::map<int, double*> ptrs;
+    int dev = 0;
+    GPE_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = ptrs.find(dev);
+    if (it != ptrs.end()) return it->second;
+    void* p = nullptr;
+    const hipError_t rc = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (rc != hipSuccess || p == nullptr)
+        throw Error(std::string("HIP error: hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(rc));
+    ptrs[dev] = static_cast<double*>(p);
+    return static_cast<double*>(p);
+}
+
+}  // namespace
+
+extern "C" {
+
+int gpe_project_f64_noalloc(double* x_dev, int64_t ldx, int64_t n, int32_t n_cols, const double* a_dev,
+                            int64_t lda, const double* b_dev, int64_t ldb, int32_t p, int32_t center,
+                            void* stream) {
+    return guarded([&] {
+        require(n >= 0 && n_cols >= 0 && p >= 0, "projection sizes must be non-negative");
+        require(n_cols <= 65535, "at most 65535 columns per projection call");
+        const size_t need = (size_t)std::max(p, 1) * (size_t)n_cols + 2 * (size_t)n_cols + (size_t)std::max(p, 1);
+        require(need <= kNoAllocScratchDoubles,
+                "projection needs " + std::to_string(need) + " scratch doubles; the allocation-free entry holds " +
+                    std::to_string(kNoAllocScratchDoubles) + " ((max(p,1)+2) * n_cols + max(p,1) <= 524288)");
+        if (n == 0 || n_cols == 0 || (p == 0 && !center)) return;
+        require(x_dev != nullptr && ldx >= n, "projected matrix needs ldx >= n");
+        if (p > 0) require(a_dev != nullptr && b_dev != nullptr && lda >= n && ldb >= n, "projection factor shapes");
+        // every scratch value is written by this call's kernels before it is read
+        // (coefficients, column sums and means), so nothing is cleared
+        double* T = noalloc_scratch_ptr();
+        double* Ssum = T + (size_t)std::max(p, 1) * n_cols;
+        double* mean = Ssum + n_cols;
+        double* a = mean + n_cols;
+        launch_project(x_dev, ldx, n, n_cols, a_dev, lda, b_dev, ldb, p, center != 0, T, Ssum, a, mean, S(stream));
+    });
+}
+
 int gpe_col_sums_f64(const double* x_dev, int64_t ldx, int64_t n, int32_t n_cols, double* out_dev, void* stream) {
     return guarded([&] {
         require(n >= 0 && n_cols >= 0, "column-sum sizes must be non-negative");
