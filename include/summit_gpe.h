@@ -245,6 +245,17 @@ int gpe_xtv_f64(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const double* v
 int gpe_windowed_ld(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const int64_t* bp_host, double window_bp,
                     int64_t window_snps, const double* annot_dev, int32_t n_bins, double* out_dev, void* stream);
 
+/* gpe_windowed_ld with covariates, as compute_windowed_ld_chr with C and R
+ * (src/native/winldcore.cpp:783-931, panels :341-372): each SNP's ddof-0
+ * standardized column x is projected, y = x - C (R x), and restandardized over
+ * all N rows (centred, population variance 1), r_ij the correlation of the
+ * projected columns, r~2 = r^2 - (1 - r^2)/(N - p - 1).  C column-major (N, p)
+ * (c_dev[k * ldc + n]), R row-major (p, N) (r_dev[k * ldr + n]), device FP64.
+ * The exact integer Gram gets a rank-2p FP64 correction per pair.  Synchronizes. */
+int gpe_windowed_ld_cov(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const int64_t* bp_host, double window_bp,
+                        int64_t window_snps, const double* annot_dev, int32_t n_bins, const double* c_dev,
+                        int64_t ldc, const double* r_dev, int64_t ldr, int32_t p, double* out_dev, void* stream);
+
 /* The same for n_chr chromosomes in ONE banded-Gram launch (every
  * chromosome's tile pairs in one work list): chromosome c is SNPs
  * [chr_bounds_host[2c], chr_bounds_host[2c+1]) (sorted, non-overlapping);

@@ -541,13 +541,23 @@ def restandardize_cols(X):
 
 
 def windowed_ld_chr(bed, n_total: int, chr_start: int, chr_end: int, bp, annot_chr, ld_wind_kb: float = None,
-                    rows=None, impute: str = "mean", impute_seed: int = 0, window_snps: int = 0):
-    """L[j, b] = sum_{i in window(j)} (r_ij^2 - (1 - r_ij^2)/(N - 2)) a[i, b]."""
+                    rows=None, impute: str = "mean", impute_seed: int = 0, window_snps: int = 0, C=None, R=None):
+    """L[j, b] = sum_{i in window(j)} (r_ij^2 - (1 - r_ij^2)/(corr_dim - 1)) a[i, b].
+
+    With covariates C (N, p) and R (p, N) every panel is projected, X <- X - C (R X),
+    and restandardized (winldcore.cpp:341-372); corr_dim = N - p, else N - 1
+    (winldcore.cpp:848-851)."""
     X = read_block_standardized(bed, n_total, chr_start, chr_end, rows, 0, impute, impute_seed)
-    if impute == "mean":
+    p = 0
+    if C is not None and R is not None:
+        C = np.asarray(C, dtype=np.float64)
+        R = np.asarray(R, dtype=np.float64)
+        p = C.shape[1]
+        X = restandardize_cols(X - C @ (R @ X))
+    elif impute == "mean":
         X = restandardize_cols(X)
     N = X.shape[0]
-    corr_dim = N - 1
+    corr_dim = N - (p if p > 0 else 1)
     r = (X.T @ X) / N
     r2 = r * r
     r2u = r2 - (1.0 - r2) / (corr_dim - 1)

@@ -1202,12 +1202,22 @@ void copy_rows(const double* src, int64_t lds, int64_t rows, int cols, double* d
 // non-overlapping, absolute SNP ids) in ONE banded-Gram launch: every
 // chromosome's 64x64 tile pairs within the window go into one work list.
 // Rows of bp (when used), annot and out are SNP ids relative to bounds[0].
+// Covariates (p > 0): C column-major (N, p) with ldc, R row-major (p, N) with ldr;
+// every panel is projected, X <- X - C (R X), and restandardized
+// (src/native/winldcore.cpp:341-372), corr_dim = N - p (:848-851).  The projection
+// enters the exact integer Gram as a rank-2p FP64 correction of each pair's cross
+// product plus per-SNP means and scales (k_win_cov_prep).
 void windowed_ld_impl(gpe_geno* g, int n_chr, const int64_t* bounds, const int64_t* bp_host, double window_bp,
                       int64_t window_snps, const double* annot_dev, int32_t n_bins, double* out_dev,
-                      hipStream_t st) {
+                      hipStream_t st, const double* c_dev = nullptr, int64_t ldc = 0, const double* r_dev = nullptr,
+                      int64_t ldr = 0, int p = 0) {
     require(n_bins > 0, "annot_chr must have at least one column");
     require(g->n > 2, "Too few rows for windowed LD score computation");
-    const double corr_dim = (double)g->n - 1.0;
+    require(p >= 0 && p <= 1024, "covariate count out of range");
+    if (p > 0)
+        require(c_dev != nullptr && r_dev != nullptr && ldc >= g->n && ldr >= g->n,
+                "C/R shape mismatch in compute_windowed_ld_chr");
+    const double corr_dim = (double)g->n - (p > 0 ? (double)p : 1.0);
     require(corr_dim > 1.0, "Too few residual dimensions for unbiased windowed r^2 correction");
     const int64_t row0 = bounds[0];
     for (int c = 0; c < n_chr; ++c) {
@@ -1280,6 +1290,35 @@ void windowed_ld_impl(gpe_geno* g, int n_chr, const int64_t* bounds, const int64
     a.n_bins = n_bins;
     a.out = out_dev;
     a.corr_dim = corr_dim;
+    if (p > 0) {
+        // u = C^T x_j and v = R x_j for every SNP of the rows (exact-limb FP64 products),
+        // then the per-SNP correction rows, means and scales of the projected columns
+        double* u = static_cast<double*>(g->scratch("win_u", (size_t)rows * p * sizeof(double)));
+        double* v = static_cast<double*>(g->scratch("win_v", (size_t)rows * p * sizeof(double)));
+        for (int pass = 0; pass < 2; ++pass) {
+            ContractArgs w{};
+            w.out_f64 = pass == 0 ? u : v;
+            w.ldo = p;
+            w.cols_out = p;
+            w.out_row0 = row0;
+            run_phase2(g, row0, row0 + rows, pass == 0 ? c_dev : r_dev, pass == 0 ? ldc : ldr, 1, p, 6, EPI_WORK, w,
+                       st, nullptr, nullptr);
+        }
+        double* G = static_cast<double*>(g->scratch("win_G", ((size_t)p * p + p) * sizeof(double)));
+        double* csum = G + (size_t)p * p;
+        launch_cov_gram(c_dev, ldc, g->n, p, G, csum, st);
+        double* A = static_cast<double*>(g->scratch("win_A", (size_t)rows * 2 * p * sizeof(double)));
+        double* B = static_cast<double*>(g->scratch("win_B", (size_t)rows * 2 * p * sizeof(double)));
+        double* yb = static_cast<double*>(g->scratch("win_ybar", (size_t)rows * 2 * sizeof(double)));
+        launch_win_cov_prep(u, v, p, G, csum, g->nobs, g->sum, g->sumsq, g->inv_sd, completed, row0, row0 + rows,
+                            g->n, A, B, yb, yb + rows, st);
+        a.scale = g->inv_sd;
+        a.cov_a = A;
+        a.cov_b = B;
+        a.cov_k = 2 * p;
+        a.ybar = yb;
+        a.tsc = yb + rows;
+    }
     const bool miss = g->impute_mode == GPE_IMPUTE_MEAN && g->n_missing > 0;
     ProfScope ps(GPE_PROF_GRAM, st);
     launch_gram_band(a, np, miss, st);
@@ -1697,6 +1736,18 @@ int gpe_windowed_ld(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const int64
         require(snp_end - snp_begin > 0, "Chromosome slice is empty");
         const int64_t bounds[2] = {snp_begin, snp_end};
         windowed_ld_impl(g, 1, bounds, bp_host, window_bp, window_snps, annot_dev, n_bins, out_dev, S(stream));
+    });
+}
+
+int gpe_windowed_ld_cov(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const int64_t* bp_host, double window_bp,
+                        int64_t window_snps, const double* annot_dev, int32_t n_bins, const double* c_dev,
+                        int64_t ldc, const double* r_dev, int64_t ldr, int32_t p, double* out_dev, void* stream) {
+    return guarded([&] {
+        require_prepared(g);
+        require(p > 0, "C/R shape mismatch in compute_windowed_ld_chr");
+        const int64_t b[2] = {snp_begin, snp_end};
+        windowed_ld_impl(g, 1, b, bp_host, window_bp, window_snps, annot_dev, n_bins, out_dev, S(stream), c_dev, ldc,
+                         r_dev, ldr, p);
     });
 }
 

@@ -105,6 +105,13 @@ struct GramArgs {
     int n_bins;
     double* out;           // row-major (rows, n_bins), relative to row0
     double corr_dim;
+    // covariates (cov_k > 0): r_ij = (s_i s_j dot + cov_a[i] . cov_b[j] - N ybar_i ybar_j) t_i t_j / N,
+    // per-SNP rows of cov_k values indexed by absolute SNP minus row0
+    const double* cov_a;
+    const double* cov_b;
+    int cov_k;
+    const double* ybar;
+    const double* tsc;
 };
 
 // Launchers (gpe_kernels.hip)
@@ -200,6 +207,16 @@ void launch_synth_markov(uint8_t* snp, int64_t stride, int64_t m, int64_t n, uin
                          double maf_hi, double miss_rate, int block_len, double copy_prob, int iid,
                          hipStream_t st);
 void launch_gram_band(const GramArgs& a, int64_t n_pairs, bool miss, hipStream_t st);
+// G[k * p + q] = sum_i C[k ldc + i] C[q ldc + i] and csum[k] = sum_i C[k ldc + i], i < n
+void launch_cov_gram(const double* C, int64_t ldc, int64_t n, int p, double* G, double* csum, hipStream_t st);
+// Per-SNP terms of the projected, restandardized windowed panel for SNPs [s, e)
+// (rows relative to s): u = C^T x_j, v = R x_j (row-major (L, p)), ||x_j||^2 from the
+// integer moments and inv_sd; A_j = [G v_j - u_j, -v_j], B_j = [v_j, u_j], ybar_j =
+// -(C^T 1) . v_j / n, t_j = 1 / sd of the projected column (1 when it has no variance)
+void launch_win_cov_prep(const double* u, const double* v, int p, const double* G, const double* csum,
+                         const int32_t* nobs, const int64_t* sum, const int64_t* sumsq, const double* inv_sd,
+                         int completed, int64_t s, int64_t e, int64_t n, double* A, double* B, double* ybar,
+                         double* tsc, hipStream_t st);
 void launch_philox_rademacher(const uint64_t* keys, const int64_t* starts, const int64_t* lens, int n_blocks,
                               int v_count, int64_t max_len, int64_t row0, int8_t* z, int64_t ldz, hipStream_t st);
 void launch_weighted_sumsq(const uint8_t* snp, int64_t stride, int64_t r0, int64_t r1, int64_t n, const double* mean,

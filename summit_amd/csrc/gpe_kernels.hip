@@ -2172,7 +2172,7 @@ void launch_quant_i32(const int32_t* u, int64_t ldu, int64_t n, int n_cols, int 
 // ---------------------------------------------------------------------------
 
 
-template <bool MISS>
+template <bool MISS, bool COV>
 __global__ __launch_bounds__(256) void k_gram_band(GramArgs a) {
     __shared__ double r2t[4][32][33];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, half = lane >> 5, lr = lane & 31;
@@ -2239,7 +2239,19 @@ __global__ __launch_bounds__(256) void k_gram_band(GramArgs a) {
                 } else {
                     dot = (double)gg[e] - mu_j * sum_i - mu_i * sum_j + mu_i * mu_j * dn;
                 }
-                const double r = a.scale[i] * s_j * dot / dn;
+                double r;
+                if (COV) {
+                    // projected panel: cross product of the projected columns, minus their
+                    // means, times the restandardizing scales
+                    const double* ai = a.cov_a + ir * a.cov_k;
+                    const double* bj = a.cov_b + jr * a.cov_k;
+                    double corr = 0.0;
+                    for (int k = 0; k < a.cov_k; ++k) corr = fma(ai[k], bj[k], corr);
+                    const double yy = a.scale[i] * s_j * dot + corr - dn * a.ybar[ir] * a.ybar[jr];
+                    r = yy * a.tsc[ir] * a.tsc[jr] / dn;
+                } else {
+                    r = a.scale[i] * s_j * dot / dn;
+                }
                 const double r2 = r * r;
                 v = r2 - (1.0 - r2) / (a.corr_dim - 1.0);
             }
@@ -2282,12 +2294,106 @@ void launch_gram_band(const GramArgs& a, int64_t n_pairs, bool miss, hipStream_t
         GramArgs b = a;
         b.pairs = a.pairs + 3 * p0;
         const int64_t np = (n_pairs - p0) < (1 << 30) ? (n_pairs - p0) : (1 << 30);
-        if (miss)
-            hipLaunchKernelGGL(k_gram_band<true>, dim3((unsigned)np), dim3(256), 0, st, b);
+        const bool cov = a.cov_k > 0;
+        if (miss && cov)
+            hipLaunchKernelGGL((k_gram_band<true, true>), dim3((unsigned)np), dim3(256), 0, st, b);
+        else if (miss)
+            hipLaunchKernelGGL((k_gram_band<true, false>), dim3((unsigned)np), dim3(256), 0, st, b);
+        else if (cov)
+            hipLaunchKernelGGL((k_gram_band<false, true>), dim3((unsigned)np), dim3(256), 0, st, b);
         else
-            hipLaunchKernelGGL(k_gram_band<false>, dim3((unsigned)np), dim3(256), 0, st, b);
+            hipLaunchKernelGGL((k_gram_band<false, false>), dim3((unsigned)np), dim3(256), 0, st, b);
         GPE_HIP(hipGetLastError());
     }
+}
+
+// G = C^T C and C^T 1 for the covariate projection of windowed panels: one
+// workgroup per (k, q) entry with q >= k (the k == q workgroup also forms the sum).
+__global__ __launch_bounds__(256) void k_cov_gram(const double* C, int64_t ldc, int64_t n, int p, double* G,
+                                                  double* csum) {
+    __shared__ double red[2][4];
+    const int k = blockIdx.x, q = blockIdx.y;
+    if (q < k) return;
+    const int tid = threadIdx.x;
+    const double* ck = C + (int64_t)k * ldc;
+    const double* cq = C + (int64_t)q * ldc;
+    double acc = 0.0, sum = 0.0;
+    for (int64_t i = tid; i < n; i += 256) {
+        acc = fma(ck[i], cq[i], acc);
+        sum += ck[i];
+    }
+    acc = wave_sum_f64(acc);
+    sum = wave_sum_f64(sum);
+    if ((tid & 63) == 0) {
+        red[0][tid >> 6] = acc;
+        red[1][tid >> 6] = sum;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const double g = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+        G[k * p + q] = g;
+        G[q * p + k] = g;
+        if (q == k) csum[k] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+    }
+}
+void launch_cov_gram(const double* C, int64_t ldc, int64_t n, int p, double* G, double* csum, hipStream_t st) {
+    if (p <= 0) return;
+    hipLaunchKernelGGL(k_cov_gram, dim3((unsigned)p, (unsigned)p), dim3(256), 0, st, C, ldc, n, p, G, csum);
+    GPE_HIP(hipGetLastError());
+}
+
+__global__ void k_win_cov_prep(const double* u, const double* v, int p, const double* G, const double* csum,
+                               const int32_t* nobs, const int64_t* sum, const int64_t* sumsq, const double* inv_sd,
+                               int completed, int64_t s, int64_t e, int64_t n, double* A, double* B, double* ybar,
+                               double* tsc) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t j = s + r;
+    if (j >= e) return;
+    const double* uj = u + r * p;
+    const double* vj = v + r * p;
+    double* Aj = A + r * 2 * p;
+    double* Bj = B + r * 2 * p;
+    // ||x_j||^2 = inv_sd^2 M2 with M2 the centred sum of squares of the (observed) counts
+    const double cnt = completed ? (double)n : (double)nobs[j];
+    double M2 = 0.0;
+    if (cnt > 0) {
+        const double sm = (double)sum[j];
+        M2 = (double)sumsq[j] - sm * sm / cnt;
+        if (M2 < 0.0) M2 = 0.0;
+    }
+    const double isd = inv_sd[j];
+    double yy = isd * isd * M2, cv = 0.0;
+    for (int k = 0; k < p; ++k) {
+        double gv = 0.0;
+        for (int q = 0; q < p; ++q) gv = fma(G[k * p + q], vj[q], gv);
+        yy += vj[k] * (gv - 2.0 * uj[k]);
+        cv = fma(csum[k], vj[k], cv);
+        Aj[k] = gv - uj[k];
+        Aj[p + k] = -vj[k];
+        Bj[k] = vj[k];
+        Bj[p + k] = uj[k];
+    }
+    // the column sum of x_j is 0 (centred on its own mean), so that of y_j is -(C^T 1) . v_j
+    const double ys = -cv;
+    const double dn = (double)n;
+    ybar[r] = ys / dn;
+    double ss = yy - ys * ys / dn;
+    if (ss < 0.0 && ss > -1e-12) ss = 0.0;
+    double t = 1.0;
+    if (ss > 0.0 && isfinite(ss)) {
+        const double var = ss / dn;
+        if (var > 0.0 && isfinite(var)) t = 1.0 / sqrt(var);
+    }
+    tsc[r] = t;
+}
+void launch_win_cov_prep(const double* u, const double* v, int p, const double* G, const double* csum,
+                         const int32_t* nobs, const int64_t* sum, const int64_t* sumsq, const double* inv_sd,
+                         int completed, int64_t s, int64_t e, int64_t n, double* A, double* B, double* ybar,
+                         double* tsc, hipStream_t st) {
+    if (e <= s) return;
+    hipLaunchKernelGGL(k_win_cov_prep, dim3((unsigned)ceil_div(e - s, 128)), dim3(128), 0, st, u, v, p, G, csum,
+                       nobs, sum, sumsq, inv_sd, completed, s, e, n, A, B, ybar, tsc);
+    GPE_HIP(hipGetLastError());
 }
 
 // ---------------------------------------------------------------------------

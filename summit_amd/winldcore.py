@@ -5,8 +5,10 @@ src/native/winldcore.cpp:964-994 and returns an (m, B) Fortran FP64 array.
 The banded Gram runs on int8 MFMA with exact integer cross products
 (``gpe_windowed_ld``); panel caching (panel_cols, cache_mb) has no role on a
 device-resident matrix and those arguments are accepted and ignored.
-Covariate projection is not provided (it re-standardizes after a dense
-projection, src/native/winldcore.cpp:362-365).
+With covariates (``C`` (N, p) and ``R`` (p, N) both given) every SNP column is
+projected and restandardized as the reference's panels are
+(src/native/winldcore.cpp:341-372), with corr_dim = N - p
+(``gpe_windowed_ld_cov``).
 """
 from __future__ import annotations
 
@@ -32,8 +34,9 @@ def progress_active() -> bool:
 
 
 def windowed_ld(g: engine.Genotype, snp_begin: int, snp_end: int, bp=None, ld_wind_kb: float | None = None,
-                annot=None, window_snps: int = 0) -> np.ndarray:
-    """Device-level call: (m, B) FP64 windowed LD scores of SNPs [snp_begin, snp_end)."""
+                annot=None, window_snps: int = 0, C=None, R=None) -> np.ndarray:
+    """Device-level call: (m, B) FP64 windowed LD scores of SNPs [snp_begin, snp_end);
+    with C (N, p) and R (p, N) the panels are projected off the covariates."""
     m = snp_end - snp_begin
     if annot is None:
         annot = np.ones((m, 1))
@@ -49,9 +52,21 @@ def windowed_ld(g: engine.Genotype, snp_begin: int, snp_end: int, bp=None, ld_wi
     a_d = torch.from_numpy(annot).to(engine.default_device())
     out = torch.empty((m, B), dtype=torch.float64, device=a_d.device)
     window_bp = float(ld_wind_kb) * 1000.0 if ld_wind_kb is not None else 0.0
-    engine.call("gpe_windowed_ld", g.handle, int(snp_begin), int(snp_end),
-                0 if bp_arr is None else int(bp_arr.ctypes.data), window_bp, int(window_snps), engine._ptr(a_d), B,
-                engine._ptr(out), engine._stream())
+    bp_ptr = 0 if bp_arr is None else int(bp_arr.ctypes.data)
+    if C is not None and R is not None:
+        Cn = np.asarray(C.cpu().numpy() if isinstance(C, torch.Tensor) else C, dtype=np.float64)
+        Rn = np.asarray(R.cpu().numpy() if isinstance(R, torch.Tensor) else R, dtype=np.float64)
+        if Cn.ndim != 2 or Rn.ndim != 2 or Cn.shape[0] != g.n or Rn.shape != (Cn.shape[1], g.n):
+            raise RuntimeError("C/R shape mismatch in compute_windowed_ld_chr")
+        p = Cn.shape[1]
+        c_d = torch.from_numpy(np.ascontiguousarray(Cn.T)).to(a_d.device)  # column k at k * N
+        r_d = torch.from_numpy(np.ascontiguousarray(Rn)).to(a_d.device)    # row k at k * N
+        engine.call("gpe_windowed_ld_cov", g.handle, int(snp_begin), int(snp_end), bp_ptr, window_bp,
+                    int(window_snps), engine._ptr(a_d), B, engine._ptr(c_d), g.n, engine._ptr(r_d), g.n, p,
+                    engine._ptr(out), engine._stream())
+        return out.cpu().numpy()
+    engine.call("gpe_windowed_ld", g.handle, int(snp_begin), int(snp_end), bp_ptr, window_bp, int(window_snps),
+                engine._ptr(a_d), B, engine._ptr(out), engine._stream())
     return out.cpu().numpy()
 
 
@@ -102,14 +117,16 @@ def compute_windowed_ld_chr(bed_prefix, fam_path, chr_start, chr_end, bp, annot_
     wbp = float(ld_wind_kb) * 1000.0
     if not np.isfinite(wbp) or wbp <= 0.0:
         raise RuntimeError("ld_wind_kb must define a finite positive BP window")
-    if C is not None or R is not None:
-        raise RuntimeError("covariate projection is not provided by the GPU windowed-LD engine")
     g, _, m_total = _geno(bed_prefix, fam_path, row_sel, 0, impute_mode, impute_seed)
     if chr_end > m_total:
         raise RuntimeError("chr_end exceeds #SNPs in BIM")
+    projecting = C is not None and R is not None
+    if projecting and g.n - np.asarray(C).shape[1] <= 1:
+        raise RuntimeError("Too few residual dimensions for unbiased windowed r^2 correction")
     _PROGRESS.update(total=1, done=0, active=True)
     try:
-        out = windowed_ld(g, chr_start, chr_end, bp, ld_wind_kb, annot_chr)
+        out = windowed_ld(g, chr_start, chr_end, bp, ld_wind_kb, annot_chr, C=C if projecting else None,
+                          R=R if projecting else None)
     finally:
         _PROGRESS.update(done=1, active=False)
     return np.asfortranarray(out)

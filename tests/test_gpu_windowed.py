@@ -84,3 +84,56 @@ def test_windowed_all_chromosomes_in_one_launch(oracle, mode):
     with pytest.raises(RuntimeError, match="sorted and must not overlap"):
         winldcore.windowed_ld_multi(g, [(100, 200, None), (150, 250, None)], window_snps=5)
     g.close()
+
+
+@pytest.mark.parametrize("mode", ["mean", "hwe"])
+def test_windowed_covariates_match_oracle(oracle, mode):
+    """Covariate-projected, restandardized panels (winldcore.cpp:341-372) with a
+    non-orthonormal C and R = (C^T C)^-1 C^T, a BP window and two annotation
+    columns, against the oracle's dense projection."""
+    print(f"windowed covariates: mode={mode}", flush=True)
+    from summit_amd import engine, winldcore
+    m, n = 520, 389
+    codes = random_codes(m, n, seed=44, miss_rate=0.03, monomorphic=2)
+    rng = np.random.default_rng(5)
+    bp = np.cumsum(rng.integers(1000, 5000, size=m)).astype(np.int64)
+    annot = np.column_stack([np.ones(m), (rng.random(m) < 0.3).astype(float)])
+    C = np.column_stack([np.ones(n), rng.standard_normal((n, 3)) * 2.0 + 0.5])
+    R = np.linalg.solve(C.T @ C, C.T)
+    g = engine.Genotype.from_bed_bytes(oracle.pack_codes(codes), n, m).prepare(ddof=0, impute=mode, impute_seed=13)
+    lo, hi = 21, 500
+    got = winldcore.windowed_ld(g, lo, hi, bp[lo:hi], 60.0, annot[lo:hi], C=C, R=R)
+    g.close()
+    ref = oracle.windowed_ld_chr(oracle.pack_codes(codes), n, lo, hi, bp[lo:hi], annot[lo:hi], 60.0, impute=mode,
+                                 impute_seed=13, C=C, R=R)
+    np.testing.assert_allclose(got, ref, rtol=1e-10, atol=1e-10)
+    plain = oracle.windowed_ld_chr(oracle.pack_codes(codes), n, lo, hi, bp[lo:hi], annot[lo:hi], 60.0, impute=mode,
+                                   impute_seed=13)
+    assert np.max(np.abs(plain - ref)) > 1e-3  # the covariates matter at this size
+
+
+def test_windowed_covariates_dropin_orthonormal_basis(oracle):
+    """The drop-in with the reference's own covariate form, C = Q and R = Q^T for
+    Q = QR([1, covariates]) (gw_ldscore.py:1015-1035), row selection and a SNP
+    slice; a bad C/R shape raises the reference's message."""
+    from summit_amd import winldcore
+    m, n = 260, 300
+    codes = random_codes(m, n, seed=45, miss_rate=0.02)
+    d = tempfile.mkdtemp()
+    prefix = os.path.join(d, "wc")
+    oracle.write_bed(prefix, codes)
+    rows = np.sort(np.random.default_rng(1).choice(n, 240, replace=False)).astype(np.int64)
+    rng = np.random.default_rng(9)
+    Q, _ = np.linalg.qr(np.column_stack([np.ones(240), rng.standard_normal((240, 2))]))
+    C = np.asfortranarray(Q)
+    R = np.asfortranarray(Q.T)
+    bp = np.arange(m, dtype=np.int64) * 2000 + 7
+    out = winldcore.compute_windowed_ld_chr(prefix, prefix + ".fam", 5, 250, bp[5:250], np.ones((245, 1)), 40.0, 50,
+                                            row_sel=rows, C=C, R=R, impute_mode="mean")
+    ref = oracle.windowed_ld_chr(oracle.pack_codes(codes), n, 5, 250, bp[5:250], np.ones((245, 1)), 40.0, rows=rows,
+                                 C=C, R=R)
+    assert out.flags.f_contiguous
+    np.testing.assert_allclose(out, ref, rtol=1e-10, atol=1e-10)
+    with pytest.raises(RuntimeError, match="C/R shape mismatch"):
+        winldcore.compute_windowed_ld_chr(prefix, prefix + ".fam", 5, 250, bp[5:250], np.ones((245, 1)), 40.0, 50,
+                                          row_sel=rows, C=C[:-1], R=R, impute_mode="mean")

@@ -68,3 +68,53 @@ def test_gxe_oracle_within_five_se_of_kernel_expectation(oracle):
         mean = c2.sum(axis=1)
         var = 2.0 * (mean ** 2 - (c2 * c2).sum(axis=1)) / nvecs
         assert np.all(np.abs(out[name][:, 0] - mean) <= 5.0 * np.sqrt(var) + 1e-12), name
+
+
+def test_windowed_covariates_match_reference_test_formula(oracle):
+    """tests/test_pgen_gwld.py:899-916: project the ddof-0 standardized genotypes
+    off q = QR([1, covariate]), re-centre and rescale to N, corr_dim = N - q.shape[1]."""
+    rng = np.random.default_rng(7)
+    codes = random_codes(25, 40, seed=11, miss_rate=0.0)
+    g = oracle.codes_to_geno(codes).astype(np.float64).T
+    raw = g - g.mean(axis=0)
+    raw /= raw.std(axis=0, ddof=0)
+    age = rng.standard_normal(40)
+    q, _ = np.linalg.qr(np.column_stack([np.ones(40), age]))
+    x = raw - q @ (q.T @ raw)
+    x -= x.mean(axis=0)
+    x *= np.sqrt(40.0 / np.sum(x * x, axis=0))
+    corr = x.T @ x / 40.0
+    unb = corr * corr - (1.0 - corr * corr) / float(40 - 2 - 1)
+    bp = np.cumsum(rng.integers(20, 80, size=25))
+    annot = np.column_stack([np.ones(25), (np.arange(25) % 3 == 0).astype(float)])
+    expect = (unb * (np.abs(bp[:, None] - bp[None, :]) <= 100)) @ annot
+    got = oracle.windowed_ld_chr(oracle.pack_codes(codes), 40, 0, 25, bp, annot, ld_wind_kb=0.1, C=q, R=q.T)
+    np.testing.assert_allclose(got, expect, rtol=2e-12, atol=2e-12)
+    # a dropped projection or the uncorrected degrees of freedom no longer match
+    plain = oracle.windowed_ld_chr(oracle.pack_codes(codes), 40, 0, 25, bp, annot, ld_wind_kb=0.1)
+    assert np.max(np.abs(plain - expect)) > 1e-3
+
+
+def test_windowed_covariates_missing_calls_general_factor(oracle):
+    """Missing calls (mean imputation) and a non-orthonormal C with R = (C^T C)^-1 C^T:
+    the oracle equals the textbook residual correlation of mean-imputed genotypes."""
+    rng = np.random.default_rng(8)
+    codes = random_codes(18, 60, seed=12, miss_rate=0.1)
+    g = oracle.codes_to_geno(codes).astype(np.float64).T
+    obs = g != -9
+    x = np.zeros_like(g)
+    for j in range(18):
+        v = g[obs[:, j], j]
+        c = v - v.mean()
+        x[obs[:, j], j] = c / np.sqrt((c @ c) / obs[:, j].sum())
+    C = np.column_stack([np.ones(60), rng.standard_normal((60, 2)) * 3.0 + 1.0])
+    R = np.linalg.solve(C.T @ C, C.T)
+    y = x - C @ (R @ x)
+    y -= y.mean(axis=0)
+    y *= np.sqrt(60.0 / np.sum(y * y, axis=0))
+    corr = y.T @ y / 60.0
+    unb = corr * corr - (1.0 - corr * corr) / float(60 - 3 - 1)
+    expect = (unb * (np.abs(np.arange(18)[:, None] - np.arange(18)[None, :]) <= 4)) @ np.ones((18, 1))
+    got = oracle.windowed_ld_chr(oracle.pack_codes(codes), 60, 0, 18, None, np.ones((18, 1)), window_snps=4, C=C,
+                                 R=R)
+    np.testing.assert_allclose(got, expect, rtol=1e-11, atol=1e-12)
