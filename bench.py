@@ -352,6 +352,58 @@ def partitioned(engine, ldscore, torch) -> dict:
     return out
 
 
+def h2_end_to_end(engine, ldscore, torch) -> dict:
+    """The LD scores' consumer end to end at N=50,000, M=200,000 (block-Markov
+    genotypes, seed 202, 0.5% missing): partitioned LD scores (a base bin plus 8
+    random 10% annotations, B=100 Rademacher probes) written to and read back from
+    .gw.ldscore.gz / .gw.M, a GWAS simulated on the same genotypes with
+    heritability 0.3 spread over all SNPs plus 0.2 on the first annotation, and
+    the moment solve with a 200-block delete-one jackknife."""
+    import tempfile
+
+    import pandas as pd
+
+    from summit_amd import h2, simulate
+    N, M, B = 50_000, 200_000, 100
+    t_all = time.perf_counter()
+    g = engine.Genotype.synthetic(N, M, seed=202, maf_lo=0.01, maf_hi=0.5, miss_rate=0.005, block_len=100,
+                                  copy_prob=0.9)
+    g.prepare(ddof=1, impute="mean")
+    annot = np.column_stack([np.ones(M), (np.random.default_rng(1202).random((M, 8)) < 0.1).astype(np.float64)])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = ldscore.genomewide_ldscore(g, B, seed=1202, annot=annot)
+    t_ld = time.perf_counter() - t0
+    names = ["base"] + [f"a{k}" for k in range(1, 9)]
+    with tempfile.TemporaryDirectory() as d:
+        out = os.path.join(d, "c2")
+        t0 = time.perf_counter()
+        snp = pd.DataFrame({"CHR": np.ones(M, dtype=np.int64), "SNP": [f"rs{j}" for j in range(M)],
+                            "BP": np.arange(M, dtype=np.int64) * 100})
+        ldscore.write_gw_ldscore_table(out, snp, pd.DataFrame(res.ldscore, columns=names))
+        ldscore.write_gw_annotation_masses(out, annot.sum(axis=0))
+        _, _, L, bins = h2.read_gw_ldscores(out)
+        t_io = time.perf_counter() - t0
+    h2_bins = np.zeros(9)
+    h2_bins[0], h2_bins[1] = 0.3, 0.2
+    t0 = time.perf_counter()
+    sim = simulate.gwas(g, annot, h2_bins, seed=505)
+    t_gwas = time.perf_counter() - t0
+    g.close()
+    t0 = time.perf_counter()
+    y = h2.h2_response(sim.beta, sim.se, sim.n, sim.n)
+    fit = h2.fit_h2(L, annot, y, h2.effective_n_scale(sim.n), h2.JackknifeDesign.blocks(M, 200))
+    t_fit = time.perf_counter() - t0
+    truth_a1 = 0.2 + 0.3 * annot[:, 1].sum() / M
+    return {"workload": "N=50000 M=200000, base + 8 random 10% annotations, B=100, 200 jackknife blocks",
+            "h2": round(fit.h2, 5), "h2_se": round(fit.h2_se, 5), "h2_true": 0.5,
+            "h2_simulated": round(sim.h2_total, 5), "z_h2": round((fit.h2 - 0.5) / fit.h2_se, 3),
+            "h2_a1": round(float(fit.h2_cat[1]), 5), "h2_a1_se": round(float(fit.h2_cat_se[1]), 5),
+            "h2_a1_true": round(truth_a1, 5), "enrichment_a1": round(float(fit.enrichment[1]), 4),
+            "bins_read": len(bins), "ldscore_s": round(t_ld, 3), "files_s": round(t_io, 3),
+            "gwas_s": round(t_gwas, 3), "fit_s": round(t_fit, 3), "total_s": round(time.perf_counter() - t_all, 3)}
+
+
 def learned_cv(engine, ldscore, torch, dev, plain_max: int = 98304, plain_tile: int = 2048) -> dict:
     """Learned control variate at N=50,000, M=200,000 (block-Markov genotypes,
     MAF ~ U(0.01, 0.5), 0.5% missing, 8 random 10% annotations).  The network
@@ -469,7 +521,7 @@ def main() -> int:
     ap.add_argument("--no-tune", action="store_true", help="keep the built-in schedule heuristics")
     ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph replay timing of the same steps")
     ap.add_argument("--only", choices=["learned_cv", "partitioned", "gxe", "windowed", "wall_to_1pct",
-                                       "cpu_mailman_c1"],
+                                       "cpu_mailman_c1", "h2_end_to_end"],
                     help="run only one extra measurement and print it")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the 1%%-error wall-time run and the host->HBM ingest measurement")
@@ -501,6 +553,9 @@ def main() -> int:
         return 0
     if args.only == "partitioned":
         print(json.dumps({"partitioned": partitioned(engine, ldscore, torch)}), flush=True)
+        return 0
+    if args.only == "h2_end_to_end":
+        print(json.dumps({"h2_end_to_end": h2_end_to_end(engine, ldscore, torch)}), flush=True)
         return 0
     if args.only == "gxe":
         print(json.dumps({"gxe": gxe_rate(engine, torch)}), flush=True)
@@ -685,6 +740,8 @@ def main() -> int:
     if not args.no_extras:
         extra["partitioned"] = partitioned(engine, ldscore, torch)
         log(f"partitioned: {extra['partitioned']}")
+        extra["h2_end_to_end"] = h2_end_to_end(engine, ldscore, torch)
+        log(f"h2 end to end: {extra['h2_end_to_end']}")
     if not args.no_extras and not args.no_cv:
         extra["learned_cv"] = learned_cv(engine, ldscore, torch, dev)
     line = {

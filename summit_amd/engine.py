@@ -41,6 +41,12 @@ def _ptr(t) -> int:
     return int(t.data_ptr())
 
 
+def _ld(t) -> int:
+    """Leading dimension of a row-major 2-D tensor; a single row's stride is
+    arbitrary in torch (size-1 dimensions ignore it), so it is the row length."""
+    return int(t.stride(0)) if t.shape[0] > 1 else max(int(t.stride(0)), int(t.shape[1]))
+
+
 def _stream(stream=None) -> int:
     if stream is None:
         stream = torch.cuda.current_stream()
@@ -164,7 +170,7 @@ def gz_i32(g: Genotype, z: torch.Tensor, snp_begin: int = 0, snp_end: int | None
     if out is None:
         out = torch.zeros((B, g.n), dtype=torch.int32, device=z.device)
         accumulate = True
-    call("gpe_gz_i32", g.handle, snp_begin, snp_end, _ptr(z), B, B, _ptr(out), out.stride(0), int(accumulate),
+    call("gpe_gz_i32", g.handle, snp_begin, snp_end, _ptr(z), B, B, _ptr(out), _ld(out), int(accumulate),
          _stream(stream))
     return out
 
@@ -178,7 +184,7 @@ def gtu_i64(g: Genotype, u: torch.Tensor, snp_begin: int = 0, snp_end: int | Non
     u = u.contiguous()
     B = u.shape[0]
     w = torch.empty((snp_end - snp_begin, B), dtype=torch.int64, device=u.device)
-    call("gpe_gtu_i64", g.handle, snp_begin, snp_end, _ptr(u), u.stride(0), B, _ptr(w), B, _stream(stream))
+    call("gpe_gtu_i64", g.handle, snp_begin, snp_end, _ptr(u), _ld(u), B, _ptr(w), B, _stream(stream))
     return w
 
 
@@ -215,11 +221,11 @@ def phase1_xz(g: Genotype, snp_begin: int, snp_end: int, step_size: int, annot: 
         if rand_dist != "rademacher" or v_start_dev.dtype != torch.int32 or v_start_dev.numel() != 1:
             raise RuntimeError("v_start_dev needs Rademacher probes and a one-element int32 device tensor")
         call("gpe_phase1_xz_indirect", g.handle, snp_begin, snp_end, step_size, _ptr(annot), B, _ptr(inv),
-             _ptr(v_start_dev), v_count, tile, int(seed) & 0xFFFFFFFFFFFFFFFF, _ptr(xz), xz.stride(0),
+             _ptr(v_start_dev), v_count, tile, int(seed) & 0xFFFFFFFFFFFFFFFF, _ptr(xz), _ld(xz),
              _ptr(bin_mode), limbs, _stream(stream))
         return xz
     call("gpe_phase1_xz_tiled", g.handle, snp_begin, snp_end, step_size, _ptr(annot), B, _ptr(inv), v_start, v_count,
-         tile, rand_code(rand_dist, probe_fp32), int(seed) & 0xFFFFFFFFFFFFFFFF, _ptr(xz), xz.stride(0),
+         tile, rand_code(rand_dist, probe_fp32), int(seed) & 0xFFFFFFFFFFFFFFFF, _ptr(xz), _ld(xz),
          _ptr(bin_mode), limbs, _stream(stream))
     return xz
 
@@ -257,7 +263,7 @@ def phase2_accum(g: Genotype, snp_begin: int, snp_end: int, inv_left: torch.Tens
     counts = (ctypes.c_ulonglong * 2)()
     guards = ctypes.cast(counts, c_vp) if want_guards else None
     if sum_xz is None:
-        call("gpe_phase2_accum", g.handle, snp_begin, snp_end, _ptr(inv_left), tile_nvecs, _ptr(xz), xz.stride(0),
+        call("gpe_phase2_accum", g.handle, snp_begin, snp_end, _ptr(inv_left), tile_nvecs, _ptr(xz), _ld(xz),
              B, int(n_denom), base, B, _ptr(mc_aggregate_sumsq), mcp, float(extreme_warn), int(bool(drop_extreme)),
              limbs, guards, _stream(stream))
     else:
@@ -265,7 +271,7 @@ def phase2_accum(g: Genotype, snp_begin: int, snp_end: int, inv_left: torch.Tens
         if sum_xz.numel() != Q:
             raise RuntimeError("sum_Xz shape mismatch in phase2_accum_XtXz_bed_mailman_rm")
         call("gpe_phase2_accum_colsums", g.handle, snp_begin, snp_end, _ptr(inv_left), tile_nvecs, _ptr(xz),
-             xz.stride(0), B, _ptr(sum_xz), int(n_denom), base, B, _ptr(mc_aggregate_sumsq), mcp,
+             _ld(xz), B, _ptr(sum_xz), int(n_denom), base, B, _ptr(mc_aggregate_sumsq), mcp,
              float(extreme_warn), int(bool(drop_extreme)), limbs, guards, _stream(stream))
     if want_guards:
         return int(counts[0]), int(counts[1])
@@ -282,7 +288,7 @@ def xc(g: Genotype, c: torch.Tensor, snp_begin: int = 0, snp_end: int | None = N
     Q = c.shape[1]
     if out is None:
         out = torch.zeros((Q, g.n), dtype=torch.float64, device=c.device)
-    call("gpe_xc_f64", g.handle, snp_begin, snp_end, _ptr(c), Q, Q, _ptr(out), out.stride(0), int(bool(accumulate)),
+    call("gpe_xc_f64", g.handle, snp_begin, snp_end, _ptr(c), Q, Q, _ptr(out), _ld(out), int(bool(accumulate)),
          limbs, _stream(stream))
     return out
 
@@ -296,7 +302,7 @@ def xtv(g: Genotype, v: torch.Tensor, snp_begin: int = 0, snp_end: int | None = 
         raise RuntimeError("v must be (Q, n_samples)")
     Q = v.shape[0]
     out = torch.empty((snp_end - snp_begin, Q), dtype=torch.float64, device=v.device)
-    call("gpe_xtv_f64", g.handle, snp_begin, snp_end, _ptr(v), v.stride(0), Q, _ptr(out), Q, limbs,
+    call("gpe_xtv_f64", g.handle, snp_begin, snp_end, _ptr(v), _ld(v), Q, _ptr(out), Q, limbs,
          _stream(stream))
     return out
 
@@ -321,7 +327,7 @@ def project(x_qn: torch.Tensor, A: torch.Tensor | None, B: torch.Tensor | None, 
         a_ptr, b_ptr = _ptr(A_cm), _ptr(B_rm)
     if p == 0 and not center:
         return x_qn
-    call("gpe_project_f64", _ptr(x_qn), x_qn.stride(0), N, Q, a_ptr, lda, b_ptr, ldb, p, int(bool(center)),
+    call("gpe_project_f64", _ptr(x_qn), _ld(x_qn), N, Q, a_ptr, lda, b_ptr, ldb, p, int(bool(center)),
          _stream(stream))
     if A is not None and stream is not None:
         # the factor copies were made on torch's current stream; keep them until `stream` is done
@@ -336,7 +342,7 @@ def col_sums(x_qn: torch.Tensor, stream=None) -> torch.Tensor:
         raise RuntimeError("column sums need a (Q, N) FP64 matrix with unit stride along N")
     Q, N = x_qn.shape
     out = torch.empty(Q, dtype=torch.float64, device=x_qn.device)
-    call("gpe_col_sums_f64", _ptr(x_qn), x_qn.stride(0), N, Q, _ptr(out), _stream(stream))
+    call("gpe_col_sums_f64", _ptr(x_qn), _ld(x_qn), N, Q, _ptr(out), _stream(stream))
     return out
 
 

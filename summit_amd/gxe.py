@@ -123,7 +123,7 @@ def weighted_sumsq(g: engine.Genotype, w: torch.Tensor, snp_begin: int = 0, snp_
     w2 = (w[None, :] if single else w).to(torch.float64).contiguous()
     L = snp_end - snp_begin
     out = torch.empty((w2.shape[0], L), dtype=torch.float64, device=w2.device)
-    engine.call("gpe_weighted_sumsq", g.handle, snp_begin, snp_end, engine._ptr(w2), w2.shape[0], w2.stride(0),
+    engine.call("gpe_weighted_sumsq", g.handle, snp_begin, snp_end, engine._ptr(w2), w2.shape[0], engine._ld(w2),
                 engine._ptr(out), L, engine._stream())
     return out[0] if single else out
 
