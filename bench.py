@@ -33,20 +33,34 @@ HBM_PEAK_GBS = 8000.0
 INT8_PEAK_TOPS = 5000.0  # dense int8 MFMA, 5.0 POP/s
 
 
+def kernel_source_sha() -> str:
+    """SHA-256 of the kernel sources (the contraction kernels and their launch
+    code): profiles/traffic.json carries the hash of the build its counters
+    were read from, and a value from any other build is not reported."""
+    import hashlib
+    h = hashlib.sha256()
+    for name in ("gpe_kernels.hip", "gpe_api.hip", "gpe_internal.h"):
+        with open(os.path.join(ROOT, "summit_amd", "csrc", name), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(geno, n_snps_sample: int, b: int, seed: int, step: int = 1000) -> dict:
-    """The reference's CPU dense path on a bounded sample of the same workload:
-    decode + standardize each SNP block (the reference's own genotype.cpp when
-    oracle/_ref is built, else the oracle's restatement), probes, phase-1 GEMM
-    Xz += X (inv*Z), a second decode and the phase-2 GEMM Work = X^T Xz with the
-    squared-norm reduce, in FP64 through NumPy's OpenBLAS on all host cores."""
+def cpu_baseline(geno, n_snps_sample: int, b: int, seed: int, step: int = 1000, repeats: int = 3) -> dict:
+    """The reference's CPU dense path on a bounded sample of the same workload,
+    in float32 (the reference's default dtype, src/summit/cli.py:796) and in
+    float64: per SNP block the reference's own in-place standardized decode
+    (genotype.cpp through oracle/_ref when built, else the oracle's
+    restatement), the residual scale, the block's probes, the phase-1 GEMM
+    Xz += X (inv*Z); then per block a second decode, the phase-2 GEMM
+    Work = X^T Xz and the squared-norm reduce in float64 -- GEMMs through
+    NumPy's OpenBLAS on every host core.  One warm-up block, then the best of
+    `repeats` full passes per precision."""
     import ctypes
     import tempfile
-
-    import numpy as np
 
     from oracle import oracle as o
 
@@ -67,42 +81,58 @@ def cpu_baseline(geno, n_snps_sample: int, b: int, seed: int, step: int = 1000) 
     use_ref = os.path.exists(ref_path)
     if use_ref:
         ref = ctypes.CDLL(ref_path)
-        ref.ref_read_block_standardized_double.argtypes = [
+        ref.ref_read_block_standardized_into.argtypes = [
             ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
-            ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_void_p]
+            ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
         rows = np.arange(n, dtype=np.int32)
+    bed_b, fam_b = (prefix + ".bed").encode(), (prefix + ".fam").encode()
 
-    def read_block(s, e):
+    def read_block(s, e, dt, buf):
         if use_ref:
-            out = np.empty((e - s, n), dtype=np.float64)
-            rc = ref.ref_read_block_standardized_double((prefix + ".bed").encode(), (prefix + ".fam").encode(), s, e,
-                                                        rows.ctypes.data, n, 1, 0, 0, out.ctypes.data)
+            rc = ref.ref_read_block_standardized_into(bed_b, fam_b, s, e, rows.ctypes.data, n, 1, 0, 0,
+                                                      buf.ctypes.data, buf.size, int(dt == np.float32))
             if rc != 0:
                 raise RuntimeError("reference decoder failed")
-            return out.T
-        return o.read_block_standardized(body, n, s, e, None, 1, "mean", 0)
+            return buf[:(e - s) * n].reshape(e - s, n).T
+        return o.read_block_standardized(body, n, s, e, None, 1, "mean", 0).astype(dt)
 
     blocks = [(s, min(L, s + step)) for s in range(0, L, step)]
-    t0 = time.perf_counter()
-    invs = {}
-    xz = np.zeros((n, b), dtype=np.float64, order="F")
-    for s, e in blocks:
-        X = read_block(s, e)
-        inv = o.inv_resvar(X)
-        invs[s] = inv
-        Z = o.probe_block(seed, s, 0, e - s, b)
-        xz += X @ (inv[:, None] * Z)
-    meansq = np.zeros((L, 1))
-    for s, e in blocks:
-        X = read_block(s, e)
-        o.phase2_accum(X, invs[s], b, xz, meansq, n, s)
-    dt = time.perf_counter() - t0
-    pairs = 2.0 * n * L * b
-    return {"value": pairs / dt / 1e9, "unit": "Gop/s", "cores": cores,
+    probes = {s: o.probe_block(seed, s, 0, e - s, b) for s, e in blocks}
+
+    def one_pass(dt, blks):
+        buf = np.empty(step * n, dtype=dt)
+        xz = np.zeros((n, b), dtype=dt, order="F")
+        invs = {}
+        for s, e in blks:
+            X = read_block(s, e, dt, buf)
+            ss = np.einsum("ij,ij->j", X, X, dtype=np.float64)
+            inv = 1.0 / np.sqrt(np.maximum(ss / (n - 1), 1e-10))
+            invs[s] = inv
+            xz += X @ (inv[:, None] * probes[s]).astype(dt)
+        meansq = np.zeros(L)
+        for s, e in blks:
+            X = read_block(s, e, dt, buf)
+            W = X.T @ xz
+            meansq[s:e] += np.einsum("ij,ij->i", W, W, dtype=np.float64) * invs[s] ** 2 / (n - 1.0) ** 2
+        return meansq
+
+    out = {}
+    t_all = time.perf_counter()
+    for name, dt in (("float32", np.float32), ("float64", np.float64)):
+        one_pass(dt, blocks[:1])  # warm-up: decoder, BLAS threads, page cache
+        best = np.inf
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            one_pass(dt, blocks)
+            best = min(best, time.perf_counter() - t0)
+        out[name] = {"gop_s": round(2.0 * n * L * b / best / 1e9, 3), "best_s": round(best, 3)}
+    return {"value": out["float32"]["gop_s"], "unit": "Gop/s", "cores": cores,
             "kind": "reference" if use_ref else "port",
+            "float32": out["float32"], "float64": out["float64"],
             "sample": f"N={n} x first {L} SNPs of the same genotypes, B={b}, both passes "
-                      f"({'reference genotype.cpp decode' if use_ref else 'oracle decode'} + FP64 OpenBLAS GEMMs), "
-                      f"{dt:.1f} s"}
+                      f"({'reference genotype.cpp in-place decode' if use_ref else 'oracle decode'} + OpenBLAS GEMMs), "
+                      f"value = float32 (the reference default); best of {repeats} after a warm-up block, "
+                      f"{time.perf_counter() - t_all:.1f} s in all"}
 
 
 def wall_to_one_percent(engine, ldscore, torch, dev) -> dict:
@@ -352,6 +382,76 @@ def partitioned(engine, ldscore, torch) -> dict:
     return out
 
 
+def dropin_loop(engine, ldscore, torch) -> dict:
+    """What a caller of the drop-in module gets: the reference driver's per-block
+    loop (src/summit/ldscore/gw_ldscore.py:1905-2133) through summit_amd.gwldcore
+    with NumPy arrays, at N=50,000, M=200,000 (block-Markov genotypes, seed 202,
+    0.5% missing, mean imputation), one bin, B=100 Rademacher probes in one
+    V-tile, blocks of 1,000 SNPs, the residual scales fused into phase 1; the BED
+    is written to a temporary file and made resident by the first call (timed
+    separately).  Beside it the engine's fast path (one call per phase over all
+    blocks) on the same genotypes."""
+    import tempfile
+
+    from oracle import oracle as o
+    from summit_amd import gwldcore as gw
+    N, M, B, step, seed = 50_000, 200_000, 100, 1000, 1202
+    g = engine.Genotype.synthetic(N, M, seed=202, maf_lo=0.01, maf_hi=0.5, miss_rate=0.005, block_len=100,
+                                  copy_prob=0.9)
+    g.prepare(ddof=1, impute="mean")
+    body = g.download_rows("snp_raw", 0, M)
+    with tempfile.TemporaryDirectory() as d:
+        prefix = os.path.join(d, "c2")
+        with open(prefix + ".bed", "wb") as f:
+            f.write(o.BED_MAGIC)
+            f.write(body.tobytes())
+        with open(prefix + ".fam", "w") as f:
+            f.write("x\n" * N)
+        with open(prefix + ".bim", "w") as f:
+            f.write("x\n" * M)
+        del body
+        kw = dict(bed_prefix=prefix, fam_path=prefix + ".fam", row_sel=None, ddof=1, impute_mode="mean",
+                  impute_seed=None)
+        t0 = time.perf_counter()
+        gw.prefetch_bed_block(prefix, prefix + ".fam", 0, step, 1, "mean", 1, None)
+        t_resident = time.perf_counter() - t0
+        blocks = [(s, min(M, s + step)) for s in range(0, M, step)]
+        annot = np.ones((M, 1))
+
+        def run():
+            inv_all = np.empty(M)
+            meansq = np.zeros((M, 1))
+            xz = np.empty((N, B), order="F")
+            init = np.zeros(1, dtype=np.uint8)
+            for s, e in blocks:
+                gw.phase1_compute_Xz_bed_chunk(blk_start=s, blk_end=e, annot_blk=annot[s:e], inv_right=inv_all[s:e],
+                                               v_start=0, v_count=B, kmax_hint=e - s, seed=seed, Xz2d_chunk=xz,
+                                               bin_init_mask=init, inv_out=inv_all[s:e], **kw)
+            for s, e in blocks:
+                gw.phase2_accum_XtXz_bed(blk_start=s, blk_end=e, inv_left=inv_all[s:e], tile_nvecs=B, Xz2d=xz,
+                                         meansq_accum=meansq, N_denom=N, **kw)
+            return meansq
+
+        run()  # warm-up: workspaces, schedules
+        t0 = time.perf_counter()
+        ms_dropin = run()
+        t_dropin = time.perf_counter() - t0
+        gw._clear_bed_mapping_cache()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fast = ldscore.genomewide_ldscore(g, B, seed=seed, annot=annot)
+    t_fast = time.perf_counter() - t0
+    g.close()
+    pairs = 2.0 * N * M * B
+    return {"workload": "N=50000 M=200000, one bin, B=100, 200 blocks of 1000 SNPs, NumPy arrays through "
+                        "summit_amd.gwldcore (the reference driver's loop)",
+            "dropin_s": round(t_dropin, 3), "dropin_gop_s": round(pairs / t_dropin / 1e9, 1),
+            "fast_path_s": round(t_fast, 3), "fast_path_gop_s": round(pairs / t_fast / 1e9, 1),
+            "bed_resident_s": round(t_resident, 3),
+            "max_rel_diff_vs_fast_path": float(np.max(np.abs(ms_dropin[:, 0] / B - (fast.ldscore[:, 0] + M / (N - 1.0)))
+                                                      / np.abs(fast.ldscore[:, 0] + M / (N - 1.0))))}
+
+
 def h2_end_to_end(engine, ldscore, torch) -> dict:
     """The LD scores' consumer end to end at N=50,000, M=200,000 (block-Markov
     genotypes, seed 202, 0.5% missing): partitioned LD scores (a base bin plus 8
@@ -514,14 +614,14 @@ def main() -> int:
     ap.add_argument("--b", type=int, default=64)
     ap.add_argument("--limbs", type=int, default=5)
     ap.add_argument("--impute", default="mean")
-    ap.add_argument("--cpu-snps", type=int, default=4000)
+    ap.add_argument("--cpu-snps", type=int, default=20000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-exact", action="store_true")
     ap.add_argument("--no-cv", action="store_true", help="skip the learned control-variate measurement")
     ap.add_argument("--no-tune", action="store_true", help="keep the built-in schedule heuristics")
     ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph replay timing of the same steps")
     ap.add_argument("--only", choices=["learned_cv", "partitioned", "gxe", "windowed", "wall_to_1pct",
-                                       "cpu_mailman_c1", "h2_end_to_end"],
+                                       "cpu_mailman_c1", "h2_end_to_end", "dropin_loop"],
                     help="run only one extra measurement and print it")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the 1%%-error wall-time run and the host->HBM ingest measurement")
@@ -553,6 +653,9 @@ def main() -> int:
         return 0
     if args.only == "partitioned":
         print(json.dumps({"partitioned": partitioned(engine, ldscore, torch)}), flush=True)
+        return 0
+    if args.only == "dropin_loop":
+        print(json.dumps({"dropin_loop": dropin_loop(engine, ldscore, torch)}), flush=True)
         return 0
     if args.only == "h2_end_to_end":
         print(json.dumps({"h2_end_to_end": h2_end_to_end(engine, ldscore, torch)}), flush=True)
@@ -652,21 +755,31 @@ def main() -> int:
     avg_ms = kms / max(kn, 1)
     ops = 2.0 * N * M * B
     achieved = ops / (avg_ms / 1000.0) / 1e12
+    # HBM bytes per launch of the dominant kernel from the counter runs of
+    # profiles/traffic.json, reported only when they were read from this build
     traffic = None
+    traffic_note = "no counter run recorded for this shape"
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
-        try:
-            with open(tpath) as f:
-                traffic = json.load(f).get(f"N={N},M={M},B={B},limbs={args.limbs},impute={args.impute}")
-        except Exception:
-            traffic = None
+        with open(tpath) as f:
+            tj = json.load(f)
+        entry = tj.get("kernels", {}).get(f"N={N},M={M},B={B},limbs={args.limbs},impute={args.impute}")
+        if entry is None:
+            pass
+        elif tj.get("source_sha256") != kernel_source_sha():
+            traffic_note = "counter run of another build of the kernels: not reported"
+        else:
+            key = "p2_contract" if kname.startswith("k_contract phase 2") else "p1_contract"
+            traffic = entry.get(key)
+            traffic_note = tj.get("measured", "")
     # int8 MFMA multiply-adds the kernel issues per algorithmic genotype-probe pair: one per limb of
     # the real-valued operand, doubled in phase 2 when the missing-call plane is contracted too (with
     # sparse missing calls their sums are gathered over per-SNP lists instead: p2_miss_sums)
     miss_plane = args.impute == "mean" and geno.n_missing() > 0 and pm_n == 0
     per_pair = args.limbs * (2 if (kname.startswith("k_contract phase 2") and miss_plane) else 1)
     roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / INT8_PEAK_TOPS, 4), "traffic": traffic, "kernel": kname,
+                "frac": round(achieved / INT8_PEAK_TOPS, 4), "traffic": traffic, "traffic_source": traffic_note,
+                "kernel": kname,
                 "int8_macs_per_pair": per_pair, "mfma_pipe_frac": round(achieved * per_pair / INT8_PEAK_TOPS, 4),
                 "avg_ms": round(avg_ms, 3),
                 "phase_ms": {"p1_contract": round(p1_ms / max(p1_n, 1), 3),
@@ -740,6 +853,8 @@ def main() -> int:
     if not args.no_extras:
         extra["partitioned"] = partitioned(engine, ldscore, torch)
         log(f"partitioned: {extra['partitioned']}")
+        extra["dropin_loop"] = dropin_loop(engine, ldscore, torch)
+        log(f"drop-in loop: {extra['dropin_loop']}")
         extra["h2_end_to_end"] = h2_end_to_end(engine, ldscore, torch)
         log(f"h2 end to end: {extra['h2_end_to_end']}")
     if not args.no_extras and not args.no_cv:

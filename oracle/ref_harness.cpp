@@ -42,6 +42,29 @@ extern "C" int ref_read_block_standardized_double(const char* bed_path, const ch
     }
 }
 
+// The reference's in-place standardized decode (the call its phase 1 and phase 2
+// make per block, src/native/gwldcore.cpp:1582-1861, :2329-2600) in float
+// (is_float != 0) or double into out (N x L column-major, capacity elems).
+extern "C" int ref_read_block_standardized_into(const char* bed_path, const char* fam_path, int blk_start,
+                                                int blk_end, const int* rows, int nrows, int ddof, int impute_mode,
+                                                uint64_t impute_seed, void* out, size_t elems, int is_float) {
+    try {
+        std::vector<int> r(rows, rows + nrows);
+        int N = 0, L = 0;
+        const ImputeMode mode = impute_mode == 1 ? ImputeMode::Hwe : ImputeMode::Mean;
+        if (is_float)
+            read_block_standardized_into<float>(bed_path, fam_path, blk_start, blk_end, r, ddof, mode, impute_seed,
+                                                static_cast<float*>(out), elems, N, L);
+        else
+            read_block_standardized_into<double>(bed_path, fam_path, blk_start, blk_end, r, ddof, mode, impute_seed,
+                                                 static_cast<double*>(out), elems, N, L);
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
 extern "C" int ref_read_block_mailman_hwe(const char* bed_path, const char* fam_path,
                                           int blk_start, int blk_end, const int* rows, int nrows,
                                           int ddof, uint64_t impute_seed, double* mean_out,
