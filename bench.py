@@ -561,8 +561,18 @@ def learned_cv(engine, ldscore, torch, dev, plain_max: int = 98304, plain_tile: 
     t_cv = time.perf_counter() - t0
     feats_b = cv.snp_features(gb, annot_b)
     base = feats_b[:, 0] + cv.remainder_scale(feats_b[:, 0]) * model.resid_mean  # same features, no network
+    # the method without the network: the same combination with the in-window sum plus
+    # the mean remainder as the prediction, its squared error calibrated on the training labels
+    base_a = fa[:, 0] + cv.remainder_scale(fa[:, 0]) * model.resid_mean
+    tau2_base = max(float(np.mean((lab.ldscore[:, 0] - base_a) ** 2) - np.mean(lab.mc_var[:, 0])),
+                    (1e-3 * float(np.mean(np.abs(lab.ldscore[:, 0])))) ** 2)
+    est_base, _, _ = cv.combine(base, res.plain, tau2_base, 8, M, N)
     out["errors"] = {"cv_b8": round(err(res.ldscore), 5), "plain_b8": round(err(res.plain), 5),
                      "prediction": round(err(res.prediction), 5), "inwindow_plus_mean_resid": round(err(base), 5),
+                     "cv_b8_without_network": round(err(est_base), 5),
+                     "network_gain_prediction": round(1.0 - err(res.prediction) / err(base), 4),
+                     "network_gain_cv_b8": round(1.0 - err(res.ldscore) / err(est_base), 4),
+                     "shrinkage_bias_bound": round(res.bias_bound, 5),
                      "plain_b1000": round(err(p1000.ldscore[:, 0]), 5),
                      "cv_b8_vs_plain_b1000": round(float(np.mean(np.abs(res.ldscore - p1000.ldscore[:, 0]) /
                                                                 np.abs(p1000.ldscore[:, 0]))), 5)}

@@ -245,6 +245,14 @@ int gpe_xtv_f64(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const double* v
 int gpe_windowed_ld(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const int64_t* bp_host, double window_bp,
                     int64_t window_snps, const double* annot_dev, int32_t n_bins, double* out_dev, void* stream);
 
+/* gpe_windowed_ld summed over one side of each SNP's window: side 1 takes the
+ * window SNPs before j, side 2 those after j (j itself in neither; side 0 is
+ * gpe_windowed_ld).  No counterpart in the reference: the asymmetric in-window
+ * features of the learned LD-score predictor (how LD decays on each side). */
+int gpe_windowed_ld_side(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const int64_t* bp_host, double window_bp,
+                         int64_t window_snps, int32_t side, const double* annot_dev, int32_t n_bins, double* out_dev,
+                         void* stream);
+
 /* gpe_windowed_ld with covariates, as compute_windowed_ld_chr with C and R
  * (src/native/winldcore.cpp:783-931, panels :341-372): each SNP's ddof-0
  * standardized column x is projected, y = x - C (R x), and restandardized over

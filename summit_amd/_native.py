@@ -57,6 +57,7 @@ SIGNATURES = {
     "gpe_weighted_sumsq": (ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_i32, c_i64, c_vp, c_i64, c_vp]),
     "gpe_windowed_ld": (ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_dbl, c_i64, c_vp, c_i32, c_vp, c_vp]),
     "gpe_windowed_ld_multi": (ctypes.c_int, [c_vp, c_i32, c_vp, c_vp, c_dbl, c_i64, c_vp, c_i32, c_vp, c_vp]),
+    "gpe_windowed_ld_side": (ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_dbl, c_i64, c_i32, c_vp, c_i32, c_vp, c_vp]),
     "gpe_windowed_ld_cov": (ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_dbl, c_i64, c_vp, c_i32, c_vp, c_i64, c_vp,
                                            c_i64, c_i32, c_vp, c_vp]),
     "gpe_xc_f64":(ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_i64, c_i32, c_vp, c_i64, c_i32, c_i32, c_vp]),

@@ -1210,7 +1210,7 @@ void copy_rows(const double* src, int64_t lds, int64_t rows, int cols, double* d
 void windowed_ld_impl(gpe_geno* g, int n_chr, const int64_t* bounds, const int64_t* bp_host, double window_bp,
                       int64_t window_snps, const double* annot_dev, int32_t n_bins, double* out_dev,
                       hipStream_t st, const double* c_dev = nullptr, int64_t ldc = 0, const double* r_dev = nullptr,
-                      int64_t ldr = 0, int p = 0) {
+                      int64_t ldr = 0, int p = 0, int side = 0) {
     require(n_bins > 0, "annot_chr must have at least one column");
     require(g->n > 2, "Too few rows for windowed LD score computation");
     require(p >= 0 && p <= 1024, "covariate count out of range");
@@ -1290,6 +1290,8 @@ void windowed_ld_impl(gpe_geno* g, int n_chr, const int64_t* bounds, const int64
     a.n_bins = n_bins;
     a.out = out_dev;
     a.corr_dim = corr_dim;
+    require(side >= 0 && side <= 2, "window side must be 0, 1 or 2");
+    a.side = side;
     if (p > 0) {
         // u = C^T x_j and v = R x_j for every SNP of the rows (exact-limb FP64 products),
         // then the per-SNP correction rows, means and scales of the projected columns
@@ -1736,6 +1738,18 @@ int gpe_windowed_ld(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const int64
         require(snp_end - snp_begin > 0, "Chromosome slice is empty");
         const int64_t bounds[2] = {snp_begin, snp_end};
         windowed_ld_impl(g, 1, bounds, bp_host, window_bp, window_snps, annot_dev, n_bins, out_dev, S(stream));
+    });
+}
+
+int gpe_windowed_ld_side(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const int64_t* bp_host, double window_bp,
+                         int64_t window_snps, int32_t side, const double* annot_dev, int32_t n_bins, double* out_dev,
+                         void* stream) {
+    return guarded([&] {
+        require_prepared(g);
+        require(snp_end - snp_begin > 0, "Chromosome slice is empty");
+        const int64_t b[2] = {snp_begin, snp_end};
+        windowed_ld_impl(g, 1, b, bp_host, window_bp, window_snps, annot_dev, n_bins, out_dev, S(stream), nullptr, 0,
+                         nullptr, 0, 0, side);
     });
 }
 

@@ -112,6 +112,8 @@ struct GramArgs {
     int cov_k;
     const double* ybar;
     const double* tsc;
+    // 0: out[j] sums over the whole window; 1: only window SNPs before j; 2: only after j
+    int side;
 };
 
 // Launchers (gpe_kernels.hip)

@@ -2254,14 +2254,20 @@ __global__ __launch_bounds__(256) void k_gram_band(GramArgs a) {
                 }
                 const double r2 = r * r;
                 v = r2 - (1.0 - r2) / (a.corr_dim - 1.0);
+                // one-sided sums: in a diagonal tile the rows take only the pairs on their side
+                if (ti == tj && a.side != 0 && (a.side == 1 ? j >= i : j <= i)) v = 0.0;
             }
         }
         r2t[wave][il][lr] = v;
     }
     __syncthreads();
-    // rows: L_i += sum_j r2[i][j] a_j ; columns (off-diagonal tiles): L_j += sum_i r2[i][j] a_i
+    // rows: L_i += sum_j r2[i][j] a_j ; columns (off-diagonal tiles): L_j += sum_i r2[i][j] a_i.
+    // In an off-diagonal tile every row SNP precedes every column SNP: the row sums are
+    // right-side sums, the column sums left-side sums.
     const bool diag = ti == tj;
-    if (half == 0) {
+    const bool rows_on = diag || a.side != 1;
+    const bool cols_on = !diag && a.side != 2;
+    if (half == 0 && rows_on) {
         const int64_t i = i0 + lr;
         if (i < ce) {
             for (int b = 0; b < a.n_bins; ++b) {
@@ -2273,7 +2279,7 @@ __global__ __launch_bounds__(256) void k_gram_band(GramArgs a) {
                 atomicAdd(&a.out[(i - a.row0) * a.n_bins + b], acc);
             }
         }
-    } else if (!diag) {
+    } else if (half == 1 && cols_on) {
         const int64_t jc = j0 + lr;
         if (jc < ce) {
             for (int b = 0; b < a.n_bins; ++b) {

@@ -1,10 +1,13 @@
-"""Learned control variate for the stochastic genome-wide LD scores.
+"""Learned prediction of the stochastic genome-wide LD scores, combined with a
+cheap plain run (a learned shrinkage estimator: the "control variate" of the
+plan).
 
 A small network predicts each SNP's LD score from features that are cheap
 and exact on the GPU:
 
 * the exact in-window LD sums over the +-50 neighbouring SNPs (unbiased r^2,
-  total and per annotation) and the total over +-25, from the banded
+  total and per annotation), the total over +-25, and the sums over each
+  side of the SNP (before / after it) at +-50 and +-25, from the banded
   int8-MFMA Gram of the windowed engine;
 * the minor allele frequency;
 * the annotation bits.
@@ -16,14 +19,18 @@ genotype set: minimizing squared error against unbiased noisy labels fits
 their conditional mean, so the prediction can be far more accurate than any
 single label.
 
-The prediction ``f_j`` then serves as the control variate of a cheap plain
-run with B probes (estimate ``Lp_j``, Monte Carlo variance ``v_j``):
+The prediction ``f_j`` is then combined with a cheap plain run with B probes
+(estimate ``Lp_j``, Monte Carlo variance ``v_j``):
 
     L_j^cv = f_j + lam_j (Lp_j - f_j),   lam_j = tau^2 / (tau^2 + v_j),
 
-the minimum-variance combination of an unbiased estimate with a prediction
-of squared error tau^2 (calibrated on held-out training SNPs after removing
-the labels' own Monte Carlo variance).  As B grows, v_j -> 0 and the
+the mean-squared-error-optimal shrinkage of an unbiased estimate towards a
+prediction of squared error tau^2 (calibrated on held-out training SNPs after
+removing the labels' own Monte Carlo variance).  It is biased per SNP by
+(1 - lam_j)(f_j - L_j): unlike a classical control variate (which subtracts
+a zero-mean correction) it trades that bias for variance, and
+``CVResult.bias_bound`` reports its size, (1 - lam_j) tau averaged over SNPs
+relative to the mean LD score.  As B grows, v_j -> 0, lam_j -> 1 and the
 estimate becomes the plain one; a SNP whose plain estimate departs from the
 prediction by more than ``guard`` standard deviations (default 10: with few
 probes the plain estimate has chi-square-like tails) keeps the plain
@@ -47,8 +54,9 @@ FEATURE_WINDOW = 50
 
 def snp_features(geno: engine.Genotype, annot_bits: np.ndarray | None = None,
                  window: int = FEATURE_WINDOW) -> np.ndarray:
-    """(M, 3 + 2K) FP64: in-window unbiased r^2 sums (total, then per
-    annotation), the total over +-window/2, MAF, annotation bits."""
+    """(M, 7 + 2K) FP64: in-window unbiased r^2 sums (total, then per
+    annotation), the total over +-window/2, the one-sided sums (before, after)
+    over window and window/2, MAF, annotation bits."""
     M = geno.m
     K = 0 if annot_bits is None else int(np.asarray(annot_bits).shape[1])
     ann = np.ones((M, 1 + K), dtype=np.float64)
@@ -58,10 +66,14 @@ def snp_features(geno: engine.Genotype, annot_bits: np.ndarray | None = None,
     # the total over the inner half of the window: with the full-window total it
     # describes how fast LD decays, which is what sets the part beyond the window
     inner = winldcore.windowed_ld(geno, 0, M, window_snps=max(1, int(window) // 2))
+    # each side's sums: LD that decays differently before and after a SNP (a
+    # haplotype-block boundary on one side) sets a different part beyond the window
+    sided = [winldcore.windowed_ld(geno, 0, M, window_snps=w, side=side)
+             for w in (int(window), max(1, int(window) // 2)) for side in (1, 2)]
     st = geno.stats()
     p = 0.5 * st["mean"]
     maf = np.minimum(p, 1.0 - p)
-    parts = [win, inner, maf[:, None]]
+    parts = [win, inner, *sided, maf[:, None]]
     if K:
         parts.append(ann[:, 1:])
     return np.ascontiguousarray(np.concatenate(parts, axis=1))
@@ -186,13 +198,14 @@ def fit(feats: np.ndarray, labels: np.ndarray, label_var: np.ndarray | None = No
 
 @dataclass
 class CVResult:
-    ldscore: np.ndarray      # control-variate estimate (M,)
+    ldscore: np.ndarray      # combined estimate (M,)
     prediction: np.ndarray   # network prediction f (M,)
     plain: np.ndarray        # plain B-probe estimate (M,)
     weight: np.ndarray       # lam per SNP
     fallback: int            # SNPs that kept the plain estimate
     nvecs: int
     seconds: dict
+    bias_bound: float = 0.0  # mean (1 - lam) tau / mean |L|: the shrinkage's typical relative bias
 
 
 def combine(pred: np.ndarray, plain: np.ndarray, tau2: float, nvecs: int, n_snps: int, n_samples: int,
@@ -227,4 +240,5 @@ def cv_ldscore(geno: engine.Genotype, model: LDScoreModel, annot_bits: np.ndarra
     est, lam, nfall = combine(pred, plain, model.tau2, nvecs, geno.m, geno.n, guard=guard)
     t4 = time.perf_counter()
     secs = {"features": t1 - t0, "predict": t2 - t1, "plain": t3 - t2, "combine": t4 - t3, "total": t4 - t0}
-    return CVResult(est, pred, plain, lam, nfall, int(nvecs), secs)
+    bias = float(np.mean((1.0 - lam) * np.sqrt(model.tau2)) / max(float(np.mean(np.abs(est))), 1e-300))
+    return CVResult(est, pred, plain, lam, nfall, int(nvecs), secs, bias)

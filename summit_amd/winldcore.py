@@ -34,9 +34,10 @@ def progress_active() -> bool:
 
 
 def windowed_ld(g: engine.Genotype, snp_begin: int, snp_end: int, bp=None, ld_wind_kb: float | None = None,
-                annot=None, window_snps: int = 0, C=None, R=None) -> np.ndarray:
+                annot=None, window_snps: int = 0, C=None, R=None, side: int = 0) -> np.ndarray:
     """Device-level call: (m, B) FP64 windowed LD scores of SNPs [snp_begin, snp_end);
-    with C (N, p) and R (p, N) the panels are projected off the covariates."""
+    with C (N, p) and R (p, N) the panels are projected off the covariates; side 1
+    (2) sums only the window SNPs before (after) each SNP."""
     m = snp_end - snp_begin
     if annot is None:
         annot = np.ones((m, 1))
@@ -64,6 +65,10 @@ def windowed_ld(g: engine.Genotype, snp_begin: int, snp_end: int, bp=None, ld_wi
         engine.call("gpe_windowed_ld_cov", g.handle, int(snp_begin), int(snp_end), bp_ptr, window_bp,
                     int(window_snps), engine._ptr(a_d), B, engine._ptr(c_d), g.n, engine._ptr(r_d), g.n, p,
                     engine._ptr(out), engine._stream())
+        return out.cpu().numpy()
+    if side:
+        engine.call("gpe_windowed_ld_side", g.handle, int(snp_begin), int(snp_end), bp_ptr, window_bp,
+                    int(window_snps), int(side), engine._ptr(a_d), B, engine._ptr(out), engine._stream())
         return out.cpu().numpy()
     engine.call("gpe_windowed_ld", g.handle, int(snp_begin), int(snp_end), bp_ptr, window_bp, int(window_snps),
                 engine._ptr(a_d), B, engine._ptr(out), engine._stream())

@@ -54,7 +54,7 @@ def test_control_variate_beats_plain_at_same_probes():
     print(f"control variate N={N} M={M}: train on seed 505 (1000-probe labels), evaluate seed 506", flush=True)
     ga, annot_a = _data(505, N, M)
     fa = cv.snp_features(ga, annot_a)
-    assert fa.shape == (M, 19)
+    assert fa.shape == (M, 23)
     lab = ldscore.genomewide_ldscore(ga, 1000, seed=1505, vtiles=ldscore.bounded_vtile_sizes(1000, 512), mc=True)
     model = cv.fit(fa, lab.ldscore[:, 0], lab.mc_var[:, 0], epochs=40, batch=1024, seed=0)
     ga.close()
@@ -80,21 +80,23 @@ def test_control_variate_beats_plain_at_same_probes():
 
 
 def test_network_learns_ld_beyond_the_window():
-    """LD that reaches past +-50 SNPs (1000-SNP haplotype blocks, copy
-    probability 0.98): the remainder L - W is no longer negligible.  The
-    network, trained on 1,000-probe labels whose noise is shared by SNPs in LD,
-    must do at least as well as its own starting point (the in-window sum plus
-    the mean remainder), and the control variate must beat the plain estimator
+    """LD that reaches far past +-50 SNPs (1000-SNP haplotype blocks, copy
+    probability 0.99: the part of the LD score beyond the window is about half
+    of the in-window part).  Trained on labels from a 32,768-probe plain run of
+    the method itself, the network must cut the mean relative error of its own
+    starting point (the in-window sum plus the mean remainder) by at least 15%,
+    and the combination with an 8-probe plain run must beat the plain estimator
     at the same probe count by a wide margin."""
     from summit_amd import cv, ldscore
     N, M = 3000, 20000
     print(f"long-range LD N={N} M={M}", flush=True)
-    ga, annot_a = _data(515, N, M, block_len=1000, copy_prob=0.98)
+    ga, annot_a = _data(515, N, M, block_len=1000, copy_prob=0.99)
     fa = cv.snp_features(ga, annot_a)
-    lab = ldscore.genomewide_ldscore(ga, 1000, seed=1515, vtiles=ldscore.bounded_vtile_sizes(1000, 512), mc=True)
-    model = cv.fit(fa, lab.ldscore[:, 0], lab.mc_var[:, 0], epochs=60, batch=2048, seed=0)
+    lab = ldscore.genomewide_ldscore(ga, 32768, seed=1515, vtiles=ldscore.bounded_vtile_sizes(32768, 512), mc=True,
+                                     traversal_width=4096)
+    model = cv.fit(fa, lab.ldscore[:, 0], lab.mc_var[:, 0], epochs=120, batch=1024, lr=3e-4, seed=0, patience=20)
     ga.close()
-    gb, annot_b = _data(516, N, M, block_len=1000, copy_prob=0.98)
+    gb, annot_b = _data(516, N, M, block_len=1000, copy_prob=0.99)
     exact = cv.exact_ldscore(gb)
     fb = cv.snp_features(gb, annot_b)
     res = cv.cv_ldscore(gb, model, annot_b, nvecs=8, seed=1516, feats=fb)
@@ -105,7 +107,8 @@ def test_network_learns_ld_beyond_the_window():
 
     e_base = err(fb[:, 0] + cv.remainder_scale(fb[:, 0]) * model.resid_mean)
     e_pred, e_cv, e_plain = err(res.prediction), err(res.ldscore), err(res.plain)
-    print(f"mean rel err: in-window + mean remainder {e_base:.4f}, network {e_pred:.4f}, control variate B=8 "
-          f"{e_cv:.4f}, plain B=8 {e_plain:.4f}; best epoch {model.info['best_epoch']}", flush=True)
-    assert e_pred <= 1.02 * e_base
+    print(f"mean rel err: in-window + mean remainder {e_base:.4f}, network {e_pred:.4f} "
+          f"(x{e_pred / e_base:.3f}), combined B=8 {e_cv:.4f}, plain B=8 {e_plain:.4f}; best epoch "
+          f"{model.info['best_epoch']}, shrinkage bias bound {res.bias_bound:.4f}", flush=True)
+    assert e_pred <= 0.85 * e_base
     assert e_cv < e_plain / 10.0

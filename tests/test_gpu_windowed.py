@@ -137,3 +137,28 @@ def test_windowed_covariates_dropin_orthonormal_basis(oracle):
     with pytest.raises(RuntimeError, match="C/R shape mismatch"):
         winldcore.compute_windowed_ld_chr(prefix, prefix + ".fam", 5, 250, bp[5:250], np.ones((245, 1)), 40.0, 50,
                                           row_sel=rows, C=C[:-1], R=R, impute_mode="mean")
+
+
+@pytest.mark.parametrize("mode", ["mean", "hwe"])
+def test_one_sided_window_sums(oracle, mode):
+    """Sums over the window SNPs before (side 1) and after (side 2) each SNP,
+    against the oracle's masked unbiased r^2 matrix; with the SNP itself
+    (r~2_jj = 1) the two sides give the two-sided sum."""
+    from summit_amd import engine, winldcore
+    m, n = 300, 333
+    codes = random_codes(m, n, seed=46, miss_rate=0.02, monomorphic=1)
+    annot = np.column_stack([np.ones(m), (np.arange(m) % 4 == 0).astype(float)])
+    g = engine.Genotype.from_bed_bytes(oracle.pack_codes(codes), n, m).prepare(ddof=0, impute=mode, impute_seed=3)
+    lo, hi = 7, 290
+    left = winldcore.windowed_ld(g, lo, hi, annot=annot[lo:hi], window_snps=40, side=1)
+    right = winldcore.windowed_ld(g, lo, hi, annot=annot[lo:hi], window_snps=40, side=2)
+    both = winldcore.windowed_ld(g, lo, hi, annot=annot[lo:hi], window_snps=40)
+    g.close()
+    mm = hi - lo
+    R = oracle.windowed_ld_chr(oracle.pack_codes(codes), n, lo, hi, None, np.eye(mm), window_snps=40, impute=mode,
+                               impute_seed=3)  # row j: r~2_ji within the window
+    before = np.tril(R, -1) @ annot[lo:hi]
+    after = np.triu(R, 1) @ annot[lo:hi]
+    np.testing.assert_allclose(left, before, rtol=1e-11, atol=1e-11)
+    np.testing.assert_allclose(right, after, rtol=1e-11, atol=1e-11)
+    np.testing.assert_allclose(left + right + annot[lo:hi] * np.diag(R)[:, None], both, rtol=1e-11, atol=1e-11)
