@@ -285,7 +285,11 @@ int gpe_philox_rademacher(const uint64_t* keys_host, const int64_t* blk_starts_h
                           int32_t v_count, int8_t* z_dev, int64_t ldz, void* stream);
 
 /* Phase 1 with caller-provided int8 probes z_dev (row-major (L, v_count), row 0
- * = snp_begin), otherwise as gpe_phase1_xz.  The GxE sources
+ * = snp_begin), otherwise as gpe_phase1_xz.  Probe values must be -1, 0 or +1
+ * (Rademacher signs): the missing-call terms keep each probe as a 2-bit sign and
+ * all columns of a bin share one fixed-point scale.  Outside graph capture the
+ * call checks this on the device and returns GPE_EINVAL otherwise (the call then
+ * synchronizes); under capture the caller guarantees it.  The GxE sources
  * (src/native/gxeldcore.cpp:7418-7576) use it with their Philox probes. */
 int gpe_phase1_xz_z8(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const double* annot_dev, int32_t n_bins,
                      const double* inv_dev, const int8_t* z_dev, int64_t ldz, int32_t v_count, double* xz_dev,
@@ -440,6 +444,14 @@ int gpe_schedule_set(int32_t knob, int64_t rows, int64_t length, int32_t columns
 /* The value in force for a key (0 = the heuristic decides). */
 int gpe_schedule_get(int32_t knob, int64_t rows, int64_t length, int32_t columns, int32_t limbs, int32_t* value);
 int gpe_schedule_clear(void);
+/* The built-in K-split rules the traversals use without a schedule entry (host
+ * only, no device work): rule 0 the fill rule of the exact integer products (the
+ * smallest split count whose last round of workgroups fills >= 90% of the
+ * resident slots), rule 1 the phase-1 rule (splits until the grid holds 16 rounds
+ * of workgroups while every split keeps >= 128 chunks, at most 16); both then
+ * raised until no split exceeds max_chunks_per_split. */
+int gpe_schedule_split_rule(int32_t rule, int64_t grid_xy, int64_t kcount, int64_t max_chunks_per_split,
+                            int64_t slots, int32_t* splits);
 
 #ifdef __cplusplus
 }

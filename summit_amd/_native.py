@@ -75,6 +75,7 @@ SIGNATURES = {
                                           c_vp, c_vp]),
     "gpe_schedule_set": (ctypes.c_int, [c_i32, c_i64, c_i64, c_i32, c_i32, c_i32]),
     "gpe_schedule_get": (ctypes.c_int, [c_i32, c_i64, c_i64, c_i32, c_i32, ctypes.POINTER(c_i32)]),
+    "gpe_schedule_split_rule": (ctypes.c_int, [c_i32, c_i64, c_i64, c_i64, c_i64, c_vp]),
     "gpe_schedule_clear": (ctypes.c_int, []),
     "gpe_profile_enable": (ctypes.c_int, [c_i32]),
     "gpe_profile_read": (ctypes.c_int, [c_i32, ctypes.POINTER(c_dbl), ctypes.POINTER(c_i64)]),

@@ -268,9 +268,6 @@ struct ProfScope {
     }
 };
 
-// Real-valued probes of blocks [starts[b], next start or blk_end) for V-tile
-// v_start, written to dst[(row - starts[0]) * ldz + c] for c < v_count; every
-// block's stream is independent, so blocks are spread over host threads.
 // K splits of the MFMA-bound phase-1 contraction: workgroups of equal work
 // finish unevenly, so the last wave's idle slots cost less the more waves
 // there are.  Add splits until the grid holds 16 waves of resident
@@ -286,6 +283,9 @@ int pick_splits_deep(int64_t grid_xy, int64_t kcount, int64_t max_chunks_per_spl
     return (int)s;
 }
 
+// Real-valued probes of blocks [starts[b], next start or blk_end) for V-tile
+// v_start, written to dst[(row - starts[0]) * ldz + c] for c < v_count; every
+// block's stream is independent, so blocks are spread over host threads.
 void real_probes_into(int rand_dist, uint64_t root, const int64_t* starts, int n_blocks, int64_t blk_end, int v_start,
                       int v_count, double* dst, int64_t ldz);
 
@@ -398,11 +398,6 @@ void parallel_memcpy(uint8_t* dst, const uint8_t* src, int64_t bytes, int n_thre
     for (auto& th : pool) th.join();
 }
 
-// K splits of a contraction whose partial sums are added atomically.  Equal-work
-// workgroups finish in rounds of `slots` (the device's resident capacity), so the
-// split count is the smallest one whose last round is at least 90% full (the best
-// such fill up to 16 splits otherwise), then raised until no split exceeds
-// max_chunks_per_split.
 // Schedule overrides written by the autotuner: (knob, rows, length, columns,
 // limbs) -> value.  Read on every traversal; guarded by a mutex.
 struct SchedKey {
@@ -428,6 +423,11 @@ int32_t sched_lookup(int32_t knob, int64_t rows, int64_t length, int32_t columns
     return it == t.end() ? 0 : it->second;
 }
 
+// K splits of a contraction whose partial sums are added atomically.  Equal-work
+// workgroups finish in rounds of `slots` (the device's resident capacity), so the
+// split count is the smallest one whose last round is at least 90% full (the best
+// such fill up to 16 splits otherwise), then raised until no split exceeds
+// max_chunks_per_split.
 int pick_splits(int64_t grid_xy, int64_t kcount, int64_t max_chunks_per_split, int64_t slots) {
     int64_t best = 1;
     double best_fill = 0.0;
@@ -535,6 +535,17 @@ int gpe_schedule_clear(void) {
     return guarded([&] {
         std::lock_guard<std::mutex> lock(sched_mu());
         sched_table().clear();
+    });
+}
+
+int gpe_schedule_split_rule(int32_t rule, int64_t grid_xy, int64_t kcount, int64_t max_chunks_per_split,
+                            int64_t slots, int32_t* splits) {
+    return guarded([&] {
+        require(splits != nullptr && grid_xy > 0 && kcount > 0 && max_chunks_per_split > 0 && slots > 0,
+                "split rule needs positive sizes");
+        require(rule == 0 || rule == 1, "split rule must be 0 (fill) or 1 (deep)");
+        *splits = rule == 0 ? pick_splits(grid_xy, kcount, max_chunks_per_split, slots)
+                            : pick_splits_deep(grid_xy, kcount, max_chunks_per_split, slots);
     });
 }
 
@@ -1557,6 +1568,18 @@ int gpe_phase1_xz_z8(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const doub
         hipStream_t st = S(stream);
         const int64_t L = snp_end - snp_begin;
         const int v_pad = (int)round_up(v_count, 32);
+        // the per-call missing-call records hold each probe as a 2-bit sign and every
+        // column of a bin shares one fixed-point scale: only -1, 0 and +1 are exact
+        CaptureScope cap(g, st);
+        if (!cap.active) {
+            unsigned int* bad = static_cast<unsigned int*>(g->scratch("z8_check", sizeof(unsigned int)));
+            launch_fill_u8(reinterpret_cast<uint8_t*>(bad), sizeof(unsigned int), 0, st);
+            launch_z8_check(z_dev, ldz, L, v_count, bad, st);
+            unsigned int bad_h = 0;
+            GPE_HIP(hipMemcpyAsync(&bad_h, bad, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+            GPE_HIP(hipStreamSynchronize(st));
+            require(bad_h == 0, "int8 probes must be -1, 0 or +1");
+        }
         double* w = static_cast<double*>(g->scratch("p1_w", g->m_alloc * n_bins * sizeof(double)));
         hipLaunchKernelGGL(k_make_w, dim3((unsigned)ceil_div(L, 256)), dim3(256), 0, st, annot_dev, n_bins, inv_dev,
                            g->inv_sd, g->nobs, g->sum, g->sumsq, snp_begin, snp_end, w);

@@ -192,6 +192,7 @@ void launch_colmax_f64(const double* x, int64_t ldx, int64_t n, int n_cols, int 
 void launch_quant_p2(const double* x, int64_t ldx, int64_t n, int n_bins, int v_count, int v_pad,
                      const int32_t* shift, int nl, int ctc, int64_t coef_ctiles, int8_t* coef,
                      int64_t* col_sum, hipStream_t st);
+void launch_z8_check(const int8_t* z, int64_t ldz, int64_t rows, int cols, unsigned int* bad, hipStream_t st);
 void launch_col_sum_override(const double* sum, int n_bins, int v_count, int v_pad, const int32_t* shift,
                              double* col_scale, int64_t* col_sum, hipStream_t st);
 void launch_shift_from_colmax(const uint64_t* colmax_bits, int n, int precision_bits, int32_t* shift,

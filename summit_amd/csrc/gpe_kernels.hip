@@ -1842,6 +1842,20 @@ void launch_col_sum_override(const double* sum, int n_bins, int v_count, int v_p
     GPE_HIP(hipGetLastError());
 }
 
+// Flag int8 probe values outside {-1, 0, +1} in rows x cols of z (leading dimension ldz).
+__global__ void k_z8_check(const int8_t* z, int64_t ldz, int64_t rows, int cols, unsigned int* bad) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= rows * cols) return;
+    const int8_t v = z[(idx / cols) * ldz + idx % cols];
+    if (v < -1 || v > 1) atomicOr(bad, 1u);
+}
+void launch_z8_check(const int8_t* z, int64_t ldz, int64_t rows, int cols, unsigned int* bad, hipStream_t st) {
+    if (rows <= 0 || cols <= 0) return;
+    hipLaunchKernelGGL(k_z8_check, dim3((unsigned)ceil_div(rows * cols, 256)), dim3(256), 0, st, z, ldz, rows, cols,
+                       bad);
+    GPE_HIP(hipGetLastError());
+}
+
 // Limb rows of the phase-2 operand for gathers by sample.  One workgroup per
 // 64-sample x 64-column tile: q and its digits as k_quant_p2_t forms them, staged
 // in LDS and written out so that each sample's nl x 64 bytes are contiguous.

@@ -59,11 +59,13 @@ def test_tune_phase1_caches_and_keeps_results():
 @pytest.mark.parametrize("n,impute", [(1000, "mean"), (1100, "mean"), (1100, "hwe")])
 def test_two_chunks_per_barrier_is_bit_identical(n, impute):
     """The two-chunks-per-barrier contraction (a schedule knob) gives bit-identical
-    phase-1 and phase-2 results: even and odd chunk counts (N = 1000 -> 8 chunks,
-    1100 -> 9), K splits with odd chunk counts per split, missing plane or not."""
+    phase-1 and phase-2 results: even and odd phase-2 chunk counts (N = 1000 -> 8
+    chunks, 1100 -> 9) and phase-1 K splits of the 23 SNP chunks of M = 2900 into
+    3 (8/8/7 chunks) and 5 (5/5/5/5/3) -- splits whose last pair of chunks is a
+    single chunk -- with the missing plane or not."""
     from summit_amd import autotune, engine
     dev = engine.default_device()
-    M, V = 3000, 40
+    M, V = 2900, 40
     print(f"cpb parity N={n} M={M} {impute}", flush=True)
     g = engine.Genotype.synthetic(n, M, seed=21, maf_lo=0.05, maf_hi=0.5, miss_rate=0.01, block_len=100,
                                   copy_prob=0.5)
@@ -75,7 +77,7 @@ def test_two_chunks_per_barrier_is_bit_identical(n, impute):
     autotune.clear()
     outs = {}
     for cpb in (1, 2):
-        for splits in (1, 3):
+        for splits in (1, 3, 5):
             autotune.set_value("p1_cpb", k1, cpb)
             autotune.set_value("p2_cpb", k2, cpb)
             autotune.set_value("p1_splits", k1, splits)

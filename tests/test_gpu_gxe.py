@@ -89,3 +89,24 @@ def test_weighted_sumsq_matches_dense(oracle, n_w, impute):
     ref = W @ (X[:, 20:290] ** 2)
     np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-9)
     g.close()
+
+
+def test_phase1_int8_probes_must_be_signs(oracle):
+    """gpe_phase1_xz_z8 takes caller-provided int8 probes; a value outside
+    {-1, 0, +1} would not fit its missing-call records and is refused."""
+    from summit_amd import _native, engine
+    m, n, V = 300, 200, 8
+    codes = random_codes(m, n, seed=61, miss_rate=0.02)
+    g = engine.Genotype.from_bed_bytes(oracle.pack_codes(codes), n, m).prepare(ddof=1, impute="mean")
+    dev = engine.default_device()
+    annot = torch.ones((m, 1), dtype=torch.float64, device=dev)
+    inv = engine.resvar_inv(g, 0, m)
+    z = torch.from_numpy(np.random.default_rng(1).choice([-1, 0, 1], size=(m, V)).astype(np.int8)).to(dev)
+    xz = torch.zeros((V, n), dtype=torch.float64, device=dev)
+    engine.call("gpe_phase1_xz_z8", g.handle, 0, m, engine._ptr(annot), 1, engine._ptr(inv), engine._ptr(z), V, V,
+                engine._ptr(xz), n, None, 5, engine._stream())
+    z[123, 5] = 2
+    with pytest.raises(_native.GpeError, match="int8 probes must be -1, 0 or \\+1"):
+        engine.call("gpe_phase1_xz_z8", g.handle, 0, m, engine._ptr(annot), 1, engine._ptr(inv), engine._ptr(z), V,
+                    V, engine._ptr(xz), n, None, 5, engine._stream())
+    g.close()
