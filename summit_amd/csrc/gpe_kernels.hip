@@ -29,6 +29,7 @@ namespace gpe {
 typedef int32_t v4i __attribute__((ext_vector_type(4)));
 typedef int32_t v16i __attribute__((ext_vector_type(16)));
 typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+typedef uint16_t v2u16 __attribute__((ext_vector_type(2)));
 
 // ---------------------------------------------------------------------------
 // helpers
@@ -1857,9 +1858,10 @@ void launch_z8_check(const int8_t* z, int64_t ldz, int64_t rows, int cols, unsig
 }
 
 // Limb rows of the phase-2 operand for gathers by sample.  One workgroup per
-// 64-sample x 64-column tile: q and its digits as k_quant_p2_t forms them, staged
-// in LDS and written out so that each sample's nl x 64 bytes are contiguous.
-// Rows run to n inclusive: row n is all zeros (the pad entry of the gathers).
+// 64-sample x 64-column tile: the quantized q as k_quant_p2_t forms it, as the
+// NL unsigned base-256 digits of q + 2^(8 NL - 1), staged in LDS and written out
+// so that each sample's nl x 64 bytes are contiguous.  Rows run to n inclusive:
+// row n is all zeros (the pad entry of the gathers).
 template <int NL>
 __global__ __launch_bounds__(256) void k_xrow_limbs(const double* __restrict__ x, int64_t ldx, int64_t n, int ncols,
                                                     int v_count, int v_pad, const int32_t* __restrict__ shift,
@@ -1878,10 +1880,11 @@ __global__ __launch_bounds__(256) void k_xrow_limbs(const double* __restrict__ x
             const int k = c / v_pad, v = c % v_pad;
             if (v < v_count) q = quant_rn(x[(int64_t)(k * v_count + v) * ldx + smp], shift[c]);
         }
-        int8_t d[NL];
-        balanced_digits<NL>(q, d);
+        // offset-binary digits of u = q + 2^(8 NL - 1) (unsigned, most significant first):
+        // the gathers add them as unsigned bytes; the pad row (smp == n) is all zero
+        const uint64_t u = smp < n ? (uint64_t)q + ((uint64_t)1 << (8 * NL - 1)) : 0;
 #pragma unroll
-        for (int l = 0; l < NL; ++l) tile[sl][l * 64 + cc] = d[l];
+        for (int l = 0; l < NL; ++l) tile[sl][l * 64 + cc] = (int8_t)(uint8_t)(u >> (8 * (NL - 1 - l)));
     }
     __syncthreads();
     for (int i = tid; i < 64 * DW; i += 256) {
@@ -1949,11 +1952,12 @@ void launch_range_ptrs(const int64_t* ptr, const int32_t* idx, int64_t m, int pa
 // stay in that XCD's L2 (at N = 100,000, P = 8: 90% L2 hits).  The gathers are
 // then bound by the misses a CU keeps in flight to L2, about 7 ms for 2.5e8
 // calls of 320 bytes, against about 8 ms for the missing-flag MFMA plane.
-template <int NL>
+template <int NL, bool BUF>
 __global__ __launch_bounds__(256) void k_miss_sums(const int64_t* __restrict__ rptr, int parts,
                                                    const int32_t* __restrict__ idx, int64_t r0, int64_t r1,
                                                    const int8_t* __restrict__ xrow, int n, int nch,
-                                                   int64_t* __restrict__ out, int64_t part_stride, int64_t ld_ms) {
+                                                   int64_t* __restrict__ out, int64_t part_stride, int64_t ld_ms,
+                                                   uint32_t table_bytes) {
     constexpr int D = NL * 16;
     constexpr int X = D - 64;        // tail dwords per entry (16 or 32)
     constexpr int E = 64 / X;        // entries per tail load
@@ -1987,33 +1991,59 @@ __global__ __launch_bounds__(256) void k_miss_sums(const int64_t* __restrict__ r
     const uint32_t* tab = reinterpret_cast<const uint32_t*>(xrow) + (uint32_t)h * D;
     const uint32_t tail_off = 64u + (uint32_t)(lane % X);
     const int tail_grp = lane / X;
+    // BUF (table below 4 GiB): buffer loads with the row's byte offset in a scalar
+    // register and the lane's dword as the vector offset -- no address arithmetic per
+    // entry on the vector ALU
+    // (the descriptor starts at this wave's column group; unused when !BUF)
+    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<uint32_t*>(tab), (short)0, (int)(table_bytes - 4u * (uint32_t)h * D), 0x00020000);
+    // 32-bit sums of the dword's four unsigned digit bytes, and packed 16-bit sums of
+    // bytes 0, 2 (even) and 1, 3 (odd) within a batch of at most 64 entries
+    // (64 x 255 < 2^16): a mask, a byte permute (bytes 1 and 3 to 0 and 2, zeros
+    // between) and two packed adds per dword and entry
     int32_t s0[4] = {0, 0, 0, 0}, s1[4] = {0, 0, 0, 0};
-    auto add4 = [](int32_t* acc, uint32_t v) {
-        acc[0] += (int32_t)(int8_t)(v & 0xffu);
-        acc[1] += (int32_t)(int8_t)((v >> 8) & 0xffu);
-        acc[2] += (int32_t)(int8_t)((v >> 16) & 0xffu);
-        acc[3] += (int32_t)(int8_t)(v >> 24);
+    auto flush = [](int32_t* acc, v2u16 ev, v2u16 od) {
+        acc[0] += (int32_t)ev.x;
+        acc[1] += (int32_t)od.x;
+        acc[2] += (int32_t)ev.y;
+        acc[3] += (int32_t)od.y;
     };
     for (int64_t p0 = b0; p0 < b1; p0 += 64) {
         const int cnt = (int)((b1 - p0) < 64 ? (b1 - p0) : 64);
         const int my = lane < cnt ? idx[p0 + lane] : n;
+        v2u16 e0 = v2u16{0, 0}, o0 = v2u16{0, 0}, e1 = v2u16{0, 0}, o1 = v2u16{0, 0};
         for (int t = 0; t < cnt; t += U) {
             uint32_t v0[U], v1[U / E];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const uint32_t smp = (uint32_t)__builtin_amdgcn_readlane(my, t + u);
-                v0[u] = tab[smp * rstride + (uint32_t)lane];
+                if constexpr (BUF)
+                    v0[u] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, (int)(4u * (uint32_t)lane),
+                                                                 (int)(4u * smp * rstride), 0);
+                else
+                    v0[u] = tab[smp * rstride + (uint32_t)lane];
             }
 #pragma unroll
             for (int w = 0; w < U / E; ++w) {
                 const uint32_t smp = (uint32_t)__shfl(my, t + w * E + tail_grp, 64);
-                v1[w] = tab[smp * rstride + tail_off];
+                if constexpr (BUF)
+                    v1[w] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, (int)(4u * (smp * rstride + tail_off)), 0, 0);
+                else
+                    v1[w] = tab[smp * rstride + tail_off];
             }
 #pragma unroll
-            for (int u = 0; u < U; ++u) add4(s0, v0[u]);
+            for (int u = 0; u < U; ++u) {
+                e0 += __builtin_bit_cast(v2u16, v0[u] & 0x00FF00FFu);
+                o0 += __builtin_bit_cast(v2u16, __builtin_amdgcn_perm(0u, v0[u], 0x0C030C01u));
+            }
 #pragma unroll
-            for (int w = 0; w < U / E; ++w) add4(s1, v1[w]);
+            for (int w = 0; w < U / E; ++w) {
+                e1 += __builtin_bit_cast(v2u16, v1[w] & 0x00FF00FFu);
+                o1 += __builtin_bit_cast(v2u16, __builtin_amdgcn_perm(0u, v1[w], 0x0C030C01u));
+            }
         }
+        flush(s0, e0, o0);
+        flush(s1, e1, o1);
     }
     // fold the tail lane groups: lanes < X then hold dword 64 + lane
 #pragma unroll
@@ -2032,9 +2062,11 @@ __global__ __launch_bounds__(256) void k_miss_sums(const int64_t* __restrict__ r
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // sum of the offset values minus the offset of every real entry (the pad row is 0)
     int64_t me = 0;
 #pragma unroll
     for (int l = 0; l < NL; ++l) me = me * 256 + (int64_t)rd[l * 64 + lane];
+    me -= (b1 - b0) * ((int64_t)1 << (8 * NL - 1));
     out[(int64_t)p * part_stride + (j - r0) * ld_ms + h * 64 + lane] = me;
 }
 // out[i] = sum_p parts[p * stride + i] for i < n, into parts[i] (part 0), in order
@@ -2059,14 +2091,23 @@ void launch_miss_sums(const int64_t* rptr, int parts, const int32_t* idx, int64_
     const int64_t blocks = parts % kXcds == 0 ? kXcds * (int64_t)(parts / kXcds) * ceil_div(per_range, 4)
                                               : ceil_div(per_range * parts, 4);
     const dim3 grid((unsigned)blocks);
-    if (nl == 5)
-        hipLaunchKernelGGL(k_miss_sums<5>, grid, dim3(256), 0, st, rptr, parts, idx, r0, r1, xrow, (int)n, nch, out,
-                           part_stride, ld_ms);
+    const int64_t table_bytes = (n + 1) * nch * nl * 64;
+    const bool buf = table_bytes <= (int64_t)0xFFFFFFFF;
+    const uint32_t tb = buf ? (uint32_t)table_bytes : 0u;
+#define GPE_MS(NL_, BUF_)                                                                                           \
+    hipLaunchKernelGGL((k_miss_sums<NL_, BUF_>), grid, dim3(256), 0, st, rptr, parts, idx, r0, r1, xrow, (int)n,  \
+                       nch, out, part_stride, ld_ms, tb)
+    if (nl == 5 && buf)
+        GPE_MS(5, true);
+    else if (nl == 5)
+        GPE_MS(5, false);
+    else if (nl == 6 && buf)
+        GPE_MS(6, true);
     else if (nl == 6)
-        hipLaunchKernelGGL(k_miss_sums<6>, grid, dim3(256), 0, st, rptr, parts, idx, r0, r1, xrow, (int)n, nch, out,
-                           part_stride, ld_ms);
+        GPE_MS(6, false);
     else
         throw Error("phase-2 operand limbs must be 5 or 6");
+#undef GPE_MS
     GPE_HIP(hipGetLastError());
 }
 

@@ -7,7 +7,7 @@ import sys
 import numpy as np
 import pytest
 
-from tests.helpers import random_codes
+from tests.helpers import assert_ld_close, random_codes
 
 pytestmark = pytest.mark.gpu
 
@@ -286,8 +286,7 @@ def test_ldscore_matches_oracle(engine, oracle, mode, limbs):
                             impute=mode, vtiles=[12, 8])
     np.testing.assert_allclose(res.inv, ref.inv, rtol=1e-13)
     np.testing.assert_allclose(res.meansq_accum, ref.meansq_accum, rtol=1e-11)
-    tol = 1e-10 * np.maximum(np.abs(ref.ldscore), 1.0)
-    assert np.all(np.abs(res.ldscore - ref.ldscore) <= tol)
+    assert_ld_close(res.ldscore, ref.ldscore, np.full(1, m / (n - 1.0)))
     g.close()
 
 
@@ -312,8 +311,7 @@ def test_wide_traversals_keep_the_vtile_streams(engine, oracle, rand_dist):
                                      rand_dist=rand_dist)
     ref = oracle.gw_ldscore(oracle.pack_codes(codes), n, m, annot=annot, nvecs=28, step_size=step, seed=seed,
                             impute="mean", vtiles=vt, rand_dist=rand_dist)
-    tol = 1e-10 * np.maximum(np.abs(ref.ldscore), 1.0)
-    assert np.all(np.abs(res.ldscore - ref.ldscore) <= tol)
+    assert_ld_close(res.ldscore, ref.ldscore, annot.sum(axis=0) / (n - 1.0))
     one = ldscore.genomewide_ldscore(g, 28, seed, annot=annot, step_size=step, vtiles=vt, rand_dist=rand_dist)
     np.testing.assert_allclose(res.meansq_accum, one.meansq_accum, rtol=1e-12)
     g.close()

@@ -9,7 +9,7 @@ import tempfile
 import numpy as np
 import pytest
 
-from tests.helpers import random_codes
+from tests.helpers import assert_ld_close, random_codes
 
 pytestmark = pytest.mark.gpu
 
@@ -66,7 +66,7 @@ def test_dropin_driver_matches_oracle(oracle, impute):
     np.testing.assert_allclose(inv, ref.inv, rtol=1e-13)
     np.testing.assert_allclose(meansq, ref.meansq_accum, rtol=5e-11)
     np.testing.assert_allclose(mc, ref.mc_per_snp_sumsq, rtol=2e-10)
-    assert np.all(np.abs(ld - ref.ldscore) <= 1e-10 * np.maximum(np.abs(ref.ldscore), 1.0))
+    assert_ld_close(ld, ref.ldscore, annot.sum(axis=0) / (n - 1.0))
 
 
 def test_dropin_row_selection_and_float32(oracle):
@@ -79,10 +79,10 @@ def test_dropin_row_selection_and_float32(oracle):
     ld, meansq, inv, _ = _reference_style_run(gw, prefix, n, m, annot, nvecs, step, seed, "mean", [8], rows=rows)
     ref = oracle.gw_ldscore(oracle.pack_codes(codes), n, m, nvecs=nvecs, step_size=step, seed=seed, impute="mean",
                             rows=rows)
-    assert np.all(np.abs(ld - ref.ldscore) <= 1e-10 * np.maximum(np.abs(ref.ldscore), 1.0))
+    assert_ld_close(ld, ref.ldscore, np.full(1, m / (190 - 1.0)))
     ld32, *_ = _reference_style_run(gw, prefix, n, m, annot, nvecs, step, seed, "mean", [8], dtype=np.float32,
                                     rows=rows)
-    assert np.all(np.abs(ld32 - ref.ldscore) <= 1e-4 * np.maximum(np.abs(ref.ldscore), 1.0))
+    assert_ld_close(ld32, ref.ldscore, np.full(1, m / (190 - 1.0)), rtol=1e-6)
 
 
 def test_dropin_errors_mirror_reference(oracle):
