@@ -29,7 +29,6 @@ namespace gpe {
 typedef int32_t v4i __attribute__((ext_vector_type(4)));
 typedef int32_t v16i __attribute__((ext_vector_type(16)));
 typedef uint32_t v4u __attribute__((ext_vector_type(4)));
-typedef uint16_t v2u16 __attribute__((ext_vector_type(2)));
 
 // ---------------------------------------------------------------------------
 // helpers
@@ -1858,10 +1857,10 @@ void launch_z8_check(const int8_t* z, int64_t ldz, int64_t rows, int cols, unsig
 }
 
 // Limb rows of the phase-2 operand for gathers by sample.  One workgroup per
-// 64-sample x 64-column tile: the quantized q as k_quant_p2_t forms it, as the
-// NL unsigned base-256 digits of q + 2^(8 NL - 1), staged in LDS and written out
-// so that each sample's nl x 64 bytes are contiguous.  Rows run to n inclusive:
-// row n is all zeros (the pad entry of the gathers).
+// 64-sample x 64-column tile: q and its balanced digits as k_quant_p2_t forms
+// them, staged in LDS and written out so that each sample's nl x 64 bytes are
+// contiguous (byte 64 l + c: digit l, most significant first, of column c).
+// Rows run to n inclusive: row n is all zeros (the pad entry of the gathers).
 template <int NL>
 __global__ __launch_bounds__(256) void k_xrow_limbs(const double* __restrict__ x, int64_t ldx, int64_t n, int ncols,
                                                     int v_count, int v_pad, const int32_t* __restrict__ shift,
@@ -1880,11 +1879,10 @@ __global__ __launch_bounds__(256) void k_xrow_limbs(const double* __restrict__ x
             const int k = c / v_pad, v = c % v_pad;
             if (v < v_count) q = quant_rn(x[(int64_t)(k * v_count + v) * ldx + smp], shift[c]);
         }
-        // offset-binary digits of u = q + 2^(8 NL - 1) (unsigned, most significant first):
-        // the gathers add them as unsigned bytes; the pad row (smp == n) is all zero
-        const uint64_t u = smp < n ? (uint64_t)q + ((uint64_t)1 << (8 * NL - 1)) : 0;
+        int8_t d[NL];
+        balanced_digits<NL>(q, d);
 #pragma unroll
-        for (int l = 0; l < NL; ++l) tile[sl][l * 64 + cc] = (int8_t)(uint8_t)(u >> (8 * (NL - 1 - l)));
+        for (int l = 0; l < NL; ++l) tile[sl][l * 64 + cc] = d[l];
     }
     __syncthreads();
     for (int i = tid; i < 64 * DW; i += 256) {
@@ -1935,35 +1933,46 @@ void launch_range_ptrs(const int64_t* ptr, const int32_t* idx, int64_t m, int pa
     GPE_HIP(hipGetLastError());
 }
 
-// Exact missing-call sums of the phase-2 operand over one sample range, one
-// wave per (range p, column chunk h, SNP j):
+// Exact missing-call sums of the phase-2 operand over sample ranges p:
 //   part_p[j - r0][64 h + cc] = sum over samples n in range p missing at j of q[n][64 h + cc].
-// A sample's limb row is D = nl x 16 dwords (k_xrow_limbs).  Entries are taken
-// U = 16 at a time: each entry's first 64 dwords are one full-wave load (lane
-// = dword), and its last D - 64 dwords share a full-wave load with the next
-// 64 / (D - 64) - 1 entries (lane groups of D - 64).  Each lane sums the four
-// signed bytes of its dwords in int32 (|sum| <= 128 n < 2^31); the tail lane
-// groups are folded with shuffles and the limbs combined to int64 at the end.
-// A batch's entries come 64 at a time (lane t loads entry t's sample index,
-// entries past the list read the all-zero row n), so every group is full and
-// its 16 + 16 (D - 64) / 64 loads are in flight together.  With P a multiple of
-// the XCD count, workgroup b runs on XCD b % 8 and the ranges are dealt so that
-// each XCD gathers only from its own ranges, one after another: a range's rows
-// stay in that XCD's L2 (at N = 100,000, P = 8: 90% L2 hits).  The gathers are
-// then bound by the misses a CU keeps in flight to L2, about 7 ms for 2.5e8
-// calls of 320 bytes, against about 8 ms for the missing-flag MFMA plane.
+// One wave per (range p, column chunk h, SNP j).
+// A sample's limb row (k_xrow_limbs) is P = 4 NL pieces of 16 bytes.  The sums
+// run on the int8 matrix cores: v_mfma_i32_16x16x64_i8 with A the constant
+// selector A[i][k] = 1 where k is byte i of a lane's 16 (byte (lane % 16) of
+// each lane set) gives D[i][c] += sum over the four lanes c, c+16, c+32, c+48
+// of byte i of their B registers -- the same (row, column) entry of D for the
+// same byte of four different gathered pieces, whatever order the instruction
+// gives the K index within a lane (A and B share it).  So, per round of 16
+// entries, lane class c = lane % 16 loads piece c of entries 4m + lane / 16 in
+// four MFMAs into D0 (D0[i][c]: byte 16 c + i of the row), and the T = P - 16
+// tail pieces of all 16 entries go through T / 4 more MFMAs into D1, class c
+// loading piece 16 + c % T (D1[i][c]: byte 16 (16 + c % T) + i, summed over the
+// 16 / T classes of that piece at the end).  Per entry that is 5/16 (NL = 5) of
+// an MFMA and the row's 20 lanes of 16-byte loads, where summing bytes on the
+// vector ALU took about 8 instructions per entry; the limbs are combined to
+// int64 at the end (|sum| <= 128 n < 2^31 per digit).  A batch's entries come 64
+// at a time (lane t loads entry t's sample index; entries past the list read the
+// all-zero row n).  With P a multiple of the XCD count, workgroup b runs on XCD
+// b % 8 and the ranges are dealt so that each XCD gathers only from its own
+// ranges, one after another: a range's rows stay in that XCD's L2.
+// BUF (table below 4 GiB): buffer loads with 32-bit offsets.
 template <int NL, bool BUF>
 __global__ __launch_bounds__(256) void k_miss_sums(const int64_t* __restrict__ rptr, int parts,
                                                    const int32_t* __restrict__ idx, int64_t r0, int64_t r1,
                                                    const int8_t* __restrict__ xrow, int n, int nch,
                                                    int64_t* __restrict__ out, int64_t part_stride, int64_t ld_ms,
                                                    uint32_t table_bytes) {
-    constexpr int D = NL * 16;
-    constexpr int X = D - 64;        // tail dwords per entry (16 or 32)
-    constexpr int E = 64 / X;        // entries per tail load
-    constexpr int U = 16;            // entries per group
-    static_assert(X > 0 && 64 % X == 0 && U % E == 0, "limb rows of 5 or 6 limbs");
-    __shared__ int32_t red[4][NL * 64];
+    constexpr int P = 4 * NL;      // 16-byte pieces per row
+    constexpr int T = P - 16;      // tail pieces
+    constexpr int R = 16 / T;      // lane classes per tail piece
+    constexpr int TM = T / 4;      // tail MFMAs per round of 16 entries
+    constexpr int TE = 64 / T;     // entries per tail MFMA
+    // rounds of 16 entries per step: occupancy beats loads in flight per wave
+    // (N = 100,000, M = 500,000, 64 columns, 0.5% missing: 1 round 4.78 ms with
+    // the range sums, 2 rounds 4.86 ms, 4 rounds 6.34 ms)
+    constexpr int RPS = 1;
+    static_assert(T == 4 || T == 8, "limb rows of 5 or 6 limbs");
+    __shared__ int32_t red[4][512];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int64_t rows = r1 - r0;
@@ -1986,87 +1995,76 @@ __global__ __launch_bounds__(256) void k_miss_sums(const int64_t* __restrict__ r
     const int h = (int)(item / rows);
     const int64_t j = r0 + item % rows;
     const int64_t b0 = rptr[j * (parts + 1) + p], b1 = rptr[j * (parts + 1) + p + 1];
-    // dword offsets (the table holds fewer than 2^32 dwords)
-    const uint32_t rstride = (uint32_t)(nch * D);
-    const uint32_t* tab = reinterpret_cast<const uint32_t*>(xrow) + (uint32_t)h * D;
-    const uint32_t tail_off = 64u + (uint32_t)(lane % X);
-    const int tail_grp = lane / X;
-    // BUF (table below 4 GiB): buffer loads with the row's byte offset in a scalar
-    // register and the lane's dword as the vector offset -- no address arithmetic per
-    // entry on the vector ALU
-    // (the descriptor starts at this wave's column group; unused when !BUF)
+    const uint32_t rstride = (uint32_t)nch * (uint32_t)(NL * 64);  // bytes per sample row
+    const int8_t* base = xrow + (int64_t)h * (NL * 64);
     const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<uint32_t*>(tab), (short)0, (int)(table_bytes - 4u * (uint32_t)h * D), 0x00020000);
-    // 32-bit sums of the dword's four unsigned digit bytes, and packed 16-bit sums of
-    // bytes 0, 2 (even) and 1, 3 (odd) within a batch of at most 64 entries
-    // (64 x 255 < 2^16): a mask, a byte permute (bytes 1 and 3 to 0 and 2, zeros
-    // between) and two packed adds per dword and entry
-    int32_t s0[4] = {0, 0, 0, 0}, s1[4] = {0, 0, 0, 0};
-    auto flush = [](int32_t* acc, v2u16 ev, v2u16 od) {
-        acc[0] += (int32_t)ev.x;
-        acc[1] += (int32_t)od.x;
-        acc[2] += (int32_t)ev.y;
-        acc[3] += (int32_t)od.y;
+        const_cast<int8_t*>(base), (short)0, (int)(table_bytes - (uint32_t)h * (uint32_t)(NL * 64)), 0x00020000);
+    const int c16 = lane & 15, g = lane >> 4;
+    // the selector: byte c16 of the lane's 16 is 1
+    const int32_t one = 1 << (8 * (c16 & 3));
+    const v4i sel = v4i{(c16 >> 2) == 0 ? one : 0, (c16 >> 2) == 1 ? one : 0, (c16 >> 2) == 2 ? one : 0,
+                        (c16 >> 2) == 3 ? one : 0};
+    const uint32_t off_head = 16u * (uint32_t)c16;
+    const uint32_t off_tail = 16u * (uint32_t)(16 + c16 % T);
+    const int slot_tail = g * R + c16 / T;
+    auto load = [&](uint32_t smp, uint32_t off) -> v4i {
+        if constexpr (BUF)
+            return __builtin_bit_cast(v4i, __builtin_amdgcn_raw_buffer_load_b128(rsrc, (int)(smp * rstride + off), 0, 0));
+        else
+            return *reinterpret_cast<const v4i*>(base + (int64_t)smp * rstride + off);
     };
+    v4i d0a = v4i{0, 0, 0, 0}, d0b = v4i{0, 0, 0, 0}, d1 = v4i{0, 0, 0, 0};
     for (int64_t p0 = b0; p0 < b1; p0 += 64) {
         const int cnt = (int)((b1 - p0) < 64 ? (b1 - p0) : 64);
         const int my = lane < cnt ? idx[p0 + lane] : n;
-        v2u16 e0 = v2u16{0, 0}, o0 = v2u16{0, 0}, e1 = v2u16{0, 0}, o1 = v2u16{0, 0};
-        for (int t = 0; t < cnt; t += U) {
-            uint32_t v0[U], v1[U / E];
+        // RPS rounds of 16 entries per step: RPS (4 + TM) loads in flight
+        for (int t0 = 0; t0 < cnt; t0 += 16 * RPS) {
+            v4i hv[4 * RPS], tv[RPS * TM];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const uint32_t smp = (uint32_t)__builtin_amdgcn_readlane(my, t + u);
-                if constexpr (BUF)
-                    v0[u] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, (int)(4u * (uint32_t)lane),
-                                                                 (int)(4u * smp * rstride), 0);
-                else
-                    v0[u] = tab[smp * rstride + (uint32_t)lane];
+            for (int r = 0; r < RPS; ++r) {
+#pragma unroll
+                for (int m = 0; m < 4; ++m)
+                    hv[4 * r + m] = load((uint32_t)__shfl(my, t0 + 16 * r + 4 * m + g, 64), off_head);
+#pragma unroll
+                for (int m = 0; m < TM; ++m)
+                    tv[TM * r + m] = load((uint32_t)__shfl(my, t0 + 16 * r + TE * m + slot_tail, 64), off_tail);
+            }
+            // every load issued before the first MFMA waits on one
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int m = 0; m < 4 * RPS; m += 2) {
+                d0a = __builtin_amdgcn_mfma_i32_16x16x64_i8(sel, hv[m], d0a, 0, 0, 0);
+                d0b = __builtin_amdgcn_mfma_i32_16x16x64_i8(sel, hv[m + 1], d0b, 0, 0, 0);
             }
 #pragma unroll
-            for (int w = 0; w < U / E; ++w) {
-                const uint32_t smp = (uint32_t)__shfl(my, t + w * E + tail_grp, 64);
-                if constexpr (BUF)
-                    v1[w] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, (int)(4u * (smp * rstride + tail_off)), 0, 0);
-                else
-                    v1[w] = tab[smp * rstride + tail_off];
-            }
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                e0 += __builtin_bit_cast(v2u16, v0[u] & 0x00FF00FFu);
-                o0 += __builtin_bit_cast(v2u16, __builtin_amdgcn_perm(0u, v0[u], 0x0C030C01u));
-            }
-#pragma unroll
-            for (int w = 0; w < U / E; ++w) {
-                e1 += __builtin_bit_cast(v2u16, v1[w] & 0x00FF00FFu);
-                o1 += __builtin_bit_cast(v2u16, __builtin_amdgcn_perm(0u, v1[w], 0x0C030C01u));
-            }
+            for (int m = 0; m < RPS * TM; ++m) d1 = __builtin_amdgcn_mfma_i32_16x16x64_i8(sel, tv[m], d1, 0, 0, 0);
         }
-        flush(s0, e0, o0);
-        flush(s1, e1, o1);
     }
-    // fold the tail lane groups: lanes < X then hold dword 64 + lane
-#pragma unroll
-    for (int o = X; o < 64; o <<= 1) {
-#pragma unroll
-        for (int k = 0; k < 4; ++k) s1[k] += __shfl_down(s1[k], o, 64);
-    }
-    // dword d holds limb d / 16 of columns 4 (d % 16) .. +3, i.e. red[4 d + k]
+    // D[i][c] sits in lane c + 16 (i / 4), register i % 4
     int32_t* rd = red[wave];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) rd[4 * lane + k] = s0[k];
-    if (lane < X) {
-#pragma unroll
-        for (int k = 0; k < 4; ++k) rd[4 * (64 + lane) + k] = s1[k];
+    for (int r = 0; r < 4; ++r) {
+        rd[(4 * g + r) * 16 + c16] = d0a[r] + d0b[r];
+        rd[256 + (4 * g + r) * 16 + c16] = d1[r];
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // sum of the offset values minus the offset of every real entry (the pad row is 0)
+    // lane = column: digit l is row byte 64 l + lane
     int64_t me = 0;
 #pragma unroll
-    for (int l = 0; l < NL; ++l) me = me * 256 + (int64_t)rd[l * 64 + lane];
-    me -= (b1 - b0) * ((int64_t)1 << (8 * NL - 1));
+    for (int l = 0; l < NL; ++l) {
+        const int bb = 64 * l + lane;
+        int32_t v = 0;
+        if (l < 4) {
+            v = rd[(bb & 15) * 16 + (bb >> 4)];
+        } else {
+            const int q = (bb >> 4) - 16;
+#pragma unroll
+            for (int a = 0; a < R; ++a) v += rd[256 + (bb & 15) * 16 + a * T + q];
+        }
+        me = me * 256 + (int64_t)v;
+    }
     out[(int64_t)p * part_stride + (j - r0) * ld_ms + h * 64 + lane] = me;
 }
 // out[i] = sum_p parts[p * stride + i] for i < n, into parts[i] (part 0), in order
