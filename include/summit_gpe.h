@@ -325,6 +325,18 @@ int gpe_project_f64_noalloc(double* x_dev, int64_t ldx, int64_t n, int32_t n_col
  * (:3134-3171) after gpe_project_f64. */
 int gpe_col_sums_f64(const double* x_dev, int64_t ldx, int64_t n, int32_t n_cols, double* out_dev, void* stream);
 
+/* out_dev[k * ldo + i] += sum_{snp_begin <= j < snp_end} xt_ij^2 w_dev[(j - snp_begin) + k * ldw]
+ * for k < n_bins and every sample i (FP64), xt the standardized working genotype
+ * (0 at a missing call under mean imputation) and, when p > 0, its projection
+ * xt = x - C (R x) with C column-major (N, p) (ldc) and rx_dev[(j - snp_begin) *
+ * ldrx + a] = (R x_j)_a (p <= 64).  With w = a_jk / resvar_j these are the
+ * per-sample sums S of the fourth-moment (mu22) statistics of
+ * precompute_residual_variances_bed(compute_mu22=True)
+ * (src/native/gwldcore.cpp:3496-3521; mu22 = S^T S / (N Msum_a Msum_b), :3528-3544). */
+int gpe_sq_rowsums_f64(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const double* w_dev, int64_t ldw,
+                       int32_t n_bins, const double* c_dev, int64_t ldc, const double* rx_dev, int64_t ldrx, int32_t p,
+                       double* out_dev, int64_t ldo, void* stream);
+
 /* out[e * ldo + (j - snp_begin)] = sum_n w[e * ldw + n] x_jn^2 for each of
  * n_weights weight vectors (standardized working codes): the ||e . x_j||^2
  * feature moments of the GxE interaction columns, every environment in one

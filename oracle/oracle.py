@@ -34,6 +34,11 @@ Pins (each function is checked by something other than a retyping of itself):
   ``1^T x = 0``, ``||x_j||^2 / (N - p - 1) = 1``, and planted mistakes (no
   centring, environment applied after the projection, a dropped degree of
   freedom) that the formula rejects.
+* Fused residual-variance pass (``resvar_pass``): hand-computed mu22 of one SNP
+  with and without a missing call, invariance to SNP duplication and block
+  size, the Cauchy-Schwarz bound of complete data, the (N - 2)/(N - 1) scaling
+  of an intercept-only projection, and V against the reference test's formula
+  (``tests/test_snp_var_weights.py:52-67``).
 """
 from __future__ import annotations
 
@@ -320,6 +325,52 @@ def inv_resvar(X, eps: float = 1e-10):
     var = ss / float(N - 1)
     safe = np.where(np.isnan(var), np.nan, np.maximum(var, eps))
     return 1.0 / np.sqrt(safe)
+
+
+# ----------------------------------------------------------------------------
+# Fused residual-variance pass: src/native/gwldcore.cpp:3335-3566
+# ----------------------------------------------------------------------------
+def resvar_pass(bed, n_total: int, nsnps: int, step_size: int, rows=None, ddof: int = 1, eps: float = 1e-10,
+                annot=None, C=None, R=None, mode: str = "mean", impute_seed: int = 0):
+    """(inv, mu22 or None, snp_var) of precompute_residual_variances_bed_impl:
+    per block the standardized genotypes X, raw = ||x_j||^2, X <- X - C (R X)
+    when C/R are given (:3468-3469), ss = ||x_j||^2 over proj_df = N - p - 1
+    (N - 1 without covariates) into inv (eps floor, NaN kept, :3478-3490),
+    snp_var = min(1, ss / raw) (0 where raw is 0 or ss is not finite, :3491-3493);
+    with annot, S (N, B) += (X * X) (a * inv^2) (:3496-3521) and
+    mu22[a, b] = (S^T S)[a, b] / (N Msum_a Msum_b), 0 where the denominator is 0
+    or the value is not finite (:3528-3544)."""
+    Nr = n_total if rows is None else len(rows)
+    p = 0 if C is None else np.asarray(C).shape[1]
+    proj_df = float(Nr - p - 1) if C is not None else float(Nr - 1)
+    inv = np.empty(nsnps)
+    snp_var = np.empty(nsnps)
+    S = None
+    if annot is not None:
+        A = np.asarray(annot, dtype=np.float64)
+        S = np.zeros((Nr, A.shape[1]))
+    for s in range(0, nsnps, step_size):
+        e = min(nsnps, s + step_size)
+        X = np.array(read_block_standardized(bed, n_total, s, e, rows, ddof, mode, impute_seed), dtype=np.float64)
+        raw = np.einsum("ij,ij->j", X, X)
+        if C is not None:
+            X = X - np.asarray(C, dtype=np.float64) @ (np.asarray(R, dtype=np.float64) @ X)
+        ss = np.einsum("ij,ij->j", X, X)
+        var = ss / proj_df
+        safe = np.where(np.isnan(var), np.nan, np.maximum(var, eps))
+        inv[s:e] = 1.0 / np.sqrt(safe)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            snp_var[s:e] = np.where((raw > 0.0) & np.isfinite(ss), np.minimum(1.0, ss / raw), 0.0)
+        if S is not None:
+            S += (X * X) @ (A[s:e] * (inv[s:e] ** 2)[:, None])
+    if S is None:
+        return inv, None, snp_var
+    msum = A[:nsnps].sum(axis=0)
+    denom = float(Nr) * msum[:, None] * msum[None, :]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        mu22 = np.where(denom != 0.0, (S.T @ S) / denom, 0.0)
+    mu22[~np.isfinite(mu22)] = 0.0
+    return inv, mu22, snp_var
 
 
 # ----------------------------------------------------------------------------

@@ -55,6 +55,8 @@ SIGNATURES = {
                                                c_vp]),
     "gpe_col_sums_f64": (ctypes.c_int, [c_vp, c_i64, c_i64, c_i32, c_vp, c_vp]),
     "gpe_weighted_sumsq": (ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_i32, c_i64, c_vp, c_i64, c_vp]),
+    "gpe_sq_rowsums_f64": (ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_i64, c_i32, c_vp, c_i64, c_vp, c_i64, c_i32,
+                                          c_vp, c_i64, c_vp]),
     "gpe_windowed_ld": (ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_dbl, c_i64, c_vp, c_i32, c_vp, c_vp]),
     "gpe_windowed_ld_multi": (ctypes.c_int, [c_vp, c_i32, c_vp, c_vp, c_dbl, c_i64, c_vp, c_i32, c_vp, c_vp]),
     "gpe_windowed_ld_side": (ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_dbl, c_i64, c_i32, c_vp, c_i32, c_vp, c_vp]),

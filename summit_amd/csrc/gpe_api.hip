@@ -1689,6 +1689,26 @@ int gpe_col_sums_f64(const double* x_dev, int64_t ldx, int64_t n, int32_t n_cols
     });
 }
 
+int gpe_sq_rowsums_f64(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const double* w_dev, int64_t ldw,
+                       int32_t n_bins, const double* c_dev, int64_t ldc, const double* rx_dev, int64_t ldrx, int32_t p,
+                       double* out_dev, int64_t ldo, void* stream) {
+    return guarded([&] {
+        require_prepared(g);
+        require(snp_begin >= 0 && snp_end <= g->m && snp_begin <= snp_end, "SNP range out of bounds");
+        const int64_t L = snp_end - snp_begin;
+        require(n_bins >= 0 && (n_bins == 0 || (w_dev != nullptr && out_dev != nullptr && ldw >= L && ldo >= g->n)),
+                "bad squared-row-sum shape");
+        require(p >= 0 && p <= 64, "squared row sums take at most 64 covariates");
+        if (p > 0)
+            require(c_dev != nullptr && rx_dev != nullptr && ldc >= g->n && ldrx >= p, "C/R shape mismatch");
+        if (n_bins == 0 || L == 0) return;
+        const int64_t ny = ceil_div(L, sq_rowsums_chunk(g->n, L));
+        double* part = static_cast<double*>(g->scratch("sq_part", (size_t)ny * 8 * g->n * sizeof(double)));
+        launch_sq_rowsums(g->smp_work, g->smp_stride, g->n, snp_begin, snp_end, g->mean, g->inv_sd, w_dev, ldw, n_bins,
+                          c_dev, ldc, rx_dev, ldrx, p, part, out_dev, ldo, S(stream));
+    });
+}
+
 int gpe_weighted_sumsq(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const double* w_dev, int32_t n_weights,
                        int64_t ldw, double* out_dev, int64_t ldo, void* stream) {
     return guarded([&] {

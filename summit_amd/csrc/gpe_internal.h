@@ -154,6 +154,14 @@ void launch_miss_sums(const int64_t* rptr, int parts, const int32_t* idx, int64_
                       int64_t n, int nl, int nch, int64_t* out, int64_t part_stride, int64_t ld_ms, hipStream_t st);
 // parts[i] = sum_p parts[p * stride + i] for i < n (the sample ranges' sums into part 0)
 void launch_sum_parts(int64_t* parts, int n_parts, int64_t stride, int64_t n, hipStream_t st);
+// out[k * ldo + i] += sum_{j in [s, e)} xt_ij^2 w[(j - s) + k ldw] (k < n_bins), xt the standardized
+// working genotype minus C (R x) when p > 0 (rx[(j - s) ldrx + a] = (R x_j)_a, p <= 64); part holds
+// 8 n ceil((e - s) / sq_rowsums_chunk(n, e - s)) doubles
+int64_t sq_rowsums_chunk(int64_t n, int64_t L);
+void launch_sq_rowsums(const uint8_t* smp, int64_t stride, int64_t n, int64_t s, int64_t e, const double* mean,
+                       const double* inv_sd, const double* w, int64_t ldw, int n_bins, const double* C, int64_t ldc,
+                       const double* rx, int64_t ldrx, int p, double* part, double* out, int64_t ldo, hipStream_t st);
+
 void launch_miss_fill(const uint8_t* smp, int64_t stride, int64_t n, const int64_t* ptr, int32_t* idx,
                       hipStream_t st);
 void launch_fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t st);

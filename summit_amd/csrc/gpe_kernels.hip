@@ -2067,6 +2067,110 @@ __global__ __launch_bounds__(256) void k_miss_sums(const int64_t* __restrict__ r
     }
     out[(int64_t)p * part_stride + (j - r0) * ld_ms + h * 64 + lane] = me;
 }
+// Per-sample sums of squared standardized genotypes weighted per SNP and bin,
+// the X2 * W panel products of the fused residual-variance pass
+// (src/native/gwldcore.cpp:3496-3521):
+//   part[y][k][i] = sum over SNPs j of chunk y of xt_ij^2 w[(j - s) + (kb0 + k) ldw]
+// with xt_ij = x_ij - sum_a C[i + a ldc] rx[(j - s) ldrx + a] (no projection when
+// p = 0) and x_ij the standardized genotype of the working codes (0 at a missing
+// call under mean imputation).  One thread per sample: its row of the
+// sample-major layout is read 16 bytes (64 SNPs) at a time, and the SNP-wise
+// values (mean, 1/sd, weights, R x) are the same for every lane of a wave.
+template <int KB, int PMAX>
+__global__ __launch_bounds__(256) void k_sq_rowsums(const uint8_t* __restrict__ smp, int64_t stride, int64_t n,
+                                                    int64_t s, int64_t e, int64_t chunk,
+                                                    const double* __restrict__ mean,
+                                                    const double* __restrict__ inv_sd, const double* __restrict__ w,
+                                                    int64_t ldw, int kb0, int kb, const double* __restrict__ C,
+                                                    int64_t ldc, const double* __restrict__ rx, int64_t ldrx, int p,
+                                                    double* __restrict__ part, int64_t ldp, int64_t part_stride) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t j0 = s + (int64_t)blockIdx.y * chunk;
+    const int64_t j1 = j0 + chunk < e ? j0 + chunk : e;
+    if (i >= n) return;
+    double ci[PMAX > 0 ? PMAX : 1];
+#pragma unroll
+    for (int a = 0; a < PMAX; ++a) ci[a] = a < p ? C[(int64_t)a * ldc + i] : 0.0;
+    double acc[KB];
+#pragma unroll
+    for (int k = 0; k < KB; ++k) acc[k] = 0.0;
+    const uint8_t* row = smp + i * stride;
+    for (int64_t jb = j0 & ~(int64_t)63; jb < j1; jb += 64) {
+        const v4u wv = *reinterpret_cast<const v4u*>(row + jb / 4);
+        const int t0 = jb < j0 ? (int)(j0 - jb) : 0;
+        const int t1 = jb + 64 > j1 ? (int)(j1 - jb) : 64;
+        for (int t = t0; t < t1; ++t) {
+            const int64_t j = jb + t;
+            const uint32_t code = (wv[t >> 4] >> (2 * (t & 15))) & 3u;
+            double x = code == 1u ? 0.0 : ((double)(code == 0u ? 0 : (int)code - 1) - mean[j]) * inv_sd[j];
+#pragma unroll
+            for (int a = 0; a < PMAX; ++a)
+                if (a < p) x -= ci[a] * rx[(j - s) * ldrx + a];
+            const double x2 = x * x;
+#pragma unroll
+            for (int k = 0; k < KB; ++k)
+                if (k < kb) acc[k] = fma(x2, w[(j - s) + (int64_t)(kb0 + k) * ldw], acc[k]);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < KB; ++k)
+        if (k < kb) part[(int64_t)blockIdx.y * part_stride + (int64_t)k * ldp + i] = acc[k];
+}
+// out[(kb0 + k) ldo + i] += sum_y part[y][k][i], the chunks added in order
+__global__ void k_sq_rowsums_reduce(const double* __restrict__ part, int ny, int64_t part_stride, int64_t ldp,
+                                    int64_t n, int kb0, int kb, double* __restrict__ out, int64_t ldo) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * kb) return;
+    const int k = (int)(t / n);
+    const int64_t i = t % n;
+    double v = 0.0;
+    for (int y = 0; y < ny; ++y) v += part[(int64_t)y * part_stride + (int64_t)k * ldp + i];
+    out[(int64_t)(kb0 + k) * ldo + i] += v;
+}
+int64_t sq_rowsums_chunk(int64_t n, int64_t L) {
+    // about 2048 workgroups, chunks of at least 1024 SNPs
+    const int64_t gx = ceil_div(n, 256);
+    const int64_t ny = std::max<int64_t>(1, std::min<int64_t>(ceil_div(L, 1024), ceil_div(2048, gx)));
+    return round_up(ceil_div(L, ny), 64);
+}
+void launch_sq_rowsums(const uint8_t* smp, int64_t stride, int64_t n, int64_t s, int64_t e, const double* mean,
+                       const double* inv_sd, const double* w, int64_t ldw, int n_bins, const double* C, int64_t ldc,
+                       const double* rx, int64_t ldrx, int p, double* part, double* out, int64_t ldo, hipStream_t st) {
+    const int64_t L = e - s;
+    if (n <= 0 || L <= 0 || n_bins <= 0) return;
+    const int64_t chunk = sq_rowsums_chunk(n, L);
+    const int64_t ny = ceil_div(L, chunk);
+    const int64_t ldp = n;
+    const int64_t part_stride = 8 * n;
+    const dim3 grid((unsigned)ceil_div(n, 256), (unsigned)ny);
+    for (int kb0 = 0; kb0 < n_bins; kb0 += 8) {
+        const int kb = std::min(8, n_bins - kb0);
+#define GPE_SQ(KB_, P_)                                                                                          \
+    hipLaunchKernelGGL((k_sq_rowsums<KB_, P_>), grid, dim3(256), 0, st, smp, stride, n, s, e, chunk, mean, inv_sd, \
+                       w, ldw, kb0, kb, C, ldc, rx, ldrx, p, part, ldp, part_stride)
+#define GPE_SQP(KB_)                     \
+    if (p == 0) GPE_SQ(KB_, 0);          \
+    else if (p <= 8) GPE_SQ(KB_, 8);     \
+    else if (p <= 32) GPE_SQ(KB_, 32);   \
+    else GPE_SQ(KB_, 64)
+        if (kb == 1) {
+            GPE_SQP(1);
+        } else if (kb == 2) {
+            GPE_SQP(2);
+        } else if (kb <= 4) {
+            GPE_SQP(4);
+        } else {
+            GPE_SQP(8);
+        }
+#undef GPE_SQP
+#undef GPE_SQ
+        GPE_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_sq_rowsums_reduce, dim3((unsigned)ceil_div(n * kb, 256)), dim3(256), 0, st, part,
+                           (int)ny, part_stride, ldp, n, kb0, kb, out, ldo);
+        GPE_HIP(hipGetLastError());
+    }
+}
+
 // out[i] = sum_p parts[p * stride + i] for i < n, into parts[i] (part 0), in order
 __global__ void k_sum_parts(int64_t* parts, int n_parts, int64_t stride, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -307,6 +307,36 @@ def xtv(g: Genotype, v: torch.Tensor, snp_begin: int = 0, snp_end: int | None = 
     return out
 
 
+def sq_rowsums(g: Genotype, w_kl: torch.Tensor, snp_begin: int = 0, snp_end: int | None = None,
+               C: torch.Tensor | None = None, rx: torch.Tensor | None = None, out=None, stream=None) -> torch.Tensor:
+    """out[k, i] += sum_j xt[i, j]^2 w_kl[k, j - snp_begin] over SNPs j of [snp_begin, snp_end): per-sample
+    sums of squared standardized genotypes, projected xt = x - C (R x) when C (N, p) and rx (L, p) = (R x_j)_a
+    are given.  w_kl (K, L) FP64 = column-major (L, K); out (K, N) FP64, zeros when None."""
+    snp_end = g.m if snp_end is None else snp_end
+    L = snp_end - snp_begin
+    w_kl = w_kl.to(torch.float64).contiguous()
+    if w_kl.dim() != 2 or w_kl.shape[1] != L:
+        raise RuntimeError("weights must be (K, snp_end - snp_begin)")
+    K = w_kl.shape[0]
+    if out is None:
+        out = torch.zeros((K, g.n), dtype=torch.float64, device=w_kl.device)
+    if out.shape != (K, g.n) or out.dtype != torch.float64 or out.stride(1) != 1:
+        raise RuntimeError("out must be (K, n_samples) FP64")
+    p = 0
+    c_ptr = r_ptr = 0
+    ldc = ldr = 0
+    if C is not None:
+        c_cm = C.to(torch.float64).t().contiguous()  # (p, N) rows = columns of C
+        rx = rx.to(torch.float64).contiguous()
+        p = int(c_cm.shape[0])
+        if c_cm.shape[1] != g.n or rx.shape != (L, p):
+            raise RuntimeError("C must be (N, p) and rx (snp_end - snp_begin, p)")
+        c_ptr, ldc, r_ptr, ldr = _ptr(c_cm), g.n, _ptr(rx), p
+    call("gpe_sq_rowsums_f64", g.handle, snp_begin, snp_end, _ptr(w_kl), _ld(w_kl), K, c_ptr, ldc, r_ptr, ldr, p,
+         _ptr(out), _ld(out), _stream(stream))
+    return out
+
+
 def project(x_qn: torch.Tensor, A: torch.Tensor | None, B: torch.Tensor | None, center: bool = False,
             stream=None) -> torch.Tensor:
     """In place: x_q <- x_q - A (B x_q) [then minus its mean] for the rows q of

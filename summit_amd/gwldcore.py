@@ -17,6 +17,10 @@ libsummit_gpe on the GPU.  Differences a caller can see:
   the GPU in the library's own FP64 kernels (gpe_project_f64, gpe_col_sums_f64,
   the phase-2 contraction); the Cholesky factor of the small p x p
   ``resvar_gram`` is taken on the host in FP64.
+* ``precompute_residual_variances_bed(compute_mu22=True)`` forms the per-sample
+  sums S of the fourth-moment statistics on the GPU (gpe_sq_rowsums_f64, at
+  most 64 covariates) and the B x B ratios on the host in FP64;
+  ``compute_block_corrections_binary`` is host FP64.
 """
 from __future__ import annotations
 
@@ -358,31 +362,31 @@ def phase1_compute_Xz_bed_chunk_rowmajor(bed_prefix, fam_path, blk_start, blk_en
 
 
 # --------------------------------------------------------------------------- residual variances
-def residual_inv_block(g: engine.Genotype, s: int, e: int, eps: float, C=None, R=None, gram=None) -> torch.Tensor:
-    """inv_j = 1/sqrt(max(||P x_j||^2 / df, eps)) (src/native/gwldcore.cpp:3220-3333).
+def _residual_sums(g: engine.Genotype, s: int, e: int, C=None, R=None, gram=None):
+    """(ss, ss_proj, df) for SNPs [s, e): ||x_j||^2 of the standardized working
+    genotypes, ||P x_j||^2 with P x = x - C (R x) (= ss without covariates) and
+    the residual degrees of freedom N - p - 1 (N - 1 without covariates).
 
-    With covariates, ||P x||^2 = ||x||^2 + q - 2h with h = ||R x||^2 and
-    q = h or (R x)^T G (R x); both are sums of squares of phase-2 products
-    against transformed covariate rows, so they run on the packed-contraction
-    kernel (G = L L^T, q = ||L^T R x||^2)."""
+    ||P x||^2 = ||x||^2 + q - 2h with h = ||R x||^2 and q = h or (R x)^T G (R x);
+    both are sums of squares of phase-2 products against transformed covariate
+    rows, so they run on the packed-contraction kernel (G = L L^T, q = ||L^T R x||^2)."""
     if (C is None) != (R is None):
         raise RuntimeError("C and R must be provided together for fused residual variances")
-    if C is None:
-        return engine.resvar_inv(g, s, e, eps)
     N = g.n
+    # plain ||x_j||^2 from the integer moments: inv0 = 1/sqrt(ss/(N-1)) -> ss
+    inv0 = engine.resvar_inv(g, s, e, 0.0).cpu().numpy()
+    with np.errstate(divide="ignore", invalid="ignore"):
+        ss = float(N - 1) / (inv0 * inv0)
+    ss = np.where(np.isfinite(ss), ss, 0.0)
+    if C is None:
+        return ss, ss, float(N - 1)
     Rd = _to_dev(R)
     p = Rd.shape[0]
     if tuple(_to_dev(C).shape) != (N, p) or Rd.shape[1] != N:
         raise RuntimeError("C/R shape mismatch in fused residual variances")
     if N - p <= 1:
         raise RuntimeError("N_eff must be > 1 in fused residual variances")
-    df = float(N - p - 1)
     ones = torch.ones(e - s, dtype=torch.float64, device=_dev())
-    # plain ||x_j||^2 from the integer moments: inv0 = 1/sqrt(max(ss/(N-1), eps)) -> recover ss
-    inv0 = engine.resvar_inv(g, s, e, 0.0).cpu().numpy()
-    with np.errstate(divide="ignore", invalid="ignore"):
-        ss = float(N - 1) / (inv0 * inv0)
-    ss = np.where(np.isfinite(ss), ss, 0.0)
 
     def sumsq_rows(Mrows: torch.Tensor) -> np.ndarray:
         acc = torch.zeros((g.m, 1), dtype=torch.float64, device=_dev())
@@ -399,30 +403,140 @@ def residual_inv_block(g: engine.Genotype, s: int, e: int, eps: float, C=None, R
         q = sumsq_rows(_to_dev(Lc.T @ Rh))
     else:
         q = h
-    var = (ss + q - 2.0 * h) / df
+    return ss, ss + q - 2.0 * h, float(N - p - 1)
+
+
+def residual_inv_block(g: engine.Genotype, s: int, e: int, eps: float, C=None, R=None, gram=None) -> torch.Tensor:
+    """inv_j = 1/sqrt(max(||P x_j||^2 / df, eps)) (src/native/gwldcore.cpp:3220-3333)."""
+    if (C is None) != (R is None):
+        raise RuntimeError("C and R must be provided together for fused residual variances")
+    if C is None:
+        return engine.resvar_inv(g, s, e, eps)
+    _, ssp, df = _residual_sums(g, s, e, C, R, gram)
+    var = ssp / df
     with np.errstate(invalid="ignore"):
         safe = np.where(np.isnan(var), var, np.maximum(var, eps))
     return _to_dev(1.0 / np.sqrt(safe))
 
 
+def _mu22(g: engine.Genotype, nsnps: int, step_size: int, annot: np.ndarray, inv: np.ndarray, C, R) -> np.ndarray:
+    """mu22[a, b] = (S^T S)[a, b] / (N Msum_a Msum_b) with S[i, k] = sum_j xt_ij^2 a_jk inv_j^2
+    (src/native/gwldcore.cpp:3496-3544): S on the GPU (gpe_sq_rowsums_f64, projected
+    genotypes when C/R are given), the B x B product and ratios on the host in FP64."""
+    N = g.n
+    B = annot.shape[1]
+    dev = _dev()
+    S = torch.zeros((B, N), dtype=torch.float64, device=dev)
+    Cd = rx = None
+    if C is not None:
+        Cd = _to_dev(C)
+        if Cd.shape[1] > 64:
+            raise RuntimeError("compute_mu22 supports at most 64 covariates")
+        rx = engine.xtv(g, _to_dev(R), 0, nsnps, limbs=6)
+    inv2 = inv * inv
+    # weights a_jk inv_j^2 uploaded in SNP slices of at most 2^22 values
+    step = max(1, min(nsnps, (1 << 22) // max(1, B)))
+    for s0 in range(0, nsnps, step):
+        e0 = min(nsnps, s0 + step)
+        w = torch.from_numpy(np.ascontiguousarray((annot[s0:e0] * inv2[s0:e0, None]).T)).to(dev)
+        engine.sq_rowsums(g, w, s0, e0, C=Cd, rx=None if rx is None else rx[s0:e0], out=S)
+    Sh = S.cpu().numpy()
+    msum = annot.sum(axis=0)
+    denom = float(N) * msum[:, None] * msum[None, :]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        mu22 = np.where(denom != 0.0, (Sh @ Sh.T) / denom, 0.0)
+    mu22[~np.isfinite(mu22)] = 0.0
+    return np.ascontiguousarray(mu22)
+
+
 def precompute_residual_variances_bed(bed_prefix, fam_path, nsnps, step_size, row_sel=None, ddof=1, eps=1e-10,
                                       compute_mu22=False, annot_all=None, C=None, R=None, impute_mode="hwe",
                                       impute_seed=None, snp_var_out=None):
-    """(inv, None) for the first nsnps SNPs (src/native/gwldcore.cpp:3335-3566)."""
+    """(inv, mu22 or None) for the first nsnps SNPs (src/native/gwldcore.cpp:3335-3566);
+    snp_var_out (nsnps,) receives ||P x_j||^2 / ||x_j||^2 = 1 - R2_j, 0 for a
+    monomorphic SNP (:3364-3373, :3491-3493)."""
     g, n_total, m_total = _geno(bed_prefix, fam_path, row_sel, ddof, impute_mode, impute_seed)
     nsnps = int(nsnps)
     if nsnps < 0 or nsnps > m_total:
         raise RuntimeError("nsnps is out of range in precompute_residual_variances_bed")
+    if snp_var_out is not None and np.shape(snp_var_out) != (nsnps,):
+        raise RuntimeError("snp_var_out shape mismatch in precompute_residual_variances_bed")
     if int(step_size) <= 0:
         raise RuntimeError("step_size must be > 0 in precompute_residual_variances_bed")
     if g.n <= 1:
         raise RuntimeError("Too few rows in precompute_residual_variances_bed")
+    have_proj = C is not None and R is not None
+    if have_proj:
+        p = np.shape(C)[1]
+        if np.shape(C)[0] != g.n or np.shape(R) != (p, g.n):
+            raise RuntimeError("C/R shape mismatch in precompute_residual_variances_bed")
+        if g.n - p <= 1:
+            raise RuntimeError("N_eff must be > 1 in precompute_residual_variances_bed")
+    else:
+        C = R = None
+    annot = None
     if compute_mu22:
-        raise RuntimeError("compute_mu22 (skew corrections) is not provided by the GPU engine")
+        if annot_all is None:
+            raise RuntimeError("annot_all must be provided when compute_mu22=True")
+        annot = np.asarray(annot_all.cpu().numpy() if isinstance(annot_all, torch.Tensor) else annot_all,
+                           dtype=np.float64)
+        if annot.ndim != 2 or annot.shape[0] != nsnps:
+            raise RuntimeError("annot_all shape mismatch in precompute_residual_variances_bed")
+        if annot.shape[1] <= 0:
+            raise RuntimeError("annot_all must have at least one column")
+    if nsnps == 0:
+        inv = np.empty(0)
+        ss = ssp = np.empty(0)
+    elif C is None:
+        inv = engine.resvar_inv(g, 0, nsnps, float(eps)).cpu().numpy()
+        if snp_var_out is None and annot is None:
+            return inv, None
+        ss, ssp, _ = _residual_sums(g, 0, nsnps)
+    else:
+        ss, ssp, df = _residual_sums(g, 0, nsnps, C, R)
+        var = ssp / df
+        with np.errstate(invalid="ignore", divide="ignore"):
+            inv = 1.0 / np.sqrt(np.where(np.isnan(var), var, np.maximum(var, float(eps))))
     if snp_var_out is not None:
-        raise RuntimeError("snp_var_out is not provided by the GPU engine")
-    inv = residual_inv_block(g, 0, nsnps, float(eps), C, R)
-    return inv.cpu().numpy(), None
+        with np.errstate(divide="ignore", invalid="ignore"):
+            v = np.where((ss > 0.0) & np.isfinite(ssp), np.minimum(1.0, ssp / np.where(ss > 0.0, ss, 1.0)), 0.0)
+        snp_var_out[...] = torch.from_numpy(v) if isinstance(snp_var_out, torch.Tensor) else v
+    mu22 = None
+    if annot is not None:
+        mu22 = _mu22(g, nsnps, int(step_size), annot, inv, C, R) if nsnps > 0 else np.zeros((annot.shape[1],) * 2)
+    return inv, mu22
+
+
+def compute_block_corrections_binary(meansq_raw, mu22, annot_all, N, d):
+    """Block-level fourth-moment diagnostics (R2, rho2, bias, delta), each B x B
+    (src/native/gwldcore.cpp:3568-3633): over the SNPs i with a_ia != 0,
+    S_ab = sum meansq_raw[i, b] and M_a their count; for M_a, M_b > 0 and
+    S_ab != 0: R2 = S / (M_a M_b), rho2 = (d^2 R2 - N mu22) / (N (N - 1)),
+    bias = S - M_a M_b rho2, delta = mu22 - (1 + 2 rho2); 0 elsewhere."""
+    m = np.asarray(meansq_raw, dtype=np.float64)
+    if m.ndim != 2:
+        raise RuntimeError("meansq_raw must be 2-D")
+    M, B = m.shape
+    u = np.asarray(mu22, dtype=np.float64)
+    if u.shape != (B, B):
+        raise RuntimeError("mu22 shape mismatch")
+    A = np.asarray(annot_all)
+    if A.shape != (M, B):
+        raise RuntimeError("annot_all shape mismatch")
+    if int(N) <= 1:
+        raise RuntimeError("N must be > 1 in compute_block_corrections_binary")
+    ind = (A != 0).astype(np.float64)
+    Ma = ind.sum(axis=0)
+    S = ind.T @ m
+    Nf = float(N)
+    ok = (Ma[:, None] > 0.0) & (Ma[None, :] > 0.0) & (S != 0.0)
+    MM = Ma[:, None] * Ma[None, :]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        R2 = np.where(ok, S / np.where(ok, MM, 1.0), 0.0)
+        rho2 = np.where(ok, (float(d) * float(d) * R2 - Nf * u) / (Nf * (Nf - 1.0)), 0.0)
+    bias = np.where(ok, S - MM * rho2, 0.0)
+    delta = np.where(ok, u - (1.0 + 2.0 * rho2), 0.0)
+    return tuple(np.ascontiguousarray(x) for x in (R2, rho2, bias, delta))
 
 
 # --------------------------------------------------------------------------- phase 2

@@ -186,6 +186,19 @@ def write_gw_annotation_masses(outpath, masses) -> str:
     return path
 
 
+def write_gw_delta(outpath, delta, l2cols) -> str:
+    """``<outpath>.gw.delta``: the block-level fourth-moment matrix
+    delta = mu22 - (1 + 2 rho2) (``compute_block_corrections_binary``), rows and
+    columns named by the annotation columns, tab-separated, %.8e
+    (src/summit/ldscore/gw_ldscore.py:2343-2349)."""
+    import pandas as pd
+
+    path = f"{outpath}.gw.delta"
+    pd.DataFrame(np.asarray(delta, dtype=np.float64), index=list(l2cols), columns=list(l2cols)).to_csv(
+        path, sep="\t", float_format="%.8e")
+    return path
+
+
 def mc_variance_from_probe_sums(probe_sum, probe_sumsq, nvecs: int, return_info: bool = False):
     """Monte Carlo variance of a probe mean, sample_var(Y_v) / V, from S1 = sum_v Y_v
     and S2 = sum_v Y_v^2 per element: (S2 - S1^2 / V) / (V (V - 1))

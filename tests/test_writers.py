@@ -25,3 +25,16 @@ def test_ldscore_table_format():
     assert lines[3].split("\t")[4] == "-0.500000"
     mpath = write_gw_annotation_masses(os.path.join(d, "out"), [10000.0, 1234.5678901234, 3])
     assert open(mpath).read() == "10000\t1234.56789\t3\n"
+
+
+def test_delta_matrix_format():
+    """<out>.gw.delta: the B x B delta matrix with the annotation columns as row and
+    column names, tab-separated, %.8e (src/summit/ldscore/gw_ldscore.py:2343-2349)."""
+    from summit_amd.ldscore import write_gw_delta
+    d = tempfile.mkdtemp()
+    path = write_gw_delta(os.path.join(d, "out"), np.array([[1.5, -2.0e-3], [0.0, 12345.678]]), ["L2_0", "L2_1"])
+    assert path.endswith(".gw.delta")
+    lines = open(path).read().splitlines()
+    assert lines[0] == "\tL2_0\tL2_1"
+    assert lines[1] == "L2_0\t1.50000000e+00\t-2.00000000e-03"
+    assert lines[2] == "L2_1\t0.00000000e+00\t1.23456780e+04"
