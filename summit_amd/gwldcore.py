@@ -681,40 +681,66 @@ def project_colmajor_inplace(Xz2d, C=None, R=None) -> None:
 
 
 # --------------------------------------------------------------------------- GRM panel apply
-def _apply_grm(bed_prefix, fam_path, nsnps, step_size, row_sel, ddof, inv_all, panel_in, panel_out, C, R, impute_mode,
-               impute_seed):
-    g, n_total, m_total = _geno(bed_prefix, fam_path, row_sel, ddof, impute_mode, impute_seed)
-    nsnps = int(nsnps)
-    if nsnps < 0 or nsnps > m_total:
-        raise RuntimeError("nsnps is out of range in apply_grm_bed_panel")
-    if int(step_size) <= 0:
-        raise RuntimeError("step_size must be > 0 in apply_grm_bed_panel")
-    if len(inv_all) < nsnps:
-        raise RuntimeError("inv_all shape mismatch in apply_grm_bed_panel")
-    if panel_in.shape[0] != g.n or panel_out.shape[0] != g.n:
-        raise RuntimeError("panel row mismatch in apply_grm_bed_panel")
-    if panel_in.shape[1] != panel_out.shape[1]:
-        raise RuntimeError("panel col mismatch in apply_grm_bed_panel")
-    vin, _ = _xz_colmajor_dev(panel_in)
+def _panel_shape(panel) -> tuple:
+    """(rows N, columns Q) of a Fortran (N, Q) array or a (Q, N) device tensor."""
+    if isinstance(panel, torch.Tensor):
+        return int(panel.shape[1]), int(panel.shape[0])
+    return int(np.shape(panel)[0]), int(np.shape(panel)[1])
+
+
+def grm_apply_dev(g: engine.Genotype, vin: torch.Tensor, inv_all, nsnps: int, step_size: int, C=None,
+                  R=None) -> torch.Tensor:
+    """P (1/M) X diag(inv^2) X^T P v for a (Q, N) device panel v (P = I - C R when
+    C/R are given), into a new (Q, N) tensor: the device core of apply_grm_bed_panel."""
     projecting = C is not None and R is not None
     if projecting:
         vin = _project_dev(vin, C, R)
-    out = engine.apply_grm(g, vin, _to_dev(np.asarray(inv_all)[:nsnps] if not isinstance(inv_all, torch.Tensor)
-                                           else inv_all[:nsnps]), nsnps, int(step_size), limbs=max(LIMBS, 6))
+    inv = inv_all[:nsnps] if isinstance(inv_all, torch.Tensor) else np.asarray(inv_all)[:nsnps]
+    out = engine.apply_grm(g, vin, _to_dev(inv), nsnps, int(step_size), limbs=max(LIMBS, 6))
     if projecting:
         out = _project_dev(out, C, R)
+    return out
+
+
+def _apply_grm(name, bed_prefix, fam_path, nsnps, step_size, row_sel, ddof, inv_all, panel_in, panel_out, C, R,
+               impute_mode, impute_seed):
+    g, n_total, m_total = _geno(bed_prefix, fam_path, row_sel, ddof, impute_mode, impute_seed)
+    nsnps = int(nsnps)
+    if nsnps < 0 or nsnps > m_total:
+        raise RuntimeError(f"nsnps is out of range in {name}")
+    if int(step_size) <= 0:
+        raise RuntimeError(f"step_size must be > 0 in {name}")
+    if g.n <= 0:
+        raise RuntimeError(f"No rows selected in {name}")
+    if len(inv_all) < nsnps:
+        raise RuntimeError(f"inv_all shape mismatch in {name}")
+    (n_in, q_in), (n_out, q_out) = _panel_shape(panel_in), _panel_shape(panel_out)
+    if n_in != g.n or n_out != g.n:
+        raise RuntimeError(f"panel row mismatch in {name}")
+    if q_in != q_out:
+        raise RuntimeError(f"panel col mismatch in {name}")
+    if C is not None and R is not None:
+        p = np.shape(C)[1]
+        if np.shape(C)[0] != g.n or tuple(np.shape(R)) != (p, g.n):
+            raise RuntimeError(f"C/R shape mismatch in {name}")
+    else:
+        C = R = None
+    vin, _ = _xz_colmajor_dev(panel_in)
+    out = grm_apply_dev(g, vin, inv_all, nsnps, int(step_size), C, R)
     _write_back_colmajor(panel_out, out)
 
 
 def apply_grm_bed_panel(bed_prefix, fam_path, nsnps, step_size, row_sel=None, ddof=1, *, inv_all, panel_in,
                         panel_out, C=None, R=None, impute_mode="hwe", impute_seed=None):
     """panel_out = (1/M) X diag(inv^2) X^T panel_in (src/native/gwldcore.cpp:1134-1257)."""
-    _apply_grm(bed_prefix, fam_path, nsnps, step_size, row_sel, ddof, inv_all, panel_in, panel_out, C, R,
-               impute_mode, impute_seed)
+    _apply_grm("apply_grm_bed_panel", bed_prefix, fam_path, nsnps, step_size, row_sel, ddof, inv_all, panel_in,
+               panel_out, C, R, impute_mode, impute_seed)
 
 
 def apply_grm_bed_panel_mailman(bed_prefix, fam_path, nsnps, step_size, row_sel=None, ddof=1, *, inv_all, panel_in,
                                 panel_out, C=None, R=None, impute_seed=None):
-    """HWE-completed GRM apply (src/native/gwldcore.cpp:1259-1497)."""
-    _apply_grm(bed_prefix, fam_path, nsnps, step_size, row_sel, ddof, inv_all, panel_in, panel_out, C, R, "hwe",
-               impute_seed)
+    """HWE-completed GRM apply (src/native/gwldcore.cpp:1259-1497): the same product
+    as apply_grm_bed_panel with impute_mode="hwe" (the Mailman decomposition of the
+    reference is replaced by the exact int8-MFMA products)."""
+    _apply_grm("apply_grm_bed_panel_mailman", bed_prefix, fam_path, nsnps, step_size, row_sel, ddof, inv_all,
+               panel_in, panel_out, C, R, "hwe", impute_seed)
