@@ -7,12 +7,13 @@ src/summit/ldscore/gwe_ldscore.py:3354-3905) on the packed-contraction engine:
   off an orthonormal centred covariate basis C plus the intercept,
   X_j = P G_j inv_x_j and W_j = P(e . G_j) inv_w_j, inv = 1/sqrt(||P .||^2/df),
   df = N - p - 1 (gwe_ldscore.py:2164-2191, :2323-2375).  The norms come from
-  one phase-2 traversal against [C, 1, C.e, e] plus two weighted sums of
-  squares -- no N x M matrix is formed.
+  one phase-2 traversal against every environment's [C, 1, C.e, e] plus one
+  weighted sum of squares per environment -- no N x M matrix is formed.
 * sources S = [P(G D_x), P(e . (G D_w))] with D = inv sqrt(a) Z: one phase-1
   traversal with 2K bins (gwe_ldscore.py:2561-2579).
-* targets X^T S and W^T S = inv_w G^T (e . S): two phase-2 traversals whose
-  fused epilogues give XX, XW, WX, WW (gwe_ldscore.py:2581-2586, :3738-3759).
+* targets X^T S and W^T S = inv_w G^T (e . S): one phase-2 traversal over
+  [S; e . S] whose fused epilogue gives XX, XW, WX, WW (gwe_ldscore.py:2581-2586,
+  :3738-3759).
 Probes are NumPy Philox streams per (SNP block, probe id), generated on the
 GPU from the keys NumPy's seeding yields.
 """
@@ -139,77 +140,13 @@ def _project(M_qn: torch.Tensor, C: torch.Tensor | None) -> torch.Tensor:
     return engine.project(out, None, None, center=True)
 
 
-def features(g: engine.Genotype, env: torch.Tensor, C: torch.Tensor | None, eps_var: float = 1e-10,
-             limbs: int = 6):
-    """inv_x, inv_w (M,) and df for one environment."""
-    N = g.n
-    p = 0 if C is None else int(C.shape[1])
-    df = float(N - p - 1)
-    if df <= 0:
-        raise ValueError(f"Residual correlation degrees of freedom are non-positive: nsamp={N}, p_eff={p}, df={df}.")
-    ones = torch.ones(N, dtype=torch.float64, device=env.device)
-    rows = [ones[None, :]]
-    if p:
-        rows.insert(0, C.T)
-    rows_e = [env[None, :]]
-    if p:
-        rows_e.insert(0, (C * env[:, None]).T)
-    V = torch.cat(rows + rows_e, dim=0).contiguous()
-    T = engine.xtv(g, V, limbs=limbs)  # (M, 2p + 2)
-    ssg, sseg = weighted_sumsq(g, torch.stack([ones, env * env]))
-    cg = T[:, :p]
-    one_g = T[:, p]
-    ceg = T[:, p + 1:2 * p + 1]
-    e_g = T[:, 2 * p + 1]
-    ssx = ssg - (cg * cg).sum(dim=1) - one_g * one_g / N
-    ssw = sseg - (ceg * ceg).sum(dim=1) - e_g * e_g / N
-    varx = ssx / df
-    varw = ssw / df
-    bad = ~(torch.isfinite(varx) & (varx > eps_var) & torch.isfinite(varw) & (varw > eps_var))
-    if bool(bad.any()):
-        n_bad = int(bad.sum())
-        raise ValueError(f"{n_bad} variant feature(s) have zero or invalid projected variance. GxE kernel "
-                         "normalization cannot retain zero columns; QC/remove these variants.")
-    return 1.0 / torch.sqrt(varx), 1.0 / torch.sqrt(varw), df
-
-
 def gxe_ldscore(g: engine.Genotype, env, C_int=None, annot=None, nvecs: int = 8, seed: int = 1,
                 step_size: int = 1000, vtiles=None, probe_offset: int = 0, limbs: int = 5, eps_var: float = 1e-10):
-    """XX, XW, WX, WW directional scores (M, K) for one environment (raw, divided by nvecs)."""
-    dev = engine.default_device()
-    M, N = g.m, g.n
-    env_d = torch.as_tensor(np.asarray(env, dtype=np.float64), device=dev)
-    C_d = None if C_int is None else torch.as_tensor(np.asarray(C_int, dtype=np.float64), device=dev)
-    if annot is None:
-        annot = np.ones((M, 1))
-    annot = np.asarray(annot, dtype=np.float64)
-    K = annot.shape[1]
-    inv_x, inv_w, df = features(g, env_d, C_d, eps_var=eps_var, limbs=max(limbs, 6))
-    a_pos = torch.as_tensor(np.maximum(annot, 0.0), device=dev)
-    annot2 = torch.cat([a_pos * (inv_x * inv_x)[:, None], a_pos * (inv_w * inv_w)[:, None]], dim=1).contiguous()
-    ones_inv = torch.ones(M, dtype=torch.float64, device=dev)
-    vtiles = [nvecs] if vtiles is None else [int(v) for v in vtiles]
-    acc_a = torch.zeros((M, 2 * K), dtype=torch.float64, device=dev)
-    acc_b = torch.zeros((M, 2 * K), dtype=torch.float64, device=dev)
-    v0 = 0
-    for Vt in vtiles:
-        kt = K * Vt
-        z = probes_device(seed, 0, M, step_size, [probe_offset + v0 + v for v in range(Vt)], dev)
-        y = torch.zeros((2 * kt, N), dtype=torch.float64, device=dev)
-        engine.call("gpe_phase1_xz_z8", g.handle, 0, M, engine._ptr(annot2), 2 * K, engine._ptr(ones_inv),
-                    engine._ptr(z), Vt, Vt, engine._ptr(y), N, None, limbs, engine._stream())
-        s = torch.empty_like(y)
-        s[:kt] = _project(y[:kt], C_d)
-        s[kt:] = _project(y[kt:] * env_d[None, :], C_d)
-        engine.phase2_accum(g, 0, M, inv_x, Vt, s, acc_a, int(df) + 1, extreme_warn=0.0, limbs=limbs)
-        es = (s * env_d[None, :]).contiguous()
-        engine.phase2_accum(g, 0, M, inv_w, Vt, es, acc_b, int(df) + 1, extreme_warn=0.0, limbs=limbs)
-        v0 += Vt
-    torch.cuda.synchronize()
-    a = acc_a.cpu().numpy() / float(nvecs)
-    b = acc_b.cpu().numpy() / float(nvecs)
-    return {"xx": a[:, :K], "xw": a[:, K:], "wx": b[:, :K], "ww": b[:, K:],
-            "inv_x": inv_x.cpu().numpy(), "inv_w": inv_w.cpu().numpy(), "df": df}
+    """XX, XW, WX, WW directional scores (M, K) for one environment (raw, divided by
+    nvecs): gxe_ldscore_multi with one environment."""
+    env = np.asarray(env, dtype=np.float64).reshape(-1, 1)
+    return gxe_ldscore_multi(g, env, C_int, annot, nvecs=nvecs, seed=seed, step_size=step_size, vtiles=vtiles,
+                             probe_offset=probe_offset, limbs=limbs, eps_var=eps_var)[0]
 
 
 def _features_multi(g: engine.Genotype, envs_d: torch.Tensor, Cs, eps_var: float, limbs: int):
@@ -262,9 +199,10 @@ def gxe_ldscore_multi(g: engine.Genotype, envs, C_ints=None, annot=None, nvecs: 
     """E environments (envs (N, E)) served together, as the reference's
     MultiEnvironmentKernel does (src/native/gxeldcore.cpp:10281-13197): per probe
     tile one phase-1 traversal produces the x- and w-sources of every
-    environment (2*K*E bins), and two phase-2 traversals score every
-    environment's targets (columns of all environments side by side; each
-    environment's residual scale and df^2 applied to its own bins afterwards).
+    environment (2*K*E bins), and one phase-2 traversal scores every
+    environment's targets X^T S and G^T (e . S) (columns of all environments and
+    both targets side by side; each environment's residual scale and df^2
+    applied to its own bins afterwards).
     C_ints is one basis shared by every environment or a list of per-environment
     bases.  Returns one score dict per environment, as gxe_ldscore."""
     dev = engine.default_device()
@@ -286,28 +224,38 @@ def gxe_ldscore_multi(g: engine.Genotype, envs, C_ints=None, annot=None, nvecs: 
                                      dim=1) for k in range(E)], dim=1).contiguous()
     ones_m = torch.ones(M, dtype=torch.float64, device=dev)
     vtiles = [nvecs] if vtiles is None else [int(v) for v in vtiles]
-    acc_a = torch.zeros((M, 2 * K * E), dtype=torch.float64, device=dev)
-    acc_b = torch.zeros((M, 2 * K * E), dtype=torch.float64, device=dev)
+    nb = 2 * K * E  # bins of S (and of e . S)
+    acc = torch.zeros((M, 2 * nb), dtype=torch.float64, device=dev)
     v0 = 0
     for Vt in vtiles:
         kt = K * Vt
         z = probes_device(seed, 0, M, step_size, [probe_offset + v0 + v for v in range(Vt)], dev)
         y = torch.zeros((2 * kt * E, N), dtype=torch.float64, device=dev)
-        engine.call("gpe_phase1_xz_z8", g.handle, 0, M, engine._ptr(annot_all), 2 * K * E, engine._ptr(ones_m),
+        engine.call("gpe_phase1_xz_z8", g.handle, 0, M, engine._ptr(annot_all), nb, engine._ptr(ones_m),
                     engine._ptr(z), Vt, Vt, engine._ptr(y), N, None, limbs, engine._stream())
-        s = torch.empty_like(y)
-        es = torch.empty_like(y)
+        # targets S (bins [0, nb)) and e . S (bins [nb, 2 nb)) side by side: one
+        # phase-2 traversal reads the genotypes once for both products
+        se = torch.empty((2 * y.shape[0], N), dtype=torch.float64, device=dev)
+        s, es = se[:y.shape[0]], se[y.shape[0]:]
         for k in range(E):
             b = 2 * kt * k
             e = envs_d[:, k][None, :]
             s[b:b + kt] = _project(y[b:b + kt], Cs[k])
             s[b + kt:b + 2 * kt] = _project(y[b + kt:b + 2 * kt] * e, Cs[k])
             es[b:b + 2 * kt] = s[b:b + 2 * kt] * e
+        del y
         # n_denom 2: the (n_denom - 1)^2 divisor is 1; scales and df^2 are applied per environment below
-        engine.phase2_accum(g, 0, M, ones_m, Vt, s, acc_a, 2, extreme_warn=0.0, limbs=limbs)
-        engine.phase2_accum(g, 0, M, ones_m, Vt, es, acc_b, 2, extreme_warn=0.0, limbs=limbs)
+        if 2 * nb * ((Vt + 31) // 32 * 32) <= 65535:
+            engine.phase2_accum(g, 0, M, ones_m, Vt, se, acc, 2, extreme_warn=0.0, limbs=limbs)
+        else:  # beyond one traversal's column limit: S and e . S separately
+            part = torch.zeros((M, nb), dtype=torch.float64, device=dev)
+            for h, tgt in enumerate((s, es)):
+                part.zero_()
+                engine.phase2_accum(g, 0, M, ones_m, Vt, tgt, part, 2, extreme_warn=0.0, limbs=limbs)
+                acc[:, h * nb:(h + 1) * nb] += part
         v0 += Vt
     torch.cuda.synchronize()
+    acc_a, acc_b = acc[:, :nb], acc[:, nb:]
     out = []
     for k in range(E):
         cols = slice(2 * K * k, 2 * K * (k + 1))
