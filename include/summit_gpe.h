@@ -295,6 +295,14 @@ int gpe_phase1_xz_z8(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const doub
                      const double* inv_dev, const int8_t* z_dev, int64_t ldz, int32_t v_count, double* xz_dev,
                      int64_t ldxz, const uint8_t* bin_mode_dev, int32_t limbs, void* stream);
 
+/* Phase 1 with caller-provided real-valued probes z_dev (FP64 row-major
+ * (L, v_count), row 0 = snp_begin), otherwise as gpe_phase1_xz: the GxE
+ * sources with Gaussian or spherical NumPy Philox probes
+ * (src/summit/ldscore/gwe_ldscore.py:2484-2505, src/native/gxeldcore.cpp:7418-7576). */
+int gpe_phase1_xz_f64(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const double* annot_dev, int32_t n_bins,
+                      const double* inv_dev, const double* z_dev, int64_t ldz, int32_t v_count, double* xz_dev,
+                      int64_t ldxz, const uint8_t* bin_mode_dev, int32_t limbs, void* stream);
+
 /* Small-rank FP64 projection of n_cols columns of length n, in place:
  *   x_q <- x_q - A (B x_q),  then, when center != 0, x_q <- x_q - mean(x_q),
  * x column q at x_dev[q * ldx + i]; A n x p column-major (a_dev[i + k * lda]);

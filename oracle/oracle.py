@@ -631,13 +631,24 @@ def windowed_ld_chr(bed, n_total: int, chr_start: int, chr_end: int, bp, annot_c
 # G is the mean-imputed standardized genotype block; C_int an orthonormal,
 # centred covariate basis (N x p); env the environment vector.
 # ----------------------------------------------------------------------------
-def gxe_probe_block(root: int, blk_start: int, probe_ids, L: int) -> np.ndarray:
-    """(L, V) +-1 probes from NumPy Philox(_make_seed(root, blk_start, probe_id))."""
+def gxe_probe_block(root: int, blk_start: int, probe_ids, L: int, dist: str = "rademacher",
+                    fp32: bool = False) -> np.ndarray:
+    """(L, V) probes from NumPy Philox(_make_seed(root, blk_start, probe_id))
+    (src/summit/ldscore/gwe_ldscore.py:2484-2505): +-1 from integers {0, 1},
+    or standard normals (rescaled to norm sqrt(L) for "spherical"); fp32
+    rounds to float32 as a float32 run stores them."""
     Z = np.empty((L, len(probe_ids)), dtype=np.float64, order="F")
     for c, pid in enumerate(probe_ids):
         rng = np.random.Generator(np.random.Philox(make_seed(root, blk_start, int(pid))))
-        Z[:, c] = 2.0 * rng.integers(0, 2, size=L, dtype=np.int8).astype(np.float64) - 1.0
-    return Z
+        if dist == "rademacher":
+            Z[:, c] = 2.0 * rng.integers(0, 2, size=L, dtype=np.int8).astype(np.float64) - 1.0
+        else:
+            v = rng.standard_normal(size=L)
+            if dist == "spherical":
+                nrm = float(np.linalg.norm(v))
+                v = v * (np.sqrt(float(L)) / nrm) if nrm > 0.0 else np.zeros_like(v)
+            Z[:, c] = v
+    return Z.astype(np.float32).astype(np.float64) if fp32 else Z
 
 
 def gxe_project_center(M, C_int):
@@ -650,7 +661,7 @@ def gxe_project_center(M, C_int):
 
 def gxe_scores(bed, n_total: int, m_total: int, env, C_int, annot=None, nvecs: int = 8, seed: int = 1,
                step_size: int = 1000, vtiles=None, ddof: int = 1, impute: str = "mean", rows=None,
-               probe_offset: int = 0, eps_var: float = 1e-10):
+               probe_offset: int = 0, eps_var: float = 1e-10, rand_dist: str = "rademacher", probe_fp32: bool = False):
     env = np.asarray(env, dtype=np.float64)
     C_int = np.zeros((env.size, 0)) if C_int is None else np.asarray(C_int, dtype=np.float64)
     if annot is None:
@@ -674,7 +685,7 @@ def gxe_scores(bed, n_total: int, m_total: int, env, C_int, annot=None, nvecs: i
         kt = K * Vt
         src = np.zeros((N, 2 * kt))
         for s, e in blocks:
-            Z = gxe_probe_block(seed, s, [probe_offset + v0 + v for v in range(Vt)], e - s)
+            Z = gxe_probe_block(seed, s, [probe_offset + v0 + v for v in range(Vt)], e - s, rand_dist, probe_fp32)
             sq = np.sqrt(np.maximum(annot[s:e], 0.0))
             for k in range(K):
                 wz = sq[:, k:k + 1] * Z

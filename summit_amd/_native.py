@@ -50,6 +50,8 @@ SIGNATURES = {
     "gpe_philox_rademacher": (ctypes.c_int, [c_vp, c_vp, c_i32, c_i64, c_i32, c_vp, c_i64, c_vp]),
     "gpe_phase1_xz_z8": (ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_i32, c_vp, c_vp, c_i64, c_i32, c_vp, c_i64, c_vp,
                                         c_i32, c_vp]),
+    "gpe_phase1_xz_f64": (ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_i32, c_vp, c_vp, c_i64, c_i32, c_vp, c_i64, c_vp,
+                                         c_i32, c_vp]),
     "gpe_project_f64": (ctypes.c_int, [c_vp, c_i64, c_i64, c_i32, c_vp, c_i64, c_vp, c_i64, c_i32, c_i32, c_vp]),
     "gpe_project_f64_noalloc": (ctypes.c_int, [c_vp, c_i64, c_i64, c_i32, c_vp, c_i64, c_vp, c_i64, c_i32, c_i32,
                                                c_vp]),

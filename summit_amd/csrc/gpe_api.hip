@@ -1591,6 +1591,30 @@ int gpe_phase1_xz_z8(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const doub
     });
 }
 
+int gpe_phase1_xz_f64(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const double* annot_dev, int32_t n_bins,
+                      const double* inv_dev, const double* z_dev, int64_t ldz, int32_t v_count, double* xz_dev,
+                      int64_t ldxz, const uint8_t* bin_mode_dev, int32_t limbs, void* stream) {
+    return guarded([&] {
+        require_prepared(g);
+        require(snp_begin >= 0 && snp_end <= g->m && snp_begin <= snp_end, "blk_end exceeds #SNPs in BIM");
+        require(n_bins > 0 && v_count > 0 && ldz >= v_count, "bad probe matrix");
+        require(ldxz >= g->n, "Xz2d_chunk shape must be (N, B*v_count)");
+        if (snp_end == snp_begin) return;
+        (void)limb_cfg(limbs);
+        hipStream_t st = S(stream);
+        const int64_t L = snp_end - snp_begin;
+        const int v_pad = (int)round_up(v_count, 32);
+        double* w = static_cast<double*>(g->scratch("p1_w", g->m_alloc * n_bins * sizeof(double)));
+        hipLaunchKernelGGL(k_make_w, dim3((unsigned)ceil_div(L, 256)), dim3(256), 0, st, annot_dev, n_bins, inv_dev,
+                           g->inv_sd, g->nobs, g->sum, g->sumsq, snp_begin, snp_end, w);
+        GPE_HIP(hipGetLastError());
+        double* z = static_cast<double*>(g->scratch("p1_zc", g->m_alloc * v_pad * sizeof(double)));
+        copy_rows(z_dev, ldz, L, v_count, z, v_pad, snp_begin, st);
+        run_phase1(g, snp_begin, snp_end, w, n_bins, nullptr, z, v_pad, v_count, xz_dev, ldxz, bin_mode_dev, limbs,
+                   st);
+    });
+}
+
 int gpe_project_f64(double* x_dev, int64_t ldx, int64_t n, int32_t n_cols, const double* a_dev, int64_t lda,
                     const double* b_dev, int64_t ldb, int32_t p, int32_t center, void* stream) {
     return guarded([&] {

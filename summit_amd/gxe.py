@@ -14,8 +14,9 @@ src/summit/ldscore/gwe_ldscore.py:3354-3905) on the packed-contraction engine:
 * targets X^T S and W^T S = inv_w G^T (e . S): one phase-2 traversal over
   [S; e . S] whose fused epilogue gives XX, XW, WX, WW (gwe_ldscore.py:2581-2586,
   :3738-3759).
-Probes are NumPy Philox streams per (SNP block, probe id), generated on the
-GPU from the keys NumPy's seeding yields.
+Probes are NumPy Philox streams per (SNP block, probe id): Rademacher signs
+generated on the GPU from the keys NumPy's seeding yields, Gaussian and
+spherical values drawn by NumPy on the host.
 """
 from __future__ import annotations
 
@@ -116,6 +117,28 @@ def probes_device(root: int, snp_begin: int, snp_end: int, step_size: int, probe
     return z
 
 
+def probes_host(root: int, snp_begin: int, snp_end: int, step_size: int, probe_ids, rand_dist: str,
+                fp32: bool = False) -> np.ndarray:
+    """Real-valued (L, V) GxE probes drawn as the reference draws them
+    (src/summit/ldscore/gwe_ldscore.py:2484-2505): per (SNP block, probe id) a NumPy
+    Philox generator seeded with make_seed(root, block start, probe id) gives the
+    block's standard normals, rescaled to norm sqrt(L) for "spherical"; fp32 rounds
+    them to float32 as a float32 run stores them."""
+    if rand_dist not in ("gaussian", "normal", "spherical"):
+        raise ValueError(f"Unsupported rand_dist: {rand_dist}")
+    L = snp_end - snp_begin
+    z = np.empty((L, len(probe_ids)), dtype=np.float64)
+    for s in range(snp_begin, snp_end, step_size):
+        e = min(snp_end, s + step_size)
+        for c, pid in enumerate(probe_ids):
+            v = np.random.Generator(np.random.Philox(make_seed(root, s, int(pid)))).standard_normal(size=e - s)
+            if rand_dist == "spherical":
+                nrm = float(np.linalg.norm(v))
+                v = v * (np.sqrt(float(e - s)) / nrm) if nrm > 0.0 else np.zeros_like(v)
+            z[s - snp_begin:e - snp_begin, c] = v
+    return z.astype(np.float32).astype(np.float64) if fp32 else z
+
+
 def weighted_sumsq(g: engine.Genotype, w: torch.Tensor, snp_begin: int = 0, snp_end: int | None = None):
     """sum_n w_n x_jn^2 over SNPs [snp_begin, snp_end): w (N,) -> (L,), or (E, N) -> (E, L)
     with every weight vector served by the same pass over the genotypes."""
@@ -141,12 +164,14 @@ def _project(M_qn: torch.Tensor, C: torch.Tensor | None) -> torch.Tensor:
 
 
 def gxe_ldscore(g: engine.Genotype, env, C_int=None, annot=None, nvecs: int = 8, seed: int = 1,
-                step_size: int = 1000, vtiles=None, probe_offset: int = 0, limbs: int = 5, eps_var: float = 1e-10):
+                step_size: int = 1000, vtiles=None, probe_offset: int = 0, limbs: int = 5, eps_var: float = 1e-10,
+                rand_dist: str = "rademacher", probe_fp32: bool = False):
     """XX, XW, WX, WW directional scores (M, K) for one environment (raw, divided by
     nvecs): gxe_ldscore_multi with one environment."""
     env = np.asarray(env, dtype=np.float64).reshape(-1, 1)
     return gxe_ldscore_multi(g, env, C_int, annot, nvecs=nvecs, seed=seed, step_size=step_size, vtiles=vtiles,
-                             probe_offset=probe_offset, limbs=limbs, eps_var=eps_var)[0]
+                             probe_offset=probe_offset, limbs=limbs, eps_var=eps_var, rand_dist=rand_dist,
+                             probe_fp32=probe_fp32)[0]
 
 
 def _features_multi(g: engine.Genotype, envs_d: torch.Tensor, Cs, eps_var: float, limbs: int):
@@ -195,7 +220,7 @@ def _features_multi(g: engine.Genotype, envs_d: torch.Tensor, Cs, eps_var: float
 
 def gxe_ldscore_multi(g: engine.Genotype, envs, C_ints=None, annot=None, nvecs: int = 8, seed: int = 1,
                       step_size: int = 1000, vtiles=None, probe_offset: int = 0, limbs: int = 5,
-                      eps_var: float = 1e-10):
+                      eps_var: float = 1e-10, rand_dist: str = "rademacher", probe_fp32: bool = False):
     """E environments (envs (N, E)) served together, as the reference's
     MultiEnvironmentKernel does (src/native/gxeldcore.cpp:10281-13197): per probe
     tile one phase-1 traversal produces the x- and w-sources of every
@@ -204,7 +229,12 @@ def gxe_ldscore_multi(g: engine.Genotype, envs, C_ints=None, annot=None, nvecs: 
     both targets side by side; each environment's residual scale and df^2
     applied to its own bins afterwards).
     C_ints is one basis shared by every environment or a list of per-environment
-    bases.  Returns one score dict per environment, as gxe_ldscore."""
+    bases.  rand_dist "rademacher" draws the probes on the GPU (int8 signs);
+    "gaussian" / "spherical" draws them on the host (probes_host, float32-rounded
+    with probe_fp32) for the FP64-probe phase 1.  Returns one score dict per
+    environment, as gxe_ldscore."""
+    if rand_dist not in ("rademacher", "gaussian", "normal", "spherical"):
+        raise ValueError(f"Unsupported rand_dist: {rand_dist!r}.")
     dev = engine.default_device()
     M, N = g.m, g.n
     envs = np.asarray(envs, dtype=np.float64)
@@ -229,10 +259,16 @@ def gxe_ldscore_multi(g: engine.Genotype, envs, C_ints=None, annot=None, nvecs: 
     v0 = 0
     for Vt in vtiles:
         kt = K * Vt
-        z = probes_device(seed, 0, M, step_size, [probe_offset + v0 + v for v in range(Vt)], dev)
+        ids = [probe_offset + v0 + v for v in range(Vt)]
         y = torch.zeros((2 * kt * E, N), dtype=torch.float64, device=dev)
-        engine.call("gpe_phase1_xz_z8", g.handle, 0, M, engine._ptr(annot_all), nb, engine._ptr(ones_m),
-                    engine._ptr(z), Vt, Vt, engine._ptr(y), N, None, limbs, engine._stream())
+        if rand_dist == "rademacher":
+            z = probes_device(seed, 0, M, step_size, ids, dev)
+            engine.call("gpe_phase1_xz_z8", g.handle, 0, M, engine._ptr(annot_all), nb, engine._ptr(ones_m),
+                        engine._ptr(z), Vt, Vt, engine._ptr(y), N, None, limbs, engine._stream())
+        else:
+            z = torch.from_numpy(probes_host(seed, 0, M, step_size, ids, rand_dist, probe_fp32)).to(dev)
+            engine.call("gpe_phase1_xz_f64", g.handle, 0, M, engine._ptr(annot_all), nb, engine._ptr(ones_m),
+                        engine._ptr(z), Vt, Vt, engine._ptr(y), N, None, limbs, engine._stream())
         # targets S (bins [0, nb)) and e . S (bins [nb, 2 nb)) side by side: one
         # phase-2 traversal reads the genotypes once for both products
         se = torch.empty((2 * y.shape[0], N), dtype=torch.float64, device=dev)

@@ -80,7 +80,8 @@ def windowed_ld_multi(g: engine.Genotype, chroms, ld_wind_kb: float | None = Non
     """Every chromosome in one banded-Gram launch.  chroms: [(snp_begin,
     snp_end, bp or None), ...] sorted and non-overlapping; returns (rows, B)
     FP64 for SNPs [chroms[0][0], chroms[-1][1]) (rows between chromosomes 0),
-    equal to one ``windowed_ld`` call per chromosome."""
+    equal to one ``windowed_ld`` call per chromosome up to FP64 round-off (tile
+    pairs add into each SNP's score by FP64 atomics, in no fixed order)."""
     bounds = np.ascontiguousarray(np.asarray([[int(c[0]), int(c[1])] for c in chroms], dtype=np.int64).reshape(-1))
     if bounds.size == 0:
         raise RuntimeError("at least one chromosome is required")

@@ -44,6 +44,30 @@ def test_gxe_scores_match_oracle(oracle, with_cov):
     g.close()
 
 
+@pytest.mark.parametrize("dist,fp32", [("gaussian", True), ("gaussian", False), ("spherical", False)])
+def test_gxe_real_valued_probes_match_oracle(oracle, dist, fp32):
+    """Gaussian (float32-rounded as a float32 run stores them, or FP64) and spherical
+    Philox probes drawn on the host, through the FP64-probe phase 1."""
+    from summit_amd import engine, gxe
+    m, n, nvecs, step = 500, 300, 10, 170
+    print(f"gxe real probes {dist} fp32={fp32}", flush=True)
+    codes = random_codes(m, n, seed=92, miss_rate=0.01)
+    rng = np.random.default_rng(6)
+    env = (rng.random(n) < 0.3).astype(float)
+    raw = rng.standard_normal((n, 2))
+    raw -= raw.mean(axis=0)
+    C, _ = np.linalg.qr(raw)
+    annot = np.column_stack([np.ones(m), (rng.random(m) < 0.3).astype(float)])
+    g = engine.Genotype.from_bed_bytes(oracle.pack_codes(codes), n, m).prepare(impute="mean")
+    got = gxe.gxe_ldscore(g, env, C, annot, nvecs=nvecs, seed=77, step_size=step, vtiles=[6, 4], rand_dist=dist,
+                          probe_fp32=fp32)
+    ref = oracle.gxe_scores(oracle.pack_codes(codes), n, m, env, C, annot, nvecs=nvecs, seed=77, step_size=step,
+                            vtiles=[6, 4], rand_dist=dist, probe_fp32=fp32)
+    for k in ("xx", "xw", "wx", "ww"):
+        np.testing.assert_allclose(got[k], ref[k], rtol=1e-10, err_msg=k)
+    g.close()
+
+
 @pytest.mark.parametrize("per_env_cov", [False, True])
 def test_gxe_multi_environment_matches_oracle(oracle, per_env_cov):
     """Several environments served by one set of traversals (the reference's
