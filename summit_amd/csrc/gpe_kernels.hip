@@ -949,6 +949,7 @@ void launch_scan_counts(const int32_t* counts, int64_t n, int64_t* ptr, hipStrea
 // ---------------------------------------------------------------------------
 constexpr int kMtN = 312;
 constexpr int kMtM = 156;
+constexpr int kMtStage = 65536;  // bytes of a probe tile staged in LDS
 constexpr uint64_t kMtA = 0xB5026F5AA96619E9ULL;
 constexpr uint64_t kMtUM = 0xFFFFFFFF80000000ULL;
 constexpr uint64_t kMtLM = 0x7FFFFFFFULL;
@@ -964,6 +965,10 @@ __global__ __launch_bounds__(64) void k_mt_rademacher(uint64_t root, const int64
                                                      int64_t blk_end_all, int v_start_arg, const int32_t* v_start_dev,
                                                      int v_count_all, int sub, int8_t* z_all, int64_t ldz) {
     __shared__ uint64_t mt[kMtN];
+    // the block's probe tile, row-major (L, v_count), when it fits: the draws fill it
+    // column by column and it is written out as whole rows (contiguous stores)
+    // instead of one byte per draw at a stride of ldz
+    __shared__ __attribute__((aligned(16))) int8_t stage[kMtStage];
     const int b = blockIdx.x;
     const int lane = threadIdx.x;
     if (b >= n_blocks) return;
@@ -985,6 +990,7 @@ __global__ __launch_bounds__(64) void k_mt_rademacher(uint64_t root, const int64
     }
     __syncthreads();
     const int64_t total = L * (int64_t)v_count;
+    const bool staged = total <= kMtStage;
     // output t = v L + r of this lane's slots i = lane + 64 q, advanced by kMtN per
     // twist without a division
     int32_t ov[5], orow[5];
@@ -994,46 +1000,68 @@ __global__ __launch_bounds__(64) void k_mt_rademacher(uint64_t root, const int64
         ov[q] = (int32_t)(i / L);
         orow[q] = (int32_t)(i % L);
     }
+    // the state in registers: lane l holds mt[l + 64 q] in m[q] (q < 5, index < 312);
+    // a twist exchanges the neighbours it needs across lanes instead of through LDS
+    uint64_t m[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) m[q] = lane + 64 * q < kMtN ? mt[lane + 64 * q] : 0ULL;
+    auto shfl64 = [](uint64_t v, int src) -> uint64_t {
+        const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, src, 64);
+        const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), src, 64);
+        return ((uint64_t)hi << 32) | lo;
+    };
+    auto lane64 = [](uint64_t v, int src) -> uint64_t {
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, src);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), src);
+        return ((uint64_t)hi << 32) | lo;
+    };
+    const int up1 = (lane + 1) & 63;
+    const int up28 = (lane + 28) & 63;  // i + 156 = (lane + 28) + 64 (q + 2 or q + 3)
+    const int up36 = (lane + 36) & 63;  // i - 156 = (lane + 36) + 64 (q - 3 or q - 2)
     for (int64_t base = 0; base < total; base += kMtN) {
-        uint64_t nv[3];
-        // phase A: i in [0, 156) reads old values only
+        // old mt[i + 1] of every slot
+        uint64_t nb[5];
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            const int i = lane + 64 * q;
-            if (i < kMtN - kMtM) nv[q] = mt_twist_one(mt[i], mt[i + 1], mt[i + kMtM]);
+        for (int q = 0; q < 5; ++q) {
+            const uint64_t x = shfl64(m[q], up1);
+            const uint64_t w = q < 4 ? lane64(m[q + 1], 0) : 0ULL;
+            nb[q] = lane == 63 ? w : x;
         }
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            const int i = lane + 64 * q;
-            if (i < kMtN - kMtM) mt[i] = nv[q];
-        }
-        __syncthreads();
-        // phase B: i in [156, 311) reads new mt[i-156] and old mt[i], mt[i+1]
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            const int i = kMtN - kMtM + lane + 64 * q;
-            if (i < kMtN - 1) nv[q] = mt_twist_one(mt[i], mt[i + 1], mt[i + (kMtM - kMtN)]);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            const int i = kMtN - kMtM + lane + 64 * q;
-            if (i < kMtN - 1) mt[i] = nv[q];
-        }
-        __syncthreads();
-        if (lane == 0) mt[kMtN - 1] = mt_twist_one(mt[kMtN - 1], mt[0], mt[kMtM - 1]);
-        __syncthreads();
+        // i < 156 (slots 0, 1; slot 2 lanes < 28) from old mt[i + 156]; every exchange
+        // runs with all lanes active (a lane-permute reads nothing from an inactive lane)
+        const uint64_t g2 = shfl64(m[2], up28), g3 = shfl64(m[3], up28), g4 = shfl64(m[4], up28);
+        const uint64_t f0 = lane < 36 ? g2 : g3;
+        const uint64_t f1 = lane < 36 ? g3 : g4;
+        const uint64_t f2 = g4;
+        const uint64_t a0 = mt_twist_one(m[0], nb[0], f0);
+        const uint64_t a1 = mt_twist_one(m[1], nb[1], f1);
+        const uint64_t a2 = mt_twist_one(m[2], nb[2], f2);
+        // 156 <= i < 311 (slot 2 lanes >= 28, slot 3, slot 4 lanes < 55) from new mt[i - 156]
+        const uint64_t s0 = shfl64(a0, up36), s1 = shfl64(a1, up36), s2 = shfl64(a2, up36);
+        const uint64_t b2 = mt_twist_one(m[2], nb[2], s0);
+        const uint64_t b3 = mt_twist_one(m[3], nb[3], lane < 28 ? s0 : s1);
+        const uint64_t b4 = mt_twist_one(m[4], nb[4], lane < 28 ? s1 : s2);
+        // i = 311 from old mt[311], new mt[0] and new mt[155]
+        const uint64_t n311 = mt_twist_one(lane64(m[4], 55), lane64(a0, 0), lane64(a2, 27));
+        m[0] = a0;
+        m[1] = a1;
+        m[2] = lane < 28 ? a2 : b2;
+        m[3] = b3;
+        m[4] = lane < 55 ? b4 : (lane == 55 ? n311 : 0ULL);
 #pragma unroll
         for (int q = 0; q < 5; ++q) {
             const int i = lane + 64 * q;
             if (i < kMtN && ov[q] < v_count) {
-                uint64_t y = mt[i];
+                uint64_t y = m[q];
                 y ^= (y >> 29) & 0x5555555555555555ULL;
                 y ^= (y << 17) & 0x71D67FFFEDA60000ULL;
                 y ^= (y << 37) & 0xFFF7EEE000000000ULL;
                 y ^= (y >> 43);
-                z[(s + orow[q]) * ldz + ov[q]] = (y & 1ULL) ? (int8_t)1 : (int8_t)-1;
+                const int8_t val = (y & 1ULL) ? (int8_t)1 : (int8_t)-1;
+                if (staged)
+                    stage[orow[q] * v_count + ov[q]] = val;
+                else
+                    z[(s + orow[q]) * ldz + ov[q]] = val;
             }
             orow[q] += kMtN;
             while (orow[q] >= L) {
@@ -1041,7 +1069,20 @@ __global__ __launch_bounds__(64) void k_mt_rademacher(uint64_t root, const int64
                 ++ov[q];
             }
         }
-        __syncthreads();
+    }
+    __syncthreads();
+    if (staged) {
+        // whole dwords when the rows allow it, else bytes (consecutive lanes, consecutive bytes)
+        if (v_count % 4 == 0 && ldz % 4 == 0 && (reinterpret_cast<uintptr_t>(z) & 3) == 0) {
+            const int dpr = v_count / 4;
+            for (int64_t d = lane; d < total / 4; d += 64) {
+                const int64_t r = d / dpr;
+                const int c = (int)(d - r * dpr);
+                *reinterpret_cast<uint32_t*>(z + (s + r) * ldz + 4 * c) = reinterpret_cast<const uint32_t*>(stage)[d];
+            }
+        } else {
+            for (int64_t d = lane; d < total; d += 64) z[(s + d / v_count) * ldz + d % v_count] = stage[d];
+        }
     }
 }
 void launch_mt_rademacher(uint64_t root, const int64_t* blk_starts, int n_blocks, int64_t blk_end_all,
