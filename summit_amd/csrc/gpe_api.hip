@@ -1150,21 +1150,21 @@ void run_phase2(gpe_geno* g, int64_t s, int64_t e, const double* x, int64_t ldx,
         int8_t* xrow = static_cast<int8_t*>(g->scratch("p2_xrow", (size_t)(g->n + 1) * nch * lc.nl * 64));
         launch_xrow_limbs(x, ldx, g->n, n_bins, v_count, v_pad, shift, lc.nl, nch, xrow, st);
         const int64_t ld_ms = (int64_t)nch * 64;
-        // SNP slices keep the per-range sums near 4 GiB
-        const int64_t slice =
-            std::max<int64_t>(256, round_up(((int64_t)1 << 32) / ((int64_t)parts * ld_ms * 8), 256));
+        // SNP slices keep the sums near 4 GiB and the gather grid within 2^25 waves
+        // (2^31 threads in x)
+        const int64_t by_bytes = round_up(((int64_t)1 << 32) / (ld_ms * 8), 256);
+        const int64_t by_waves = (((int64_t)1 << 25) / ((int64_t)parts * nch)) / 256 * 256;
+        const int64_t slice = std::max<int64_t>(256, std::min(by_bytes, by_waves));
         const int64_t rows = std::min<int64_t>(slice, e - s);
-        int64_t* msum =
-            static_cast<int64_t*>(g->scratch("p2_msum", (size_t)parts * rows * ld_ms * sizeof(int64_t)));
+        int64_t* msum = static_cast<int64_t*>(g->scratch("p2_msum", (size_t)rows * ld_ms * sizeof(int64_t)));
         a.msum = msum;
         a.ld_ms = ld_ms;
         for (int64_t rs = s; rs < e; rs += slice) {
             const int64_t re = std::min(e, rs + slice);
             ProfScope pm(GPE_PROF_P2_MISS, st);
-            launch_miss_sums(g->smiss_rptr, parts, g->smiss_idx, rs, re, xrow, g->n, lc.nl, nch, msum, rows * ld_ms,
-                             ld_ms, st);
-            // the sample ranges' sums added into the first (the epilogue reads one value)
-            launch_sum_parts(msum, parts, rows * ld_ms, (re - rs) * ld_ms, st);
+            // the sample ranges' sums meet in one zeroed buffer (the epilogue reads one value)
+            launch_fill_u64(reinterpret_cast<uint64_t*>(msum), (re - rs) * ld_ms, 0, st);
+            launch_miss_sums(g->smiss_rptr, parts, g->smiss_idx, rs, re, xrow, g->n, lc.nl, nch, msum, ld_ms, st);
             pm.stop();
             a.r0 = rs;
             a.r1 = re;

@@ -147,13 +147,11 @@ void launch_xrow_limbs(const double* x, int64_t ldx, int64_t n, int n_bins, int 
 // by sample) at a sample >= p * range, p = 0..parts
 void launch_range_ptrs(const int64_t* ptr, const int32_t* idx, int64_t m, int parts, int64_t range, int64_t* rptr,
                        hipStream_t st);
-// out[p * part_stride + (j - r0) * ld_ms + c] = sum over the missing calls (n, j) with n in sample
-// range p of q[n][c], exact, for SNPs j in [r0, r1), from the range pointers of launch_range_ptrs
-// and the limb rows of launch_xrow_limbs
+// out[(j - r0) * ld_ms + c] += sum over the missing calls (n, j) of q[n][c], exact (int64 atomic
+// adds of the per-sample-range sums; out zeroed by the caller), for SNPs j in [r0, r1), from the
+// range pointers of launch_range_ptrs and the limb rows of launch_xrow_limbs
 void launch_miss_sums(const int64_t* rptr, int parts, const int32_t* idx, int64_t r0, int64_t r1, const int8_t* xrow,
-                      int64_t n, int nl, int nch, int64_t* out, int64_t part_stride, int64_t ld_ms, hipStream_t st);
-// parts[i] = sum_p parts[p * stride + i] for i < n (the sample ranges' sums into part 0)
-void launch_sum_parts(int64_t* parts, int n_parts, int64_t stride, int64_t n, hipStream_t st);
+                      int64_t n, int nl, int nch, int64_t* out, int64_t ld_ms, hipStream_t st);
 // out[k * ldo + i] += sum_{j in [s, e)} xt_ij^2 w[(j - s) + k ldw] (k < n_bins), xt the standardized
 // working genotype minus C (R x) when p > 0 (rx[(j - s) ldrx + a] = (R x_j)_a, p <= 64); part holds
 // 8 n ceil((e - s) / sq_rowsums_chunk(n, e - s)) doubles

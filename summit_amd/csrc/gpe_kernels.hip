@@ -1974,9 +1974,10 @@ void launch_range_ptrs(const int64_t* ptr, const int32_t* idx, int64_t m, int pa
     GPE_HIP(hipGetLastError());
 }
 
-// Exact missing-call sums of the phase-2 operand over sample ranges p:
-//   part_p[j - r0][64 h + cc] = sum over samples n in range p missing at j of q[n][64 h + cc].
-// One wave per (range p, column chunk h, SNP j).
+// Exact missing-call sums of the phase-2 operand:
+//   out[j - r0][64 h + cc] += sum over samples n missing at j of q[n][64 h + cc]
+// (out zeroed first).  One wave per (sample range p, column chunk h, SNP j); the
+// ranges' int64 sums meet in out by atomic adds, exact in any order.
 // A sample's limb row (k_xrow_limbs) is P = 4 NL pieces of 16 bytes.  The sums
 // run on the int8 matrix cores: v_mfma_i32_16x16x64_i8 with A the constant
 // selector A[i][k] = 1 where k is byte i of a lane's 16 (byte (lane % 16) of
@@ -2001,8 +2002,7 @@ template <int NL, bool BUF>
 __global__ __launch_bounds__(256) void k_miss_sums(const int64_t* __restrict__ rptr, int parts,
                                                    const int32_t* __restrict__ idx, int64_t r0, int64_t r1,
                                                    const int8_t* __restrict__ xrow, int n, int nch,
-                                                   int64_t* __restrict__ out, int64_t part_stride, int64_t ld_ms,
-                                                   uint32_t table_bytes) {
+                                                   int64_t* __restrict__ out, int64_t ld_ms, uint32_t table_bytes) {
     constexpr int P = 4 * NL;      // 16-byte pieces per row
     constexpr int T = P - 16;      // tail pieces
     constexpr int R = 16 / T;      // lane classes per tail piece
@@ -2106,7 +2106,9 @@ __global__ __launch_bounds__(256) void k_miss_sums(const int64_t* __restrict__ r
         }
         me = me * 256 + (int64_t)v;
     }
-    out[(int64_t)p * part_stride + (j - r0) * ld_ms + h * 64 + lane] = me;
+    // every sample range adds into the one zeroed sum: integer atomics, exact in any order
+    atomicAdd(reinterpret_cast<unsigned long long*>(&out[(j - r0) * ld_ms + h * 64 + lane]),
+              (unsigned long long)me);
 }
 // Per-sample sums of squared standardized genotypes weighted per SNP and bin,
 // the X2 * W panel products of the fused residual-variance pass
@@ -2212,22 +2214,8 @@ void launch_sq_rowsums(const uint8_t* smp, int64_t stride, int64_t n, int64_t s,
     }
 }
 
-// out[i] = sum_p parts[p * stride + i] for i < n, into parts[i] (part 0), in order
-__global__ void k_sum_parts(int64_t* parts, int n_parts, int64_t stride, int64_t n) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int64_t acc = parts[i];
-    for (int p = 1; p < n_parts; ++p) acc += parts[(int64_t)p * stride + i];
-    parts[i] = acc;
-}
-void launch_sum_parts(int64_t* parts, int n_parts, int64_t stride, int64_t n, hipStream_t st) {
-    if (n <= 0 || n_parts <= 1) return;
-    hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, parts, n_parts, stride, n);
-    GPE_HIP(hipGetLastError());
-}
-
 void launch_miss_sums(const int64_t* rptr, int parts, const int32_t* idx, int64_t r0, int64_t r1, const int8_t* xrow,
-                      int64_t n, int nl, int nch, int64_t* out, int64_t part_stride, int64_t ld_ms, hipStream_t st) {
+                      int64_t n, int nl, int nch, int64_t* out, int64_t ld_ms, hipStream_t st) {
     const int64_t per_range = (r1 - r0) * nch;
     if (per_range <= 0) return;
     if ((n + 1) * nch * nl * 16 >= ((int64_t)1 << 32)) throw Error("missing-call gather table exceeds 2^32 dwords");
@@ -2239,7 +2227,7 @@ void launch_miss_sums(const int64_t* rptr, int parts, const int32_t* idx, int64_
     const uint32_t tb = buf ? (uint32_t)table_bytes : 0u;
 #define GPE_MS(NL_, BUF_)                                                                                           \
     hipLaunchKernelGGL((k_miss_sums<NL_, BUF_>), grid, dim3(256), 0, st, rptr, parts, idx, r0, r1, xrow, (int)n,  \
-                       nch, out, part_stride, ld_ms, tb)
+                       nch, out, ld_ms, tb)
     if (nl == 5 && buf)
         GPE_MS(5, true);
     else if (nl == 5)
