@@ -867,7 +867,10 @@ int gpe_gz_i32(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const int8_t* z_
         const int splits = pick_splits(gx * groups, kcount, 1 << 30, contract_slots(1, ctc, false, EPI_I32_ATOMIC));
         a.kc_per_split = ceil_div(kcount, splits);
         ProfScope ps(GPE_PROF_GZ, st);
-        launch_contract(1, ctc, false, EPI_I32_ATOMIC, a, groups, splits, st);
+        // two 128-value chunks per LDS slot: a sample row's two 32-byte pieces are
+        // loaded together, so a 64-byte run of its L2 line is used at once
+        // (N=100,000, M=500,000, 64 columns: 3.88 ms against 4.23 ms with one chunk)
+        launch_contract(1, ctc, false, EPI_I32_ATOMIC, a, groups, splits, st, 2);
         ps.stop();
     });
 }
@@ -910,7 +913,9 @@ int gpe_gtu_i64(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const int32_t* 
         a.cols_out = n_cols;
         a.out_row0 = snp_begin;
         ProfScope ps(GPE_PROF_GTU, st);
-        launch_contract(nl, ctc, false, EPI_I64_ROWMAJOR, a, groups, 1, st);
+        // two chunks per LDS slot for the 3-limb product (7.55 ms against 7.76 ms at the
+        // same shape)
+        launch_contract(nl, ctc, false, EPI_I64_ROWMAJOR, a, groups, 1, st, nl == 3 ? 2 : 1);
         ps.stop();
     });
 }

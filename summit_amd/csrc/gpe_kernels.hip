@@ -521,6 +521,14 @@ void launch_contract(int nl, int ctc, bool miss, EpiMode mode, const ContractArg
             launch_one<5, 1, false, EPI_I64_ATOMIC_Q, 2>(a, n_col_groups, n_splits, st);
             return;
         }
+        if (nl == 1 && ctc == 2 && !miss && mode == EPI_I32_ATOMIC) {
+            launch_one<1, 2, false, EPI_I32_ATOMIC, 2>(a, n_col_groups, n_splits, st);
+            return;
+        }
+        if (nl == 3 && ctc == 1 && !miss && mode == EPI_I64_ROWMAJOR) {
+            launch_one<3, 1, false, EPI_I64_ROWMAJOR, 2>(a, n_col_groups, n_splits, st);
+            return;
+        }
     }
 #define GPE_CASE(NL_, CTC_, MISS_, EPI_)                                                          \
     if (nl == NL_ && ctc == CTC_ && miss == MISS_ && mode == EPI_) {                              \
