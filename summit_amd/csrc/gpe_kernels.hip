@@ -138,6 +138,11 @@ __device__ __forceinline__ int64_t wave_sum_i64(int64_t v) {
 // consecutive K values of half (l >> 5); result register e of lane l holds
 // (row 8*(e>>2) + 4*(l>>5) + (e&3), column l & 31).
 // ---------------------------------------------------------------------------
+// column groups per band of the contraction's workgroup order (GxE at N=100,000,
+// M=500,000, 2,048 phase-2 columns: 0.83 s with 4, 0.82 s with 2, 0.88 s with 8,
+// 0.91 s unbanded; partitioned, 800 columns: 0.117-0.119 s with 4, 0.119-0.123 s with 2)
+constexpr int kGroupBand = 4;
+
 constexpr uint32_t waitcnt_vm(uint32_t n) {  // s_waitcnt vmcnt(n), other counters unconstrained
     return (n & 15u) | ((n >> 4) << 14) | (7u << 4) | (15u << 8);
 }
@@ -202,18 +207,26 @@ __global__ __launch_bounds__(256, (NL * CTC <= 2) ? 4 : 2) void k_contract(Contr
     const int lr = lane & 31;
     // Workgroups are dispatched in flattened (x, y, z) order and dealt round-robin
     // to the XCDs, each with its own L2.  The flattened index is remapped so that
-    // every XCD runs a contiguous range of (split, row block, column group) tiles:
-    // the column groups of a row block (sharing its packed rows) and neighbouring
-    // row blocks of a split (sharing coefficient chunks) then meet in one L2.
+    // every XCD runs a contiguous range of (split, row block, column group) tiles,
+    // in bands of at most kGroupBand column groups: within a band the groups of a
+    // row block run together (sharing its packed rows) and so do neighbouring row
+    // blocks (sharing the band's coefficient chunks), so the ~64 workgroups an XCD
+    // holds span about 64 / kGroupBand row blocks whatever the column count.
     const int n_groups = (int)gridDim.y;
     const int64_t gx = gridDim.x;
     const int64_t total = gx * n_groups * gridDim.z;
     const int64_t d = blockIdx.x + gx * (blockIdx.y + (int64_t)n_groups * blockIdx.z);
     const int64_t per_xcd = total / kXcds;
     const int64_t lin = d < per_xcd * kXcds ? (d % kXcds) * per_xcd + d / kXcds : d;
-    const int cg = (int)(lin % n_groups);
-    const int64_t rblk = (lin / n_groups) % gx;
-    const int64_t split = lin / n_groups / gx;
+    const int64_t per_split = gx * n_groups;
+    const int64_t split = lin / per_split;
+    const int64_t t = lin - split * per_split;
+    const int gband = n_groups < kGroupBand ? n_groups : kGroupBand;
+    const int band = (int)(t / (gx * gband));
+    const int gb_eff = n_groups - band * gband < gband ? n_groups - band * gband : gband;
+    const int64_t rem = t - (int64_t)band * gx * gband;
+    const int cg = band * gband + (int)(rem % gb_eff);
+    const int64_t rblk = rem / gb_eff;
     const int64_t rbase = a.r0 + rblk * RPB + (int64_t)wave * RPW;
     const int64_t kc_begin = a.kc0 + split * a.kc_per_split;
     const int64_t kc_end = kc_begin + a.kc_per_split < a.kc1 ? kc_begin + a.kc_per_split : a.kc1;
