@@ -46,9 +46,9 @@ def effective_n_scale(nsamp: float, cov_rank: int = 0) -> float:
     return n
 
 
-def h2_response(beta, se, n_obs, nsamp: float, cov_rank: int = 0) -> np.ndarray:
-    """Per-SNP moment response from GWAS effect sizes and standard errors:
-    z*^2 minus the excess of its null variance over one (NaN where undefined)."""
+def score_z(beta, se, n_obs, nsamp: float, cov_rank: int = 0) -> np.ndarray:
+    """Score statistic on the reference sample-size scale,
+    ``z*_j = sqrt(N*) b_j / sqrt(b_j^2 + (n_j - q - 2) se_j^2)`` (NaN where undefined)."""
     beta = np.asarray(beta, dtype=np.float64)
     se = np.asarray(se, dtype=np.float64)
     n_obs = np.broadcast_to(np.asarray(n_obs, dtype=np.float64), beta.shape)
@@ -58,10 +58,26 @@ def h2_response(beta, se, n_obs, nsamp: float, cov_rank: int = 0) -> np.ndarray:
     ok = np.isfinite(beta) & np.isfinite(se) & (se > 0) & np.isfinite(n_obs) & (n_obs > 0) & (nu > 0) & (den > 0)
     z = np.full(beta.shape, np.nan)
     z[ok] = np.sqrt(n_star) * beta[ok] / np.sqrt(den[ok])
-    local = nu + 1.0
-    noise = np.full(beta.shape, np.nan)
+    return z
+
+
+def score_noise_scale(n_obs, nsamp: float, cov_rank: int = 0, shape=None) -> np.ndarray:
+    """Null variance of ``z*_j``: ``N* / (n_j - q - 1)`` (NaN where ``n_j - q - 1 <= 1``)."""
+    n_obs = np.asarray(n_obs, dtype=np.float64)
+    if shape is not None:
+        n_obs = np.broadcast_to(n_obs, shape)
+    n_star = effective_n_scale(nsamp, cov_rank)
+    local = n_obs - float(cov_rank) - 1.0
+    noise = np.full(n_obs.shape, np.nan)
     np.divide(n_star, local, out=noise, where=np.isfinite(local) & (local > 1.0))
-    y = z * z - (noise - 1.0)
+    return noise
+
+
+def h2_response(beta, se, n_obs, nsamp: float, cov_rank: int = 0) -> np.ndarray:
+    """Per-SNP moment response from GWAS effect sizes and standard errors:
+    z*^2 minus the excess of its null variance over one (NaN where undefined)."""
+    z = score_z(beta, se, n_obs, nsamp, cov_rank)
+    y = z * z - (score_noise_scale(n_obs, nsamp, cov_rank, shape=z.shape) - 1.0)
     y[~np.isfinite(y)] = np.nan
     return y
 
@@ -236,6 +252,30 @@ def symmetrize_traces(trace: np.ndarray, sizes: np.ndarray) -> np.ndarray:
     return out
 
 
+def unit_sums(A, L, y, active, design: JackknifeDesign):
+    """Per jackknife unit over the active SNPs: the count ``m``, ``A_k = sum a``,
+    ``A^T A``, ``A^T L`` and ``A^T y`` (src/summit/inference/h2core.py:453-543,
+    src/summit/inference/rgcore.py:528-541)."""
+    U, K = design.nunit, A.shape[1]
+    m_u = np.zeros(U)
+    Ak_u = np.zeros((U, K))
+    AA_u = np.zeros((U, K, K))
+    AL_u = np.zeros((U, K, K))
+    Ay_u = np.zeros((U, K))
+    for u in range(U):
+        s, e = int(design.starts[u]), int(design.ends[u])
+        act = active[s:e]
+        if not act.any():
+            continue
+        Au, Lu, yu = A[s:e][act], L[s:e][act], y[s:e][act]
+        m_u[u] = float(Au.shape[0])
+        Ak_u[u] = Au.sum(axis=0)
+        AA_u[u] = Au.T @ Au
+        AL_u[u] = Au.T @ Lu
+        Ay_u[u] = Au.T @ yu
+    return m_u, Ak_u, AA_u, AL_u, Ay_u
+
+
 def _solve_each(lhs: np.ndarray, rhs: np.ndarray) -> np.ndarray:
     out = np.full(rhs.shape, np.nan)
     for i in range(lhs.shape[0]):
@@ -266,6 +306,8 @@ class H2Fit:
     n_scale: float
     nsnps: int
     reps_h2: np.ndarray       # (U + 1,) replicate totals, full estimate last
+    sigma_reps: np.ndarray = None  # (U + 1, K + 1) replicate solutions, full estimate last
+    unit_sizes: np.ndarray = None  # (U,) SNPs with a finite response per jackknife unit
 
 
 def fit_h2(ldscores, annot, y, n_scale: float, design: JackknifeDesign) -> H2Fit:
@@ -291,22 +333,7 @@ def fit_h2(ldscores, annot, y, n_scale: float, design: JackknifeDesign) -> H2Fit
         raise RuntimeError(f"Invalid univariate n_scale={n_scale}")
     active = np.isfinite(y)
     U = design.nunit
-    m_u = np.zeros(U)
-    Ak_u = np.zeros((U, K))
-    AA_u = np.zeros((U, K, K))
-    AL_u = np.zeros((U, K, K))
-    Ay_u = np.zeros((U, K))
-    for u in range(U):
-        s, e = int(design.starts[u]), int(design.ends[u])
-        act = active[s:e]
-        if not act.any():
-            continue
-        Au, Lu, yu = A[s:e][act], L[s:e][act], y[s:e][act]
-        m_u[u] = float(Au.shape[0])
-        Ak_u[u] = Au.sum(axis=0)
-        AA_u[u] = Au.T @ Au
-        AL_u[u] = Au.T @ Lu
-        Ay_u[u] = Au.T @ yu
+    m_u, Ak_u, AA_u, AL_u, Ay_u = unit_sums(A, L, y, active, design)
     M_rep = _replicates(m_u)
     Ak_rep = _replicates(Ak_u)
     AA_rep = _replicates(AA_u)
@@ -336,7 +363,7 @@ def fit_h2(ldscores, annot, y, n_scale: float, design: JackknifeDesign) -> H2Fit
     return H2Fit(h2=float(h2_tot[-1]), h2_se=float(jackknife_se(h2_tot[:, None], m_u)[0]),
                  h2_cat=h2_cat[-1], h2_cat_se=jackknife_se(h2_cat, m_u), enrichment=enr[-1],
                  enrichment_se=jackknife_se(enr, m_u), sigma=sigma[-1], sigma_se=jackknife_se(sigma, m_u),
-                 n_scale=n, nsnps=int(active.sum()), reps_h2=h2_tot)
+                 n_scale=n, nsnps=int(active.sum()), reps_h2=h2_tot, sigma_reps=sigma, unit_sizes=m_u)
 
 
 # --------------------------------------------------------------------------- the engine's files
