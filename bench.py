@@ -504,6 +504,29 @@ def h2_end_to_end(engine, ldscore, torch) -> dict:
             "gwas_s": round(t_gwas, 3), "fit_s": round(t_fit, 3), "total_s": round(time.perf_counter() - t_all, 3)}
 
 
+def vendor_int8_gemm(torch, dev, rows: int = 131072, k: int = 102400, n: int = 320) -> dict:
+    """The ROCm library int8 GEMM (torch._int_mm, int32 out) on the limb product of the phase-2
+    contraction with its operands already unpacked: SNP rows x samples (K = N rounded to a multiple
+    of 8) times samples x (5 limbs x 64 columns), on a 131,072-row slice (the rate is flat in the row
+    count).  It does the same MFMA work per row as the contraction without the 2-bit unpack, the
+    missing-call terms or the fused epilogue; the square 16384^3 int8 GEMM is its best case."""
+    out = {}
+    for name, (m_, k_, n_) in (("phase2_limb_shape", (rows, k, n)), ("square_16384", (16384, 16384, 16384))):
+        a = torch.randint(-128, 128, (m_, k_), dtype=torch.int8, device=dev)
+        b = torch.randint(-128, 128, (k_, n_), dtype=torch.int8, device=dev)
+        torch._int_mm(a, b)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            torch._int_mm(a, b)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1000.0 / 5
+        out[name] = {"m": m_, "k": k_, "n": n_, "ms": round(ms, 3),
+                     "tops": round(2.0 * m_ * k_ * n_ / (ms / 1000.0) / 1e12, 1)}
+        del a, b
+    return out
+
+
 def learned_cv(engine, ldscore, torch, dev, plain_max: int = 98304, plain_tile: int = 2048) -> dict:
     """Learned control variate at N=50,000, M=200,000 (block-Markov genotypes,
     MAF ~ U(0.01, 0.5), 0.5% missing, 8 random 10% annotations).  The network
@@ -631,7 +654,7 @@ def main() -> int:
     ap.add_argument("--no-tune", action="store_true", help="keep the built-in schedule heuristics")
     ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph replay timing of the same steps")
     ap.add_argument("--only", choices=["learned_cv", "partitioned", "gxe", "windowed", "wall_to_1pct",
-                                       "cpu_mailman_c1", "h2_end_to_end", "dropin_loop"],
+                                       "cpu_mailman_c1", "h2_end_to_end", "dropin_loop", "vendor_int8_gemm"],
                     help="run only one extra measurement and print it")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the 1%%-error wall-time run and the host->HBM ingest measurement")
@@ -666,6 +689,9 @@ def main() -> int:
         return 0
     if args.only == "dropin_loop":
         print(json.dumps({"dropin_loop": dropin_loop(engine, ldscore, torch)}), flush=True)
+        return 0
+    if args.only == "vendor_int8_gemm":
+        print(json.dumps({"vendor_int8_gemm": vendor_int8_gemm(torch, dev)}), flush=True)
         return 0
     if args.only == "h2_end_to_end":
         print(json.dumps({"h2_end_to_end": h2_end_to_end(engine, ldscore, torch)}), flush=True)
@@ -867,6 +893,11 @@ def main() -> int:
         log(f"drop-in loop: {extra['dropin_loop']}")
         extra["h2_end_to_end"] = h2_end_to_end(engine, ldscore, torch)
         log(f"h2 end to end: {extra['h2_end_to_end']}")
+        vg = vendor_int8_gemm(torch, dev)
+        # the contraction's own int8 MFMA rate (limb products issued per second) beside the library's
+        vg["phase2_contract_tops"] = round(roofline["achieved"] * roofline["int8_macs_per_pair"], 1)
+        extra["vendor_int8_gemm"] = vg
+        log(f"vendor int8 gemm: {vg}")
     if not args.no_extras and not args.no_cv:
         extra["learned_cv"] = learned_cv(engine, ldscore, torch, dev)
     line = {
