@@ -302,3 +302,76 @@ def gxe_ldscore_multi(g: engine.Genotype, envs, C_ints=None, annot=None, nvecs: 
         out.append({"xx": a[:, :K], "xw": a[:, K:], "wx": b[:, :K], "ww": b[:, K:],
                     "inv_x": inv_x[k].cpu().numpy(), "inv_w": inv_w[k].cpu().numpy(), "df": dfs[k]})
     return out
+
+
+def summary_inputs(g: engine.Genotype, env, pheno, C_int=None, annot=None, nvecs: int = 64, seed: int = 1,
+                   step_size: int = 1000, limbs: int = 5, eps_var: float = 1e-10, panels=None) -> dict:
+    """Every per-SNP and scalar input of the GxE normal equations
+    (summit_amd.gxe_fit.assemble_normal_equations) for one environment e and
+    phenotype, from the device-resident genotypes:
+
+    * the phenotype projected off [1/sqrt(N), C] and scaled to y'y = r = N - p - 1
+      (src/summit/ldscore/gxe_score.py:1520-1538);
+    * score_x = f_x'y / sqrt(r) = inv_x g'y / sqrt(r) and score_w = inv_w g'(e.y) / sqrt(r)
+      (y lies in the range of P, so P drops out of both products);
+    * norm = f'f / r (one by the normalization) and the NxE diagonals
+      f' diag(e^2) f / r = inv^2 (sum e^2 v^2 - 2 c'(Q' diag(e^2) v) + c' Q'diag(e^2)Q c) / r
+      for v = g and e.g, c = Q'v;
+    * q_nxe = |e.y|^2, q_residual = r, and the exact NxE traces.
+    One phase-2 traversal against the rows [y, e.y, Q', (e Q)', (e^2 Q)', (e^3 Q)']
+    and one weighted sum of squares with weights [1, e^2, e^4] give every per-SNP
+    array; the LD panels come from gxe_ldscore (nvecs probes) unless `panels`
+    (a gxe_ldscore result) is passed."""
+    from .gxe_fit import nxe_traces
+    dev = engine.default_device()
+    N, M = g.n, g.m
+    e = torch.as_tensor(np.asarray(env, dtype=np.float64).ravel(), device=dev)
+    y0 = torch.as_tensor(np.asarray(pheno, dtype=np.float64).ravel(), device=dev)
+    if e.numel() != N or y0.numel() != N:
+        raise ValueError("env and pheno need one value per sample.")
+    Cn = None if C_int is None else np.asarray(C_int, dtype=np.float64).reshape(N, -1)
+    Q = torch.full((N, 1), 1.0 / np.sqrt(N), dtype=torch.float64, device=dev)
+    if Cn is not None and Cn.shape[1]:
+        Q = torch.cat([Q, torch.as_tensor(Cn, device=dev)], dim=1)
+    q = Q.shape[1]
+    r = N - q
+    if r <= 0:
+        raise ValueError(f"Residual correlation degrees of freedom are non-positive: nsamp={N}, p_eff={q - 1}, df={r}.")
+    y = y0 - Q @ (Q.T @ y0)
+    yy = float(y @ y)
+    if not (np.isfinite(yy) and yy > 0.0):
+        raise ValueError("the projected phenotype is zero.")
+    y = y * np.sqrt(r / yy)
+    rows = torch.cat([y[None, :], (e * y)[None, :], Q.T, (Q * e[:, None]).T, (Q * (e * e)[:, None]).T,
+                      (Q * (e * e * e)[:, None]).T], dim=0).contiguous()
+    T = engine.xtv(g, rows, limbs=max(limbs, 6))
+    ss = weighted_sumsq(g, torch.stack([torch.ones_like(e), e * e, e ** 4]))
+    gy, gey = T[:, 0], T[:, 1]
+    cq, ceq, ce2q, ce3q = (T[:, 2 + i * q:2 + (i + 1) * q] for i in range(4))
+    var_x = (ss[0] - (cq * cq).sum(dim=1)) / r
+    var_w = (ss[1] - (ceq * ceq).sum(dim=1)) / r
+    bad = ~(torch.isfinite(var_x) & (var_x > eps_var) & torch.isfinite(var_w) & (var_w > eps_var))
+    if bool(bad.any()):
+        raise ValueError(f"{int(bad.sum())} variant feature(s) have zero or invalid projected variance. GxE "
+                         "kernel normalization cannot retain zero columns; QC/remove these variants.")
+    ix2, iw2 = 1.0 / var_x, 1.0 / var_w
+    QDQ = Q.T @ (Q * (e * e)[:, None])
+    nxe_x = ix2 * (ss[1] - 2.0 * (cq * ce2q).sum(dim=1) + ((cq @ QDQ) * cq).sum(dim=1)) / r
+    nxe_w = iw2 * (ss[2] - 2.0 * (ceq * ce3q).sum(dim=1) + ((ceq @ QDQ) * ceq).sum(dim=1)) / r
+    t1, t2 = nxe_traces(e.cpu().numpy(), Cn)
+    if annot is None:
+        annot = np.ones((M, 1))
+    annot = np.asarray(annot, dtype=np.float64)
+    if panels is None:
+        panels = gxe_ldscore(g, e.cpu().numpy(), Cn, annot, nvecs=nvecs, seed=seed, step_size=step_size,
+                             limbs=limbs, eps_var=eps_var)
+    sq = float(np.sqrt(r))
+    return {"annotations": annot,
+            "score_x": (torch.sqrt(ix2) * gy / sq).cpu().numpy(),
+            "score_w": (torch.sqrt(iw2) * gey / sq).cpu().numpy(),
+            "ld_xx": panels["xx"], "ld_xw": panels["xw"], "ld_wx": panels["wx"], "ld_ww": panels["ww"],
+            "norm_x": (var_x * ix2).cpu().numpy(), "norm_w": (var_w * iw2).cpu().numpy(),
+            "diag_nxe_x": torch.clamp(nxe_x, min=0.0).cpu().numpy(),
+            "diag_nxe_w": torch.clamp(nxe_w, min=0.0).cpu().numpy(),
+            "residual_rank": r, "q_nxe": float(((e * y) ** 2).sum()), "q_residual": float(y @ y),
+            "trace_nxe": t1, "trace_nxe_sq": t2}

@@ -32,3 +32,43 @@ def assert_ld_close(ld, ref, null, rtol: float = 1e-10):
     bad = err > rtol * scale
     assert not np.any(bad), (f"{int(bad.sum())} LD scores beyond {rtol:g} relative; worst "
                              f"{float(np.max(err / np.maximum(scale, 1e-300))):.3g}")
+
+
+def gxe_basis(rng, n, p):
+    """[1/sqrt(N), C] with C orthonormal and centred."""
+    C = rng.standard_normal((n, p))
+    C -= C.mean(axis=0)
+    C, _ = np.linalg.qr(C)
+    return C
+
+
+def gxe_features(G, e, C):
+    n = G.shape[0]
+    Q = np.column_stack([np.full(n, 1.0 / np.sqrt(n)), C])
+    P = np.eye(n) - Q @ Q.T
+    r = n - Q.shape[1]
+    fx = P @ G
+    fw = P @ (e[:, None] * G)
+    fx *= np.sqrt(r / np.sum(fx * fx, axis=0))
+    fw *= np.sqrt(r / np.sum(fw * fw, axis=0))
+    return P, r, fx, fw
+
+
+def gxe_dense_inputs(G, e, C, y_raw, annot):
+    """Exact summary arrays from dense features (the panels without probes)."""
+    P, r, fx, fw = gxe_features(G, e, C)
+    y = P @ y_raw
+    y *= np.sqrt(r / (y @ y))
+    d = e * e
+    ld = {}
+    for key, (lf, sf) in {"xx": (fx, fx), "xw": (fx, fw), "wx": (fw, fx), "ww": (fw, fw)}.items():
+        ld[key] = ((lf.T @ sf / r) ** 2) @ annot
+    from summit_amd import gxe_fit
+    t1, t2 = gxe_fit.nxe_traces(e, C)
+    return {"annotations": annot, "score_x": fx.T @ y / np.sqrt(r), "score_w": fw.T @ y / np.sqrt(r),
+            "ld_xx": ld["xx"], "ld_xw": ld["xw"], "ld_wx": ld["wx"], "ld_ww": ld["ww"],
+            "norm_x": np.sum(fx * fx, axis=0) / r, "norm_w": np.sum(fw * fw, axis=0) / r,
+            "diag_nxe_x": np.sum(d[:, None] * fx * fx, axis=0) / r,
+            "diag_nxe_w": np.sum(d[:, None] * fw * fw, axis=0) / r,
+            "residual_rank": r, "q_nxe": float(np.sum((e * y) ** 2)), "q_residual": float(y @ y),
+            "trace_nxe": t1, "trace_nxe_sq": t2}, (P, r, fx, fw, y)
