@@ -40,6 +40,9 @@ struct gpe_geno {
     uint8_t* snp_raw = nullptr;
     uint8_t* snp_work = nullptr;  // == snp_raw unless HWE-completed
     uint8_t* smp_work = nullptr;
+    // chunk-tiled copies of snp_work / smp_work read by the contractions (built by prepare)
+    uint8_t* snp_tiled = nullptr;
+    uint8_t* smp_tiled = nullptr;
     bool prepared = false;
     int ddof = 1;
     int impute_mode = GPE_IMPUTE_MEAN;
@@ -205,6 +208,10 @@ __global__ void k_absmax_i32(const int32_t* u, int64_t ldu, int64_t n, int n_col
 void free_prepared(gpe_geno* g) {
     if (g->snp_work && g->snp_work != g->snp_raw) dfree(g->snp_work);
     g->snp_work = g->snp_raw;
+    dfree(g->snp_tiled);
+    dfree(g->smp_tiled);
+    g->snp_tiled = nullptr;
+    g->smp_tiled = nullptr;
     dfree(g->miss_ptr);
     dfree(g->miss_idx);
     dfree(g->smiss_ptr);
@@ -761,6 +768,12 @@ int gpe_geno_prepare(gpe_geno* g, int32_t ddof, int32_t impute_mode, uint64_t im
             g->smiss_idx = dalloc<int32_t>(total);
             launch_miss_fill(g->snp_work, g->snp_stride, g->m, g->smiss_ptr, g->smiss_idx, st);
         }
+        // the contractions stream chunk-tiled copies: one step's 32 rows x 32 bytes are
+        // 1 KiB of consecutive bytes instead of 32 pieces 32 bytes long, one per row
+        g->snp_tiled = dalloc<uint8_t>(g->m_alloc * g->snp_stride);
+        launch_tile_packed(g->snp_work, g->snp_stride, g->m_alloc, g->snp_tiled, st);
+        g->smp_tiled = dalloc<uint8_t>(g->n_alloc * g->smp_stride);
+        launch_tile_packed(g->smp_work, g->smp_stride, g->n_alloc, g->smp_tiled, st);
         GPE_HIP(hipStreamSynchronize(st));
         g->prepared = true;
     });
@@ -850,7 +863,8 @@ int gpe_gz_i32(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const int8_t* z_
             for (int c = 0; c < n_cols; ++c)
                 launch_fill_u8(reinterpret_cast<uint8_t*>(u_dev + (int64_t)c * ldu), g->n * 4, 0, st);
         ContractArgs a{};
-        a.packed = g->smp_work;
+        a.packed = g->smp_tiled;
+        a.tiled = 1;
         a.stride = g->smp_stride;
         a.r0 = 0;
         a.r1 = g->n;
@@ -898,7 +912,8 @@ int gpe_gtu_i64(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const int32_t* 
         int8_t* coef = static_cast<int8_t*>(g->scratch("coef", kcount * coef_ctiles * 4096));
         launch_quant_i32(u_dev, ldu, g->n, n_cols, nl, ctc, coef_ctiles, coef, st);
         ContractArgs a{};
-        a.packed = g->snp_work;
+        a.packed = g->snp_tiled;
+        a.tiled = 1;
         a.stride = g->snp_stride;
         a.r0 = snp_begin;
         a.r1 = snp_end;
@@ -997,7 +1012,8 @@ void run_phase1(gpe_geno* g, int64_t s, int64_t e, const double* w, int n_bins, 
     int64_t* y = static_cast<int64_t*>(g->scratch("p1_y", (int64_t)ncols_pad * ldy * sizeof(int64_t)));
     launch_fill_u64(reinterpret_cast<uint64_t*>(y), (int64_t)ncols_pad * ldy, 0, st);
     ContractArgs a{};
-    a.packed = g->smp_work;
+    a.packed = g->smp_tiled;
+    a.tiled = 1;
     a.stride = g->smp_stride;
     a.r0 = 0;
     a.r1 = g->n;
@@ -1115,7 +1131,8 @@ void run_phase2(gpe_geno* g, int64_t s, int64_t e, const double* x, int64_t ldx,
     if (ncols_pad != ncols) launch_fill_u8(reinterpret_cast<uint8_t*>(coef), kcount * coef_ctiles * 4096, 0, st);
     launch_quant_p2(x, ldx, g->n, n_bins, v_count, v_pad, shift, lc.nl, lc.ctc, coef_ctiles, coef, col_sum, st);
     if (sum_override) launch_col_sum_override(sum_override, n_bins, v_count, v_pad, shift, col_scale, col_sum, st);
-    a.packed = g->snp_work;
+    a.packed = g->snp_tiled;
+    a.tiled = 1;
     a.stride = g->snp_stride;
     a.r0 = s;
     a.r1 = e;
@@ -1776,7 +1793,8 @@ int gpe_weighted_sumsq(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const do
             int64_t* me = static_cast<int64_t*>(g->scratch("wss_me", (size_t)L * v_pad * sizeof(int64_t)));
             int64_t* g2 = static_cast<int64_t*>(g->scratch("wss_g2", (size_t)L * v_pad * sizeof(int64_t)));
             ContractArgs a{};
-            a.packed = g->snp_work;
+            a.packed = g->snp_tiled;
+            a.tiled = 1;
             a.stride = g->snp_stride;
             a.r0 = snp_begin;
             a.r1 = snp_end;

@@ -57,6 +57,7 @@ enum EpiMode : int {
 struct ContractArgs {
     const uint8_t* packed;   // packed rows (either layout)
     int64_t stride;          // bytes per packed row
+    int32_t tiled;           // 0: row-major rows; 1: chunk-tiled copy (tile_packed)
     int64_t r0, r1;          // rows [r0, r1) are produced
     const int8_t* coef;      // swizzled coefficient limbs
     int64_t coef_ctiles;     // c-tiles per K chunk in the swizzled layout
@@ -124,6 +125,11 @@ void launch_contract(int nl, int ctc, bool miss, EpiMode mode, const ContractArg
 // workgroups of one contraction variant resident on the whole device at once
 int contract_slots(int nl, int ctc, bool miss, EpiMode mode);
 void launch_clear_tail(uint8_t* snp, int64_t m, int64_t stride, int64_t n, hipStream_t st);
+// Chunk-tiled copy of a packed matrix of `rows` rows (a multiple of 32) and `stride`
+// bytes per row (a multiple of 32): dst[r / 32][c][r % 32][32 B] = src[r][32 c .. 32 c + 32),
+// so the 32 rows of a tile read by one contraction step for one 128-value chunk are
+// 1 KiB of consecutive bytes.
+void launch_tile_packed(const uint8_t* src, int64_t stride, int64_t rows, uint8_t* dst, hipStream_t st);
 void launch_select_rows(const uint8_t* src, int64_t src_stride, int64_t m, const int64_t* rows, int64_t n_sel,
                         uint8_t* dst, int64_t dst_stride, hipStream_t st);
 void launch_snp_stats(const uint8_t* snp, int64_t stride, int64_t m, int64_t n, int32_t* nobs, int64_t* sum,

@@ -185,189 +185,12 @@ __device__ __forceinline__ void chunk_steps(const int8_t* L, int lane, const v4i
     }
 }
 
-// CPB = chunks per barrier: 1 -> a three-slot LDS ring of one chunk per slot,
-// loads issued two chunks ahead, one barrier per chunk; 2 -> a two-slot ring of
-// two chunks per slot, loads issued one slot (two chunks) ahead, one barrier
-// per two chunks (needs CT <= 5 to keep two workgroups per CU in 160 KiB of LDS).
-template <int NL, int CTC, bool MISS, int EPI, int CPB = 1>
-__global__ __launch_bounds__(256, (NL * CTC <= 2) ? 4 : 2) void k_contract(ContractArgs a) {
-    constexpr int CT = NL * CTC;
-    constexpr int RPW = MISS ? 32 : 64;
-    constexpr int RPB = 4 * RPW;
-    constexpr int G = MISS ? 1 : 2;  // packed-row loads per lane per chunk
-    constexpr uint32_t TBL = EPI == EPI_I64_SQ ? kSqTable : (EPI == EPI_I64_ATOMIC_Q ? kMissTable : kGenoTable);
-    static_assert(CPB == 1 || CPB == 2, "one or two chunks per barrier");
-    constexpr int SLOTS = CPB == 1 ? 3 : 2;
-    __shared__ __attribute__((aligned(16))) int8_t lds[SLOTS][CPB * CT * 4096];
-
-    const int tid = threadIdx.x;
-    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int lane = tid & 63;
-    const int half = lane >> 5;
-    const int lr = lane & 31;
-    // Workgroups are dispatched in flattened (x, y, z) order and dealt round-robin
-    // to the XCDs, each with its own L2.  The flattened index is remapped so that
-    // every XCD runs a contiguous range of (split, row block, column group) tiles,
-    // in bands of at most kGroupBand column groups: within a band the groups of a
-    // row block run together (sharing its packed rows) and so do neighbouring row
-    // blocks (sharing the band's coefficient chunks), so the ~64 workgroups an XCD
-    // holds span about 64 / kGroupBand row blocks whatever the column count.
-    const int n_groups = (int)gridDim.y;
-    const int64_t gx = gridDim.x;
-    const int64_t total = gx * n_groups * gridDim.z;
-    const int64_t d = blockIdx.x + gx * (blockIdx.y + (int64_t)n_groups * blockIdx.z);
-    const int64_t per_xcd = total / kXcds;
-    const int64_t lin = d < per_xcd * kXcds ? (d % kXcds) * per_xcd + d / kXcds : d;
-    const int64_t per_split = gx * n_groups;
-    const int64_t split = lin / per_split;
-    const int64_t t = lin - split * per_split;
-    const int gband = n_groups < kGroupBand ? n_groups : kGroupBand;
-    const int band = (int)(t / (gx * gband));
-    const int gb_eff = n_groups - band * gband < gband ? n_groups - band * gband : gband;
-    const int64_t rem = t - (int64_t)band * gx * gband;
-    const int cg = band * gband + (int)(rem % gb_eff);
-    const int64_t rblk = rem / gb_eff;
-    const int64_t rbase = a.r0 + rblk * RPB + (int64_t)wave * RPW;
-    const int64_t kc_begin = a.kc0 + split * a.kc_per_split;
-    const int64_t kc_end = kc_begin + a.kc_per_split < a.kc1 ? kc_begin + a.kc_per_split : a.kc1;
-
-    v16i acc[CT][2];
-#pragma unroll
-    for (int t = 0; t < CT; ++t) {
-        acc[t][0] = v16i{};
-        acc[t][1] = v16i{};
-    }
-
-    const uint8_t* prow0 = a.packed + (rbase + lr) * a.stride + half * 16;
-    const uint8_t* prow1 = MISS ? prow0 : prow0 + 32 * a.stride;
-    const int8_t* coef_grp = a.coef + (int64_t)cg * CT * 4096;
-    const int64_t coef_chunk_bytes = a.coef_ctiles * 4096;
-
-    if constexpr (CPB == 1) {
-        // packed words of chunks kc (g0), kc+1 (g1); plain unrolled code, no lambdas,
-        // so these stay in VGPRs.  Prefetches past the end re-read the last chunk
-        // (in bounds, never used).
-        v4i g0a = v4i{}, g0b = v4i{}, g1a = v4i{}, g1b = v4i{};
-        if (kc_begin < kc_end) {
-            const int64_t k1 = kc_begin + 1 < kc_end ? kc_begin + 1 : kc_end - 1;
-            coef_chunk_to_lds<CT>(coef_grp + (kc_begin - a.kc0) * coef_chunk_bytes, lds[0], tid, wave);
-            g0a = *reinterpret_cast<const v4i*>(prow0 + kc_begin * 32);
-            if (!MISS) g0b = *reinterpret_cast<const v4i*>(prow1 + kc_begin * 32);
-            coef_chunk_to_lds<CT>(coef_grp + (k1 - a.kc0) * coef_chunk_bytes, lds[1], tid, wave);
-            g1a = *reinterpret_cast<const v4i*>(prow0 + k1 * 32);
-            if (!MISS) g1b = *reinterpret_cast<const v4i*>(prow1 + k1 * 32);
-        }
-        // operands of step 0 of the first chunk; later ones are unpacked one step ahead
-        v4i op0, op1;
-        if (MISS) {
-            unpack16((uint32_t)g0a[0], op0, op1);
-        } else {
-            unpack16_g<TBL>((uint32_t)g0a[0], op0);
-            unpack16_g<TBL>((uint32_t)g0b[0], op1);
-        }
-        int buf = 0;  // LDS buffer of chunk kc
-        for (int64_t kc = kc_begin; kc < kc_end; ++kc) {
-            // chunk kc's coefficients (and packed words) have landed once only the
-            // CT + G loads issued for chunk kc+1 may still be in flight; the barrier
-            // then publishes every wave's part of the chunk
-            __builtin_amdgcn_s_waitcnt(waitcnt_vm(CT + G));
-            asm volatile("" ::: "memory");
-            __builtin_amdgcn_s_barrier();
-            asm volatile("" ::: "memory");
-            // chunk kc+2 into the buffer read two chunks ago (k2 passes through an empty
-            // asm so these loads cannot be hoisted above the wait and barrier)
-            int64_t k2 = kc + 2 < kc_end ? kc + 2 : kc_end - 1;
-            asm volatile("" : "+s"(k2));
-            const int buf2 = buf == 0 ? 2 : buf - 1;
-            coef_chunk_to_lds<CT>(coef_grp + (k2 - a.kc0) * coef_chunk_bytes, lds[buf2], tid, wave);
-            const v4i g2a = *reinterpret_cast<const v4i*>(prow0 + k2 * 32);
-            const v4i g2b = MISS ? v4i{} : *reinterpret_cast<const v4i*>(prow1 + k2 * 32);
-            const int8_t* L = lds[buf];
-    #pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                // unpack the operands of the next step (step 0 of the next chunk after
-                // step 3) before this step's MFMAs, so the two are independent and interleave
-                const uint32_t wn0 = s < 3 ? (uint32_t)g0a[s + 1] : (uint32_t)g1a[0];
-                v4i nx0, nx1;
-                if (MISS) {
-                    unpack16(wn0, nx0, nx1);
-                } else {
-                    const uint32_t wn1 = s < 3 ? (uint32_t)g0b[s + 1] : (uint32_t)g1b[0];
-                    unpack16_g<TBL>(wn0, nx0);
-                    unpack16_g<TBL>(wn1, nx1);
-                }
-    #pragma unroll
-                for (int t = 0; t < CT; ++t) {
-                    const v4i av = *reinterpret_cast<const v4i*>(L + ((t * 4 + s) * 64 + lane) * 16);
-                    acc[t][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, op0, acc[t][0], 0, 0, 0);
-                    acc[t][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, op1, acc[t][1], 0, 0, 0);
-                }
-                op0 = nx0;
-                op1 = nx1;
-            }
-            g0a = g1a;
-            g0b = g1b;
-            g1a = g2a;
-            g1b = g2b;
-            buf = buf == 2 ? 0 : buf + 1;
-        }
-    } else {
-        // super chunk = chunks kc, kc+1 in one slot; words of the current pair in
-        // g0/g1, of the next pair in g2/g3 (loaded one slot ahead)
-        constexpr int64_t kSlot = (int64_t)CT * 4096;
-        v4i g0a = v4i{}, g0b = v4i{}, g1a = v4i{}, g1b = v4i{};
-        if (kc_begin < kc_end) {
-            const int64_t c1 = kc_begin + 1 < kc_end ? kc_begin + 1 : kc_end - 1;
-            coef_chunk_to_lds<CT>(coef_grp + (kc_begin - a.kc0) * coef_chunk_bytes, lds[0], tid, wave);
-            coef_chunk_to_lds<CT>(coef_grp + (c1 - a.kc0) * coef_chunk_bytes, lds[0] + kSlot, tid, wave);
-            g0a = *reinterpret_cast<const v4i*>(prow0 + kc_begin * 32);
-            if (!MISS) g0b = *reinterpret_cast<const v4i*>(prow1 + kc_begin * 32);
-            g1a = *reinterpret_cast<const v4i*>(prow0 + c1 * 32);
-            if (!MISS) g1b = *reinterpret_cast<const v4i*>(prow1 + c1 * 32);
-        }
-        v4i op0, op1;
-        if (MISS) {
-            unpack16((uint32_t)g0a[0], op0, op1);
-        } else {
-            unpack16_g<TBL>((uint32_t)g0a[0], op0);
-            unpack16_g<TBL>((uint32_t)g0b[0], op1);
-        }
-        int buf = 0;
-        for (int64_t kc = kc_begin; kc < kc_end; kc += 2) {
-            // everything issued for this pair (coefficients into LDS by every wave,
-            // this lane's packed words) has landed; the barrier publishes the slot
-            // and frees the other one, read during the previous pair
-            __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
-            asm volatile("" ::: "memory");
-            __builtin_amdgcn_s_barrier();
-            asm volatile("" ::: "memory");
-            int64_t n0 = kc + 2 < kc_end ? kc + 2 : kc_end - 1;
-            int64_t n1 = kc + 3 < kc_end ? kc + 3 : kc_end - 1;
-            asm volatile("" : "+s"(n0));
-            asm volatile("" : "+s"(n1));
-            int8_t* nxt = lds[buf ^ 1];
-            coef_chunk_to_lds<CT>(coef_grp + (n0 - a.kc0) * coef_chunk_bytes, nxt, tid, wave);
-            coef_chunk_to_lds<CT>(coef_grp + (n1 - a.kc0) * coef_chunk_bytes, nxt + kSlot, tid, wave);
-            const v4i g2a = *reinterpret_cast<const v4i*>(prow0 + n0 * 32);
-            const v4i g2b = MISS ? v4i{} : *reinterpret_cast<const v4i*>(prow1 + n0 * 32);
-            const v4i g3a = *reinterpret_cast<const v4i*>(prow0 + n1 * 32);
-            const v4i g3b = MISS ? v4i{} : *reinterpret_cast<const v4i*>(prow1 + n1 * 32);
-            const int8_t* L = lds[buf];
-            chunk_steps<CT, MISS, TBL>(L, lane, g0a, g0b, (uint32_t)g1a[0], (uint32_t)g1b[0], op0, op1, acc);
-            if (kc + 1 < kc_end)
-                chunk_steps<CT, MISS, TBL>(L + kSlot, lane, g1a, g1b, (uint32_t)g2a[0], (uint32_t)g2b[0], op0, op1,
-                                           acc);
-            g0a = g2a;
-            g0b = g2b;
-            g1a = g3a;
-            g1b = g3b;
-            buf ^= 1;
-        }
-    }
-    // no LDS-load may still be writing when the workgroup ends
-    __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
-
-    // ------------------------------------------------------------- epilogues
+// Epilogues of the packed contraction, per wave: acc[l*CTC + ctc][bt] holds limb l of
+// c-tile cg*CTC + ctc for row tile bt (rows rbase + 32*bt + lr; MISS: tile 1 is the
+// missing plane of the same rows).
+template <int NL, int CTC, bool MISS, int EPI>
+__device__ __forceinline__ void contract_epilogue(const ContractArgs& a, v16i (&acc)[NL * CTC][2], int cg,
+                                                  int64_t rbase, int lr, int half) {
     if constexpr (EPI == EPI_I32_ATOMIC) {
         static_assert(NL == 1, "int32 output needs one limb");
 #pragma unroll
@@ -500,6 +323,197 @@ __global__ __launch_bounds__(256, (NL * CTC <= 2) ? 4 : 2) void k_contract(Contr
     }
 }
 
+// CPB = chunks per barrier: 1 -> a three-slot LDS ring of one chunk per slot,
+// loads issued two chunks ahead, one barrier per chunk; 2 -> a two-slot ring of
+// two chunks per slot, loads issued one slot (two chunks) ahead, one barrier
+// per two chunks (needs CT <= 5 to keep two workgroups per CU in 160 KiB of LDS).
+template <int NL, int CTC, bool MISS, int EPI, int CPB = 1>
+__global__ __launch_bounds__(256, (NL * CTC <= 2) ? 4 : 2) void k_contract(ContractArgs a) {
+    constexpr int CT = NL * CTC;
+    constexpr int RPW = MISS ? 32 : 64;
+    constexpr int RPB = 4 * RPW;
+    constexpr int G = MISS ? 1 : 2;  // packed-row loads per lane per chunk
+    constexpr uint32_t TBL = EPI == EPI_I64_SQ ? kSqTable : (EPI == EPI_I64_ATOMIC_Q ? kMissTable : kGenoTable);
+    static_assert(CPB == 1 || CPB == 2, "one or two chunks per barrier");
+    constexpr int SLOTS = CPB == 1 ? 3 : 2;
+    __shared__ __attribute__((aligned(16))) int8_t lds[SLOTS][CPB * CT * 4096];
+
+    const int tid = threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lane = tid & 63;
+    const int half = lane >> 5;
+    const int lr = lane & 31;
+    // Workgroups are dispatched in flattened (x, y, z) order and dealt round-robin
+    // to the XCDs, each with its own L2.  The flattened index is remapped so that
+    // every XCD runs a contiguous range of (split, row block, column group) tiles,
+    // in bands of at most kGroupBand column groups: within a band the groups of a
+    // row block run together (sharing its packed rows) and so do neighbouring row
+    // blocks (sharing the band's coefficient chunks), so the ~64 workgroups an XCD
+    // holds span about 64 / kGroupBand row blocks whatever the column count.
+    const int n_groups = (int)gridDim.y;
+    const int64_t gx = gridDim.x;
+    const int64_t total = gx * n_groups * gridDim.z;
+    const int64_t d = blockIdx.x + gx * (blockIdx.y + (int64_t)n_groups * blockIdx.z);
+    const int64_t per_xcd = total / kXcds;
+    const int64_t lin = d < per_xcd * kXcds ? (d % kXcds) * per_xcd + d / kXcds : d;
+    const int64_t per_split = gx * n_groups;
+    const int64_t split = lin / per_split;
+    const int64_t t = lin - split * per_split;
+    const int gband = n_groups < kGroupBand ? n_groups : kGroupBand;
+    const int band = (int)(t / (gx * gband));
+    const int gb_eff = n_groups - band * gband < gband ? n_groups - band * gband : gband;
+    const int64_t rem = t - (int64_t)band * gx * gband;
+    const int cg = band * gband + (int)(rem % gb_eff);
+    const int64_t rblk = rem / gb_eff;
+    const int64_t rbase = a.r0 + rblk * RPB + (int64_t)wave * RPW;
+    const int64_t kc_begin = a.kc0 + split * a.kc_per_split;
+    const int64_t kc_end = kc_begin + a.kc_per_split < a.kc1 ? kc_begin + a.kc_per_split : a.kc1;
+
+    v16i acc[CT][2];
+#pragma unroll
+    for (int t = 0; t < CT; ++t) {
+        acc[t][0] = v16i{};
+        acc[t][1] = v16i{};
+    }
+
+    // row r of the packed operand: row-major (a.tiled == 0) at r * stride, chunk kc at
+    // + 32 kc; chunk-tiled (a.tiled == 1, [r / 32][chunk][r % 32][32 B]) at
+    // (r / 32) * 32 * stride + (r % 32) * 32, chunk kc at + 1024 kc.  Either way the
+    // next 32-row tile is 32 * stride further.
+    const int64_t row = rbase + lr;
+    const int64_t cpitch = a.tiled ? 1024 : 32;
+    const uint8_t* prow0 = a.packed + (row >> 5) * (32 * a.stride) + (row & 31) * (a.tiled ? 32 : a.stride) + half * 16;
+    const uint8_t* prow1 = MISS ? prow0 : prow0 + 32 * a.stride;
+    const int8_t* coef_grp = a.coef + (int64_t)cg * CT * 4096;
+    const int64_t coef_chunk_bytes = a.coef_ctiles * 4096;
+
+    if constexpr (CPB == 1) {
+        // packed words of chunks kc (g0), kc+1 (g1); plain unrolled code, no lambdas,
+        // so these stay in VGPRs.  Prefetches past the end re-read the last chunk
+        // (in bounds, never used).
+        v4i g0a = v4i{}, g0b = v4i{}, g1a = v4i{}, g1b = v4i{};
+        if (kc_begin < kc_end) {
+            const int64_t k1 = kc_begin + 1 < kc_end ? kc_begin + 1 : kc_end - 1;
+            coef_chunk_to_lds<CT>(coef_grp + (kc_begin - a.kc0) * coef_chunk_bytes, lds[0], tid, wave);
+            g0a = *reinterpret_cast<const v4i*>(prow0 + kc_begin * cpitch);
+            if (!MISS) g0b = *reinterpret_cast<const v4i*>(prow1 + kc_begin * cpitch);
+            coef_chunk_to_lds<CT>(coef_grp + (k1 - a.kc0) * coef_chunk_bytes, lds[1], tid, wave);
+            g1a = *reinterpret_cast<const v4i*>(prow0 + k1 * cpitch);
+            if (!MISS) g1b = *reinterpret_cast<const v4i*>(prow1 + k1 * cpitch);
+        }
+        // operands of step 0 of the first chunk; later ones are unpacked one step ahead
+        v4i op0, op1;
+        if (MISS) {
+            unpack16((uint32_t)g0a[0], op0, op1);
+        } else {
+            unpack16_g<TBL>((uint32_t)g0a[0], op0);
+            unpack16_g<TBL>((uint32_t)g0b[0], op1);
+        }
+        int buf = 0;  // LDS buffer of chunk kc
+        for (int64_t kc = kc_begin; kc < kc_end; ++kc) {
+            // chunk kc's coefficients (and packed words) have landed once only the
+            // CT + G loads issued for chunk kc+1 may still be in flight; the barrier
+            // then publishes every wave's part of the chunk
+            __builtin_amdgcn_s_waitcnt(waitcnt_vm(CT + G));
+            asm volatile("" ::: "memory");
+            __builtin_amdgcn_s_barrier();
+            asm volatile("" ::: "memory");
+            // chunk kc+2 into the buffer read two chunks ago (k2 passes through an empty
+            // asm so these loads cannot be hoisted above the wait and barrier)
+            int64_t k2 = kc + 2 < kc_end ? kc + 2 : kc_end - 1;
+            asm volatile("" : "+s"(k2));
+            const int buf2 = buf == 0 ? 2 : buf - 1;
+            coef_chunk_to_lds<CT>(coef_grp + (k2 - a.kc0) * coef_chunk_bytes, lds[buf2], tid, wave);
+            const v4i g2a = *reinterpret_cast<const v4i*>(prow0 + k2 * cpitch);
+            const v4i g2b = MISS ? v4i{} : *reinterpret_cast<const v4i*>(prow1 + k2 * cpitch);
+            const int8_t* L = lds[buf];
+    #pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                // unpack the operands of the next step (step 0 of the next chunk after
+                // step 3) before this step's MFMAs, so the two are independent and interleave
+                const uint32_t wn0 = s < 3 ? (uint32_t)g0a[s + 1] : (uint32_t)g1a[0];
+                v4i nx0, nx1;
+                if (MISS) {
+                    unpack16(wn0, nx0, nx1);
+                } else {
+                    const uint32_t wn1 = s < 3 ? (uint32_t)g0b[s + 1] : (uint32_t)g1b[0];
+                    unpack16_g<TBL>(wn0, nx0);
+                    unpack16_g<TBL>(wn1, nx1);
+                }
+    #pragma unroll
+                for (int t = 0; t < CT; ++t) {
+                    const v4i av = *reinterpret_cast<const v4i*>(L + ((t * 4 + s) * 64 + lane) * 16);
+                    acc[t][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, op0, acc[t][0], 0, 0, 0);
+                    acc[t][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, op1, acc[t][1], 0, 0, 0);
+                }
+                op0 = nx0;
+                op1 = nx1;
+            }
+            g0a = g1a;
+            g0b = g1b;
+            g1a = g2a;
+            g1b = g2b;
+            buf = buf == 2 ? 0 : buf + 1;
+        }
+    } else {
+        // super chunk = chunks kc, kc+1 in one slot; words of the current pair in
+        // g0/g1, of the next pair in g2/g3 (loaded one slot ahead)
+        constexpr int64_t kSlot = (int64_t)CT * 4096;
+        v4i g0a = v4i{}, g0b = v4i{}, g1a = v4i{}, g1b = v4i{};
+        if (kc_begin < kc_end) {
+            const int64_t c1 = kc_begin + 1 < kc_end ? kc_begin + 1 : kc_end - 1;
+            coef_chunk_to_lds<CT>(coef_grp + (kc_begin - a.kc0) * coef_chunk_bytes, lds[0], tid, wave);
+            coef_chunk_to_lds<CT>(coef_grp + (c1 - a.kc0) * coef_chunk_bytes, lds[0] + kSlot, tid, wave);
+            g0a = *reinterpret_cast<const v4i*>(prow0 + kc_begin * cpitch);
+            if (!MISS) g0b = *reinterpret_cast<const v4i*>(prow1 + kc_begin * cpitch);
+            g1a = *reinterpret_cast<const v4i*>(prow0 + c1 * cpitch);
+            if (!MISS) g1b = *reinterpret_cast<const v4i*>(prow1 + c1 * cpitch);
+        }
+        v4i op0, op1;
+        if (MISS) {
+            unpack16((uint32_t)g0a[0], op0, op1);
+        } else {
+            unpack16_g<TBL>((uint32_t)g0a[0], op0);
+            unpack16_g<TBL>((uint32_t)g0b[0], op1);
+        }
+        int buf = 0;
+        for (int64_t kc = kc_begin; kc < kc_end; kc += 2) {
+            // everything issued for this pair (coefficients into LDS by every wave,
+            // this lane's packed words) has landed; the barrier publishes the slot
+            // and frees the other one, read during the previous pair
+            __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
+            asm volatile("" ::: "memory");
+            __builtin_amdgcn_s_barrier();
+            asm volatile("" ::: "memory");
+            int64_t n0 = kc + 2 < kc_end ? kc + 2 : kc_end - 1;
+            int64_t n1 = kc + 3 < kc_end ? kc + 3 : kc_end - 1;
+            asm volatile("" : "+s"(n0));
+            asm volatile("" : "+s"(n1));
+            int8_t* nxt = lds[buf ^ 1];
+            coef_chunk_to_lds<CT>(coef_grp + (n0 - a.kc0) * coef_chunk_bytes, nxt, tid, wave);
+            coef_chunk_to_lds<CT>(coef_grp + (n1 - a.kc0) * coef_chunk_bytes, nxt + kSlot, tid, wave);
+            const v4i g2a = *reinterpret_cast<const v4i*>(prow0 + n0 * cpitch);
+            const v4i g2b = MISS ? v4i{} : *reinterpret_cast<const v4i*>(prow1 + n0 * cpitch);
+            const v4i g3a = *reinterpret_cast<const v4i*>(prow0 + n1 * cpitch);
+            const v4i g3b = MISS ? v4i{} : *reinterpret_cast<const v4i*>(prow1 + n1 * cpitch);
+            const int8_t* L = lds[buf];
+            chunk_steps<CT, MISS, TBL>(L, lane, g0a, g0b, (uint32_t)g1a[0], (uint32_t)g1b[0], op0, op1, acc);
+            if (kc + 1 < kc_end)
+                chunk_steps<CT, MISS, TBL>(L + kSlot, lane, g1a, g1b, (uint32_t)g2a[0], (uint32_t)g2b[0], op0, op1,
+                                           acc);
+            g0a = g2a;
+            g0b = g2b;
+            g1a = g3a;
+            g1b = g3b;
+            buf ^= 1;
+        }
+    }
+    // no LDS-load may still be writing when the workgroup ends
+    __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
+    contract_epilogue<NL, CTC, MISS, EPI>(a, acc, cg, rbase, lr, half);
+}
+
+
 template <int NL, int CTC, bool MISS, int EPI, int CPB = 1>
 static void launch_one(const ContractArgs& a, int n_col_groups, int n_splits, hipStream_t st) {
     constexpr int RPB = 4 * (MISS ? 32 : 64);
@@ -630,6 +644,31 @@ void launch_fill_u64(uint64_t* p, int64_t n, uint64_t v, hipStream_t st) {
     if (n <= 0) return;
     const int64_t blocks = ceil_div(n, 256) < 65536 ? ceil_div(n, 256) : 65536;
     hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)blocks), dim3(256), 0, st, p, n, v);
+    GPE_HIP(hipGetLastError());
+}
+
+// One thread per 16 bytes of the tiled copy: writes are consecutive, and the 64 lanes
+// of a wave read the 32-byte chunk pieces of 32 rows.
+__global__ void k_tile_packed(const uint8_t* __restrict__ src, int64_t stride, int64_t pieces,
+                              uint8_t* __restrict__ dst) {
+    const int64_t chunks = stride / 32;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < pieces; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t off = t * 16;
+        const int64_t tile = off / (chunks * 1024);
+        const int64_t rem = off - tile * chunks * 1024;
+        const int64_t c = rem >> 10;
+        const int64_t lr = (rem & 1023) >> 5;
+        const int64_t b = rem & 31;
+        *reinterpret_cast<v4u*>(dst + off) =
+            *reinterpret_cast<const v4u*>(src + (tile * 32 + lr) * stride + c * 32 + b);
+    }
+}
+void launch_tile_packed(const uint8_t* src, int64_t stride, int64_t rows, uint8_t* dst, hipStream_t st) {
+    if (rows <= 0) return;
+    if (rows % 32 != 0 || stride % 32 != 0) throw Error("tiled copy needs rows and stride in multiples of 32");
+    const int64_t pieces = rows * stride / 16;
+    const int64_t blocks = std::min<int64_t>(ceil_div(pieces, 256), (int64_t)1 << 20);
+    hipLaunchKernelGGL(k_tile_packed, dim3((unsigned)blocks), dim3(256), 0, st, src, stride, pieces, dst);
     GPE_HIP(hipGetLastError());
 }
 
