@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -768,12 +769,23 @@ int gpe_geno_prepare(gpe_geno* g, int32_t ddof, int32_t impute_mode, uint64_t im
             g->smiss_idx = dalloc<int32_t>(total);
             launch_miss_fill(g->snp_work, g->snp_stride, g->m, g->smiss_ptr, g->smiss_idx, st);
         }
-        // the contractions stream chunk-tiled copies: one step's 32 rows x 32 bytes are
-        // 1 KiB of consecutive bytes instead of 32 pieces 32 bytes long, one per row
-        g->snp_tiled = dalloc<uint8_t>(g->m_alloc * g->snp_stride);
-        launch_tile_packed(g->snp_work, g->snp_stride, g->m_alloc, g->snp_tiled, st);
-        g->smp_tiled = dalloc<uint8_t>(g->n_alloc * g->smp_stride);
-        launch_tile_packed(g->smp_work, g->smp_stride, g->n_alloc, g->smp_tiled, st);
+        // the contractions stream chunk-tiled copies (one step's 32 rows x 32 bytes are
+        // 1 KiB of consecutive bytes instead of 32 pieces of 32 bytes, one per row) when
+        // both copies fit with a quarter of the device's memory to spare; otherwise they
+        // read the row-major layouts (the same results).  SUMMIT_AMD_TILED=0 turns the
+        // copies off.
+        const char* tiled_env = std::getenv("SUMMIT_AMD_TILED");
+        const bool want_tiled = !(tiled_env && tiled_env[0] == '0');
+        const size_t tiled_bytes = (size_t)(g->m_alloc * g->snp_stride + g->n_alloc * g->smp_stride);
+        size_t free_b = 0, total_b = 0;
+        GPE_HIP(hipStreamSynchronize(st));
+        GPE_HIP(hipMemGetInfo(&free_b, &total_b));
+        if (want_tiled && tiled_bytes + total_b / 4 <= free_b) {
+            g->snp_tiled = dalloc<uint8_t>(g->m_alloc * g->snp_stride);
+            launch_tile_packed(g->snp_work, g->snp_stride, g->m_alloc, g->snp_tiled, st);
+            g->smp_tiled = dalloc<uint8_t>(g->n_alloc * g->smp_stride);
+            launch_tile_packed(g->smp_work, g->smp_stride, g->n_alloc, g->smp_tiled, st);
+        }
         GPE_HIP(hipStreamSynchronize(st));
         g->prepared = true;
     });
@@ -863,8 +875,8 @@ int gpe_gz_i32(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const int8_t* z_
             for (int c = 0; c < n_cols; ++c)
                 launch_fill_u8(reinterpret_cast<uint8_t*>(u_dev + (int64_t)c * ldu), g->n * 4, 0, st);
         ContractArgs a{};
-        a.packed = g->smp_tiled;
-        a.tiled = 1;
+        a.packed = g->smp_tiled ? g->smp_tiled : g->smp_work;
+        a.tiled = g->smp_tiled ? 1 : 0;
         a.stride = g->smp_stride;
         a.r0 = 0;
         a.r1 = g->n;
@@ -912,8 +924,8 @@ int gpe_gtu_i64(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const int32_t* 
         int8_t* coef = static_cast<int8_t*>(g->scratch("coef", kcount * coef_ctiles * 4096));
         launch_quant_i32(u_dev, ldu, g->n, n_cols, nl, ctc, coef_ctiles, coef, st);
         ContractArgs a{};
-        a.packed = g->snp_tiled;
-        a.tiled = 1;
+        a.packed = g->snp_tiled ? g->snp_tiled : g->snp_work;
+        a.tiled = g->snp_tiled ? 1 : 0;
         a.stride = g->snp_stride;
         a.r0 = snp_begin;
         a.r1 = snp_end;
@@ -1012,8 +1024,8 @@ void run_phase1(gpe_geno* g, int64_t s, int64_t e, const double* w, int n_bins, 
     int64_t* y = static_cast<int64_t*>(g->scratch("p1_y", (int64_t)ncols_pad * ldy * sizeof(int64_t)));
     launch_fill_u64(reinterpret_cast<uint64_t*>(y), (int64_t)ncols_pad * ldy, 0, st);
     ContractArgs a{};
-    a.packed = g->smp_tiled;
-    a.tiled = 1;
+    a.packed = g->smp_tiled ? g->smp_tiled : g->smp_work;
+    a.tiled = g->smp_tiled ? 1 : 0;
     a.stride = g->smp_stride;
     a.r0 = 0;
     a.r1 = g->n;
@@ -1131,8 +1143,8 @@ void run_phase2(gpe_geno* g, int64_t s, int64_t e, const double* x, int64_t ldx,
     if (ncols_pad != ncols) launch_fill_u8(reinterpret_cast<uint8_t*>(coef), kcount * coef_ctiles * 4096, 0, st);
     launch_quant_p2(x, ldx, g->n, n_bins, v_count, v_pad, shift, lc.nl, lc.ctc, coef_ctiles, coef, col_sum, st);
     if (sum_override) launch_col_sum_override(sum_override, n_bins, v_count, v_pad, shift, col_scale, col_sum, st);
-    a.packed = g->snp_tiled;
-    a.tiled = 1;
+    a.packed = g->snp_tiled ? g->snp_tiled : g->snp_work;
+    a.tiled = g->snp_tiled ? 1 : 0;
     a.stride = g->snp_stride;
     a.r0 = s;
     a.r1 = e;
@@ -1793,8 +1805,8 @@ int gpe_weighted_sumsq(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const do
             int64_t* me = static_cast<int64_t*>(g->scratch("wss_me", (size_t)L * v_pad * sizeof(int64_t)));
             int64_t* g2 = static_cast<int64_t*>(g->scratch("wss_g2", (size_t)L * v_pad * sizeof(int64_t)));
             ContractArgs a{};
-            a.packed = g->snp_tiled;
-            a.tiled = 1;
+            a.packed = g->snp_tiled ? g->snp_tiled : g->snp_work;
+            a.tiled = g->snp_tiled ? 1 : 0;
             a.stride = g->snp_stride;
             a.r0 = snp_begin;
             a.r1 = snp_end;

@@ -230,6 +230,34 @@ def test_phase1_missing_methods_match_oracle(engine, oracle, method):
     test_phase1_matches_oracle(engine, oracle, "mean", 2, 600, method)
 
 
+@pytest.mark.parametrize("impute", ["mean", "hwe"])
+def test_tiled_and_row_major_layouts_agree(engine, oracle, monkeypatch, impute):
+    """The contractions read chunk-tiled copies of both packed layouts when they
+    fit in device memory and the row-major layouts otherwise (SUMMIT_AMD_TILED=0
+    forces that): phase 1, phase 2 and the exact integer products are
+    bit-identical either way, with a ragged last row block (N = 1100, M = 2900)."""
+    m, n, V = 2900, 1100, 40
+    _say(f"tiled vs row-major layouts N={n} M={m} {impute}")
+    codes = random_codes(m, n, seed=41, miss_rate=0.01)
+    outs = []
+    for flag in ("0", "1"):
+        monkeypatch.setenv("SUMMIT_AMD_TILED", flag)
+        g = _geno(engine, oracle, codes).prepare(impute=impute, impute_seed=5)
+        annot = torch.ones((m, 1), dtype=torch.float64, device="cuda")
+        inv = engine.resvar_inv(g, 0, m)
+        xz = torch.zeros((V, n), dtype=torch.float64, device="cuda")
+        engine.phase1_xz(g, 0, m, 500, annot, inv, 0, V, 17, xz)
+        ms = torch.zeros((m, 1), dtype=torch.float64, device="cuda")
+        engine.phase2_accum(g, 0, m, inv, V, xz, ms, n)
+        z = (torch.randint(0, 2, (m, 64), generator=torch.Generator().manual_seed(3)) * 2 - 1).to(torch.int8).cuda()
+        u = engine.gz_i32(g, z)
+        outs.append((xz.cpu().numpy(), ms.cpu().numpy(), u.cpu().numpy()))
+        g.close()
+    monkeypatch.delenv("SUMMIT_AMD_TILED")
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("K", [7, 17])
 def test_phase1_many_bins_match_oracle(engine, oracle, K):
     """Bin counts the missing-call finalize splits into groups (7 = 4 + 2 + 1, 17 = 16 + 1)."""
