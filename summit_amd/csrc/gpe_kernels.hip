@@ -147,13 +147,21 @@ constexpr uint32_t waitcnt_vm(uint32_t n) {  // s_waitcnt vmcnt(n), other counte
     return (n & 15u) | ((n >> 4) << 14) | (7u << 4) | (15u << 8);
 }
 
+// One K chunk of a column group's coefficients (CT x 4 KiB, contiguous at src_chunk)
+// into LDS at dst by direct buffer loads (MUBUF with the LDS bit; the 256 threads take
+// 16 bytes each per 4 KiB).  The buffer form, not global_load_lds: the compiler counts
+// a global_load_lds as a FLAT access that may touch LDS, and while one is in flight it
+// turns every wait for an LDS operand read into lgkmcnt(0) -- every read issued since
+// must land -- which serializes the operand reads of the MFMA steps.  Buffer loads
+// are counted by vmcnt only, so those waits stay exact.
 template <int CT>
 __device__ __forceinline__ void coef_chunk_to_lds(const int8_t* src_chunk, int8_t* dst, int tid, int wave) {
+    const __amdgpu_buffer_rsrc_t rsrc =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<int8_t*>(src_chunk), (short)0, CT * 4096, 0x00020000);
 #pragma unroll
     for (int t = 0; t < CT; ++t)
-        __builtin_amdgcn_global_load_lds(
-            (const __attribute__((address_space(1))) void*)(src_chunk + (t * 256 + tid) * 16),
-            (__attribute__((address_space(3))) void*)(dst + (t * 256 + wave * 64) * 16), 16, 0, 0);
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, (__attribute__((address_space(3))) void*)(dst + (t * 256 + wave * 64) * 16),
+                                                 16, (t * 256 + tid) * 16, 0, 0, 0);
 }
 
 // The four MFMA steps of one 128-value chunk whose coefficients sit at L:
@@ -512,6 +520,7 @@ __global__ __launch_bounds__(256, (NL * CTC <= 2) ? 4 : 2) void k_contract(Contr
     __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
     contract_epilogue<NL, CTC, MISS, EPI>(a, acc, cg, rbase, lr, half);
 }
+
 
 
 template <int NL, int CTC, bool MISS, int EPI, int CPB = 1>
