@@ -333,6 +333,15 @@ int gpe_project_f64_noalloc(double* x_dev, int64_t ldx, int64_t n, int32_t n_col
  * (:3134-3171) after gpe_project_f64. */
 int gpe_col_sums_f64(const double* x_dev, int64_t ldx, int64_t n, int32_t n_cols, double* out_dev, void* stream);
 
+/* t_dev[k * ldt + q] = sum_{i < n} b_dev[k * ldb + i] x_dev[q * ldx + i] for k < p, q < n_cols,
+ * and, when sumsq_dev is not null, sumsq_dev[q] = sum_i x_dev[q * ldx + i]^2 (FP64,
+ * fixed-order reductions): the basis coefficients Q^T S and squared norms of a
+ * source panel whose ratio is the projected-source leakage diagnostic of
+ * DirectContext (src/native/gxeldcore.cpp:8244-8287, used by
+ * prepare_projected_sources :7578-7675 and target_block :7682-7812). */
+int gpe_basis_coef_f64(const double* x_dev, int64_t ldx, int64_t n, int32_t n_cols, const double* b_dev, int64_t ldb,
+                       int32_t p, double* t_dev, int64_t ldt, double* sumsq_dev, void* stream);
+
 /* out_dev[k * ldo + i] += sum_{snp_begin <= j < snp_end} xt_ij^2 w_dev[(j - snp_begin) + k * ldw]
  * for k < n_bins and every sample i (FP64), xt the standardized working genotype
  * (0 at a missing call under mean imputation) and, when p > 0, its projection

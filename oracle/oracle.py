@@ -34,6 +34,17 @@ Pins (each function is checked by something other than a retyping of itself):
   ``1^T x = 0``, ``||x_j||^2 / (N - p - 1) = 1``, and planted mistakes (no
   centring, environment applied after the projection, a dropped degree of
   freedom) that the formula rejects.
+* GxE DirectContext (``direct_decode``, ``direct_features``, ``direct_source``,
+  ``direct_target``, ``direct_leakage``): the decode against the reference
+  test's mean-filled ``std(ddof)`` route (``tests/test_gxe_native_core.py:
+  95-123``) and against the genome-wide decode up to sign and the
+  ``sqrt((N - ddof) / (nobs - ddof))`` factor; the dense features against the
+  reference's moment algebra (``Q^T E^p X`` moments, gxeldcore.cpp:8033-8243)
+  restated independently in the test; invariants (``norm_x = norm_w = 1``,
+  ``q^T source = 0``, ``target_x = diag(sx) X^T P X diag(a sx) Z`` through the
+  dense Gram); brute-force loops on a tiny block; planted mistakes (environment
+  applied after the projection, ``nobs - ddof`` in place of ``N - ddof``,
+  unprojected targets) that the checks reject.
 * Fused residual-variance pass (``resvar_pass``): hand-computed mu22 of one SNP
   with and without a missing call, invariance to SNP duplication and block
   size, the Cauchy-Schwarz bound of complete data, the (N - 2)/(N - 1) scaling
@@ -702,6 +713,101 @@ def gxe_scores(bed, n_total: int, m_total: int, env, C_int, annot=None, nvecs: i
     out["inv_x"] = inv_x
     out["inv_w"] = inv_w
     return out
+
+
+# ----------------------------------------------------------------------------
+# GxE DirectContext, the per-block native GxE backend (src/native/gxeldcore.cpp:
+# 7086-8877): decode :8656-8766, features :7349-7416 and :8033-8243, sources
+# :7418-7576, projected panels :7578-7680, targets :7682-7826 and :8341-8470,
+# phenotype scores :7839-8029.  q is the orthonormal (N, q) design basis spanning the
+# intercept and the environment, P = I - q q^T, rank = N - q.
+# ----------------------------------------------------------------------------
+def direct_decode(bed: np.ndarray, n_total: int, blk_start: int, blk_end: int, rows=None, ddof: int = 1):
+    """DirectContext::decode_block_into (gxeldcore.cpp:8674-8766): X (N, L) F-order with
+    x = (mean - g) inv_sd at observed calls (g the compact count 00->0, 10->1, 11->2,
+    so x is the standardized A1 dosage) and 0 at missing calls; mean over the observed
+    calls, inv_sd = sqrt((N - ddof) / m2) with m2 = sum g^2 - (sum g)^2 / nobs over
+    observed calls and N the selected sample count (1 when m2 <= 0 or N - ddof <= 0;
+    m2 in (-1e-12, 0) reads 0).  Returns (X, missing-call count)."""
+    codes, nobs, sm, ss = decode_block(bed, n_total, blk_start, blk_end, rows)
+    L, N = codes.shape
+    sf, ssf, dn = sm.astype(np.float64), ss.astype(np.float64), nobs.astype(np.float64)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        mean = np.where(nobs > 0, sf / np.where(nobs > 0, dn, 1.0), 0.0)
+        m2 = np.where(nobs > 0, ssf - sf * sf / np.where(nobs > 0, dn, 1.0), 0.0)
+    m2 = np.where((m2 < 0.0) & (m2 > -1e-12), 0.0, m2)
+    denom = N - ddof
+    inv = np.ones(L)
+    ok = (denom > 0) & (m2 > 0.0)
+    inv[ok] = np.sqrt(float(denom) / m2[ok])
+    val = np.array([0.0, 0.0, 1.0, 2.0])[codes]
+    X = np.where(codes == CODE_MISSING, 0.0, (mean[:, None] - val) * inv[:, None])
+    return np.asfortranarray(X.T), int((N - nobs).sum())
+
+
+def direct_features(X: np.ndarray, env, q) -> dict:
+    """Projected, scaled GxE features of a decoded block, in dense form: x = P X_j,
+    w = P (e . X_j), scale = sqrt(rank / ||.||^2), and the feature moments
+    norm = ||scaled||^2 / rank, diag_nxe = ||e . scaled||^2 / rank,
+    corr_xw = <x, w> / rank (the quantities compute_feature_moments_from_genotype,
+    gxeldcore.cpp:8033-8243, forms from Q^T E^p X moments).  The projected
+    features x, w are returned scaled."""
+    X = np.asarray(X, dtype=np.float64)
+    env = np.asarray(env, dtype=np.float64)
+    q = np.asarray(q, dtype=np.float64)
+    N = X.shape[0]
+    rank = N - q.shape[1]
+    x = X - q @ (q.T @ X)
+    ex = env[:, None] * X
+    w = ex - q @ (q.T @ ex)
+    ssx = np.sum(x * x, axis=0)
+    ssw = np.sum(w * w, axis=0)
+    scale_x = np.sqrt(rank / ssx)
+    scale_w = np.sqrt(rank / ssw)
+    x = x * scale_x
+    w = w * scale_w
+    e2 = (env * env)[:, None]
+    return {"x": x, "w": w, "scale_x": scale_x, "scale_w": scale_w,
+            "norm_x": np.sum(x * x, axis=0) / rank, "norm_w": np.sum(w * w, axis=0) / rank,
+            "diag_nxe_x": np.sum(e2 * x * x, axis=0) / rank, "diag_nxe_w": np.sum(e2 * w * w, axis=0) / rank,
+            "corr_xw": np.sum(x * w, axis=0) / rank, "ssx": ssx, "ssw": ssw, "rank": rank}
+
+
+def direct_source(X, env, q, scale_x, scale_w, sqrt_annotation, probes):
+    """DirectContext::source_block (gxeldcore.cpp:7418-7576): the fused source
+    [P X diag(a sx) Z, P (e . X diag(a sw) Z)], (N, 2V)."""
+    X = np.asarray(X, dtype=np.float64)
+    env = np.asarray(env, dtype=np.float64)
+    q = np.asarray(q, dtype=np.float64)
+    Z = np.asarray(probes, dtype=np.float64)
+    a = np.asarray(sqrt_annotation, dtype=np.float64)[:, None]
+    sx = X @ (Z * a * np.asarray(scale_x, dtype=np.float64)[:, None])
+    sw = env[:, None] * (X @ (Z * a * np.asarray(scale_w, dtype=np.float64)[:, None]))
+    S = np.hstack([sx, sw])
+    return S - q @ (q.T @ S)
+
+
+def direct_leakage(q, S) -> float:
+    """||q^T S||_F / ||S||_F (projected_source_leakage_from_coefficients,
+    gxeldcore.cpp:8244-8287)."""
+    S = np.asarray(S, dtype=np.float64)
+    c = np.asarray(q, dtype=np.float64).T @ S
+    return float(np.sqrt(np.sum(c * c) / max(np.sum(S * S), np.finfo(np.float64).tiny)))
+
+
+def direct_target(X, env, q, scale_x, scale_w, S):
+    """DirectContext::target_block (gxeldcore.cpp:7682-7826): with R = P S,
+    work_x = diag(sx) X^T R and work_w = diag(sw) X^T (e . R); target_projected_block
+    and phenotype scores (:8341-8470, :7839-8029) are the same products of an already
+    projected panel."""
+    X = np.asarray(X, dtype=np.float64)
+    env = np.asarray(env, dtype=np.float64)
+    q = np.asarray(q, dtype=np.float64)
+    S = np.asarray(S, dtype=np.float64)
+    R = S - q @ (q.T @ S)
+    wx = (X.T @ R) * np.asarray(scale_x, dtype=np.float64)[:, None]
+    ww = (X.T @ (env[:, None] * R)) * np.asarray(scale_w, dtype=np.float64)[:, None]
+    return wx, ww
 
 
 # ----------------------------------------------------------------------------

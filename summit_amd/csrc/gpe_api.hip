@@ -1747,6 +1747,17 @@ int gpe_col_sums_f64(const double* x_dev, int64_t ldx, int64_t n, int32_t n_cols
     });
 }
 
+int gpe_basis_coef_f64(const double* x_dev, int64_t ldx, int64_t n, int32_t n_cols, const double* b_dev, int64_t ldb,
+                       int32_t p, double* t_dev, int64_t ldt, double* sumsq_dev, void* stream) {
+    return guarded([&] {
+        require(n >= 0 && n_cols >= 0 && p >= 0, "basis-coefficient sizes must be non-negative");
+        if (n == 0 || n_cols == 0) return;
+        require(x_dev != nullptr && ldx >= n, "basis coefficients need ldx >= n");
+        if (p > 0) require(b_dev != nullptr && ldb >= n && t_dev != nullptr && ldt >= n_cols, "basis factor shapes");
+        launch_basis_coef(x_dev, ldx, n, n_cols, b_dev, ldb, p, t_dev, ldt, sumsq_dev, S(stream));
+    });
+}
+
 int gpe_sq_rowsums_f64(gpe_geno* g, int64_t snp_begin, int64_t snp_end, const double* w_dev, int64_t ldw,
                        int32_t n_bins, const double* c_dev, int64_t ldc, const double* rx_dev, int64_t ldrx, int32_t p,
                        double* out_dev, int64_t ldo, void* stream) {

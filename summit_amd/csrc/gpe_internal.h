@@ -259,6 +259,8 @@ void launch_resvar_inv(const int32_t* nobs, const int64_t* sum, const int64_t* s
 // S, mean (q_count) and a (p)
 // S[q] = sum_{i < n} x[q ldx + i] (fixed-order reduction, one workgroup per column)
 void launch_col_sums(const double* x, int64_t ldx, int64_t n, int q_count, double* S, hipStream_t st);
+void launch_basis_coef(const double* x, int64_t ldx, int64_t n, int q_count, const double* B, int64_t ldb, int p,
+                       double* T, int64_t ldt, double* s2, hipStream_t st);
 void launch_project(double* x, int64_t ldx, int64_t n, int q_count, const double* A, int64_t lda, const double* B,
                     int64_t ldb, int p, bool center, double* T, double* S, double* a, double* mean, hipStream_t st);
 

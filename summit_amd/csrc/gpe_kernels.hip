@@ -2982,6 +2982,35 @@ __global__ __launch_bounds__(256) void k_proj_apply(double* __restrict__ x, int6
     x[(int64_t)q * ldx + i] = v;
 }
 
+// s2[q] = sum_i x[q][i]^2 (fixed-order reduction)
+__global__ __launch_bounds__(256) void k_col_sumsq(const double* __restrict__ x, int64_t ldx, int64_t n,
+                                                   double* __restrict__ s2) {
+    __shared__ double red[4];
+    const int q = blockIdx.x, tid = threadIdx.x;
+    const double* xq = x + (int64_t)q * ldx;
+    double acc = 0.0;
+    for (int64_t i = tid; i < n; i += 256) acc = fma(xq[i], xq[i], acc);
+    const double w = wave_sum_f64(acc);
+    if ((tid & 63) == 0) red[tid >> 6] = w;
+    __syncthreads();
+    if (tid == 0) s2[q] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+void launch_basis_coef(const double* x, int64_t ldx, int64_t n, int q_count, const double* B, int64_t ldb, int p,
+                       double* T, int64_t ldt, double* s2, hipStream_t st) {
+    if (n <= 0 || q_count <= 0) return;
+    for (int k0 = 0; k0 < p; k0 += kProjRows) {
+        const int kb = std::min(kProjRows, p - k0);
+        hipLaunchKernelGGL(k_proj_coef, dim3((unsigned)q_count), dim3(256), 0, st, x, ldx, n, B, ldb, k0, kb, T, ldt,
+                           nullptr);
+        GPE_HIP(hipGetLastError());
+    }
+    if (s2) {
+        hipLaunchKernelGGL(k_col_sumsq, dim3((unsigned)q_count), dim3(256), 0, st, x, ldx, n, s2);
+        GPE_HIP(hipGetLastError());
+    }
+}
+
 void launch_col_sums(const double* x, int64_t ldx, int64_t n, int q_count, double* S, hipStream_t st) {
     if (q_count <= 0) return;
     hipLaunchKernelGGL(k_proj_coef, dim3((unsigned)q_count), dim3(256), 0, st, x, ldx, n, nullptr, (int64_t)0, 0, 0,

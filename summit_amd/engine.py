@@ -376,6 +376,23 @@ def col_sums(x_qn: torch.Tensor, stream=None) -> torch.Tensor:
     return out
 
 
+def basis_coef(x_qn: torch.Tensor, b_pn: torch.Tensor, stream=None) -> tuple[torch.Tensor, torch.Tensor]:
+    """(t, ss): t[k, q] = sum_n b_pn[k, n] x_qn[q, n] and ss[q] = sum_n x_qn[q, n]^2 for x_qn (Q, N)
+    and b_pn (p, N) FP64 with unit stride along N (fixed-order reductions)."""
+    for t in (x_qn, b_pn):
+        if t.dtype != torch.float64 or t.dim() != 2 or t.stride(1) != 1:
+            raise RuntimeError("basis coefficients need (rows, N) FP64 matrices with unit stride along N")
+    Q, N = x_qn.shape
+    p = b_pn.shape[0]
+    if b_pn.shape[1] != N:
+        raise RuntimeError("basis and panel disagree in N")
+    t_out = torch.empty((max(p, 1), Q), dtype=torch.float64, device=x_qn.device)
+    ss = torch.empty(Q, dtype=torch.float64, device=x_qn.device)
+    call("gpe_basis_coef_f64", _ptr(x_qn), _ld(x_qn), N, Q, _ptr(b_pn), _ld(b_pn), p, _ptr(t_out), Q, _ptr(ss),
+         _stream(stream))
+    return t_out[:p], ss
+
+
 def apply_grm(g: Genotype, panel: torch.Tensor, inv_all: torch.Tensor, nsnps: int | None = None,
               step_size: int | None = None, limbs: int = 5, stream=None) -> torch.Tensor:
     """panel_out = (1/M) X diag(inv^2) X^T panel_in over the first nsnps SNPs

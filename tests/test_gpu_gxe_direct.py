@@ -1,0 +1,296 @@
+"""The gxeldcore.DirectContext drop-in on the GPU against the oracle's restatement
+of the reference's per-block GxE backend (src/native/gxeldcore.cpp:7086-8877),
+following the reference's own tests (tests/test_gxe_native_core.py:619-1084).
+
+Tolerance: the engine rounds each FP64 coefficient column to a 44-bit fixed
+point (six int8 limbs) and every integer product is exact, so results agree with
+the FP64 oracle to 1e-10 of each output's largest magnitude (features, O(1),
+elementwise to 1e-10 relative)."""
+from __future__ import annotations
+
+import os
+from contextlib import contextmanager
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+TOL = 1e-10
+FEATURES = ("scale_x", "scale_w", "norm_x", "norm_w", "diag_nxe_x", "diag_nxe_w", "corr_xw")
+
+
+def _genotypes(rng, n, m, miss=0):
+    g = rng.binomial(2, rng.uniform(0.15, 0.45, size=m)[:, None], size=(m, n)).astype(np.int64)
+    if miss:
+        g[rng.integers(0, m, size=miss), rng.integers(0, n, size=miss)] = -9
+    return g
+
+
+def _write(oracle, tmp_path, g, name):
+    prefix = str(tmp_path / name)
+    oracle.write_bed(prefix, oracle.geno_to_codes(g))
+    return prefix
+
+
+def _design(n, ddof=1):
+    rng = np.random.default_rng(8931)
+    env = rng.normal(size=n)
+    env = (env - env.mean()) / env.std(ddof=ddof)
+    cov = rng.normal(size=(n, 2))
+    cov -= cov.mean(axis=0)
+    q, _ = np.linalg.qr(np.column_stack([np.ones(n), env, cov]))
+    return env, np.asfortranarray(q)
+
+
+@contextmanager
+def _context(prefix, env, q, **kw):
+    from summit_amd import gxeldcore
+    fds = [os.open(prefix + ext, os.O_RDONLY | os.O_CLOEXEC) for ext in (".bed", ".bim", ".fam")]
+    args = dict(row_sel=None, ddof=1, env=env, q_basis=q, decode_threads=2, max_workspace_bytes=1 << 30,
+                target_panel_columns=7)
+    args.update(kw)
+    ctx = None
+    try:
+        ctx = gxeldcore.DirectContext(bed_descriptor=fds[0], bim_descriptor=fds[1], fam_descriptor=fds[2], **args)
+        yield ctx
+    finally:
+        if ctx is not None:
+            ctx.close()
+        for fd in fds:
+            os.close(fd)
+
+
+def _close(got, want, what):
+    scale = max(float(np.max(np.abs(want))), 1e-300)
+    err = float(np.max(np.abs(np.asarray(got) - want)))
+    assert err <= TOL * scale, (what, err, scale)
+
+
+def _features_close(got, want, what=""):
+    for k in FEATURES:
+        err = np.max(np.abs(np.asarray(got[k]) - want[k]) / np.maximum(1.0, np.abs(want[k])))
+        assert err <= TOL, (what, k, err)
+
+
+def _probes(oracle, root, blk_start, L, V):
+    return oracle.gxe_probe_block(root, blk_start, list(range(V)), L)
+
+
+@pytest.mark.parametrize("n,m,V,blocks,strict", [
+    (43, 17, 13, [(0, 17)], True),
+    (43, 17, 13, [(0, 17)], False),
+    (1100, 300, 40, [(0, 150), (150, 300)], True),
+])
+def test_feature_source_target_match_oracle(oracle, tmp_path, n, m, V, blocks, strict):
+    print(f"direct context n={n} m={m} V={V} strict={strict}", flush=True)
+    rng = np.random.default_rng(1123)
+    g = _genotypes(rng, n, m)
+    prefix = _write(oracle, tmp_path, g, "complete")
+    env, q = _design(n)
+    bed = oracle.read_bed_matrix(prefix, n, m)
+    with _context(prefix, env, q, strict_feature_moment_verification=strict) as ctx:
+        info = ctx.info()
+        assert (info["n_total"], info["m_total"], info["n_selected"], info["q_rank"]) == (n, m, n, q.shape[1])
+        assert info["decode_threads"] == 2 and info["target_panel_columns"] == 7
+        assert info["projected_target_full_width"] is True
+        assert info["strict_feature_moment_verification"] is strict
+        assert info["feature_moment_integrity_mode"] == ("strict_duplicate" if strict
+                                                         else "deterministic_disjoint_output_tiled_gemm")
+        total = np.zeros((n, 2 * V))
+        parts = []
+        for s, e in blocks:
+            X, _ = oracle.direct_decode(bed, n, s, e)
+            want = oracle.direct_features(X, env, q)
+            feat = ctx.feature_block(blk_start=s, blk_end=e)
+            _features_close(feat, want, (s, e))
+            assert feat["missing_genotype_calls"] == 0
+            assert feat["max_projection_leakage_additive"] < 1e-12
+            assert feat["max_projection_leakage_interaction"] < 1e-12
+            z = _probes(oracle, 91, s, e - s, V)
+            a = np.sqrt(0.2 + np.arange(s, e, dtype=np.float64) / (m + 2.0))
+            sx_, sw_, missing = ctx.source_block(s, e, np.asarray(feat["scale_x"]), np.asarray(feat["scale_w"]), a,
+                                                 np.asfortranarray(z), True)
+            assert missing == 0
+            assert sw_.ctypes.data - sx_.ctypes.data == sx_.nbytes
+            expect = oracle.direct_source(X, env, q, want["scale_x"], want["scale_w"], a, z)
+            _close(sx_, expect[:, :V], "source_x")
+            _close(sw_, expect[:, V:], "source_w")
+            total += np.column_stack([sx_, sw_])
+            parts.append((s, e, X, feat))
+        sources = np.asfortranarray(total)
+        assert ctx.validate_projected_sources(sources, 1e-10) < 1e-12
+        sealed = sources.copy(order="F")
+        panel = ctx.prepare_projected_sources(sources, 1e-10)
+        assert panel.columns == 2 * V
+        sources[:] = rng.normal(size=sources.shape)  # the panel is a snapshot
+        for s, e, X, feat in parts:
+            wx, ww, missing, leakage = ctx.target_projected_block(s, e, np.asarray(feat["scale_x"]),
+                                                                  np.asarray(feat["scale_w"]), panel, True)
+            assert missing == 0 and leakage < 1e-12
+            ex, ew = oracle.direct_target(X, env, q, feat["scale_x"], feat["scale_w"], sealed)
+            _close(wx, ex, "work_x")
+            _close(ww, ew, "work_w")
+            # the generic path projects an arbitrary panel first
+            arbitrary = np.asfortranarray(rng.normal(size=(n, 5)))
+            gx, gw, _, gleak = ctx.target_block(s, e, np.asarray(feat["scale_x"]), np.asarray(feat["scale_w"]),
+                                                arbitrary, True)
+            assert gleak > 0.0
+            ex, ew = oracle.direct_target(X, env, q, feat["scale_x"], feat["scale_w"], arbitrary)
+            _close(gx, ex, "generic work_x")
+            _close(gw, ew, "generic work_w")
+            # per-panel leakage over target_panel_columns-wide panels: the largest of them
+            want_leak = max(oracle.direct_leakage(q, arbitrary[:, c:c + 7]) for c in range(0, 5, 7))
+            assert abs(gleak - want_leak) <= 1e-10 * want_leak
+
+
+def test_missing_calls_gate_and_mean_imputation(oracle, tmp_path):
+    print("direct context with missing calls", flush=True)
+    rng = np.random.default_rng(741)
+    n, m = 37, 11
+    g = _genotypes(rng, n, m)
+    g[1, 2] = g[1, 9] = g[7, 5] = -9
+    prefix = _write(oracle, tmp_path, g, "missing")
+    bed = oracle.read_bed_matrix(prefix, n, m)
+    env, q = _design(n)
+    X, miss = oracle.direct_decode(bed, n, 0, m)
+    assert miss == 3
+    want = oracle.direct_features(X, env, q)
+    with _context(prefix, env, q) as ctx:
+        with pytest.raises(RuntimeError, match="missing-free"):
+            ctx.feature_block(0, m, require_missing_free=True)
+        feat = ctx.feature_block(0, m, require_missing_free=False)
+        assert feat["missing_genotype_calls"] == 3
+        _features_close(feat, want)
+        # blocks without missing calls pass the gate
+        assert ctx.feature_block(2, 5, require_missing_free=True)["missing_genotype_calls"] == 0
+        probes = np.asfortranarray(rng.choice([-1.0, 1.0], size=(m, 6)))
+        sx_, sw_, smiss = ctx.source_block(0, m, feat["scale_x"], feat["scale_w"], np.ones(m), probes, False)
+        assert smiss == 3
+        expect = oracle.direct_source(X, env, q, want["scale_x"], want["scale_w"], np.ones(m), probes)
+        _close(sx_, expect[:, :6], "source_x")
+        _close(sw_, expect[:, 6:], "source_w")
+        sources = np.asfortranarray(np.column_stack([sx_, sw_]))
+        panel = ctx.prepare_projected_sources(sources, 1e-10)
+        wx, ww, tmiss, _ = ctx.target_projected_block(0, m, feat["scale_x"], feat["scale_w"], panel, False)
+        assert tmiss == 3
+        ex, ew = oracle.direct_target(X, env, q, want["scale_x"], want["scale_w"], sources)
+        _close(wx, ex, "work_x")
+        _close(ww, ew, "work_w")
+    env0, q0 = _design(n, ddof=0)
+    X0, _ = oracle.direct_decode(bed, n, 0, m, ddof=0)
+    with _context(prefix, env0, q0, ddof=0) as ctx:
+        _features_close(ctx.feature_block(0, m, require_missing_free=False), oracle.direct_features(X0, env0, q0))
+
+
+def test_phenotype_scores_match_oracle(oracle, tmp_path):
+    print("direct context phenotype scores", flush=True)
+    rng = np.random.default_rng(77291)
+    n, m, traits = 113, 47, 3
+    g = _genotypes(rng, n, m)
+    g[4, 3] = g[21, 77] = -9
+    prefix = _write(oracle, tmp_path, g, "pheno")
+    bed = oracle.read_bed_matrix(prefix, n, m)
+    env, q = _design(n)
+    X, _ = oracle.direct_decode(bed, n, 0, m)
+    want = oracle.direct_features(X, env, q)
+    y = rng.normal(size=(n, traits))
+    y -= q @ (q.T @ y)
+    y = np.asfortranarray(y)
+    with _context(prefix, env, q, target_panel_columns=traits, strict_feature_moment_verification=False) as ctx:
+        panel = ctx.prepare_projected_sources(y, 1e-10)
+        out = ctx.phenotype_score_block(0, m, panel, 1e-10, False)
+        info = ctx.info()
+    _close(out["score_x"], want["x"].T @ y, "score_x")
+    _close(out["score_w"], want["w"].T @ y, "score_w")
+    _features_close(out, want)
+    assert out["missing_genotype_calls"] == 2
+    assert out["repaired_feature_moment_columns"] == 0 and info["repaired_gemm_output_columns"] == 0
+
+
+def test_panel_is_bound_to_its_context_and_checks(oracle, tmp_path):
+    print("direct context panel ownership and checks", flush=True)
+    rng = np.random.default_rng(919)
+    n, m = 31, 8
+    g = _genotypes(rng, n, m)
+    prefix = _write(oracle, tmp_path, g, "opaque")
+    env, q = _design(n)
+    sources = rng.normal(size=(n, 6))
+    sources -= q @ (q.T @ sources)
+    sources = np.asfortranarray(sources)
+    with _context(prefix, env, q) as first, _context(prefix, env, q) as second:
+        feat = first.feature_block(0, m)
+        panel = first.prepare_projected_sources(sources, 1e-10)
+        sources[:] = np.nan
+        first.target_projected_block(0, m, feat["scale_x"], feat["scale_w"], panel, True)
+        with pytest.raises(RuntimeError, match="does not belong to this context"):
+            second.target_projected_block(0, m, feat["scale_x"], feat["scale_w"], panel, True)
+        with pytest.raises(RuntimeError, match="not in the projected fixed-effect complement"):
+            first.prepare_projected_sources(np.asfortranarray(q[:, :2] + 0.0), 1e-10)
+        with pytest.raises(RuntimeError, match="BED block is outside the BIM axis"):
+            first.feature_block(0, m + 1)
+        with pytest.raises(RuntimeError, match="target feature scale is invalid"):
+            first.target_projected_block(0, m, -feat["scale_x"], feat["scale_w"], panel, True)
+        bad = np.asfortranarray(rng.choice([-1.0, 1.0], size=(m, 3)))
+        bad[2, 1] = np.nan
+        with pytest.raises(RuntimeError, match="source probe contains a non-finite value"):
+            first.source_block(0, m, feat["scale_x"], feat["scale_w"], np.ones(m), bad, True)
+        with pytest.raises(RuntimeError, match="source input shape mismatch"):
+            first.source_block(0, m, feat["scale_x"], feat["scale_w"], np.ones(m - 1), bad, True)
+    with _context(prefix, env, q, max_workspace_bytes=1024) as small:
+        with pytest.raises(RuntimeError, match="feature block requires .* workspace bytes"):
+            small.feature_block(0, m)
+    with _context(prefix, env, q) as ctx:
+        pass
+    with pytest.raises(RuntimeError, match="context is closed"):
+        ctx.feature_block(0, m)
+
+
+def test_tiny_scales_stay_finite(oracle, tmp_path):
+    """Interaction weights formed directly, not through a scale ratio
+    (tests/test_gxe_native_core.py:710-731)."""
+    rng = np.random.default_rng(731)
+    n, m = 37, 9
+    prefix = _write(oracle, tmp_path, _genotypes(rng, n, m), "tiny")
+    env, q = _design(n)
+    probes = np.asfortranarray(rng.choice([-1.0, 1.0], size=(m, 5)))
+    with _context(prefix, env, q) as ctx:
+        sx_, sw_, missing = ctx.source_block(0, m, np.full(m, 1.0e-320), np.ones(m), np.ones(m), probes, True)
+    assert missing == 0
+    assert np.isfinite(sx_).all() and np.isfinite(sw_).all()
+    assert np.max(np.abs(sw_)) > 0.0
+
+
+def test_replaced_files_and_row_snapshots(oracle, tmp_path):
+    print("direct context file identity and row snapshots", flush=True)
+    rng = np.random.default_rng(334)
+    n, m = 62, 13
+    ga, gb = _genotypes(rng, n, m), _genotypes(rng, n, m)
+    prefix = _write(oracle, tmp_path, ga, "stable")
+    replacement = _write(oracle, tmp_path, gb, "replacement")
+    env, q = _design(n)
+    with _context(prefix, env, q) as ctx:
+        before = ctx.feature_block(0, m)
+        for ext in (".bed", ".bim", ".fam"):
+            os.replace(replacement + ext, prefix + ext)
+        with pytest.raises(RuntimeError, match="changed after context construction"):
+            ctx.feature_block(0, m)
+    Xa, _ = oracle.direct_decode(oracle.pack_codes(oracle.geno_to_codes(ga)), n, 0, m)
+    _features_close(before, oracle.direct_features(Xa, env, q))
+    # a row selection is a snapshot taken at construction
+    row_sel = np.arange(0, n, 2, dtype=np.int64)
+    env2, q2 = _design(row_sel.size)
+    bed_b = oracle.read_bed_matrix(prefix, n, m)
+    with _context(prefix, env2, q2, row_sel=row_sel) as ctx:
+        even = ctx.feature_block(0, m)
+        row_sel[:] = np.arange(1, n, 2, dtype=np.int64)
+        again = ctx.feature_block(0, m)
+    assert np.array_equal(even["scale_x"], again["scale_x"])
+    X_even, _ = oracle.direct_decode(bed_b, n, 0, m, rows=np.arange(0, n, 2))
+    _features_close(even, oracle.direct_features(X_even, env2, q2))
+    with _context(prefix, env2, q2, row_sel=row_sel) as ctx:
+        odd = ctx.feature_block(0, m)
+    X_odd, _ = oracle.direct_decode(bed_b, n, 0, m, rows=row_sel)
+    _features_close(odd, oracle.direct_features(X_odd, env2, q2))
+    assert not np.array_equal(even["scale_x"], odd["scale_x"])
