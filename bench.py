@@ -452,6 +452,102 @@ def dropin_loop(engine, ldscore, torch) -> dict:
                                                       / np.abs(fast.ldscore[:, 0] + M / (N - 1.0))))}
 
 
+def gxe_direct_loop(engine, torch) -> dict:
+    """What a caller of the GxE drop-in gets: the reference driver's direct
+    per-block loop (src/summit/ldscore/gwe_ldscore.py:2218-2260, :2716-2830,
+    :3683-3759) through summit_amd.gxeldcore.DirectContext with NumPy arrays, at
+    N=50,000, M=100,000 (block-Markov genotypes, seed 404, no missing calls, as the
+    production gate requires), one environment (standard normal, root 1404), the
+    design basis [1, e], one bin, B=16 Rademacher probes drawn by NumPy Philox per
+    (block, probe) as the driver draws them, blocks of 1,000 SNPs: features per
+    block, sources per block summed into the global sketch, the sketch sealed, then
+    targets per block with Work^2 accumulated into XX / XW / WX / WW.  The BED is
+    written to a temporary file and made resident when the context is built (timed
+    separately).  Beside it the engine's fast path (gxe.gxe_ldscore, one traversal
+    per phase over all blocks) on the same genotypes and probes.  Multiply-adds:
+    sources N*M*2B, targets N*M*4B."""
+    import tempfile
+
+    from oracle import oracle as o
+    from summit_amd import gxe, gxeldcore
+    N, M, B, step, root = 50_000, 100_000, 16, 1000, 1404
+    g = engine.Genotype.synthetic(N, M, seed=404, maf_lo=0.05, maf_hi=0.5, miss_rate=0.0, block_len=100,
+                                  copy_prob=0.9)
+    g.prepare(ddof=1, impute="mean")
+    body = g.download_rows("snp_raw", 0, M)
+    rng = np.random.default_rng(root)
+    env = rng.standard_normal(N)
+    env = (env - env.mean()) / env.std(ddof=1)
+    q = np.asfortranarray(np.linalg.qr(np.column_stack([np.ones(N), env]))[0])
+    blocks = [(s, min(M, s + step)) for s in range(0, M, step)]
+    df = float(N - 2)
+    with tempfile.TemporaryDirectory() as d:
+        prefix = os.path.join(d, "gxe")
+        with open(prefix + ".bed", "wb") as f:
+            f.write(o.BED_MAGIC)
+            f.write(body.tobytes())
+        with open(prefix + ".fam", "w") as f:
+            f.write("x\n" * N)
+        with open(prefix + ".bim", "w") as f:
+            f.write("x\n" * M)
+        del body
+        fds = [os.open(prefix + ext, os.O_RDONLY | os.O_CLOEXEC) for ext in (".bed", ".bim", ".fam")]
+        try:
+            def run(ctx):
+                sx_all, sw_all = np.empty(M), np.empty(M)
+                for s, e in blocks:
+                    f = ctx.feature_block(blk_start=s, blk_end=e, eps_var=1e-10, require_missing_free=False)
+                    sx_all[s:e], sw_all[s:e] = f["scale_x"], f["scale_w"]
+                sketch = np.zeros((N, 2 * B), order="F")
+                for s, e in blocks:
+                    z = np.empty((e - s, B), order="F")
+                    for v in range(B):
+                        gen = np.random.Generator(np.random.Philox(gxe.make_seed(root, s, v)))
+                        z[:, v] = 2.0 * gen.integers(0, 2, size=e - s, dtype=np.int8) - 1.0
+                    sx_, sw_, _ = ctx.source_block(s, e, sx_all[s:e], sw_all[s:e], np.ones(e - s), z, True)
+                    sketch[:, :B] += sx_
+                    sketch[:, B:] += sw_
+                panel = ctx.prepare_projected_sources(sketch, 1e-9)
+                acc = np.zeros((M, 4))
+                for s, e in blocks:
+                    wx, ww, _, _ = ctx.target_projected_block(s, e, sx_all[s:e], sw_all[s:e], panel, True)
+                    acc[s:e, 0] = np.sum(wx[:, :B] ** 2, axis=1)
+                    acc[s:e, 1] = np.sum(wx[:, B:] ** 2, axis=1)
+                    acc[s:e, 2] = np.sum(ww[:, :B] ** 2, axis=1)
+                    acc[s:e, 3] = np.sum(ww[:, B:] ** 2, axis=1)
+                return acc / (df * df * B)
+
+            warm = gxeldcore.DirectContext(fds[0], fds[1], fds[2], None, 1, env, q, 1, 64 << 30, 64)
+            run(warm)  # warm-up: workspaces, schedules
+            warm.close()
+            # a fresh context for the timed loop: its first feature call computes every SNP's features
+            t0 = time.perf_counter()
+            ctx = gxeldcore.DirectContext(fds[0], fds[1], fds[2], None, 1, env, q, 1, 64 << 30, 64)
+            t_resident = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            scores = run(ctx)
+            t_loop = time.perf_counter() - t0
+            ctx.close()
+        finally:
+            for fd in fds:
+                os.close(fd)
+    c_int = (env / np.linalg.norm(env))[:, None]
+    gxe.gxe_ldscore(g, env, C_int=c_int, nvecs=B, seed=root, step_size=step, limbs=6)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fast = gxe.gxe_ldscore(g, env, C_int=c_int, nvecs=B, seed=root, step_size=step, limbs=6)
+    t_fast = time.perf_counter() - t0
+    g.close()
+    ref = np.column_stack([fast[k][:, 0] for k in ("xx", "xw", "wx", "ww")])
+    macs = 6.0 * N * M * B
+    return {"workload": "N=50000 M=100000, one environment, basis [1, e], one bin, B=16, 100 blocks of 1000 SNPs, "
+                        "NumPy arrays through summit_amd.gxeldcore.DirectContext (the reference driver's direct loop)",
+            "dropin_s": round(t_loop, 3), "dropin_gmac_s": round(macs / t_loop / 1e9, 1),
+            "fast_path_s": round(t_fast, 3), "fast_path_gmac_s": round(macs / t_fast / 1e9, 1),
+            "bed_resident_s": round(t_resident, 3),
+            "max_rel_diff_vs_fast_path": float(np.max(np.abs(scores - ref) / np.abs(ref)))}
+
+
 def h2_end_to_end(engine, ldscore, torch) -> dict:
     """The LD scores' consumer end to end at N=50,000, M=200,000 (block-Markov
     genotypes, seed 202, 0.5% missing): partitioned LD scores (a base bin plus 8
@@ -654,7 +750,8 @@ def main() -> int:
     ap.add_argument("--no-tune", action="store_true", help="keep the built-in schedule heuristics")
     ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph replay timing of the same steps")
     ap.add_argument("--only", choices=["learned_cv", "partitioned", "gxe", "windowed", "wall_to_1pct",
-                                       "cpu_mailman_c1", "h2_end_to_end", "dropin_loop", "vendor_int8_gemm"],
+                                       "cpu_mailman_c1", "h2_end_to_end", "dropin_loop", "gxe_direct_loop",
+                                       "vendor_int8_gemm"],
                     help="run only one extra measurement and print it")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the 1%%-error wall-time run and the host->HBM ingest measurement")
@@ -689,6 +786,9 @@ def main() -> int:
         return 0
     if args.only == "dropin_loop":
         print(json.dumps({"dropin_loop": dropin_loop(engine, ldscore, torch)}), flush=True)
+        return 0
+    if args.only == "gxe_direct_loop":
+        print(json.dumps({"gxe_direct_loop": gxe_direct_loop(engine, torch)}), flush=True)
         return 0
     if args.only == "vendor_int8_gemm":
         print(json.dumps({"vendor_int8_gemm": vendor_int8_gemm(torch, dev)}), flush=True)
@@ -891,6 +991,8 @@ def main() -> int:
         log(f"partitioned: {extra['partitioned']}")
         extra["dropin_loop"] = dropin_loop(engine, ldscore, torch)
         log(f"drop-in loop: {extra['dropin_loop']}")
+        extra["gxe_direct_loop"] = gxe_direct_loop(engine, torch)
+        log(f"GxE drop-in loop: {extra['gxe_direct_loop']}")
         extra["h2_end_to_end"] = h2_end_to_end(engine, ldscore, torch)
         log(f"h2 end to end: {extra['h2_end_to_end']}")
         vg = vendor_int8_gemm(torch, dev)

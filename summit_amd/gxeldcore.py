@@ -9,11 +9,14 @@ libsummit_gpe on the GPU:
 * construction makes the selected samples of the BED body behind the descriptor
   resident in HBM once; every block is served from there (the reference
   memory-maps the file and decodes each block on the host);
-* features: the per-SNP sums sum_n e_n^p x_nj^2 (p = 0, 1, 2, 4) in one pass
+* features depend on the SNP alone, so the first call that needs them computes
+  every SNP's: the per-SNP sums sum_n e_n^p x_nj^2 (p = 0, 1, 2, 4) in one pass
   (gpe_weighted_sumsq) and the moments [Q, EQ, E^2 Q, E^3 Q]^T X in one
-  phase-2 traversal (gpe_xtv_f64); the per-SNP algebra on them is FP64;
+  phase-2 traversal (gpe_xtv_f64), the per-SNP algebra on them in FP64; each
+  block's call returns its slice, checked as the reference checks it;
 * sources X diag(a s) Z: one phase-1 traversal of FP64 coefficients (gpe_xc_f64,
-  six int8 limbs), then e . and the projection I - Q Q^T (gpe_project_f64);
+  six int8 limbs), then e . and the projection I - Q Q^T (gpe_project_f64),
+  returned in pinned host memory (one DMA);
 * targets X^T [R, e . R]: one phase-2 traversal over both halves (gpe_xtv_f64);
 * a ProjectedPanel keeps its [S, e . S] in HBM, so target blocks re-read it there.
 
@@ -176,6 +179,7 @@ class DirectContext:
         self._fds: list[int] = []
         self._map = None
         self._g = None
+        self._ftab = None
         self._lock = threading.Lock()
         self._id = next(_IDS)
         self._ddof = int(ddof)
@@ -400,15 +404,21 @@ class DirectContext:
         out.mul_(self._neg_r_d[blk_start:blk_end, None])
         return out
 
-    def _features(self, blk_start: int, blk_end: int, eps_var: float) -> dict:
-        """compute_feature_moments_from_genotype (gxeldcore.cpp:8033-8243) on the GPU
-        moments; returns host arrays."""
-        p = self._q
+    _FEATURE_NAMES = ("scale_x", "scale_w", "norm_x", "norm_w", "diag_nxe_x", "diag_nxe_w", "corr_xw")
+
+    def _feature_table(self) -> np.ndarray:
+        """compute_feature_moments_from_genotype (gxeldcore.cpp:8033-8243) for every SNP
+        at once, at the first call that needs features: one weighted-sum-of-squares
+        pass and one moment traversal over the resident matrix serve every block.
+        Rows: the seven feature vectors, varx, varw and the two leakage ratios (M,)."""
+        if self._ftab is not None:
+            return self._ftab
+        M, p = self._m_total, self._q
         rank = float(self._n - p)
-        r2 = self._neg_r_d[blk_start:blk_end] ** 2
-        sums = weighted_sumsq(self._g, self._w4_d, blk_start, blk_end) * r2[None, :]
+        r2 = self._neg_r_d ** 2
+        sums = weighted_sumsq(self._g, self._w4_d, 0, M) * r2[None, :]
         s0, s1, s2, s4 = sums[0], sums[1], sums[2], sums[3]
-        U = self._products(blk_start, blk_end, self._v4_d)
+        U = self._products(0, M, self._v4_d)
         u0, u1, u2, u3 = U[:, :p], U[:, p:2 * p], U[:, 2 * p:3 * p], U[:, 3 * p:]
         u00 = (u0 * u0).sum(dim=1)
         u11 = (u1 * u1).sum(dim=1)
@@ -425,41 +435,48 @@ class DirectContext:
         varw = ssw / rank
         sx = 1.0 / torch.sqrt(varx)
         sw = 1.0 / torch.sqrt(varw)
-        out = {
-            "scale_x": sx, "scale_w": sw,
-            "norm_x": sx * sx * ssx / rank, "norm_w": sw * sw * ssw / rank,
-            "diag_nxe_x": sx * sx * (s2 - 2.0 * u0u2 + u0e2u0) / rank,
-            "diag_nxe_w": sw * sw * (s4 - 2.0 * u1u3 + u1e2u1) / rank,
-            "corr_xw": sx * sw * (s1 - u01) / rank,
-        }
-        leak_x = torch.sqrt(torch.clamp((lx * lx).sum(dim=1), min=0.0) / torch.clamp(ssx, min=_DBL_MIN))
-        leak_w = torch.sqrt(torch.clamp((lw * lw).sum(dim=1), min=0.0) / torch.clamp(ssw, min=_DBL_MIN))
-        host = {k: v.cpu().numpy() for k, v in out.items()}
-        vx, vw = varx.cpu().numpy(), varw.cpu().numpy()
+        rows = [sx, sw, sx * sx * ssx / rank, sw * sw * ssw / rank,
+                sx * sx * (s2 - 2.0 * u0u2 + u0e2u0) / rank, sw * sw * (s4 - 2.0 * u1u3 + u1e2u1) / rank,
+                sx * sw * (s1 - u01) / rank, varx, varw,
+                torch.sqrt(torch.clamp((lx * lx).sum(dim=1), min=0.0) / torch.clamp(ssx, min=_DBL_MIN)),
+                torch.sqrt(torch.clamp((lw * lw).sum(dim=1), min=0.0) / torch.clamp(ssw, min=_DBL_MIN))]
+        self._ftab = torch.stack(rows).cpu().numpy()
+        return self._ftab
+
+    def _features(self, blk_start: int, blk_end: int, eps_var: float) -> dict:
+        """The block's features (host arrays), with the reference's per-column checks in
+        column order: zero or invalid variance, then non-finite outputs."""
+        tab = self._feature_table()[:, blk_start:blk_end]
+        k = len(self._FEATURE_NAMES)
+        vx, vw = tab[k], tab[k + 1]
         bad_var = ~(np.isfinite(vx) & np.isfinite(vw) & (vx > eps_var) & (vw > eps_var))
-        bad_out = ~np.logical_and.reduce([np.isfinite(v) for v in host.values()])
+        bad_out = ~np.isfinite(tab[:k]).all(axis=0)
         first_var = int(np.argmax(bad_var)) if bad_var.any() else None
         first_out = int(np.argmax(bad_out)) if bad_out.any() else None
         if first_var is not None and (first_out is None or first_var <= first_out):
             raise RuntimeError(f"GxE native projected feature has zero or invalid variance at block offset {first_var}")
         if first_out is not None:
             raise RuntimeError("GxE native feature output contains a non-finite value")
-        host["max_projection_leakage_additive"] = float(leak_x.max().item())
-        host["max_projection_leakage_interaction"] = float(leak_w.max().item())
+        host = {name: tab[i].copy() for i, name in enumerate(self._FEATURE_NAMES)}
+        host["max_projection_leakage_additive"] = float(np.max(tab[k + 2]))
+        host["max_projection_leakage_interaction"] = float(np.max(tab[k + 3]))
         return host
 
     def _scaled_work(self, W: torch.Tensor, columns: int, sx: np.ndarray, sw: np.ndarray):
-        """scale_target_outputs (gxeldcore.cpp:8301-8326): (L, 2C) products -> F-order
-        work_x, work_w (L, C) host arrays scaled per SNP."""
-        sx_d = torch.as_tensor(sx, device=self._dev)[:, None]
-        sw_d = torch.as_tensor(sw, device=self._dev)[:, None]
-        wx = (W[:, :columns] * sx_d).cpu().numpy()
-        ww = (W[:, columns:] * sw_d).cpu().numpy()
-        if not bool(np.isfinite(wx).all()):
+        """scale_target_outputs (gxeldcore.cpp:8301-8326): (L, 2C) products -> work_x,
+        work_w (L, C) scaled per SNP, F-order views of one pinned host array."""
+        W[:, :columns].mul_(torch.as_tensor(sx, device=self._dev)[:, None])
+        W[:, columns:].mul_(torch.as_tensor(sw, device=self._dev)[:, None])
+        ok = torch.stack([torch.isfinite(W[:, :columns]).all(), torch.isfinite(W[:, columns:]).all()]).cpu()
+        if not bool(ok[0]):
             raise RuntimeError("GxE native work_x contains a non-finite value")
-        if not bool(np.isfinite(ww).all()):
+        if not bool(ok[1]):
             raise RuntimeError("GxE native work_w contains a non-finite value")
-        return np.asfortranarray(wx), np.asfortranarray(ww)
+        wt = W.t().contiguous()  # (2C, L): column-major (L, 2C)
+        host = torch.empty(wt.shape, dtype=torch.float64, pin_memory=True)
+        host.copy_(wt)
+        both = host.numpy().T
+        return both[:, :columns], both[:, columns:]
 
     # ------------------------------------------------------------------ API
     def close(self) -> None:
@@ -568,11 +585,13 @@ class DirectContext:
                 out.mul_(torch.as_tensor(np.ldexp(1.0, part), device=self._dev)[:, None])
             out[V:].mul_(self._env_d[None, :])
             self._project(out)
-            host = out.cpu().numpy()
-            if not bool(np.isfinite(host).all()):
+            if not bool(torch.isfinite(out).all().item()):
                 raise RuntimeError("GxE native fused source contains a non-finite value")
+            # the returned arrays live in a fresh pinned host tensor: one DMA, no staging copy
+            host = torch.empty(out.shape, dtype=torch.float64, pin_memory=True)
+            host.copy_(out)
             self._check_files()
-            fused = host.T  # (N, 2V), F-order
+            fused = host.numpy().T  # (N, 2V), F-order
             return fused[:, :V], fused[:, V:], missing
 
     def prepare_projected_sources(self, sources, tolerance=1.0e-10) -> ProjectedPanel:
