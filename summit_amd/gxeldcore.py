@@ -706,3 +706,27 @@ class DirectContext:
                 "missing_genotype_calls": missing,
             })
             return res
+
+
+# --------------------------------------------------------------------------- probes
+def numpy_philox_rademacher_block(keys, rows, threads) -> np.ndarray:
+    """gxeldcore.numpy_philox_rademacher_block (gxeldcore.cpp:10070-10136): an F-order
+    (rows, probes) FP64 block of +-1 probes, column p the stream NumPy's
+    Generator(Philox(key=keys[p])).integers(0, 2, rows, dtype=int8) gives (2 x - 1).
+    The streams are generated on the GPU (gpe_philox_rademacher, one counter-based
+    thread per Philox block)."""
+    threads = int(threads)
+    if threads <= 0:
+        raise RuntimeError("Protected GxE GEMM threads must be positive")
+    if threads > len(os.sched_getaffinity(0)):
+        raise RuntimeError("Protected GxE GEMM threads exceed the OpenMP/CPU-affinity limit")
+    k = np.ascontiguousarray(np.asarray(keys, dtype=np.uint64))
+    rows = int(rows)
+    if k.ndim != 2 or rows <= 0 or k.shape[0] <= 0 or k.shape[1] != 2:
+        raise RuntimeError("Native Philox key matrix must be probes-by-two")
+    P = int(k.shape[0])
+    starts = np.zeros(1, dtype=np.int64)
+    z = torch.empty((rows, P), dtype=torch.int8, device=engine.default_device())
+    engine.call("gpe_philox_rademacher", int(k.ctypes.data), int(starts.ctypes.data), 1, rows, P, engine._ptr(z), P,
+                engine._stream())
+    return np.asfortranarray(z.to(torch.float64).cpu().numpy())

@@ -294,3 +294,21 @@ def test_replaced_files_and_row_snapshots(oracle, tmp_path):
     X_odd, _ = oracle.direct_decode(bed_b, n, 0, m, rows=row_sel)
     _features_close(odd, oracle.direct_features(X_odd, env2, q2))
     assert not np.array_equal(even["scale_x"], odd["scale_x"])
+
+
+def test_numpy_philox_rademacher_block_matches_numpy():
+    """Columns equal NumPy's own Philox streams for the given keys."""
+    from summit_amd import gxe, gxeldcore
+    seeds = np.array([[gxe.make_seed(77, 0, p) for p in range(5)]], dtype=np.uint64)
+    keys = gxe.seed_sequence_philox_keys(seeds).reshape(-1, 2)
+    rows = 1003
+    got = gxeldcore.numpy_philox_rademacher_block(keys, rows, 1)
+    assert got.shape == (rows, 5) and got.flags.f_contiguous
+    for p in range(5):
+        gen = np.random.Generator(np.random.Philox(key=keys[p]))
+        want = 2.0 * gen.integers(0, 2, size=rows, dtype=np.int8) - 1.0
+        np.testing.assert_array_equal(got[:, p], want)
+    with pytest.raises(RuntimeError, match="probes-by-two"):
+        gxeldcore.numpy_philox_rademacher_block(keys[:, :1], rows, 1)
+    with pytest.raises(RuntimeError, match="threads must be positive"):
+        gxeldcore.numpy_philox_rademacher_block(keys, rows, 0)
