@@ -342,6 +342,20 @@ int gpe_col_sums_f64(const double* x_dev, int64_t ldx, int64_t n, int32_t n_cols
 int gpe_basis_coef_f64(const double* x_dev, int64_t ldx, int64_t n, int32_t n_cols, const double* b_dev, int64_t ldb,
                        int32_t p, double* t_dev, int64_t ldt, double* sumsq_dev, void* stream);
 
+/* Per-SNP feature algebra of the GxE per-block backend (DirectContext,
+ * src/native/gxeldcore.cpp:8033-8243), one thread per SNP, FP64.  Inputs for
+ * n_snps SNPs: sums_dev rows k < 4 (sums_dev[k * lds + j]) the sums of squared
+ * standardized genotypes weighted by 1, e, e^2, e^4 (gpe_weighted_sumsq);
+ * moments_dev[j * ldm + c], c < 4q, the moments [Q, EQ, E^2 Q, E^3 Q]^T x_j
+ * (gpe_xtv_f64) in the target standardization; r2_dev[j] the factor that takes
+ * the sums to it; gram_dev / e2gram_dev the q x q matrices Q^T Q and Q^T E^2 Q
+ * (element [b * q + a]); rank = N - q.  Output rows (out_dev[r * ldo + j]):
+ * scale_x, scale_w, norm_x, norm_w, diag_nxe_x, diag_nxe_w, corr_xw, varx,
+ * varw and the two projection-leakage ratios. */
+int gpe_gxe_direct_features(const double* sums_dev, int64_t lds, const double* moments_dev, int64_t ldm,
+                            int32_t q, const double* gram_dev, const double* e2gram_dev, const double* r2_dev,
+                            double rank, int64_t n_snps, double* out_dev, int64_t ldo, void* stream);
+
 /* out_dev[k * ldo + i] += sum_{snp_begin <= j < snp_end} xt_ij^2 w_dev[(j - snp_begin) + k * ldw]
  * for k < n_bins and every sample i (FP64), xt the standardized working genotype
  * (0 at a missing call under mean imputation) and, when p > 0, its projection

@@ -56,6 +56,8 @@ SIGNATURES = {
     "gpe_project_f64_noalloc": (ctypes.c_int, [c_vp, c_i64, c_i64, c_i32, c_vp, c_i64, c_vp, c_i64, c_i32, c_i32,
                                                c_vp]),
     "gpe_col_sums_f64": (ctypes.c_int, [c_vp, c_i64, c_i64, c_i32, c_vp, c_vp]),
+    "gpe_gxe_direct_features": (ctypes.c_int, [c_vp, c_i64, c_vp, c_i64, c_i32, c_vp, c_vp, c_vp,
+                                               ctypes.c_double, c_i64, c_vp, c_i64, c_vp]),
     "gpe_basis_coef_f64": (ctypes.c_int, [c_vp, c_i64, c_i64, c_i32, c_vp, c_i64, c_i32, c_vp, c_i64, c_vp,
                                           c_vp]),
     "gpe_weighted_sumsq": (ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_i32, c_i64, c_vp, c_i64, c_vp]),

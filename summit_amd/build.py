@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 LIB_NAME = "libsummit_gpe.so"
 LIB_PATH = os.path.join(HERE, LIB_NAME)
-SOURCES = ["gpe_kernels.hip", "gpe_api.hip", "gpe_mlp.hip"]
+SOURCES = ["gpe_kernels.hip", "gpe_api.hip", "gpe_mlp.hip", "gpe_gxe.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GPE_OFFLOAD_ARCH", "gfx950")

@@ -12,7 +12,8 @@ libsummit_gpe on the GPU:
 * features depend on the SNP alone, so the first call that needs them computes
   every SNP's: the per-SNP sums sum_n e_n^p x_nj^2 (p = 0, 1, 2, 4) in one pass
   (gpe_weighted_sumsq) and the moments [Q, EQ, E^2 Q, E^3 Q]^T X in one
-  phase-2 traversal (gpe_xtv_f64), the per-SNP algebra on them in FP64; each
+  phase-2 traversal (gpe_xtv_f64), the per-SNP algebra on them in FP64
+  (gpe_gxe_direct_features, one thread per SNP); each
   block's call returns its slice, checked as the reference checks it;
 * sources X diag(a s) Z: one phase-1 traversal of FP64 coefficients (gpe_xc_f64,
   six int8 limbs), then e . and the projection I - Q Q^T (gpe_project_f64),
@@ -414,33 +415,14 @@ class DirectContext:
         if self._ftab is not None:
             return self._ftab
         M, p = self._m_total, self._q
-        rank = float(self._n - p)
         r2 = self._neg_r_d ** 2
-        sums = weighted_sumsq(self._g, self._w4_d, 0, M) * r2[None, :]
-        s0, s1, s2, s4 = sums[0], sums[1], sums[2], sums[3]
+        sums = weighted_sumsq(self._g, self._w4_d, 0, M)
         U = self._products(0, M, self._v4_d)
-        u0, u1, u2, u3 = U[:, :p], U[:, p:2 * p], U[:, 2 * p:3 * p], U[:, 3 * p:]
-        u00 = (u0 * u0).sum(dim=1)
-        u11 = (u1 * u1).sum(dim=1)
-        u01 = (u0 * u1).sum(dim=1)
-        u0u2 = (u0 * u2).sum(dim=1)
-        u1u3 = (u1 * u3).sum(dim=1)
-        u0e2u0 = (u0 * (u0 @ self._e2gram_d)).sum(dim=1)
-        u1e2u1 = (u1 * (u1 @ self._e2gram_d)).sum(dim=1)
-        lx = u0 - u0 @ self._gram_d
-        lw = u1 - u1 @ self._gram_d
-        ssx = s0 - u00
-        ssw = s2 - u11
-        varx = ssx / rank
-        varw = ssw / rank
-        sx = 1.0 / torch.sqrt(varx)
-        sw = 1.0 / torch.sqrt(varw)
-        rows = [sx, sw, sx * sx * ssx / rank, sw * sw * ssw / rank,
-                sx * sx * (s2 - 2.0 * u0u2 + u0e2u0) / rank, sw * sw * (s4 - 2.0 * u1u3 + u1e2u1) / rank,
-                sx * sw * (s1 - u01) / rank, varx, varw,
-                torch.sqrt(torch.clamp((lx * lx).sum(dim=1), min=0.0) / torch.clamp(ssx, min=_DBL_MIN)),
-                torch.sqrt(torch.clamp((lw * lw).sum(dim=1), min=0.0) / torch.clamp(ssw, min=_DBL_MIN))]
-        self._ftab = torch.stack(rows).cpu().numpy()
+        tab = torch.empty((11, M), dtype=torch.float64, device=self._dev)
+        engine.call("gpe_gxe_direct_features", engine._ptr(sums), engine._ld(sums), engine._ptr(U), engine._ld(U), p,
+                    engine._ptr(self._gram_d), engine._ptr(self._e2gram_d), engine._ptr(r2), float(self._n - p), M,
+                    engine._ptr(tab), M, engine._stream())
+        self._ftab = tab.cpu().numpy()
         return self._ftab
 
     def _features(self, blk_start: int, blk_end: int, eps_var: float) -> dict:
